@@ -1,0 +1,106 @@
+// Internal declarations of libasdpocs: error plumbing, geometry records and
+// the device-side geometry helpers shared by the projector and driver kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "asdpocs.h"
+
+namespace asd {
+
+// ---------------------------------------------------------------------------
+// Errors: every HIP call is checked; failures become a status code plus a
+// thread-local message readable through asd_last_error().
+// ---------------------------------------------------------------------------
+void set_error(int code, const std::string &msg);
+void clear_error();
+int last_status();
+
+#define ASD_HIP(call)                                                              \
+    do {                                                                           \
+        hipError_t e_ = (call);                                                    \
+        if (e_ != hipSuccess) {                                                    \
+            ::asd::set_error(ASD_ERR_HIP, std::string(#call) + ": " +              \
+                                              hipGetErrorString(e_));              \
+            return ASD_ERR_HIP;                                                    \
+        }                                                                          \
+    } while (0)
+
+#define ASD_LAUNCH_CHECK(what)                                                     \
+    do {                                                                           \
+        hipError_t e_ = hipGetLastError();                                         \
+        if (e_ != hipSuccess) {                                                    \
+            ::asd::set_error(ASD_ERR_HIP, std::string(what) + ": " +               \
+                                              hipGetErrorString(e_));              \
+            return ASD_ERR_HIP;                                                    \
+        }                                                                          \
+    } while (0)
+
+#define ASD_ARG(cond, msg)                                                         \
+    do {                                                                           \
+        if (!(cond)) {                                                             \
+            ::asd::set_error(ASD_ERR_ARG, msg);                                    \
+            return ASD_ERR_ARG;                                                    \
+        }                                                                          \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// Geometry of one view in voxel units (what the reference gateways compute,
+// src/AxAtx/Ax_main_test.cpp:138-161).
+// ---------------------------------------------------------------------------
+struct ViewGeom {
+    float cos_phi, sin_phi;  // rotation of this view
+    float SO, OD;            // source-isocentre, isocentre-detector (voxels)
+    float scale;             // voxel size in mm (outputs are multiplied by it)
+    float dz;                // axial voxel pitch (1 in the reference)
+    float dy_det, dz_det;    // detector pitch (voxels), used by the backprojector footprint
+    int nx, ny, nz, na, nb;
+};
+
+// Per-ray record for the backprojector: the six slope quotients the
+// reference's find_l_3d forms per (voxel, ray) (cu:142-196) depend only on the
+// ray, so they are formed once per ray with the same IEEE divisions.
+struct alignas(32) RayRec {
+    float s_ba, s_ab, s_ca, s_cb, s_ac, s_bc;
+    int mode;   // RAY_* below
+    float pad;
+};
+
+// find_l_3d dispatch (cu:131-200 and find_l cu:26-113)
+enum : int {
+    RAY_X_FIRST = 0,      // a,b != 0, |a| >  |b|: x faces, y faces, z faces
+    RAY_Y_FIRST = 1,      // a,b != 0, |a| <= |b|: y faces, x faces, z faces
+    RAY_A0_B0 = 2,        // a == 0, b == 0: 2-D (y,z) chord, axis-parallel in y -> dz
+    RAY_A0_C0 = 3,        // a == 0, c == 0: |pz| <= dz/2 -> dy
+    RAY_A0_YZ_U = 4,      // a == 0: (y,z) chord, |b| >  |c|
+    RAY_A0_YZ_W = 5,      // a == 0: (y,z) chord, |b| <= |c|
+    RAY_B0_C0 = 6,        // b == 0, c == 0: |pz| <= dz/2 -> dx
+    RAY_B0_XZ_U = 7,      // b == 0: (x,z) chord, |a| >  |c|
+    RAY_B0_XZ_W = 8,      // b == 0: (x,z) chord, |a| <= |c|
+};
+
+// Host helpers shared by the translation units.
+struct DeviceBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t n);   // grow-only
+    void release();
+    ~DeviceBuf() { release(); }
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+int launch_ray_table(const ViewGeom &g, const float *y_det, const float *z_det, RayRec *rt,
+                     hipStream_t s);
+int launch_ax(const ViewGeom &g, const float *x, float *y, float *sr, const float *y_det,
+              const float *z_det, hipStream_t s);
+int launch_atx(const ViewGeom &g, float *x, const float *y, float *sc, const float *y_det,
+               const float *z_det, const RayRec *rt, hipStream_t s);
+
+// Lanes cooperating on one ray of the forward projector.
+int ax_lanes_per_ray(int na, int nb, int nu);
+
+}  // namespace asd

@@ -1,0 +1,289 @@
+// Device-side geometry of the projectors.  Each function evaluates the
+// reference's FP32 expressions in the reference's order with contraction
+// off, so the GPU reproduces its rounding (the FP32 chord end points lose
+// ~1e-4 voxel to cancellation at far sources; a different evaluation order
+// would move every length by that much).
+#pragma once
+
+#include "asd_internal.hpp"
+
+namespace asd {
+
+__device__ __forceinline__ float absf(float a) { return a > 0 ? a : -a; }
+
+// ---------------------------------------------------------------------------
+// Forward projector: one unit slab of a ray (src/AxAtx/Ax_cone_gpu.cu:376-571).
+// In the slab the sorted transverse extent [ta,tb] and the sorted axial extent
+// [za,zb] (sorted independently, as the reference does) are split at the cell
+// crossings rt = (ct2-ta)/(tb-ta) and rz = (cz2-za)/(zb-za) (1 without a
+// crossing).  The pieces [0,min], [min,max], [max,1] lie in cells (ct1,cz1),
+// (ct2,cz1) or (ct1,cz2), and (ct2,cz2); each in-range piece adds w*l*x.
+// This one rule covers the reference's sixteen labelled cases 11..44.
+// ---------------------------------------------------------------------------
+struct AxRay {
+    float u1, t1s, s1, s2, z1, dz, q, tmp;
+    int nu2, nt2, nz2, nt, nz;
+    int xdriven;
+};
+
+__device__ __forceinline__ AxRay ax_ray_setup(const ViewGeom &g, float ydet, float zdet)
+{
+#pragma clang fp contract(off)
+    AxRay r;
+    float x1 = g.cos_phi * (-g.SO), y1 = g.sin_phi * (-g.SO), z1 = 0.0f;
+    float x2 = g.cos_phi * g.OD - g.sin_phi * ydet;
+    float y2 = g.sin_phi * g.OD + g.cos_phi * ydet;
+    float z2 = zdet;
+    r.tmp = (x1 - x2) * (x1 - x2) + (y1 - y2) * (y1 - y2);
+    r.q = r.tmp + (z1 - z2) * (z1 - z2);
+    r.xdriven = absf(x1 - x2) > absf(y1 - y2);
+    r.u1 = r.xdriven ? x1 : y1;
+    r.t1s = r.xdriven ? y1 : x1;
+    float du = r.xdriven ? (x2 - x1) : (y2 - y1);
+    float dt = r.xdriven ? (y2 - y1) : (x2 - x1);
+    r.s1 = dt / du;
+    r.s2 = (z2 - z1) / du;
+    r.z1 = z1;
+    r.dz = g.dz;
+    r.nu2 = r.xdriven ? g.nx / 2 : g.ny / 2;
+    r.nt2 = r.xdriven ? g.ny / 2 : g.nx / 2;
+    r.nt = r.xdriven ? g.ny : g.nx;
+    r.nz = g.nz;
+    r.nz2 = g.nz / 2;
+    return r;
+}
+
+// Accumulate slab iu of ray r into (acc, accl); X is the volume (x fastest).
+__device__ __forceinline__ void ax_slab(const AxRay &r, const float *__restrict__ X, int iu,
+                                        int nx, int ny, float &acc, float &accl)
+{
+#pragma clang fp contract(off)
+    float ua = (float)(iu - r.nu2), ub = ua + 1;
+    float ta, tb, za, zb;
+    if (r.s1 >= 0) {
+        ta = r.t1s + r.s1 * (ua - r.u1) + r.nt2;
+        tb = r.t1s + r.s1 * (ub - r.u1) + r.nt2;
+    } else {
+        ta = r.t1s + r.s1 * (ub - r.u1) + r.nt2;
+        tb = r.t1s + r.s1 * (ua - r.u1) + r.nt2;
+    }
+    if (r.s2 >= 0) {
+        za = (r.z1 + r.s2 * (ua - r.u1)) / r.dz + r.nz2;
+        zb = (r.z1 + r.s2 * (ub - r.u1)) / r.dz + r.nz2;
+    } else {
+        za = (r.z1 + r.s2 * (ub - r.u1)) / r.dz + r.nz2;
+        zb = (r.z1 + r.s2 * (ua - r.u1)) / r.dz + r.nz2;
+    }
+    int ct1 = (int)floorf(ta), ct2 = (int)floorf(tb);
+    int cz1 = (int)floorf(za), cz2 = (int)floorf(zb);
+    bool in_t1 = ct1 >= 0 && ct1 < r.nt, in_t2 = ct2 >= 0 && ct2 < r.nt;
+    bool in_z1 = cz1 >= 0 && cz1 < r.nz, in_z2 = cz2 >= 0 && cz2 < r.nz;
+    if (!((in_t1 || in_t2) && (in_z1 || in_z2))) return;
+    float rt = (ct1 == ct2) ? 1.0f : ((float)ct2 - ta) / (tb - ta);
+    float rz = (cz1 == cz2) ? 1.0f : ((float)cz2 - za) / (zb - za);
+    float d = tb - ta;
+    float l = sqrtf((d * d + 1) * r.q / r.tmp);
+    bool tfirst = !(rt > rz);  // the t crossing comes first (ties: t)
+    float lo = tfirst ? rt : rz, hi = tfirst ? rz : rt;
+    // piece 1: (ct1, cz1)
+    float w0 = lo, w1 = hi - lo, w2 = 1.0f - hi;
+    int ctm = tfirst ? ct2 : ct1, czm = tfirst ? cz1 : cz2;
+    bool inm = (tfirst ? in_t2 : in_t1) && (tfirst ? in_z1 : in_z2);
+    const size_t sxy = (size_t)nx * ny;
+    if (in_t1 && in_z1) {
+        size_t idx = r.xdriven ? (size_t)cz1 * sxy + (size_t)ct1 * nx + iu
+                               : (size_t)cz1 * sxy + (size_t)iu * nx + ct1;
+        float wl = w0 * l;
+        acc += wl * X[idx];
+        accl += wl;
+    }
+    if (inm) {
+        size_t idx = r.xdriven ? (size_t)czm * sxy + (size_t)ctm * nx + iu
+                               : (size_t)czm * sxy + (size_t)iu * nx + ctm;
+        float wl = w1 * l;
+        acc += wl * X[idx];
+        accl += wl;
+    }
+    if (in_t2 && in_z2) {
+        size_t idx = r.xdriven ? (size_t)cz2 * sxy + (size_t)ct2 * nx + iu
+                               : (size_t)cz2 * sxy + (size_t)iu * nx + ct2;
+        float wl = w2 * l;
+        acc += wl * X[idx];
+        accl += wl;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Backprojector: line ∩ voxel length (find_l_3d cu:120-209, find_l cu:16-117)
+// evaluated from the per-ray slope record.  (px,py,pz) = source - voxel centre.
+// ---------------------------------------------------------------------------
+
+// Take a candidate hit in test order; the first two define the chord.
+struct Hits {
+    float p0x, p0y, p0z, p1x, p1y, p1z;
+    int n;
+    __device__ __forceinline__ void take(bool ok, float x, float y, float z)
+    {
+        if (ok) {
+            if (n == 0) { p0x = x; p0y = y; p0z = z; }
+            else if (n == 1) { p1x = x; p1y = y; p1z = z; }
+            n++;
+        }
+    }
+};
+
+// General 2-D chord (find_l with a' != 0, b' != 0) in plane (u,w):
+// pu = u1_0 - uc, pw = w1_0 - wc, s_wu = b'/a', s_uw = a'/b'.
+__device__ __forceinline__ float chord2(float pu, float pw, float s_wu, float s_uw, float hu,
+                                        float hw, bool u_first)
+{
+#pragma clang fp contract(off)
+    float cw = s_wu * (-pu) + pw, ow = s_wu * hu;   // u faces: w coordinate
+    float cu = s_uw * (-pw) + pu, ou = s_uw * hw;   // w faces: u coordinate
+    bool um = absf(cw - ow) <= hw, up = absf(cw + ow) <= hw;
+    bool wm = absf(cu - ou) <= hu, wp = absf(cu + ou) <= hu;
+    Hits H;
+    H.n = 0;
+    H.p0z = H.p1z = 0.0f;
+    if (u_first) {
+        H.take(um, -hu, cw - ow, 0.0f);
+        H.take(up, hu, cw + ow, 0.0f);
+        H.take(wm, cu - ou, -hw, 0.0f);
+        H.take(wp, cu + ou, hw, 0.0f);
+    } else {
+        H.take(wm, cu - ou, -hw, 0.0f);
+        H.take(wp, cu + ou, hw, 0.0f);
+        H.take(um, -hu, cw - ow, 0.0f);
+        H.take(up, hu, cw + ow, 0.0f);
+    }
+    if (H.n < 2) return 0.0f;
+    float ex = H.p1x - H.p0x, ey = H.p1y - H.p0y;
+    return sqrtf(ex * ex + ey * ey);
+}
+
+__device__ __forceinline__ float find_l_3d_rec(const RayRec &R, float px, float py, float pz,
+                                               float hz, float dz)
+{
+#pragma clang fp contract(off)
+    const float hx = 0.5f, hy = 0.5f;
+    switch (R.mode) {
+    case RAY_X_FIRST:
+    case RAY_Y_FIRST: {
+        float xy_c = R.s_ba * (-px) + py, xy_o = R.s_ba * hx;
+        float xz_c = R.s_ca * (-px) + pz, xz_o = R.s_ca * hx;
+        float yx_c = R.s_ab * (-py) + px, yx_o = R.s_ab * hy;
+        float yz_c = R.s_cb * (-py) + pz, yz_o = R.s_cb * hy;
+        float zx_c = R.s_ac * (-pz) + px, zx_o = R.s_ac * hz;
+        float zy_c = R.s_bc * (-pz) + py, zy_o = R.s_bc * hz;
+        bool xm = absf(xy_c - xy_o) <= hy && absf(xz_c - xz_o) <= hz;
+        bool xp = absf(xy_c + xy_o) <= hy && absf(xz_c + xz_o) <= hz;
+        bool ym = absf(yx_c - yx_o) <= hx && absf(yz_c - yz_o) <= hz;
+        bool yp = absf(yx_c + yx_o) <= hx && absf(yz_c + yz_o) <= hz;
+        bool zm = absf(zx_c - zx_o) <= hx && absf(zy_c - zy_o) <= hy;
+        bool zp = absf(zx_c + zx_o) <= hx && absf(zy_c + zy_o) <= hy;
+        Hits H;
+        H.n = 0;
+        if (R.mode == RAY_X_FIRST) {
+            H.take(xm, -hx, xy_c - xy_o, xz_c - xz_o);
+            H.take(xp, hx, xy_c + xy_o, xz_c + xz_o);
+            H.take(ym, yx_c - yx_o, -hy, yz_c - yz_o);
+            H.take(yp, yx_c + yx_o, hy, yz_c + yz_o);
+        } else {
+            H.take(ym, yx_c - yx_o, -hy, yz_c - yz_o);
+            H.take(yp, yx_c + yx_o, hy, yz_c + yz_o);
+            H.take(xm, -hx, xy_c - xy_o, xz_c - xz_o);
+            H.take(xp, hx, xy_c + xy_o, xz_c + xz_o);
+        }
+        H.take(zm, zx_c - zx_o, zy_c - zy_o, -hz);
+        H.take(zp, zx_c + zx_o, zy_c + zy_o, hz);
+        if (H.n < 2) return 0.0f;
+        float ex = H.p1x - H.p0x, ey = H.p1y - H.p0y, ez = H.p1z - H.p0z;
+        return sqrtf(ex * ex + ey * ey + ez * ez);
+    }
+    case RAY_A0_B0: return absf(py) <= hy ? dz : 0.0f;
+    case RAY_A0_C0: return absf(pz) <= hz ? 1.0f : 0.0f;
+    case RAY_A0_YZ_U: return chord2(py, pz, R.s_cb, R.s_bc, hy, hz, true);
+    case RAY_A0_YZ_W: return chord2(py, pz, R.s_cb, R.s_bc, hy, hz, false);
+    case RAY_B0_C0: return absf(pz) <= hz ? 1.0f : 0.0f;
+    case RAY_B0_XZ_U: return chord2(px, pz, R.s_ca, R.s_ac, hx, hz, true);
+    default: return chord2(px, pz, R.s_ca, R.s_ac, hx, hz, false);
+    }
+}
+
+// Slope record of ray (ia, ib) of view g (the divisions of cu:142-196).
+__device__ __forceinline__ RayRec make_ray_rec(const ViewGeom &g, float ydet, float zdet)
+{
+#pragma clang fp contract(off)
+    float x1 = g.cos_phi * (-g.SO), y1 = g.sin_phi * (-g.SO), z1 = 0.0f;
+    float x2 = g.cos_phi * g.OD - g.sin_phi * ydet;
+    float y2 = g.sin_phi * g.OD + g.cos_phi * ydet;
+    float z2 = zdet;
+    float a = x2 - x1, b = y2 - y1, c = z2 - z1;
+    RayRec R;
+    R.s_ba = b / a; R.s_ab = a / b;
+    R.s_ca = c / a; R.s_cb = c / b;
+    R.s_ac = a / c; R.s_bc = b / c;
+    R.pad = 0.0f;
+    if (a == 0) {
+        if (b == 0) R.mode = RAY_A0_B0;
+        else if (c == 0) R.mode = RAY_A0_C0;
+        else R.mode = absf(b) > absf(c) ? RAY_A0_YZ_U : RAY_A0_YZ_W;
+    } else if (b == 0) {
+        if (c == 0) R.mode = RAY_B0_C0;
+        else R.mode = absf(a) > absf(c) ? RAY_B0_XZ_U : RAY_B0_XZ_W;
+    } else {
+        R.mode = absf(a) > absf(b) ? RAY_X_FIRST : RAY_Y_FIRST;
+    }
+    return R;
+}
+
+// Detector footprint of voxel (ix,iy,iz) (cu:230-257): bins ia in [a_lo,a_hi],
+// ib in [b_lo,b_hi] (clamped), plus the voxel centre.
+struct Footprint {
+    int a_lo, a_hi, b_lo, b_hi;
+    float xc, yc, zc;
+};
+
+__device__ __forceinline__ Footprint footprint(const ViewGeom &g, int ix, int iy, int iz)
+{
+#pragma clang fp contract(off)
+    Footprint f;
+    float SD = g.SO + g.OD;
+    int na2 = g.na / 2, nb2 = g.nb / 2;
+    int nx2 = g.nx / 2, ny2 = g.ny / 2, nz2 = g.nz / 2;
+    float d = sqrtf((1 + g.dz * g.dz) / 2);
+    f.zc = (float)(iz + 0.5 - nz2) * g.dz;
+    f.yc = (float)(iy + 0.5 - ny2);
+    f.xc = (float)(ix + 0.5 - nx2);
+    float xr = g.cos_phi * f.xc + g.sin_phi * f.yc;
+    float yr = -g.sin_phi * f.xc + g.cos_phi * f.yc;
+    float t = SD / ((xr + g.SO) * g.dy_det);
+    int a_hi = (int)floorf((yr + 1) * t + na2);
+    int a_lo = (int)floorf((yr - 1) * t + na2);
+    t = SD / ((xr + g.SO) * g.dz_det);
+    int b_hi = (int)floorf((f.zc + d) * t + nb2);
+    int b_lo = (int)floorf((f.zc - d) * t + nb2);
+    f.a_lo = a_lo < 0 ? 0 : a_lo;
+    f.b_lo = b_lo < 0 ? 0 : b_lo;
+    f.a_hi = a_hi > g.na - 1 ? g.na - 1 : a_hi;
+    f.b_hi = b_hi > g.nb - 1 ? g.nb - 1 : b_hi;
+    return f;
+}
+
+// XCD-aware block order: hardware hands consecutive workgroups to the 8 XCDs
+// round-robin; remap so that each XCD walks a contiguous range of tiles and
+// neighbouring tiles share its L2.
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nblocks)
+{
+    constexpr unsigned NXCD = 8;
+    if (nblocks < NXCD * 2) return b;
+    unsigned per = nblocks / NXCD, rem = nblocks % NXCD;
+    unsigned xcd = b % NXCD, k = b / NXCD;
+    // XCDs < rem own per+1 tiles
+    unsigned start = xcd * per + (xcd < rem ? xcd : rem);
+    unsigned cnt = per + (xcd < rem ? 1u : 0u);
+    if (k < cnt) return start + k;
+    return b;  // unreachable when nblocks is covered exactly
+}
+
+}  // namespace asd

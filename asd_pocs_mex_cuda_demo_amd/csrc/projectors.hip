@@ -1,0 +1,242 @@
+// Forward projector (ray-driven) and backprojector (voxel-driven) for gfx950,
+// plus the drop-in host-pointer launchers that replace the reference's
+// Ax_cone_mf_gpu_new / Atx_cone_mf_gpu_new (src/AxAtx/Ax_cone_gpu.cu:778, :299).
+#include <mutex>
+
+#include "projector_kernels.hpp"
+
+namespace asd {
+
+namespace {
+
+struct EpiPlain {
+    float *y, *sr;
+    float scale;
+    __device__ __forceinline__ void operator()(long id, float acc, float accl) const
+    {
+#pragma clang fp contract(off)
+        y[id] = acc * scale;
+        sr[id] = accl * scale;
+    }
+};
+
+struct EpiBackPlain {
+    float *x, *sc;
+    float scale;
+    __device__ __forceinline__ void operator()(long v, float acc, float accl) const
+    {
+#pragma clang fp contract(off)
+        x[v] = acc * scale;
+        sc[v] = accl * scale;
+    }
+};
+
+__global__ __launch_bounds__(256) void ray_table_kernel(ViewGeom g,
+                                                        const float *__restrict__ y_det,
+                                                        const float *__restrict__ z_det,
+                                                        RayRec *__restrict__ rt)
+{
+    long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= (long)g.na * g.nb) return;
+    int ia = (int)(id % g.na), ib = (int)(id / g.na);
+    rt[id] = make_ray_rec(g, y_det[ia], z_det[ib]);
+}
+
+}  // namespace
+
+int ax_lanes_per_ray(int na, int nb, int nu)
+{
+    // Aim for about 2^20 working lanes (4096 waves over 256 CUs) while keeping
+    // at least 4 slabs per lane; a 2-D view (a few hundred rays) gets a full
+    // wave per ray.
+    long rays = (long)na * nb;
+    int lpr = 1;
+    while (lpr < 64 && rays * lpr < (1L << 20) && nu / (2 * lpr) >= 4) lpr *= 2;
+    return lpr;
+}
+
+int launch_ray_table(const ViewGeom &g, const float *y_det, const float *z_det, RayRec *rt,
+                     hipStream_t s)
+{
+    long n = (long)g.na * g.nb;
+    unsigned blocks = (unsigned)((n + 255) / 256);
+    ray_table_kernel<<<blocks, 256, 0, s>>>(g, y_det, z_det, rt);
+    ASD_LAUNCH_CHECK("ray_table_kernel");
+    return ASD_OK;
+}
+
+int launch_ax(const ViewGeom &g, const float *x, float *y, float *sr, const float *y_det,
+              const float *z_det, hipStream_t s)
+{
+    int nu_max = g.nx > g.ny ? g.nx : g.ny;
+    int lpr = ax_lanes_per_ray(g.na, g.nb, nu_max);
+    return dispatch_ax(lpr, g, x, y_det, z_det, EpiPlain{y, sr, g.scale}, s);
+}
+
+int launch_atx(const ViewGeom &g, float *x, const float *y, float *sc, const float *y_det,
+               const float *z_det, const RayRec *rt, hipStream_t s)
+{
+    long N = (long)g.nx * g.ny * g.nz;
+    unsigned blocks = (unsigned)((N + 255) / 256);
+    atx_kernel<<<blocks, 256, 0, s>>>(g, y, rt, EpiBackPlain{x, sc, g.scale});
+    ASD_LAUNCH_CHECK("atx_kernel");
+    return ASD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Process-wide context of the drop-in launchers: one stream and grow-only
+// device buffers reused across calls (the reference allocates and frees five
+// buffers per call, cu:308-330 and :787-804).
+// ---------------------------------------------------------------------------
+namespace {
+
+struct DropIn {
+    std::mutex mu;
+    hipStream_t stream = nullptr;
+    DeviceBuf vol, vol2, proj, proj2, ydet, zdet, rt;
+    int init()
+    {
+        if (!stream) ASD_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        return ASD_OK;
+    }
+};
+
+DropIn &dropin()
+{
+    static DropIn *d = new DropIn();  // lives until process exit
+    return *d;
+}
+
+int check_geom(const ViewGeom &g)
+{
+    ASD_ARG(g.nx > 0 && g.ny > 0 && g.nz > 0 && g.na > 0 && g.nb > 0,
+            "sizes nx, ny, nz, na, nb must be positive");
+    ASD_ARG((long)g.nx * g.ny * g.nz < (1L << 40), "volume too large");
+    return ASD_OK;
+}
+
+ViewGeom make_geom(float cos_phi, float sin_phi, float SO, float OD, float scale,
+                   float dy_det, float dz_det, float dz, int nx, int ny, int nz, int na, int nb)
+{
+    ViewGeom g;
+    g.cos_phi = cos_phi; g.sin_phi = sin_phi; g.SO = SO; g.OD = OD; g.scale = scale;
+    g.dz = dz; g.dy_det = dy_det; g.dz_det = dz_det;
+    g.nx = nx; g.ny = ny; g.nz = nz; g.na = na; g.nb = nb;
+    return g;
+}
+
+int ax_host(float *X, float *y, float *sr, const ViewGeom &g, const float *y_det,
+            const float *z_det)
+{
+    int rc = check_geom(g);
+    if (rc) return rc;
+    ASD_ARG(X && y && sr && y_det && z_det, "NULL pointer argument");
+    DropIn &d = dropin();
+    std::lock_guard<std::mutex> lk(d.mu);
+    if ((rc = d.init())) return rc;
+    size_t N = (size_t)g.nx * g.ny * g.nz, M = (size_t)g.na * g.nb;
+    if ((rc = d.vol.ensure(N * 4)) || (rc = d.proj.ensure(M * 4)) ||
+        (rc = d.proj2.ensure(M * 4)) || (rc = d.ydet.ensure(g.na * 4)) ||
+        (rc = d.zdet.ensure(g.nb * 4)))
+        return rc;
+    ASD_HIP(hipMemcpyAsync(d.vol.p, X, N * 4, hipMemcpyHostToDevice, d.stream));
+    ASD_HIP(hipMemcpyAsync(d.ydet.p, y_det, g.na * 4, hipMemcpyHostToDevice, d.stream));
+    ASD_HIP(hipMemcpyAsync(d.zdet.p, z_det, g.nb * 4, hipMemcpyHostToDevice, d.stream));
+    if ((rc = launch_ax(g, d.vol.as<float>(), d.proj.as<float>(), d.proj2.as<float>(),
+                        d.ydet.as<float>(), d.zdet.as<float>(), d.stream)))
+        return rc;
+    ASD_HIP(hipMemcpyAsync(y, d.proj.p, M * 4, hipMemcpyDeviceToHost, d.stream));
+    ASD_HIP(hipMemcpyAsync(sr, d.proj2.p, M * 4, hipMemcpyDeviceToHost, d.stream));
+    ASD_HIP(hipStreamSynchronize(d.stream));
+    return ASD_OK;
+}
+
+int atx_host(float *X, const float *y, float *sc, const ViewGeom &g, const float *y_det,
+             const float *z_det)
+{
+    int rc = check_geom(g);
+    if (rc) return rc;
+    ASD_ARG(X && y && sc && y_det && z_det, "NULL pointer argument");
+    DropIn &d = dropin();
+    std::lock_guard<std::mutex> lk(d.mu);
+    if ((rc = d.init())) return rc;
+    size_t N = (size_t)g.nx * g.ny * g.nz, M = (size_t)g.na * g.nb;
+    if ((rc = d.vol.ensure(N * 4)) || (rc = d.vol2.ensure(N * 4)) ||
+        (rc = d.proj.ensure(M * 4)) || (rc = d.ydet.ensure(g.na * 4)) ||
+        (rc = d.zdet.ensure(g.nb * 4)) || (rc = d.rt.ensure(M * sizeof(RayRec))))
+        return rc;
+    ASD_HIP(hipMemcpyAsync(d.proj.p, y, M * 4, hipMemcpyHostToDevice, d.stream));
+    ASD_HIP(hipMemcpyAsync(d.ydet.p, y_det, g.na * 4, hipMemcpyHostToDevice, d.stream));
+    ASD_HIP(hipMemcpyAsync(d.zdet.p, z_det, g.nb * 4, hipMemcpyHostToDevice, d.stream));
+    if ((rc = launch_ray_table(g, d.ydet.as<float>(), d.zdet.as<float>(), d.rt.as<RayRec>(),
+                               d.stream)))
+        return rc;
+    if ((rc = launch_atx(g, d.vol.as<float>(), d.proj.as<float>(), d.vol2.as<float>(),
+                         d.ydet.as<float>(), d.zdet.as<float>(), d.rt.as<RayRec>(), d.stream)))
+        return rc;
+    ASD_HIP(hipMemcpyAsync(X, d.vol.p, N * 4, hipMemcpyDeviceToHost, d.stream));
+    ASD_HIP(hipMemcpyAsync(sc, d.vol2.p, N * 4, hipMemcpyDeviceToHost, d.stream));
+    ASD_HIP(hipStreamSynchronize(d.stream));
+    return ASD_OK;
+}
+
+// Device-pointer forms keep one ray table per thread of the caller.
+thread_local DeviceBuf tl_rt;
+
+}  // namespace
+}  // namespace asd
+
+using namespace asd;
+
+extern "C" {
+
+void Ax_cone_mf_gpu_new(float *X, float *y, float *sr, float cos_phi, float sin_phi,
+                        float *y_det, float *z_det, float SO, float OD, float scale, float dz,
+                        int nx, int ny, int nz, int na, int nb)
+{
+    clear_error();
+    ViewGeom g = make_geom(cos_phi, sin_phi, SO, OD, scale, 1.0f, 1.0f, dz, nx, ny, nz, na, nb);
+    ax_host(X, y, sr, g, y_det, z_det);
+}
+
+void Atx_cone_mf_gpu_new(float *X, float *y, float *sc, float cos_phi, float sin_phi,
+                         float *y_det, float *z_det, float SO, float OD, float scale,
+                         float dy_det, float dz_det, float dz, int nx, int ny, int nz, int na,
+                         int nb)
+{
+    clear_error();
+    ViewGeom g = make_geom(cos_phi, sin_phi, SO, OD, scale, dy_det, dz_det, dz, nx, ny, nz, na,
+                           nb);
+    atx_host(X, y, sc, g, y_det, z_det);
+}
+
+int asd_ax_device(const float *X, float *y, float *sr, float cos_phi, float sin_phi,
+                  const float *y_det, const float *z_det, float SO, float OD, float scale,
+                  float dz, int nx, int ny, int nz, int na, int nb, void *stream)
+{
+    clear_error();
+    ViewGeom g = make_geom(cos_phi, sin_phi, SO, OD, scale, 1.0f, 1.0f, dz, nx, ny, nz, na, nb);
+    int rc = check_geom(g);
+    if (rc) return rc;
+    ASD_ARG(X && y && sr && y_det && z_det, "NULL pointer argument");
+    return launch_ax(g, X, y, sr, y_det, z_det, (hipStream_t)stream);
+}
+
+int asd_atx_device(float *X, const float *y, float *sc, float cos_phi, float sin_phi,
+                   const float *y_det, const float *z_det, float SO, float OD, float scale,
+                   float dy_det, float dz_det, float dz, int nx, int ny, int nz, int na, int nb,
+                   void *stream)
+{
+    clear_error();
+    ViewGeom g = make_geom(cos_phi, sin_phi, SO, OD, scale, dy_det, dz_det, dz, nx, ny, nz, na,
+                           nb);
+    int rc = check_geom(g);
+    if (rc) return rc;
+    ASD_ARG(X && y && sc && y_det && z_det, "NULL pointer argument");
+    if ((rc = tl_rt.ensure((size_t)na * nb * sizeof(RayRec)))) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    if ((rc = launch_ray_table(g, y_det, z_det, tl_rt.as<RayRec>(), s))) return rc;
+    return launch_atx(g, X, y, sc, y_det, z_det, tl_rt.as<RayRec>(), s);
+}
+
+}  // extern "C"

@@ -1,0 +1,71 @@
+// Deterministic FP64 reductions: wave butterflies, one partial per block, and
+// the last block to finish folds the partials in a fixed order.  The result
+// does not depend on block scheduling, so repeated runs are bit-identical.
+#pragma once
+
+#include "asd_internal.hpp"
+
+namespace asd {
+
+constexpr int RED_THREADS = 256;
+constexpr int RED_MAX_BLOCKS = 2048;  // grid cap of the grid-stride reductions
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Sum of `val` over the block (256 threads); valid in thread 0.
+__device__ __forceinline__ double block_sum(double val)
+{
+    __shared__ double sh[RED_THREADS / 64];
+    val = wave_sum(val);
+    const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+    if (l == 0) sh[w] = val;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int i = 0; i < RED_THREADS / 64; i++) s += sh[i];
+    }
+    __syncthreads();
+    return s;
+}
+
+// Each block contributes `val`; the last block writes the total to *out and
+// re-arms *counter.  partials must hold gridDim.x doubles.
+__device__ __forceinline__ void grid_sum_to(double val, double *partials, unsigned *counter,
+                                            double *out)
+{
+    __shared__ bool last;
+    double s = block_sum(val);
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x] = s;
+        unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL,
+                                               __HIP_MEMORY_SCOPE_AGENT);
+        last = (prev == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!last) return;
+    __atomic_thread_fence(__ATOMIC_ACQUIRE);
+    double t = 0.0;
+    for (unsigned i = threadIdx.x; i < gridDim.x; i += blockDim.x)
+        t += __hip_atomic_load(&partials[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    t = block_sum(t);
+    if (threadIdx.x == 0) {
+        *out = t;
+        __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+inline unsigned red_blocks(size_t n)
+{
+    size_t b = (n + RED_THREADS - 1) / RED_THREADS;
+    if (b > RED_MAX_BLOCKS) b = RED_MAX_BLOCKS;
+    if (b == 0) b = 1;
+    return (unsigned)b;
+}
+
+}  // namespace asd

@@ -1,0 +1,274 @@
+// TV steepest descent (replaces src/tvmin/imgTVGradMin.cpp and
+// imgTVGradMin3D.cpp): the gradient of the zero-padded isotropic TV functional
+// (2-D 7-point / 3-D 13-point stencil), its FP64 norm, and the normalised step
+// I -= alpha*d*v/||v|| with d = ||I - I0|| fixed for the call.
+#include <mutex>
+
+#include "reduce_device.hpp"
+#include "tv.hpp"
+
+namespace asd {
+
+namespace {
+
+template <class T, class A>
+__device__ __forceinline__ A ld(const T *u, int nx, int ny, int nz, int x, int y, int z)
+{
+    if (x < 0 || x >= nx || y < 0 || y >= ny || z < 0 || z >= nz) return A(0);
+    return A(u[((size_t)z * ny + y) * nx + x]);
+}
+
+// 2-D stencil at row r (fastest) and column c (imgTVGradMin.cpp:84-97).
+template <class T, class A>
+__device__ __forceinline__ A tv2_grad(const T *u, int R, int C, int r, int c)
+{
+#pragma clang fp contract(off)
+    const A eps = A(1e-8);
+    A p = ld<T, A>(u, R, C, 1, r, c, 0);
+    A rm = ld<T, A>(u, R, C, 1, r - 1, c, 0), cm = ld<T, A>(u, R, C, 1, r, c - 1, 0);
+    A rp = ld<T, A>(u, R, C, 1, r + 1, c, 0), rp_cm = ld<T, A>(u, R, C, 1, r + 1, c - 1, 0);
+    A cp = ld<T, A>(u, R, C, 1, r, c + 1, 0), rm_cp = ld<T, A>(u, R, C, 1, r - 1, c + 1, 0);
+    A g1 = ((p - rm) + (p - cm)) / sqrt(eps + (p - rm) * (p - rm) + (p - cm) * (p - cm));
+    A g2 = -(rp - p) / sqrt(eps + (rp - p) * (rp - p) + (rp - rp_cm) * (rp - rp_cm));
+    A g3 = -(cp - p) / sqrt(eps + (cp - p) * (cp - p) + (cp - rm_cp) * (cp - rm_cp));
+    return g1 + g2 + g3;
+}
+
+// 3-D stencil at (x,y,z), x fastest (imgTVGradMin3D.cpp:114-140).
+template <class T, class A>
+__device__ __forceinline__ A tv3_grad(const T *u, int nx, int ny, int nz, int x, int y, int z)
+{
+#pragma clang fp contract(off)
+    const A eps = A(1e-8);
+    auto U = [&](int a, int b, int c) { return ld<T, A>(u, nx, ny, nz, a, b, c); };
+    A p = U(x, y, z);
+    A xm = U(x - 1, y, z), ym = U(x, y - 1, z), zm = U(x, y, z - 1);
+    A zp = U(x, y, z + 1), xm_zp = U(x - 1, y, z + 1), ym_zp = U(x, y - 1, z + 1);
+    A yp = U(x, y + 1, z), xm_yp = U(x - 1, y + 1, z), yp_zm = U(x, y + 1, z - 1);
+    A xp = U(x + 1, y, z), xp_ym = U(x + 1, y - 1, z), xp_zm = U(x + 1, y, z - 1);
+    A g1 = (A(6) * p - A(2) * (xm + ym + zm)) /
+           sqrt(eps + (p - xm) * (p - xm) + (p - ym) * (p - ym) + (p - zm) * (p - zm));
+    A g2 = (A(2) * p - A(2) * zp) /
+           sqrt(eps + (zp - xm_zp) * (zp - xm_zp) + (zp - ym_zp) * (zp - ym_zp) + (zp - p) * (zp - p));
+    A g3 = (A(2) * p - A(2) * yp) /
+           sqrt(eps + (yp - xm_yp) * (yp - xm_yp) + (yp - p) * (yp - p) + (yp - yp_zm) * (yp - yp_zm));
+    A g4 = (A(2) * p - A(2) * xp) /
+           sqrt(eps + (xp - p) * (xp - p) + (xp - xp_ym) * (xp - xp_ym) + (xp - xp_zm) * (xp - xp_zm));
+    return g1 + g2 + g3 + g4;
+}
+
+template <class T, class A, bool THREE_D>
+__global__ __launch_bounds__(256) void tv_grad_kernel(const T *__restrict__ u, T *__restrict__ v,
+                                                      int nx, int ny, int nz, double *partials,
+                                                      unsigned *counter, double *nrm2)
+{
+    const size_t n = (size_t)nx * ny * nz;
+    double s = 0.0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        int x = (int)(i % nx);
+        size_t r = i / nx;
+        int y = (int)(r % ny), z = (int)(r / ny);
+        A g = THREE_D ? tv3_grad<T, A>(u, nx, ny, nz, x, y, z) : tv2_grad<T, A>(u, nx, ny, x, y);
+        T gt = T(g);
+        v[i] = gt;
+        s += (double)gt * (double)gt;
+    }
+    grid_sum_to(s, partials, counter, nrm2);
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void tv_step_kernel(T *__restrict__ u, const T *__restrict__ v,
+                                                      size_t n, double alpha,
+                                                      const double *__restrict__ alpha_dev,
+                                                      const double *__restrict__ d2,
+                                                      const double *__restrict__ nrm2, int clamp)
+{
+#pragma clang fp contract(off)
+    const double a = alpha_dev ? *alpha_dev : alpha;
+    const double ad = a * sqrt(*d2);
+    const double nrm = sqrt(*nrm2);
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        double w = (double)u[i] - ad * (double)v[i] / nrm;
+        if (clamp && w < 0) w = 0;
+        u[i] = T(w);
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void sqdiff_kernel(const T *__restrict__ a,
+                                                     const T *__restrict__ b, size_t n,
+                                                     double *partials, unsigned *counter,
+                                                     double *out)
+{
+    double s = 0.0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        double e = (double)a[i] - (b ? (double)b[i] : 0.0);
+        s += e * e;
+    }
+    grid_sum_to(s, partials, counter, out);
+}
+
+unsigned stream_blocks(size_t n)
+{
+    size_t b = (n + 255) / 256;
+    if (b > 8192) b = 8192;
+    return (unsigned)(b ? b : 1);
+}
+
+}  // namespace
+
+template <class T>
+int launch_sqdiff(const T *a, const T *b, size_t n, TvScratch &s, int slot, hipStream_t st)
+{
+    unsigned blocks = red_blocks(n);
+    sqdiff_kernel<T><<<blocks, RED_THREADS, 0, st>>>(a, b, n, s.partials, s.counters + slot,
+                                                    s.scalars + slot);
+    ASD_LAUNCH_CHECK("sqdiff_kernel");
+    return ASD_OK;
+}
+
+template <class T, class A>
+int tv_descent(T *u, const T *img_old, T *v, int nx, int ny, int nz, bool three_d, double alpha,
+               const double *alpha_dev, int nTVIter, int pos, TvScratch &s, hipStream_t st,
+               bool have_d2)
+{
+    const size_t n = (size_t)nx * ny * nz;
+    int rc;
+    if (!have_d2 && (rc = launch_sqdiff<T>(u, img_old, n, s, TV_SLOT_D2, st))) return rc;
+    const unsigned gb = red_blocks(n), ub = stream_blocks(n);
+    for (int it = 0; it < nTVIter; it++) {
+        if (three_d)
+            tv_grad_kernel<T, A, true><<<gb, RED_THREADS, 0, st>>>(
+                u, v, nx, ny, nz, s.partials, s.counters + TV_SLOT_NRM2, s.scalars + TV_SLOT_NRM2);
+        else
+            tv_grad_kernel<T, A, false><<<gb, RED_THREADS, 0, st>>>(
+                u, v, nx, ny, 1, s.partials, s.counters + TV_SLOT_NRM2, s.scalars + TV_SLOT_NRM2);
+        ASD_LAUNCH_CHECK("tv_grad_kernel");
+        int clamp = (pos == 1 && it == nTVIter - 1) ? 1 : 0;
+        tv_step_kernel<T><<<ub, 256, 0, st>>>(u, v, n, alpha, alpha_dev, s.scalars + TV_SLOT_D2,
+                                              s.scalars + TV_SLOT_NRM2, clamp);
+        ASD_LAUNCH_CHECK("tv_step_kernel");
+    }
+    return ASD_OK;
+}
+
+template int launch_sqdiff<float>(const float *, const float *, size_t, TvScratch &, int,
+                                  hipStream_t);
+template int launch_sqdiff<double>(const double *, const double *, size_t, TvScratch &, int,
+                                   hipStream_t);
+template int tv_descent<float, float>(float *, const float *, float *, int, int, int, bool,
+                                      double, const double *, int, int, TvScratch &,
+                                      hipStream_t, bool);
+template int tv_descent<float, double>(float *, const float *, float *, int, int, int, bool,
+                                       double, const double *, int, int, TvScratch &,
+                                       hipStream_t, bool);
+template int tv_descent<double, double>(double *, const double *, double *, int, int, int, bool,
+                                        double, const double *, int, int, TvScratch &,
+                                        hipStream_t, bool);
+
+int TvScratch::init()
+{
+    int rc;
+    if ((rc = buf.ensure(sizeof(double) * (RED_MAX_BLOCKS + TV_NSLOTS) +
+                         sizeof(unsigned) * TV_NSLOTS)))
+        return rc;
+    partials = buf.as<double>();
+    scalars = partials + RED_MAX_BLOCKS;
+    counters = reinterpret_cast<unsigned *>(scalars + TV_NSLOTS);
+    ASD_HIP(hipMemset(buf.p, 0, buf.bytes));
+    return ASD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Host entry points (the imgTVGradMin / imgTVGradMin3D compute).
+// ---------------------------------------------------------------------------
+namespace {
+
+struct TvCtx {
+    std::mutex mu;
+    hipStream_t stream = nullptr;
+    DeviceBuf img, old, v;
+    TvScratch scratch;
+    bool ready = false;
+    int init()
+    {
+        if (ready) return ASD_OK;
+        if (!stream) ASD_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        int rc = scratch.init();
+        if (rc) return rc;
+        ready = true;
+        return ASD_OK;
+    }
+};
+
+TvCtx &tvctx()
+{
+    static TvCtx *c = new TvCtx();
+    return *c;
+}
+
+int tv_host(const double *img, int nx, int ny, int nz, bool three_d, double alpha,
+            int nTVIter, int pos, const double *img_old, double *out)
+{
+    clear_error();
+    ASD_ARG(img && out, "NULL image pointer");
+    ASD_ARG(nx > 0 && ny > 0 && nz > 0, "image dimensions must be positive");
+    TvCtx &c = tvctx();
+    std::lock_guard<std::mutex> lk(c.mu);
+    int rc = c.init();
+    if (rc) return rc;
+    size_t n = (size_t)nx * ny * nz;
+    if ((rc = c.img.ensure(n * 8)) || (rc = c.v.ensure(n * 8))) return rc;
+    if (img_old && (rc = c.old.ensure(n * 8))) return rc;
+    ASD_HIP(hipMemcpyAsync(c.img.p, img, n * 8, hipMemcpyHostToDevice, c.stream));
+    if (img_old)
+        ASD_HIP(hipMemcpyAsync(c.old.p, img_old, n * 8, hipMemcpyHostToDevice, c.stream));
+    rc = tv_descent<double, double>(c.img.as<double>(), img_old ? c.old.as<double>() : nullptr,
+                                    c.v.as<double>(), nx, ny, nz, three_d, alpha, nullptr,
+                                    nTVIter, pos, c.scratch, c.stream, false);
+    if (rc) return rc;
+    ASD_HIP(hipMemcpyAsync(out, c.img.p, n * 8, hipMemcpyDeviceToHost, c.stream));
+    ASD_HIP(hipStreamSynchronize(c.stream));
+    return ASD_OK;
+}
+
+}  // namespace
+}  // namespace asd
+
+using namespace asd;
+
+extern "C" {
+
+int asd_tv_min_2d(const double *img, int rowN, int colN, double alpha, int nTVIter, int pos,
+                  const double *img_old, double *out)
+{
+    return tv_host(img, rowN, colN, 1, false, alpha, nTVIter, pos, img_old, out);
+}
+
+int asd_tv_min_3d(const double *img, int Nx, int Ny, int Nz, double alpha, int nTVIter,
+                  int pos, const double *img_old, double *out)
+{
+    return tv_host(img, Nx, Ny, Nz, true, alpha, nTVIter, pos, img_old, out);
+}
+
+int asd_tv_min_device(double *img, const double *img_old, int nx, int ny, int nz, int three_d,
+                      double alpha, int nTVIter, int pos, void *stream)
+{
+    clear_error();
+    ASD_ARG(img, "NULL image pointer");
+    ASD_ARG(nx > 0 && ny > 0 && nz > 0, "image dimensions must be positive");
+    static thread_local TvScratch scratch;
+    static thread_local DeviceBuf v;
+    int rc;
+    if (!scratch.partials && (rc = scratch.init())) return rc;
+    size_t n = (size_t)nx * ny * nz;
+    if ((rc = v.ensure(n * 8))) return rc;
+    return tv_descent<double, double>(img, img_old, v.as<double>(), nx, ny, three_d ? nz : 1,
+                                      three_d != 0, alpha, nullptr, nTVIter, pos, scratch,
+                                      (hipStream_t)stream, false);
+}
+
+}  // extern "C"

@@ -1,0 +1,176 @@
+"""ASD-POCS iteration benchmark on one MI355X.
+
+Workload (default): 2-D fan-beam ASD-POCS at 512x512 (pixel 0.5 mm), 120
+views over 360 deg, 768 bins at 0.8 mm, SID 541 mm, SDD 949 mm, modified
+Shepp-Logan phantom (0.02 mm^-1), Poisson noise at I0 = 1e5 photons per bin
+(seeded) and log transform, 50 ASD-POCS iterations (ng = 20, alpha = 0.2).
+A step is one ASD-POCS outer iteration (a SART sweep over the 120 views,
+positivity, 20 TV sub-iterations, step adaptation) with the sinogram and
+image resident in HBM.
+
+    python bench.py [--gpus 1] [--steps 50] [--warmup 2] [--precision 0|1]
+
+Prints one JSON line (value = iterations/s) with the roofline of the
+dominant kernel (HIP events on the plan's stream over the timed region) and
+the CPU oracle timed on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "ASD-POCS iters/sec and projector GB/s (% HBM roofline), batched-mixed vs plain at RMSE"
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    "fan2d_512": dict(n=512, nz=1, vxl=0.5, nv=120, na=768, nb=1, det=0.8, SO=541.0,
+                      SDD=949.0, I0=1e5, iters=50),
+    "cone3d_256": dict(n=256, nz=256, vxl=1.0, nv=180, na=384, nb=384, det=1.2, SO=500.0,
+                       SDD=1000.0, I0=5e4, iters=30),
+}
+
+
+def make_problem(w, seed=20241203):
+    from asd_pocs_mex_cuda_demo_amd import ops, phantoms
+    n, nz = w["n"], w["nz"]
+    para = dict(nx=n, ny=n, nz=nz, SO=w["SO"], OD=w["SDD"] - w["SO"], vxlsize=w["vxl"],
+                detsize=w["det"], na=w["na"], nb=w["nb"], alpha=0.2, numOfgradients=20)
+    para["phi_vec"] = np.arange(1, w["nv"] + 1) * 360.0 / w["nv"]
+    img = phantoms.shepp_logan_2d(n) if nz == 1 else phantoms.shepp_logan_3d(n, nz=nz)
+    plan = ops.Plan(dict(para, maxnumIter=w["iters"]), para["phi_vec"])
+    # noiseless line integrals from our forward projector, then Poisson + log
+    Pclean = np.zeros((w["na"] * w["nb"], w["nv"]))
+    for i, th in enumerate(para["phi_vec"]):
+        Pclean[:, i] = ops.Ax_cone_gpu(img.astype(np.float32), dict(para, phi=-th / 180 * np.pi))[0]
+    P = phantoms.poisson_log(Pclean, w["I0"], seed)
+    return para, img, P, plan
+
+
+def stage_bytes(N, M):
+    """Algorithmic (compulsory) HBM bytes per launch of each fused stage:
+    forward projector reads m (4N) and P_v (4M) and writes cor (4M);
+    backprojector+update reads cor (4M) and reads/writes m (8N); a TV
+    gradient sub-iteration reads u and writes v (8N)."""
+    return {0: 4 * N + 8 * M, 1: 8 * N + 4 * M, 2: 8 * N}
+
+
+def cpu_baseline(para, P, sample_iters):
+    from oracle import oracle
+    threads = min(16, os.cpu_count() or 1)
+    oracle.set_threads(threads)
+    t0 = time.perf_counter()
+    oracle.asd_pocs(P, dict(para, maxnumIter=sample_iters))
+    dt = time.perf_counter() - t0
+    return {"value": sample_iters / dt, "unit": "iters/sec", "cores": threads, "kind": "port",
+            "sample": f"{sample_iters} full ASD-POCS iterations of the same workload "
+                      f"(FP32 projectors, FP64 TV/driver; OpenMP over rays/voxels), "
+                      f"{dt:.2f} s wall"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="fan2d_512", choices=sorted(WORKLOADS))
+    ap.add_argument("--precision", type=int, default=0)
+    ap.add_argument("--cpu-iters", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              f"(the multi-GPU sharding is described, not built).", file=sys.stderr)
+        sys.exit(2)
+    from asd_pocs_mex_cuda_demo_amd import _native
+    _native.lib()  # fail loudly if the HIP library is missing
+
+    w = WORKLOADS[args.workload]
+    steps = args.steps if args.steps is not None else w["iters"]
+    para, img, P, plan = make_problem(w)
+    if args.precision:
+        plan.close()
+        from asd_pocs_mex_cuda_demo_amd import ops
+        plan = ops.Plan(dict(para, maxnumIter=w["iters"], precision=args.precision),
+                        para["phi_vec"])
+    plan.set_projections(P)
+    N = para["nx"] * para["ny"] * para["nz"]
+    M = para["na"] * para["nb"]
+
+    # warm-up iterations (untimed), then restart the reconstruction from zero
+    plan.reset(None, img)
+    plan.iterate(args.warmup)
+    plan.reset(None, img)
+    plan.sync()
+
+    # timed region: `steps` outer iterations, with HIP events on the plan stream
+    plan.set_timing(True)
+    plan.sync()
+    t0 = time.perf_counter()
+    plan.iterate(steps)
+    plan.sync()
+    dt = time.perf_counter() - t0
+    out, rmse = plan.out()
+    stage = {s: plan.timing(s) for s in (0, 1, 2)}
+    plan.set_timing(False)
+
+    # the same iterations without events (reported beside)
+    plan.reset(None, img)
+    plan.sync()
+    t1 = time.perf_counter()
+    plan.iterate(steps)
+    plan.sync()
+    dt_noev = time.perf_counter() - t1
+    done = out["IterationNum"]
+
+    nbytes = stage_bytes(N, M)
+    names = {0: "ax_kernel (fused SART forward step)", 1: "atx_kernel (fused backprojection + update)",
+             2: "tv_grad_kernel"}
+    per_stage = {}
+    for s, (ms, cnt) in stage.items():
+        if cnt:
+            avg = ms / cnt
+            per_stage[names[s]] = {"avg_us": avg * 1e3, "launches": cnt,
+                                   "total_ms": ms,
+                                   "GBps": nbytes[s] / (avg * 1e-3) / 1e9}
+    dom = max(stage, key=lambda s: stage[s][0])
+    ms_d, cnt_d = stage[dom]
+    achieved = nbytes[dom] / (ms_d / cnt_d * 1e-3) / 1e9 if cnt_d else 0.0
+    roof = {"bound": "hbm", "kernel": names[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "algorithmic_bytes_per_launch": nbytes[dom]}
+
+    result = {
+        "metric": METRIC,
+        "value": done / dt,
+        "unit": "iters/sec",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt / done * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if args.precision == 0 else "f32 (fp16-stored sinogram)",
+        "data": f"synthetic: modified Shepp-Logan phantom, Poisson noise I0={w['I0']:g} (seeded), log transform",
+        "config": {"workload": args.workload, "nx": para["nx"], "ny": para["ny"],
+                   "nz": para["nz"], "views": w["nv"], "na": para["na"], "nb": para["nb"],
+                   "iterations": steps, "ng": 20, "precision": args.precision},
+        "roofline": roof,
+        "stages": per_stage,
+        "ms_per_step_without_events": dt_noev / done * 1e3,
+        "rmse_final": float(rmse[-1]) if len(rmse) else None,
+    }
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline(para, P, args.cpu_iters)
+    plan.close()
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

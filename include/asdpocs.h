@@ -1,0 +1,166 @@
+/*
+ * asdpocs.h - C ABI of the MI355X ASD-POCS hot path (libasdpocs.so).
+ *
+ * Every entry point takes plain pointers and sizes.  Geometry arguments of the
+ * projector launchers are in voxel units, exactly as the reference's MEX
+ * gateways hand them to its CUDA launchers.  Volumes are stored x fastest
+ * (MATLAB column-major nx*ny*nz), projections ia fastest (na*nb), and a
+ * sinogram is (na*nb) x nv with each view contiguous.
+ *
+ * Functions returning int return ASD_OK (0) or an ASD_ERR_* code; the text of
+ * the last error on the calling thread is available from asd_last_error().
+ */
+#ifndef ASDPOCS_H
+#define ASDPOCS_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ASD_OK 0
+#define ASD_ERR_ARG 1    /* invalid argument (sizes, NULL pointers, geometry) */
+#define ASD_ERR_HIP 2    /* a HIP runtime call or kernel launch failed */
+#define ASD_ERR_ALLOC 3  /* device or host allocation failed */
+#define ASD_ERR_STATE 4  /* plan used before it was given projections */
+
+/* ---------------------------------------------------------------------- */
+/* Drop-in projector launchers (host pointers, synchronous)                */
+/* ---------------------------------------------------------------------- */
+
+/* One view of the ray-driven forward projector: y = vxl * sum_vox l * X,
+ * sr = vxl * sum_vox l, for na*nb rays (id = ib*na + ia).
+ * Replaces Ax_cone_mf_gpu_new, src/AxAtx/Ax_cone_gpu.cu:778-805 (same
+ * signature, same argument meaning, host pointers in and out).  The reference
+ * returns no status; asd_last_status() reports this call's. */
+void Ax_cone_mf_gpu_new(float *X, float *y, float *sr, float cos_phi, float sin_phi,
+                        float *y_det, float *z_det, float SO, float OD, float scale, float dz,
+                        int nx, int ny, int nz, int na, int nb);
+
+/* One view of the voxel-driven backprojector: X = vxl * sum_bins l * y,
+ * sc = vxl * sum_bins l, over each voxel's detector footprint.
+ * Replaces Atx_cone_mf_gpu_new, src/AxAtx/Ax_cone_gpu.cu:299-331 (same
+ * signature, host pointers). */
+void Atx_cone_mf_gpu_new(float *X, float *y, float *sc, float cos_phi, float sin_phi,
+                         float *y_det, float *z_det, float SO, float OD, float scale,
+                         float dy_det, float dz_det, float dz, int nx, int ny, int nz, int na,
+                         int nb);
+
+/* Status and message of the last call made on this thread. */
+int asd_last_status(void);
+const char *asd_last_error(void);
+
+/* ---------------------------------------------------------------------- */
+/* Device-resident projector forms                                         */
+/* ---------------------------------------------------------------------- */
+
+/* Same operators on device pointers (HBM-resident X, y, y_det, z_det),
+ * enqueued on `stream` (a hipStream_t, NULL = the null stream); they return
+ * after the launch.  Same role as the kernels launched at
+ * src/AxAtx/Ax_cone_gpu.cu:796 and :318-326, without the per-call
+ * malloc/copy/free of the reference launchers. */
+int asd_ax_device(const float *X, float *y, float *sr, float cos_phi, float sin_phi,
+                  const float *y_det, const float *z_det, float SO, float OD, float scale,
+                  float dz, int nx, int ny, int nz, int na, int nb, void *stream);
+int asd_atx_device(float *X, const float *y, float *sc, float cos_phi, float sin_phi,
+                   const float *y_det, const float *z_det, float SO, float OD, float scale,
+                   float dy_det, float dz_det, float dz, int nx, int ny, int nz, int na, int nb,
+                   void *stream);
+
+/* ---------------------------------------------------------------------- */
+/* TV steepest descent                                                     */
+/* ---------------------------------------------------------------------- */
+
+/* I2 = imgTVGradMin(I, alpha, nTVIter, pos[, I0]) on host FP64 arrays of
+ * rowN x colN (column-major).  img_old may be NULL (zeros).  out may alias img.
+ * Replaces the compute of src/tvmin/imgTVGradMin.cpp:36-123. */
+int asd_tv_min_2d(const double *img, int rowN, int colN, double alpha, int nTVIter, int pos,
+                  const double *img_old, double *out);
+
+/* I2 = imgTVGradMin3D(I, alpha, nTVIter, pos[, I0]) on host FP64 arrays of
+ * Nx x Ny x Nz (x fastest).  Replaces src/tvmin/imgTVGradMin3D.cpp:44-173. */
+int asd_tv_min_3d(const double *img, int Nx, int Ny, int Nz, double alpha, int nTVIter,
+                  int pos, const double *img_old, double *out);
+
+/* In-place FP64 TV descent of a device array (three_d = 0: nx x ny 2-D stencil). */
+int asd_tv_min_device(double *img, const double *img_old, int nx, int ny, int nz, int three_d,
+                      double alpha, int nTVIter, int pos, void *stream);
+
+/* ---------------------------------------------------------------------- */
+/* ASD-POCS / SART drivers                                                 */
+/* ---------------------------------------------------------------------- */
+
+/* Mirror of the `para` struct read by ASD_POCS_cuda.m (utilities/ASD_POCS_cuda.m:20-96)
+ * and the MEX gateways.  asd_para_defaults() fills the reference defaults. */
+typedef struct asd_para {
+    int nx, ny, nz;         /* volume size (voxels) */
+    double SO, OD;          /* source-isocentre and isocentre-detector distances (mm) */
+    double vxlsize;         /* voxel size (mm) */
+    double detsize;         /* detector bin size (mm) */
+    int na, nb;             /* detector bins (horizontal, vertical) */
+    int maxnumIter;         /* default 5 */
+    int numOfgradients;     /* TV sub-iterations ng, default 20 */
+    double alpha;           /* TV step factor, default 0.02 */
+    double alpha_red;       /* default 0.95 */
+    double beta;            /* SART relaxation, default 1 */
+    double beta_red;        /* default 0.995 */
+    double r_max;           /* default 0.95 */
+    double epsilon;         /* default eps = 2^-52 */
+    double stopRelChg;      /* default eps */
+    int initBySART;         /* 1: m0 = one SART sweep (SART_cuda_inline.m) */
+    double lambda;          /* SART relaxation of the warm start, default 1 */
+    int precision;          /* 0: plain FP32 (default); 1: FP16-stored sinogram, compensated sums */
+} asd_para;
+
+typedef struct asd_out {
+    int IterationNum;       /* out.IterationNum */
+    double dataTolerance;   /* out.dataTolerance (dd2 of the last sweep) */
+    double beta;            /* out.beta */
+    double alpha;           /* out.alpha */
+} asd_out;
+
+void asd_para_defaults(asd_para *para);
+
+/* [m_res, out] = ASD_POCS_cuda(P, para).  P: host FP64 (na*nb) x nv, view
+ * contiguous; phi_vec: nv angles in degrees.  m0 / I0 may be NULL.  m_res:
+ * N doubles (the post-SART image of the last iteration).  rmse: maxnumIter
+ * doubles or NULL.  Replaces utilities/ASD_POCS_cuda.m:1-182 with the image,
+ * sinogram and all per-view work resident in HBM. */
+int asd_pocs(const double *P, const double *phi_vec, int nv, const asd_para *para,
+             const double *m0, const double *I0, double *m_res, double *rmse, asd_out *out);
+
+/* [m, out] = SART_cuda_inline(P, para): one sweep with relaxation
+ * para->lambda.  m: N doubles, m0 on entry (zeros if the caller has none),
+ * the result on exit.  Replaces utilities/SART_cuda_inline.m:56-87. */
+int asd_sart(const double *P, const double *phi_vec, int nv, const asd_para *para, double *m,
+             double *dataTolerance);
+
+/* Device-resident plan: geometry tables, sinogram and image stay in HBM
+ * across calls (the unit the iteration benchmark times). */
+typedef struct asd_plan asd_plan;
+int asd_plan_create(const asd_para *para, const double *phi_vec, int nv, asd_plan **plan);
+int asd_plan_set_projections(asd_plan *plan, const double *P_host);
+int asd_plan_reset(asd_plan *plan, const double *m0_host, const double *I0_host);
+/* Run up to n_iter outer iterations (continuing the counters of earlier calls). */
+int asd_plan_iterate(asd_plan *plan, int n_iter);
+int asd_plan_get_image(asd_plan *plan, double *m_res_host);
+int asd_plan_get_out(asd_plan *plan, asd_out *out, double *rmse_host);
+/* One SART sweep over all views with relaxation `lambda` on the plan's
+ * current image (SART_cuda_inline.m:56-87); *dataTolerance = its data error. */
+int asd_plan_sweep(asd_plan *plan, double lambda, double *dataTolerance);
+/* Current image (after TV of the last iteration, or after a sweep). */
+int asd_plan_get_current(asd_plan *plan, double *m_host);
+/* Synchronise the plan's stream (for timing). */
+int asd_plan_sync(asd_plan *plan);
+/* Per-kernel event timing on the plan stream: stage 0 = forward projector,
+ * 1 = backprojector + update, 2 = TV gradient.  Enabling resets the totals. */
+int asd_plan_set_timing(asd_plan *plan, int enable);
+int asd_plan_get_timing(asd_plan *plan, int stage, double *total_ms, long *count);
+/* Lanes per ray of the forward projector and the plan's hipStream_t. */
+int asd_plan_info(asd_plan *plan, int *lanes_per_ray, void **stream);
+void asd_plan_destroy(asd_plan *plan);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ASDPOCS_H */

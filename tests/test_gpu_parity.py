@@ -1,0 +1,133 @@
+"""GPU parity: every operator through the C ABI against the CPU oracle.
+
+Tolerances: the projectors evaluate the reference's FP32 expressions in the
+same order with contraction off, so the backprojector and the 3-D thread-per-ray
+forward projector match the oracle to rounding of the summation order only
+(relative L2 <= 1e-5 is the bar; observed is far below).  TV runs in FP64
+(relative L2 <= 1e-12).  The ASD-POCS driver keeps the image in FP32 where the
+oracle's driver keeps FP64, so reconstructions agree to <= 1e-4 relative.
+"""
+import numpy as np
+import pytest
+
+from asd_pocs_mex_cuda_demo_amd import ops, phantoms
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def geom(nx, ny, nz, na, nb, SO, OD, vxl, det, phi):
+    return dict(nx=nx, ny=ny, nz=nz, na=na, nb=nb, SO=SO, OD=OD, vxlsize=vxl, detsize=det,
+                phi=phi)
+
+
+CASES = [
+    # 2-D fan beam (nz = nb = 1), the demo-style geometry and the 512^2 one
+    ("fan64", geom(64, 64, 1, 128, 1, 405.941, 500.31, 0.5, 0.148 * 2, -0.3)),
+    ("fan512", geom(512, 512, 1, 768, 1, 541.0, 408.0, 0.5, 0.8, -1.9)),
+    ("fan_odd_na", geom(48, 40, 1, 97, 1, 500.0, 500.0, 1.0, 1.0, -2.6)),
+    # 3-D cone beam
+    ("cone32", geom(32, 32, 16, 48, 24, 500.0, 500.0, 1.0, 1.2, -0.7)),
+    ("cone_odd", geom(24, 20, 9, 37, 15, 300.0, 400.0, 1.0, 1.5, -4.0)),
+]
+
+
+@pytest.mark.parametrize("name,para", CASES, ids=[c[0] for c in CASES])
+def test_ax_matches_oracle(orc, name, para):
+    rng = np.random.default_rng(1)
+    N = para["nx"] * para["ny"] * para["nz"]
+    X = rng.random(N).astype(np.float32)
+    y, sr = ops.Ax_cone_gpu(X, para)
+    yo, sro = orc.ax_para(X, para)
+    assert rel_l2(y, yo) <= 1e-5, rel_l2(y, yo)
+    assert rel_l2(sr, sro) <= 1e-5, rel_l2(sr, sro)
+    assert np.abs(y - yo).max() <= 1e-5 * np.abs(yo).max() + 1e-30
+
+
+@pytest.mark.parametrize("name,para", CASES, ids=[c[0] for c in CASES])
+def test_atx_matches_oracle(orc, name, para):
+    rng = np.random.default_rng(2)
+    M = para["na"] * para["nb"]
+    Y = rng.standard_normal(M).astype(np.float32)
+    x, sc = ops.Atx_cone_gpu(Y, para)
+    xo, sco = orc.atx_para(Y, para)
+    # same expressions in the same order: bit-identical
+    assert np.array_equal(sc, sco), np.abs(sc - sco).max()
+    assert np.array_equal(x, xo), np.abs(x - xo).max()
+
+
+def test_tv2d_matches_oracle(orc):
+    rng = np.random.default_rng(3)
+    I = rng.random((37, 29))
+    I0 = rng.random((37, 29)) * 0.1
+    for nit, pos in [(0, 1), (1, 0), (5, 1)]:
+        out = ops.imgTVGradMin(I, 0.2, nit, pos, I0)
+        ref = orc.tv2d(I, 0.2, nit, pos, I0)
+        assert rel_l2(out, ref) <= 1e-12
+    out = ops.imgTVGradMin(I, 0.05, 3, 1)
+    assert rel_l2(out, orc.tv2d(I, 0.05, 3, 1)) <= 1e-12
+
+
+def test_tv3d_matches_oracle(orc):
+    rng = np.random.default_rng(4)
+    I = rng.random((17, 13, 11))
+    out = ops.imgTVGradMin3D(I, 0.01, 20, 1)
+    ref = orc.tv3d(I, 0.01, 20, 1)
+    assert rel_l2(out, ref) <= 1e-12
+    I0 = rng.random((17, 13, 11))
+    out = ops.imgTVGradMin3D(I, 0.3, 4, 0, I0)
+    assert rel_l2(out, orc.tv3d(I, 0.3, 4, 0, I0)) <= 1e-12
+
+
+def _small_problem(n=64, nv=24, na=96, nz=1, nb=1):
+    para = dict(nx=n, ny=n, nz=nz, SO=500.0, OD=400.0, vxlsize=1.0, detsize=1.2, na=na, nb=nb)
+    para["phi_vec"] = np.arange(1, nv + 1) * 360.0 / nv
+    img = phantoms.shepp_logan_2d(n) if nz == 1 else phantoms.shepp_logan_3d(n, nz=nz)
+    P = np.zeros((na * nb, nv))
+    for i, th in enumerate(para["phi_vec"]):
+        q = dict(para, phi=-th / 180 * np.pi)
+        P[:, i] = ops.Ax_cone_gpu(img.astype(np.float32), q)[0]
+    return para, img, P
+
+
+def test_asd_pocs_matches_oracle_driver(orc):
+    para, img, P = _small_problem()
+    para.update(maxnumIter=4, alpha=0.2, numOfgradients=10, I0=img)
+    m, out = ops.ASD_POCS_cuda(P, para)
+    mo, outo = orc.asd_pocs(P, para)
+    assert out["IterationNum"] == outo["IterationNum"] == 4
+    assert rel_l2(m.ravel(order="F"), mo) <= 1e-4, rel_l2(m.ravel(order="F"), mo)
+    assert np.allclose(out["RMSE"], outo["RMSE"], rtol=1e-4)
+    assert abs(out["alpha"] - outo["alpha"]) <= 1e-12 * abs(outo["alpha"])
+    assert abs(out["beta"] - outo["beta"]) <= 1e-12
+    assert abs(out["dataTolerance"] - outo["dataTolerance"]) <= 1e-3 * outo["dataTolerance"]
+    assert m.min() >= 0
+
+
+def test_sart_matches_oracle(orc):
+    para, img, P = _small_problem(n=48, nv=18, na=80)
+    m, out = ops.SART_cuda_inline(P, para)
+    mo, outo = orc.sart(P, para)
+    assert rel_l2(m, mo) <= 1e-5, rel_l2(m, mo)
+    assert abs(out["dataTolerance"] - outo["dataTolerance"]) <= 1e-5 * outo["dataTolerance"]
+
+
+def test_asd_pocs_3d_matches_oracle(orc):
+    para, img, P = _small_problem(n=24, nv=12, na=40, nz=8, nb=20)
+    para.update(maxnumIter=2, alpha=0.2, numOfgradients=5)
+    m, out = ops.ASD_POCS_cuda(P, para)
+    mo, outo = orc.asd_pocs(P, para)
+    assert rel_l2(m.ravel(order="F"), mo) <= 1e-4, rel_l2(m.ravel(order="F"), mo)
+
+
+def test_plan_is_deterministic():
+    para, img, P = _small_problem(n=48, nv=12, na=72)
+    para.update(maxnumIter=3)
+    a, _ = ops.ASD_POCS_cuda(P, para)
+    b, _ = ops.ASD_POCS_cuda(P, para)
+    assert np.array_equal(a, b)

@@ -1,0 +1,359 @@
+"""Pins of the CPU oracle (oracle/asd_oracle.c) from the mathematics.
+
+The reference ships no golden vectors, known-answer tests or fixtures for
+this path, and it cannot be built here (its CUDA file needs the CUDA runtime,
+its TV files need MATLAB's mex.h).  Each oracle function is therefore checked
+against an independent FP64 computation (oracle/exact64.py) chosen so that a
+dropped term, a wrong sign or index, or a transposed operand fails:
+
+  find_l / find_l_3d   Liang-Barsky clip of the line against the box
+  Ax 2-D               exact system matrix (brute force over pixels)
+  Ax 3-D               exact per-slab clip of the reference's monotone segment
+  Atx                  exact all-bins backprojection; adjoint identity in 2-D
+  TV 2-D / 3-D         finite-difference gradient of the padded TV functional
+  TV step              ||I_new - I|| = alpha * ||I - I0|| per normalised step
+  driver               the alpha/beta rules and invariants of ASD_POCS_cuda.m
+
+Tolerances reflect the reference's FP32 arithmetic: its chord end points are
+formed relative to the distant source, so lengths carry ~1e-4 voxel of
+cancellation error (SO = 500 voxels).
+"""
+import numpy as np
+import pytest
+
+from oracle import exact64 as E
+
+
+def geom(orc, phi, SO, OD, vxl, det, na, nb):
+    return orc.gateway_geometry(phi, SO, OD, vxl, det, na, nb)
+
+
+# ---------------------------------------------------------------------------
+# find_l / find_l_3d
+# ---------------------------------------------------------------------------
+
+def _random_rays(rng, n, dim):
+    """Random lines through a neighbourhood of the unit box, with a far source."""
+    p1 = rng.uniform(-1, 1, (n, dim)) * 0.6
+    d = rng.standard_normal((n, dim))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    src = p1 - 300.0 * d
+    dst = p1 + 200.0 * d
+    return src, dst
+
+
+def test_find_l_matches_exact_clip(orc):
+    rng = np.random.default_rng(10)
+    src, dst = _random_rays(rng, 3000, 2)
+    got = np.array([orc.find_l(s[0], s[1], t[0], t[1], 1.0, 1.0, 0.0, 0.0)
+                    for s, t in zip(src, dst)])
+    ref = E.clip_len(src, dst, [-0.5, -0.5], [0.5, 0.5])
+    assert np.abs(got - ref).max() < 2e-4
+    assert (ref > 0.1).sum() > 300  # the sample really crosses the box
+
+
+def test_find_l_axis_parallel(orc):
+    # a == 0: vertical line inside the closed slab gets the full side dy
+    assert orc.find_l(0.5, -9.0, 0.5, 7.0, 1.0, 2.0, 0.0, 0.0) == 2.0
+    assert orc.find_l(0.51, -9.0, 0.51, 7.0, 1.0, 2.0, 0.0, 0.0) == 0.0
+    # b == 0: horizontal line gives dx
+    assert orc.find_l(-9.0, -0.25, 7.0, -0.25, 3.0, 1.0, 0.0, 0.0) == 3.0
+
+
+def test_find_l_3d_matches_exact_clip(orc):
+    rng = np.random.default_rng(11)
+    src, dst = _random_rays(rng, 3000, 3)
+    got = np.array([orc.find_l_3d(*s, *t, 1.0, 1.0, 1.0, 0.0, 0.0, 0.0)
+                    for s, t in zip(src, dst)])
+    ref = E.clip_len(src, dst, [-0.5] * 3, [0.5] * 3)
+    assert np.abs(got - ref).max() < 2e-4
+    assert (ref > 0.1).sum() > 200
+
+
+def test_find_l_3d_degenerate_fallback(orc):
+    # a == 0: the reference takes the (y,z) chord and ignores x (cu:131-132),
+    # so a line far outside the box in x still gets the (y,z) length.
+    y1, z1, y2, z2 = -300.0, -0.2, 200.0, 0.3
+    l_in = orc.find_l_3d(0.0, y1, z1, 0.0, y2, z2, 1, 1, 1, 0.0, 0.0, 0.0)
+    l_out = orc.find_l_3d(5.0, y1, z1, 5.0, y2, z2, 1, 1, 1, 0.0, 0.0, 0.0)
+    ref = E.clip_len([y1, z1], [y2, z2], [-0.5, -0.5], [0.5, 0.5])
+    assert abs(l_in - ref) < 2e-4 and l_out == l_in
+    # b == 0: (x,z) chord, y ignored
+    l_b = orc.find_l_3d(-300.0, 7.0, -0.1, 200.0, 7.0, 0.2, 1, 1, 1, 0.0, 0.0, 0.0)
+    assert abs(l_b - E.clip_len([-300.0, -0.1], [200.0, 0.2], [-.5, -.5], [.5, .5])) < 2e-4
+
+
+def test_find_l_3d_flat_ray_on_face_counts(orc):
+    # c == 0 with the ray on the z = -dz/2 face: the closed-face ties keep it
+    # (the 2-D fan-beam case, nz = nb = 1).
+    l = orc.find_l_3d(-300.0, -0.1, 0.0, 200.0, 0.15, 0.0, 1, 1, 1, 0.0, 0.0, 0.5)
+    ref = E.clip_len([-300.0, -0.1], [200.0, 0.15], [-.5, -.5], [.5, .5])
+    assert abs(l - ref) < 2e-4 and l > 0.9
+
+
+# ---------------------------------------------------------------------------
+# Forward projector
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("phi", [-0.3, -1.2, -2.0, -3.1, -4.5, -np.pi / 4])
+def test_ax2d_matches_exact_system_matrix(orc, phi):
+    nx, ny, na = 24, 20, 40
+    g = geom(orc, phi, 500.0, 400.0, 1.0, 1.3, na, 1)
+    X = np.random.default_rng(5).random(nx * ny).astype(np.float32)
+    y, sr = orc.ax(X, g["cos_phi"], g["sin_phi"], g["y_det"], g["z_det"], g["SO"], g["OD"],
+                   g["scale"], np.float32(1), nx, ny, 1, na, 1)
+    A = E.system_matrix_2d(g["cos_phi"], g["sin_phi"], float(g["SO"]), float(g["OD"]),
+                           g["y_det"], nx, ny)
+    assert np.abs(y - A @ X).max() <= 5e-5 * np.abs(A @ X).max()
+    # row sums are image independent: the chord through the image square
+    chord = E.clip_len(np.array([-500.0 * float(g["cos_phi"]), -500.0 * float(g["sin_phi"])]),
+                       np.stack([400.0 * float(g["cos_phi"]) - float(g["sin_phi"]) * g["y_det"],
+                                 400.0 * float(g["sin_phi"]) + float(g["cos_phi"]) * g["y_det"]], -1),
+                       [-nx // 2, -ny // 2], [nx - nx // 2, ny - ny // 2])
+    assert np.abs(sr - chord).max() <= 5e-5 * chord.max()
+
+
+def test_ax2d_box_indicator_closed_form(orc):
+    # y = mu * vxl * |ray ∩ B| for the indicator of a pixel box B
+    nx = ny = 32
+    na = 64
+    vxl = 0.7
+    g = geom(orc, -0.9, 300.0 * vxl, 300.0 * vxl, vxl, 1.1 * vxl, na, 1)
+    X = np.zeros((ny, nx), np.float32)
+    X[10:17, 5:13] = 0.25  # rows iy 10..16, cols ix 5..12
+    y, _ = orc.ax(X.ravel(), g["cos_phi"], g["sin_phi"], g["y_det"], g["z_det"], g["SO"],
+                  g["OD"], g["scale"], np.float32(1), nx, ny, 1, na, 1)
+    c, s = float(g["cos_phi"]), float(g["sin_phi"])
+    src = np.array([-float(g["SO"]) * c, -float(g["SO"]) * s])
+    det = np.stack([float(g["OD"]) * c - s * g["y_det"], float(g["OD"]) * s + c * g["y_det"]], -1)
+    L = E.clip_len(src, det, [5 - 16, 10 - 16], [13 - 16, 17 - 16])
+    np.testing.assert_allclose(y, 0.25 * vxl * L, atol=3e-5)
+
+
+@pytest.mark.parametrize("SO", [300.0, 25.0])
+@pytest.mark.parametrize("phi", [-0.3, -1.2, -2.0, -3.1, -4.5])
+def test_ax3d_matches_monotone_segment_rule(orc, phi, SO):
+    """SO = 300: shallow axial slopes; SO = 25: steep slopes, so slabs often
+    cross a y and a z cell boundary at once and the split rule matters."""
+    nx, ny, nz, na, nb = 12, 10, 6, 20, 8
+    g = geom(orc, phi, SO, SO, 1.0, 1.5, na, nb)
+    X = np.random.default_rng(6).random(nx * ny * nz).astype(np.float32)
+    y, sr = orc.ax(X, g["cos_phi"], g["sin_phi"], g["y_det"], g["z_det"], g["SO"], g["OD"],
+                   g["scale"], np.float32(1), nx, ny, nz, na, nb)
+    ye, sre = E.ax3d_monotone(X, g["cos_phi"], g["sin_phi"], float(g["SO"]), float(g["OD"]),
+                              g["y_det"], g["z_det"], nx, ny, nz)
+    assert np.abs(y - ye).max() <= 3e-5 * np.abs(ye).max()
+    assert np.abs(sr - sre).max() <= 3e-5 * sre.max()
+    if SO == 300.0:
+        # with shallow slopes the split moves length between voxels but
+        # keeps the row sums equal to the exact chord lengths
+        A = E.system_matrix(g["cos_phi"], g["sin_phi"], float(g["SO"]), float(g["OD"]),
+                            g["y_det"], g["z_det"], nx, ny, nz)
+        assert np.abs(A.sum(1) - sre).max() <= 1e-9 * sre.max()
+
+
+def test_ax3d_quirk_differs_from_exact_when_slopes_disagree(orc):
+    # The reference sorts the in-plane and axial extents independently; where
+    # their slopes have opposite signs this differs from the exact projection.
+    nx, ny, nz, na, nb = 12, 10, 6, 20, 8
+    g = geom(orc, -1.2, 25.0, 25.0, 1.0, 1.5, na, nb)
+    X = np.random.default_rng(7).random(nx * ny * nz).astype(np.float32)
+    y, _ = orc.ax(X, g["cos_phi"], g["sin_phi"], g["y_det"], g["z_det"], g["SO"], g["OD"],
+                  g["scale"], np.float32(1), nx, ny, nz, na, nb)
+    A = E.system_matrix(g["cos_phi"], g["sin_phi"], float(g["SO"]), float(g["OD"]),
+                        g["y_det"], g["z_det"], nx, ny, nz)
+    assert np.abs(y - A @ X).max() > 1e-2 * np.abs(A @ X).max()
+
+
+# ---------------------------------------------------------------------------
+# Backprojector
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("phi", [-0.3, -1.2, -2.6, -np.pi / 4])
+def test_atx2d_is_exact_adjoint(orc, phi):
+    nx, ny, na = 24, 20, 40
+    g = geom(orc, phi, 500.0, 400.0, 1.0, 1.3, na, 1)
+    rng = np.random.default_rng(8)
+    X = rng.random(nx * ny).astype(np.float32)
+    Y = rng.random(na).astype(np.float32)
+    args = (g["cos_phi"], g["sin_phi"], g["y_det"], g["z_det"], g["SO"], g["OD"], g["scale"])
+    y, sr = orc.ax(X, *args, np.float32(1), nx, ny, 1, na, 1)
+    x, sc = orc.atx(Y, *args, g["det"], g["det"], np.float32(1), nx, ny, 1, na, 1)
+    lhs = np.dot(y.astype(np.float64), Y)
+    rhs = np.dot(X.astype(np.float64), x)
+    assert abs(lhs - rhs) <= 1e-4 * abs(lhs)
+    assert abs(sr.sum(dtype=np.float64) - sc.sum(dtype=np.float64)) <= 1e-4 * sc.sum()
+    A = E.system_matrix_2d(g["cos_phi"], g["sin_phi"], float(g["SO"]), float(g["OD"]),
+                           g["y_det"], nx, ny)
+    assert np.abs(x - g["scale"] * (A.T @ Y)).max() <= 3e-4 * np.abs(x).max()
+
+
+@pytest.mark.parametrize("phi", [-0.3, -1.2, -2.0, -4.5])
+def test_atx3d_matches_exact_all_bins(orc, phi):
+    nx, ny, nz, na, nb = 12, 10, 6, 20, 8
+    g = geom(orc, phi, 300.0, 300.0, 1.0, 1.5, na, nb)
+    Y = np.random.default_rng(9).random(na * nb).astype(np.float32)
+    x, sc = orc.atx(Y, g["cos_phi"], g["sin_phi"], g["y_det"], g["z_det"], g["SO"], g["OD"],
+                    g["scale"], g["det"], g["det"], np.float32(1), nx, ny, nz, na, nb)
+    A = E.system_matrix(g["cos_phi"], g["sin_phi"], float(g["SO"]), float(g["OD"]),
+                        g["y_det"], g["z_det"], nx, ny, nz)
+    assert np.abs(x - A.T @ Y).max() <= 3e-4 * np.abs(x).max()
+    assert np.abs(sc - A.sum(0)).max() <= 3e-4 * sc.max()
+
+
+def test_atx_odd_na_footprint_and_corner_ray(orc):
+    """Odd na: the footprint uses integer na/2 while the bin table uses
+    na/2.0 (cu:231 vs Ax_main_test.cpp:156), and the central bin's ray passes
+    exactly through the grid corner at the isocentre.  Away from the four
+    voxels that touch that corner the oracle equals the exact all-bins
+    backprojection (no bin is missed); at the corner the reference's FP32
+    face tests can miss the exit point and drop the chord (find_l_3d keeps
+    only two hits, cu:141-205), which the oracle reproduces."""
+    nx, ny, na = 16, 16, 63
+    g = geom(orc, -0.4, 400.0, 400.0, 1.0, 0.5, na, 1)
+    Y = np.random.default_rng(12).random(na).astype(np.float32)
+    x, _ = orc.atx(Y, g["cos_phi"], g["sin_phi"], g["y_det"], g["z_det"], g["SO"], g["OD"],
+                   g["scale"], g["det"], g["det"], np.float32(1), nx, ny, 1, na, 1)
+    A = E.system_matrix_2d(g["cos_phi"], g["sin_phi"], float(g["SO"]), float(g["OD"]),
+                           g["y_det"], nx, ny)
+    ref = A.T @ Y
+    corner = [7 * 16 + 7, 7 * 16 + 8, 8 * 16 + 7, 8 * 16 + 8]
+    keep = np.ones(nx * ny, bool)
+    keep[corner] = False
+    assert np.abs(x - ref)[keep].max() <= 3e-4 * np.abs(x).max()
+    assert np.abs(x - ref)[corner].max() > 0.1  # the dropped corner chord
+
+
+# ---------------------------------------------------------------------------
+# TV
+# ---------------------------------------------------------------------------
+
+def test_tv2d_gradient_is_tv_functional_gradient(orc):
+    u = np.random.default_rng(13).random((6, 5))
+    v = orc.tv2d_grad(u)
+    g = E.numgrad(E.tv2d_functional, u)
+    assert np.abs(v - g).max() < 1e-7
+
+
+def test_tv3d_gradient_is_twice_tv_functional_gradient(orc):
+    u = np.random.default_rng(14).random((4, 5, 3))
+    v = orc.tv3d_grad(u)
+    g = E.numgrad(E.tv3d_functional, u)
+    assert np.abs(v - 2 * g).max() < 1e-6
+
+
+def test_tv2d_transpose_symmetry_fails_for_wrong_axes(orc):
+    # the stencil's row/column roles are asymmetric only through the
+    # diagonal neighbours; its gradient of a transposed image is the
+    # transposed gradient
+    u = np.random.default_rng(15).random((7, 4))
+    np.testing.assert_allclose(orc.tv2d_grad(u.T), orc.tv2d_grad(u).T, atol=1e-12)
+
+
+def test_tv_step_length(orc):
+    rng = np.random.default_rng(16)
+    I = rng.random((9, 8))
+    I0 = rng.random((9, 8))
+    d = np.linalg.norm(I - I0)
+    out = orc.tv2d(I, 0.1, 1, 0, I0)
+    assert abs(np.linalg.norm(out - I) - 0.1 * d) <= 1e-12 * d
+    V = rng.random((5, 4, 3))
+    out = orc.tv3d(V, 0.2, 1, 0)
+    assert abs(np.linalg.norm(out - V) - 0.2 * np.linalg.norm(V)) <= 1e-12 * np.linalg.norm(V)
+    # the step direction is -v/||v||
+    v = orc.tv2d_grad(I)
+    np.testing.assert_allclose((I - orc.tv2d(I, 0.1, 1, 0, I0)) / (0.1 * d), v / np.linalg.norm(v),
+                               atol=1e-12)
+
+
+def test_tv_zero_iterations_and_positivity(orc):
+    I = np.random.default_rng(17).standard_normal((6, 6))
+    assert np.array_equal(orc.tv2d(I, 0.5, 0, 1), I)   # no positivity when nTVIter == 0
+    out = orc.tv2d(I, 0.01, 2, 1)
+    assert out.min() >= 0
+    assert (orc.tv2d(I, 0.01, 2, 0) < 0).any()
+
+
+def test_tv_constant_interior_and_nan_on_zero_norm(orc):
+    # a constant image has zero gradient inside and nonzero at the padded border
+    v = orc.tv2d_grad(np.ones((6, 7)))
+    assert np.abs(v[1:-1, 1:-1]).max() < 1e-12 and np.abs(v).max() > 0.5
+    # all-zero image: norm 0 -> 0/0 like the reference
+    out = orc.tv2d(np.zeros((4, 4)), 0.1, 1, 0, np.zeros((4, 4)))
+    assert np.isnan(out).all()
+
+
+# ---------------------------------------------------------------------------
+# SART / ASD-POCS driver
+# ---------------------------------------------------------------------------
+
+def _problem(orc, n=32, nv=12, na=48):
+    para = dict(nx=n, ny=n, nz=1, SO=500.0, OD=400.0, vxlsize=1.0, detsize=1.2, na=na, nb=1)
+    para["phi_vec"] = np.arange(1, nv + 1) * 360.0 / nv
+    xx, yy = np.meshgrid(np.arange(n) - n / 2 + 0.5, np.arange(n) - n / 2 + 0.5, indexing="ij")
+    img = 0.02 * ((xx / (0.4 * n)) ** 2 + (yy / (0.3 * n)) ** 2 <= 1.0)
+    P = np.zeros((na, nv))
+    for i, th in enumerate(para["phi_vec"]):
+        P[:, i] = orc.ax_para(img, dict(para, phi=-th / 180 * np.pi))[0]
+    return para, img, P
+
+
+def test_driver_reproduces_numpy_restatement(orc):
+    """An independent numpy ASD-POCS loop (FP64 updates around the oracle's
+    FP32 operators) gives the same image, alpha, beta and data error."""
+    para, img, P = _problem(orc)
+    para.update(maxnumIter=3, alpha=0.2, numOfgradients=4)
+    m_res, out = orc.asd_pocs(P, para)
+
+    N = img.size
+    m = np.zeros(N)
+    alpha, beta = 0.2, 1.0
+    for k in range(1, 4):
+        m_pre = m.copy()
+        dd2 = 0.0
+        for i, th in enumerate(para["phi_vec"]):
+            q = dict(para, phi=-th / 180 * np.pi)
+            proj, sumR = (a.astype(np.float64) for a in orc.ax_para(m.astype(np.float32), q))
+            sumR[sumR == 0] = 1
+            cor0 = P[:, i] - proj
+            dd2 += np.dot(cor0, cor0)
+            bp, sumC = (a.astype(np.float64) for a in orc.atx_para((cor0 / sumR).astype(np.float32), q))
+            sumC[sumC == 0] = 1
+            m = np.maximum(m + beta * bp / sumC, 0)
+        m_keep = m.copy()
+        dp = np.linalg.norm(m - m_pre)
+        if k == 1:
+            m_k1 = m_pre.copy()
+        f0 = m.copy()
+        m = orc.tv2d(m.reshape(para["nx"], para["ny"], order="F"), alpha, 4, 1,
+                     m_k1.reshape(para["nx"], para["ny"], order="F")).ravel(order="F")
+        dg = np.linalg.norm(m - f0)
+        if dg > 0.95 * dp and np.sqrt(dd2) > np.finfo(float).eps:
+            alpha *= 0.95
+        beta *= 0.995
+    np.testing.assert_allclose(m_res, m_keep, rtol=1e-10, atol=1e-14)
+    assert out["alpha"] == pytest.approx(alpha, rel=1e-14)
+    assert out["beta"] == pytest.approx(beta, rel=1e-14)
+    assert out["dataTolerance"] == pytest.approx(dd2, rel=1e-9)
+
+
+def test_driver_invariants(orc):
+    para, img, P = _problem(orc, nv=16)
+    errs = []
+    for it in (1, 2, 4):
+        m, out = orc.asd_pocs(P, dict(para, maxnumIter=it, alpha=0.2, numOfgradients=5))
+        assert m.min() >= 0
+        errs.append(out["dataTolerance"])
+        assert out["IterationNum"] == it
+        assert out["beta"] == pytest.approx(0.995 ** it)
+    assert errs[0] > errs[1] > errs[2]  # noiseless data: the data error falls
+
+
+def test_sart_single_sweep(orc):
+    para, img, P = _problem(orc)
+    m1, o1 = orc.sart(P, para)
+    m2, o2 = orc.sart(P, dict(para, maxnumIter=7))  # maxnumIter is ignored (one sweep)
+    assert np.array_equal(m1, m2) and o1["dataTolerance"] == o2["dataTolerance"]
+    mi, oi = orc.asd_pocs(P, dict(para, maxnumIter=1, initBySART=True))
+    m0, _ = orc.asd_pocs(P, dict(para, maxnumIter=1))
+    assert not np.array_equal(mi, m0)
