@@ -1,12 +1,20 @@
 // Device-resident ASD-POCS driver: the loop of utilities/ASD_POCS_cuda.m:103-171
 // and the sweep of SART_cuda_inline.m:56-87 with the image, the sinogram and
-// every per-view quantity kept in HBM.  Per view two kernels run:
-//   ax_sart   ray sums -> cor = (P - vxl*y) / (vxl*sr | 1), the data-error term
-//             and the view's ray-slope table (ASD_POCS_cuda.m:117-123)
-//   atx_sart  voxel sums -> m = max(0, m + beta * (vxl*x) / (vxl*sc | 1))
-//             (ASD_POCS_cuda.m:125-131)
-// and per outer iteration FP64 grid reductions give dd2, dp, d, dg and the
-// RMSE; the host reads them once per iteration to apply the alpha/beta rules.
+// every per-view quantity kept in HBM.
+//
+// One outer iteration is a fixed sequence of kernels with no host decision
+// in it, so it can be captured once into a hipGraph and replayed:
+//   per view   ax_sart    ray sums -> cor = (P - vxl*y) / (vxl*sr | 1), the
+//                         per-ray squared residual and the view's slope table
+//                         (ASD_POCS_cuda.m:117-123)
+//              atx_sart   voxel sums -> m = max(0, m + beta*(vxl*x)/(vxl*sc | 1))
+//                         (ASD_POCS_cuda.m:125-131)
+//   post_sart  dp = ||m - m_pre||, d = ||m - m_k1||, RMSE, dd2; f0 = m; the
+//              stop rule (m:136-149)
+//   TV         ng x (gradient + FP64 norm, normalised step) (m:155-164)
+//   post_tv    dg = ||m - f0||; m_pre = m; the alpha and beta rules (m:166-170)
+// alpha, beta, k and the stop flag live in device memory (DevCtl); once the
+// stop rule fires every later kernel returns at entry.
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -22,6 +30,21 @@ namespace {
 // Stage timing (events on the plan stream) for the dominant kernels.
 enum : int { ST_AX = 0, ST_ATX = 1, ST_TVGRAD = 2, ST_N = 3 };
 
+constexpr int HIST_CAP = 1 << 16;  // RMSE history entries kept on the device
+
+struct DevCtl {
+    double alpha;    // current TV step factor
+    double beta;     // current SART relaxation of the ASD loop
+    double relax;    // relaxation of a stand-alone sweep (SART_cuda_inline's lambda)
+    double dp, dd2, dg;
+    int k;           // outer iterations completed
+    int stopped;     // the stop rule fired
+};
+
+struct CtlRules {
+    double alpha_red, beta_red, r_max, epsilon, stopRelChg;
+};
+
 template <class PT>
 struct EpiSartAx {
     const PT *P;          // this view's measured projection
@@ -29,7 +52,9 @@ struct EpiSartAx {
     double *res2;         // per-ray accumulated squared residual of the sweep
     RayRec *rt;           // ray-slope table of this view (for atx_sart)
     const float *y_det, *z_det;
+    const int *stopped;
     ViewGeom g;
+    __device__ __forceinline__ bool skip() const { return *stopped != 0; }
     __device__ __forceinline__ void operator()(long id, float acc, float accl) const
     {
 #pragma clang fp contract(off)
@@ -45,33 +70,91 @@ struct EpiSartAx {
 
 struct EpiSartAtx {
     float *m;
-    double beta;
+    const double *relax;  // beta of the ASD loop or lambda of a stand-alone sweep
+    const int *stopped;
     float scale;
+    __device__ __forceinline__ bool skip() const { return *stopped != 0; }
     __device__ __forceinline__ void operator()(long v, float acc, float accl) const
     {
 #pragma clang fp contract(off)
         float bp = acc * scale, sc = accl * scale;
         double c = (sc == 0.0f) ? 1.0 : (double)sc;
-        double nm = (double)m[v] + beta * ((double)bp / c);
+        double nm = (double)m[v] + (*relax) * ((double)bp / c);
         m[v] = nm < 0 ? 0.0f : (float)nm;
     }
 };
 
-__global__ __launch_bounds__(256) void fill_f64(double *p, size_t n, double val)
+// After a sweep: dp^2, d^2, rmse^2 over the image (and f0 = m), dd2 over the
+// rays (and the residual buffer re-zeroed); the last block applies the stop
+// rule and records the RMSE (ASD_POCS_cuda.m:136-149).
+__global__ __launch_bounds__(256) void post_sart_kernel(
+    const float *__restrict__ m, const float *__restrict__ mpre, const float *__restrict__ mk1,
+    const float *__restrict__ I0, float *__restrict__ f0, double *__restrict__ res2, size_t N,
+    size_t M, double *partials, unsigned *counter, double *scal, DevCtl *ctl, double *hist,
+    CtlRules rules, int sweep_only)
 {
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (size_t)gridDim.x * blockDim.x)
-        p[i] = val;
+    if (ctl->stopped) return;
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += stride) {
+        float mi = m[i];
+        double a = (double)mi - (double)mpre[i];
+        double b = (double)mi - (double)mk1[i];
+        v[0] += a * a;
+        v[1] += b * b;
+        if (I0) {
+            double c = (double)mi - (double)I0[i];
+            v[2] += c * c;
+        }
+        f0[i] = mi;
+    }
+    for (size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x; j < M; j += stride) {
+        v[3] += res2[j];
+        res2[j] = 0.0;
+    }
+    double tot[4];
+    if (!grid_sums<4>(v, partials, counter, tot)) return;
+    if (threadIdx.x != 0) return;
+    scal[TV_SLOT_DP2] = tot[0];
+    scal[TV_SLOT_D2] = tot[1];
+    scal[TV_SLOT_RMSE2] = tot[2];
+    scal[TV_SLOT_DD2] = tot[3];
+    ctl->dd2 = tot[3];
+    if (sweep_only) return;
+    const int k = ctl->k + 1;
+    ctl->k = k;
+    if (I0 && k - 1 < HIST_CAP) hist[k - 1] = sqrt(tot[2] / (double)N);
+    const double dp = sqrt(tot[0]);
+    ctl->dp = dp;
+    if (dp / (double)N <= rules.stopRelChg) ctl->stopped = 1;
 }
 
-__global__ __launch_bounds__(256) void sum_f64(const double *__restrict__ a, size_t n,
-                                               double *partials, unsigned *counter, double *out)
+// After TV: dg^2 = ||m - f0||^2, m_pre = m for the next iteration, and the
+// alpha / beta rules (ASD_POCS_cuda.m:166-170).
+__global__ __launch_bounds__(256) void post_tv_kernel(const float *__restrict__ m,
+                                                      const float *__restrict__ f0,
+                                                      float *__restrict__ mpre, size_t N,
+                                                      double *partials, unsigned *counter,
+                                                      double *scal, DevCtl *ctl, CtlRules rules)
 {
-    double s = 0.0;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (size_t)gridDim.x * blockDim.x)
-        s += a[i];
-    grid_sum_to(s, partials, counter, out);
+    if (ctl->stopped) return;
+    double v[1] = {0.0};
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += stride) {
+        float mi = m[i];
+        double e = (double)mi - (double)f0[i];
+        v[0] += e * e;
+        mpre[i] = mi;
+    }
+    double tot[1];
+    if (!grid_sums<1>(v, partials, counter, tot)) return;
+    if (threadIdx.x != 0) return;
+    scal[TV_SLOT_DG2] = tot[0];
+    const double dg = sqrt(tot[0]);
+    ctl->dg = dg;
+    if (dg > rules.r_max * ctl->dp && sqrt(ctl->dd2) > rules.epsilon)
+        ctl->alpha = ctl->alpha * rules.alpha_red;
+    ctl->beta = ctl->beta * rules.beta_red;
 }
 
 __global__ __launch_bounds__(256) void f64_to_f32(const double *__restrict__ a, float *b,
@@ -80,6 +163,18 @@ __global__ __launch_bounds__(256) void f64_to_f32(const double *__restrict__ a, 
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (size_t)gridDim.x * blockDim.x)
         b[i] = (float)a[i];
+}
+
+__global__ __launch_bounds__(256) void copy3_f32(const float *__restrict__ a, float *b, float *c,
+                                                 float *d, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        float x = a[i];
+        b[i] = x;
+        c[i] = x;
+        d[i] = x;
+    }
 }
 
 unsigned grid_for(size_t n)
@@ -100,25 +195,34 @@ struct asd_plan {
     size_t N = 0, M = 0;
     std::vector<ViewGeom> views;
     hipStream_t stream = nullptr;
-    DeviceBuf ydet, zdet, P, m, mpre, mk1, f0, I0, cor, res2, rt, v, stage;
+    DeviceBuf ydet, zdet, P, m, mpre, mk1, f0, I0, cor, res2, rt, v, ctlbuf, hist;
     TvScratch red;
-    bool have_P = false, have_I0 = false, stopped = false;
-    int k = 0;
-    double alpha = 0, beta = 0, dd2 = 0;
-    std::vector<double> rmse;
+    bool have_P = false, have_I0 = false;
     int lpr = 1;
-    // optional stage timing
+    DevCtl h;  // host copy of the control block after the last sync
+    // hipGraph of one outer iteration
+    bool use_graph = true;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    // optional stage timing (eager path only)
     bool timing = false;
+    bool capturing = false;
     struct Ev { hipEvent_t a, b; int stage; };
     std::vector<Ev> ev;
     double stage_ms[ST_N] = {0, 0, 0};
     long stage_cnt[ST_N] = {0, 0, 0};
+
+    DevCtl *ctl() { return ctlbuf.as<DevCtl>(); }
+    CtlRules rules() const
+    {
+        return CtlRules{para.alpha_red, para.beta_red, para.r_max, para.epsilon, para.stopRelChg};
+    }
 };
 
 namespace {
 
-int view_geom_from_para(const asd_para &p, double phi_deg, ViewGeom &g, std::vector<float> &ydet,
-                        std::vector<float> &zdet)
+void view_geom_from_para(const asd_para &p, double phi_deg, ViewGeom &g, std::vector<float> &ydet,
+                         std::vector<float> &zdet)
 {
     // Same float casts and formulas as the MEX gateways
     // (src/AxAtx/Ax_main_test.cpp:138-161) after ASD_POCS_cuda.m:111,116.
@@ -141,7 +245,6 @@ int view_geom_from_para(const asd_para &p, double phi_deg, ViewGeom &g, std::vec
         ydet[i] = (float)(((double)(float)i - (double)(float)p.na / 2.0 + 0.5) * (double)det);
     for (int j = 0; j < p.nb; j++)
         zdet[j] = (float)(((double)(float)j - (double)(float)p.nb / 2.0 + 0.5) * (double)det);
-    return ASD_OK;
 }
 
 int check_para(const asd_para *p, int nv)
@@ -152,6 +255,7 @@ int check_para(const asd_para *p, int nv)
     ASD_ARG(nv > 0, "phi_vec is empty");
     ASD_ARG(p->vxlsize > 0 && p->detsize > 0, "para.vxlsize and para.detsize must be positive");
     ASD_ARG(p->SO > 0 && p->OD > 0, "para.SO and para.OD must be positive");
+    ASD_ARG(p->numOfgradients >= 0, "para.numOfgradients must be >= 0");
     ASD_ARG(p->precision == 0 || p->precision == 1, "para.precision must be 0 or 1");
     return ASD_OK;
 }
@@ -163,17 +267,15 @@ struct StageTimer {
     hipEvent_t a = nullptr, b = nullptr;
     StageTimer(asd_plan *pp, int st) : p(pp), stage(st)
     {
-        if (!p->timing) return;
-        if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
-            a = b = nullptr;
-            return;
-        }
-        hipEventRecord(a, p->stream);
+        if (!p->timing || p->capturing) return;
+        if (hipEventCreate(&a) != hipSuccess) { a = nullptr; return; }
+        if (hipEventCreate(&b) != hipSuccess) { (void)hipEventDestroy(a); a = nullptr; return; }
+        (void)hipEventRecord(a, p->stream);
     }
     ~StageTimer()
     {
         if (!a) return;
-        hipEventRecord(b, p->stream);
+        (void)hipEventRecord(b, p->stream);
         p->ev.push_back({a, b, stage});
     }
 };
@@ -192,20 +294,19 @@ void collect_timing(asd_plan *p)
     p->ev.clear();
 }
 
-template <class PT>
-int sweep(asd_plan *p, double beta)
+// One SART sweep over all views with relaxation *relax.
+int enqueue_sweep(asd_plan *p, const double *relax)
 {
     const size_t M = p->M;
-    fill_f64<<<grid_for(M), 256, 0, p->stream>>>(p->res2.as<double>(), M, 0.0);
-    ASD_LAUNCH_CHECK("fill_f64");
     const long N = (long)p->N;
     const unsigned atx_blocks = (unsigned)((N + 255) / 256);
+    const int *stopped = &p->ctl()->stopped;
     for (int iv = 0; iv < p->nv; iv++) {
         const ViewGeom &g = p->views[iv];
         const float *yd = p->ydet.as<float>() + (size_t)iv * p->para.na;
         const float *zd = p->zdet.as<float>() + (size_t)iv * p->para.nb;
-        EpiSartAx<PT> ea{p->P.as<PT>() + (size_t)iv * M, p->cor.as<float>(), p->res2.as<double>(),
-                         p->rt.as<RayRec>(), yd, zd, g};
+        EpiSartAx<float> ea{p->P.as<float>() + (size_t)iv * M, p->cor.as<float>(),
+                            p->res2.as<double>(), p->rt.as<RayRec>(), yd, zd, stopped, g};
         int rc;
         {
             StageTimer t(p, ST_AX);
@@ -215,75 +316,97 @@ int sweep(asd_plan *p, double beta)
             StageTimer t(p, ST_ATX);
             atx_kernel<<<atx_blocks, 256, 0, p->stream>>>(
                 g, p->cor.as<float>(), p->rt.as<RayRec>(),
-                EpiSartAtx{p->m.as<float>(), beta, g.scale});
+                EpiSartAtx{p->m.as<float>(), relax, stopped, g.scale});
             ASD_LAUNCH_CHECK("atx_kernel(sart)");
         }
     }
-    // data error of the sweep
-    sum_f64<<<red_blocks(M), RED_THREADS, 0, p->stream>>>(
-        p->res2.as<double>(), M, p->red.partials, p->red.counters + TV_SLOT_DD2,
-        p->red.scalars + TV_SLOT_DD2);
-    ASD_LAUNCH_CHECK("sum_f64");
     return ASD_OK;
 }
 
-int run_sweep(asd_plan *p, double beta)
+int enqueue_post_sart(asd_plan *p, int sweep_only)
 {
-    return sweep<float>(p, beta);
+    post_sart_kernel<<<red_blocks(p->N > p->M ? p->N : p->M), RED_THREADS, 0, p->stream>>>(
+        p->m.as<float>(), p->mpre.as<float>(), p->mk1.as<float>(),
+        p->have_I0 ? p->I0.as<float>() : nullptr, p->f0.as<float>(), p->res2.as<double>(), p->N,
+        p->M, p->red.partials, p->red.counters + TV_SLOT_DP2, p->red.scalars, p->ctl(),
+        p->hist.as<double>(), p->rules(), sweep_only);
+    ASD_LAUNCH_CHECK("post_sart_kernel");
+    return ASD_OK;
 }
 
-int read_scalars(asd_plan *p, double *h)
+// The kernels of one outer iteration (no host synchronisation inside).
+int enqueue_iteration(asd_plan *p)
 {
-    ASD_HIP(hipMemcpyAsync(h, p->red.scalars, sizeof(double) * TV_NSLOTS, hipMemcpyDeviceToHost,
-                           p->stream));
+    int rc;
+    DevCtl *ctl = p->ctl();
+    if ((rc = enqueue_sweep(p, &ctl->beta))) return rc;
+    if ((rc = enqueue_post_sart(p, 0))) return rc;
+    const asd_para &pa = p->para;
+    const bool three_d = pa.nz >= 2;
+    if ((rc = tv_descent<float, float>(p->m.as<float>(), p->mk1.as<float>(), p->v.as<float>(),
+                                       pa.nx, pa.ny, three_d ? pa.nz : 1, three_d, 0.0,
+                                       &ctl->alpha, pa.numOfgradients, 1, p->red, p->stream,
+                                       true, &ctl->stopped)))
+        return rc;
+    post_tv_kernel<<<red_blocks(p->N), RED_THREADS, 0, p->stream>>>(
+        p->m.as<float>(), p->f0.as<float>(), p->mpre.as<float>(), p->N, p->red.partials,
+        p->red.counters + TV_SLOT_DG2, p->red.scalars, ctl, p->rules());
+    ASD_LAUNCH_CHECK("post_tv_kernel");
+    return ASD_OK;
+}
+
+void drop_graph(asd_plan *p)
+{
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    p->exec = nullptr;
+    p->graph = nullptr;
+}
+
+int build_graph(asd_plan *p)
+{
+    ASD_HIP(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
+    p->capturing = true;
+    int rc = enqueue_iteration(p);
+    p->capturing = false;
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(p->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) {
+        set_error(ASD_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        return ASD_ERR_HIP;
+    }
+    p->graph = g;
+    ASD_HIP(hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0));
+    return ASD_OK;
+}
+
+int sync_ctl(asd_plan *p)
+{
+    ASD_HIP(hipMemcpyAsync(&p->h, p->ctlbuf.p, sizeof(DevCtl), hipMemcpyDeviceToHost, p->stream));
     ASD_HIP(hipStreamSynchronize(p->stream));
     if (p->timing) collect_timing(p);
     return ASD_OK;
 }
 
-// One outer iteration k = p->k + 1 (ASD_POCS_cuda.m:103-171).
-int iterate_once(asd_plan *p)
+int write_ctl(asd_plan *p)
 {
-    const int k = p->k + 1;
-    const size_t N = p->N;
-    const size_t NB = N * sizeof(float);
-    int rc;
-    ASD_HIP(hipMemcpyAsync(p->mpre.p, p->m.p, NB, hipMemcpyDeviceToDevice, p->stream));
-    if (k == 1) ASD_HIP(hipMemcpyAsync(p->mk1.p, p->m.p, NB, hipMemcpyDeviceToDevice, p->stream));
-    if ((rc = run_sweep(p, p->beta))) return rc;
-    float *m = p->m.as<float>();
-    if ((rc = launch_sqdiff<float>(m, p->mpre.as<float>(), N, p->red, TV_SLOT_DP2, p->stream)))
-        return rc;
-    if ((rc = launch_sqdiff<float>(m, p->mk1.as<float>(), N, p->red, TV_SLOT_D2, p->stream)))
-        return rc;
-    if (p->have_I0 &&
-        (rc = launch_sqdiff<float>(m, p->I0.as<float>(), N, p->red, TV_SLOT_RMSE2, p->stream)))
-        return rc;
-    ASD_HIP(hipMemcpyAsync(p->f0.p, p->m.p, NB, hipMemcpyDeviceToDevice, p->stream));
-    // TV descent from m_k1 (the image before the first sweep), then dg.
-    const asd_para &pa = p->para;
-    const bool three_d = pa.nz >= 2;
-    if ((rc = tv_descent<float, float>(m, p->mk1.as<float>(), p->v.as<float>(), pa.nx, pa.ny,
-                                       three_d ? pa.nz : 1, three_d, p->alpha, nullptr,
-                                       pa.numOfgradients, 1, p->red, p->stream, true)))
-        return rc;
-    if ((rc = launch_sqdiff<float>(m, p->f0.as<float>(), N, p->red, TV_SLOT_DG2, p->stream)))
-        return rc;
-    double h[TV_NSLOTS];
-    if ((rc = read_scalars(p, h))) return rc;
-    p->k = k;
-    p->dd2 = h[TV_SLOT_DD2];
-    const double dd = std::sqrt(h[TV_SLOT_DD2]);
-    if (p->have_I0) p->rmse.push_back(std::sqrt(h[TV_SLOT_RMSE2] / (double)N));
-    const double dp = std::sqrt(h[TV_SLOT_DP2]);
-    if (dp / (double)N <= pa.stopRelChg) {
-        // m_res is the post-SART image (f0); the TV result is discarded.
-        p->stopped = true;
-        return ASD_OK;
-    }
-    const double dg = std::sqrt(h[TV_SLOT_DG2]);
-    if (dg > pa.r_max * dp && dd > pa.epsilon) p->alpha = p->alpha * pa.alpha_red;
-    p->beta = p->beta * pa.beta_red;
+    ASD_HIP(hipMemcpyAsync(p->ctlbuf.p, &p->h, sizeof(DevCtl), hipMemcpyHostToDevice, p->stream));
+    ASD_HIP(hipStreamSynchronize(p->stream));
+    return ASD_OK;
+}
+
+// m_pre = m_k1 = f0 = m: the current image becomes the start of the loop
+// (m_k1 is m_pre of the first iteration, ASD_POCS_cuda.m:151-153).
+int rebase(asd_plan *p)
+{
+    copy3_f32<<<grid_for(p->N), 256, 0, p->stream>>>(p->m.as<float>(), p->mpre.as<float>(),
+                                                      p->mk1.as<float>(), p->f0.as<float>(), p->N);
+    ASD_LAUNCH_CHECK("copy3_f32");
+    ASD_HIP(hipStreamSynchronize(p->stream));
     return ASD_OK;
 }
 
@@ -305,6 +428,15 @@ int download_f32_as_f64(asd_plan *p, const DeviceBuf &src, double *dst, size_t n
     ASD_HIP(hipMemcpyAsync(h.data(), src.p, n * sizeof(float), hipMemcpyDeviceToHost, p->stream));
     ASD_HIP(hipStreamSynchronize(p->stream));
     for (size_t i = 0; i < n; i++) dst[i] = (double)h[i];
+    return ASD_OK;
+}
+
+int need_projections(asd_plan *p)
+{
+    if (!p->have_P) {
+        set_error(ASD_ERR_STATE, "plan has no projections (call asd_plan_set_projections)");
+        return ASD_ERR_STATE;
+    }
     return ASD_OK;
 }
 
@@ -356,6 +488,7 @@ int asd_plan_create(const asd_para *para, const double *phi_vec, int nv, asd_pla
     p->lpr = ax_lanes_per_ray(para->na, para->nb, para->nx > para->ny ? para->nx : para->ny);
     auto fail = [&](int code) { asd_plan_destroy(p); return code; };
     if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess) {
+        p->stream = nullptr;
         set_error(ASD_ERR_HIP, "hipStreamCreateWithFlags failed");
         return fail(ASD_ERR_HIP);
     }
@@ -364,17 +497,21 @@ int asd_plan_create(const asd_para *para, const double *phi_vec, int nv, asd_pla
         (rc = p->mk1.ensure(N * 4)) || (rc = p->f0.ensure(N * 4)) || (rc = p->v.ensure(N * 4)) ||
         (rc = p->cor.ensure(M * 4)) || (rc = p->res2.ensure(M * 8)) ||
         (rc = p->rt.ensure(M * sizeof(RayRec))) || (rc = p->P.ensure((size_t)nv * M * 4)) ||
+        (rc = p->ctlbuf.ensure(sizeof(DevCtl))) || (rc = p->hist.ensure(HIST_CAP * 8)) ||
         (rc = p->red.init()))
         return fail(rc);
     if (hipMemcpy(p->ydet.p, yall.data(), yall.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(p->zdet.p, zall.data(), zall.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemset(p->m.p, 0, N * 4) != hipSuccess) {
-        set_error(ASD_ERR_HIP, "geometry upload failed");
+        hipMemset(p->m.p, 0, N * 4) != hipSuccess || hipMemset(p->res2.p, 0, M * 8) != hipSuccess) {
+        set_error(ASD_ERR_HIP, "plan initialisation failed");
         return fail(ASD_ERR_HIP);
     }
-    p->alpha = para->alpha;
-    p->beta = para->beta;
     *out = p;
+    rc = asd_plan_reset(p, nullptr, nullptr);
+    if (rc) {
+        *out = nullptr;
+        return fail(rc);
+    }
     return ASD_OK;
 }
 
@@ -398,6 +535,7 @@ int asd_plan_reset(asd_plan *p, const double *m0, const double *I0)
     } else {
         ASD_HIP(hipMemsetAsync(p->m.p, 0, p->N * 4, p->stream));
     }
+    bool had_I0 = p->have_I0;
     if (I0) {
         if ((rc = p->I0.ensure(p->N * 4))) return rc;
         if ((rc = upload_f64_as_f32(p, p->I0, I0, p->N))) return rc;
@@ -405,48 +543,48 @@ int asd_plan_reset(asd_plan *p, const double *m0, const double *I0)
     } else {
         p->have_I0 = false;
     }
-    ASD_HIP(hipMemcpyAsync(p->f0.p, p->m.p, p->N * 4, hipMemcpyDeviceToDevice, p->stream));
-    ASD_HIP(hipStreamSynchronize(p->stream));
-    p->k = 0;
-    p->stopped = false;
-    p->alpha = p->para.alpha;
-    p->beta = p->para.beta;
-    p->dd2 = 0;
-    p->rmse.clear();
-    return ASD_OK;
+    if (had_I0 != p->have_I0) drop_graph(p);  // the graph bakes in the I0 pointer
+    ASD_HIP(hipMemsetAsync(p->res2.p, 0, p->M * 8, p->stream));
+    if ((rc = rebase(p))) return rc;
+    std::memset(&p->h, 0, sizeof(p->h));
+    p->h.alpha = p->para.alpha;
+    p->h.beta = p->para.beta;
+    p->h.relax = p->para.lambda;
+    return write_ctl(p);
 }
 
 int asd_plan_iterate(asd_plan *p, int n_iter)
 {
     clear_error();
     ASD_ARG(p, "NULL plan");
-    if (!p->have_P) {
-        set_error(ASD_ERR_STATE, "plan has no projections (call asd_plan_set_projections)");
-        return ASD_ERR_STATE;
+    ASD_ARG(n_iter >= 0, "n_iter must be >= 0");
+    int rc;
+    if ((rc = need_projections(p))) return rc;
+    if (p->h.stopped || n_iter == 0) return ASD_OK;
+    if (p->use_graph && !p->timing) {
+        if (!p->exec && (rc = build_graph(p))) return rc;
+        for (int i = 0; i < n_iter; i++) ASD_HIP(hipGraphLaunch(p->exec, p->stream));
+    } else {
+        for (int i = 0; i < n_iter; i++)
+            if ((rc = enqueue_iteration(p))) return rc;
     }
-    for (int i = 0; i < n_iter && !p->stopped; i++) {
-        int rc = iterate_once(p);
-        if (rc) return rc;
-    }
-    return ASD_OK;
+    return sync_ctl(p);
 }
 
 int asd_plan_sweep(asd_plan *p, double lambda, double *dataTolerance)
 {
     clear_error();
     ASD_ARG(p, "NULL plan");
-    if (!p->have_P) {
-        set_error(ASD_ERR_STATE, "plan has no projections (call asd_plan_set_projections)");
-        return ASD_ERR_STATE;
-    }
-    int rc = run_sweep(p, lambda);
-    if (rc) return rc;
-    double h[TV_NSLOTS];
-    if ((rc = read_scalars(p, h))) return rc;
-    if (dataTolerance) *dataTolerance = h[TV_SLOT_DD2];
-    ASD_HIP(hipMemcpyAsync(p->f0.p, p->m.p, p->N * 4, hipMemcpyDeviceToDevice, p->stream));
-    ASD_HIP(hipStreamSynchronize(p->stream));
-    return ASD_OK;
+    int rc;
+    if ((rc = need_projections(p))) return rc;
+    ASD_ARG(!p->h.stopped, "plan has stopped; reset it first");
+    p->h.relax = lambda;
+    if ((rc = write_ctl(p))) return rc;
+    if ((rc = enqueue_sweep(p, &p->ctl()->relax))) return rc;
+    if ((rc = enqueue_post_sart(p, 1))) return rc;
+    if ((rc = sync_ctl(p))) return rc;
+    if (dataTolerance) *dataTolerance = p->h.dd2;
+    return rebase(p);
 }
 
 int asd_plan_get_image(asd_plan *p, double *m_res)
@@ -467,12 +605,14 @@ int asd_plan_get_out(asd_plan *p, asd_out *o, double *rmse)
 {
     clear_error();
     ASD_ARG(p && o, "NULL argument");
-    o->IterationNum = p->k;
-    o->dataTolerance = p->dd2;
-    o->beta = p->beta;
-    o->alpha = p->alpha;
-    if (rmse)
-        for (size_t i = 0; i < p->rmse.size(); i++) rmse[i] = p->rmse[i];
+    o->IterationNum = p->h.k;
+    o->dataTolerance = p->h.dd2;
+    o->beta = p->h.beta;
+    o->alpha = p->h.alpha;
+    if (rmse && p->have_I0 && p->h.k > 0) {
+        int n = p->h.k < HIST_CAP ? p->h.k : HIST_CAP;
+        ASD_HIP(hipMemcpy(rmse, p->hist.p, n * sizeof(double), hipMemcpyDeviceToHost));
+    }
     return ASD_OK;
 }
 
@@ -493,6 +633,15 @@ int asd_plan_set_timing(asd_plan *p, int enable)
         p->stage_ms[i] = 0;
         p->stage_cnt[i] = 0;
     }
+    return ASD_OK;
+}
+
+int asd_plan_set_graph(asd_plan *p, int enable)
+{
+    clear_error();
+    ASD_ARG(p, "NULL plan");
+    p->use_graph = enable != 0;
+    if (!p->use_graph) drop_graph(p);
     return ASD_OK;
 }
 
@@ -521,8 +670,9 @@ void asd_plan_destroy(asd_plan *p)
     if (p->stream) {
         (void)hipStreamSynchronize(p->stream);
         collect_timing(p);
-        (void)hipStreamDestroy(p->stream);
     }
+    drop_graph(p);
+    if (p->stream) (void)hipStreamDestroy(p->stream);
     delete p;
 }
 
@@ -543,8 +693,7 @@ int asd_pocs(const double *P, const double *phi_vec, int nv, const asd_para *par
     // (ASD_POCS_cuda.m:20-32).
     if ((rc = asd_plan_reset(p, m0, I0))) return done(rc);
     if (!m0 && para->initBySART) {
-        double tol;
-        if ((rc = asd_plan_sweep(p, para->lambda, &tol))) return done(rc);
+        if ((rc = asd_plan_sweep(p, para->lambda, nullptr))) return done(rc);
     }
     if ((rc = asd_plan_iterate(p, para->maxnumIter))) return done(rc);
     if ((rc = asd_plan_get_image(p, m_res))) return done(rc);

@@ -19,6 +19,7 @@ __global__ __launch_bounds__(256) void ax_kernel(ViewGeom g, const float *__rest
                                                  const float *__restrict__ z_det, Epi epi)
 {
     constexpr int RPB = 256 / LPR;  // rays per block
+    if (epi.skip()) return;
     const long nrays = (long)g.na * g.nb;
     const int lane = threadIdx.x % LPR;
     const long id = (long)blockIdx.x * RPB + threadIdx.x / LPR;
@@ -68,6 +69,7 @@ __global__ __launch_bounds__(256) void atx_kernel(ViewGeom g, const float *__res
                                                   const RayRec *__restrict__ rt, Epi epi)
 {
 #pragma clang fp contract(off)
+    if (epi.skip()) return;
     const unsigned b = xcd_block(blockIdx.x, gridDim.x);
     const long v = (long)b * 256 + threadIdx.x;
     const long N = (long)g.nx * g.ny * g.nz;

@@ -12,6 +12,7 @@ namespace {
 struct EpiPlain {
     float *y, *sr;
     float scale;
+    __device__ __forceinline__ bool skip() const { return false; }
     __device__ __forceinline__ void operator()(long id, float acc, float accl) const
     {
 #pragma clang fp contract(off)
@@ -23,6 +24,7 @@ struct EpiPlain {
 struct EpiBackPlain {
     float *x, *sc;
     float scale;
+    __device__ __forceinline__ bool skip() const { return false; }
     __device__ __forceinline__ void operator()(long v, float acc, float accl) const
     {
 #pragma clang fp contract(off)

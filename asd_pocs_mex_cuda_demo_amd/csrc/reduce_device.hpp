@@ -9,6 +9,7 @@ namespace asd {
 
 constexpr int RED_THREADS = 256;
 constexpr int RED_MAX_BLOCKS = 2048;  // grid cap of the grid-stride reductions
+constexpr int RED_MAX_SUMS = 4;       // values reduced together by grid_sums
 
 __device__ __forceinline__ double wave_sum(double v)
 {
@@ -66,6 +67,48 @@ inline unsigned red_blocks(size_t n)
     if (b > RED_MAX_BLOCKS) b = RED_MAX_BLOCKS;
     if (b == 0) b = 1;
     return (unsigned)b;
+}
+
+}  // namespace asd
+
+namespace asd {
+
+// K simultaneous sums.  Every block stores K partials; the last block to
+// arrive folds them (fixed order) into tot[0..K-1], visible to all its
+// threads, and returns true.  partials must hold K * gridDim.x doubles.
+template <int K>
+__device__ __forceinline__ bool grid_sums(const double (&val)[K], double *partials,
+                                          unsigned *counter, double (&tot)[K])
+{
+    __shared__ bool last;
+    __shared__ double res[K];
+    double s[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) s[k] = block_sum(val[k]);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < K; k++) partials[(size_t)k * gridDim.x + blockIdx.x] = s[k];
+        unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL,
+                                               __HIP_MEMORY_SCOPE_AGENT);
+        last = (prev == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!last) return false;
+    __atomic_thread_fence(__ATOMIC_ACQUIRE);
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        double t = 0.0;
+        for (unsigned i = threadIdx.x; i < gridDim.x; i += blockDim.x)
+            t += __hip_atomic_load(&partials[(size_t)k * gridDim.x + i], __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+        t = block_sum(t);
+        if (threadIdx.x == 0) res[k] = t;
+    }
+    if (threadIdx.x == 0) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; k++) tot[k] = res[k];
+    return true;
 }
 
 }  // namespace asd

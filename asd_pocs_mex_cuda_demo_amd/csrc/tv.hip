@@ -60,8 +60,10 @@ __device__ __forceinline__ A tv3_grad(const T *u, int nx, int ny, int nz, int x,
 template <class T, class A, bool THREE_D>
 __global__ __launch_bounds__(256) void tv_grad_kernel(const T *__restrict__ u, T *__restrict__ v,
                                                       int nx, int ny, int nz, double *partials,
-                                                      unsigned *counter, double *nrm2)
+                                                      unsigned *counter, double *nrm2,
+                                                      const int *skip)
 {
+    if (skip && *skip) return;
     const size_t n = (size_t)nx * ny * nz;
     double s = 0.0;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -82,9 +84,11 @@ __global__ __launch_bounds__(256) void tv_step_kernel(T *__restrict__ u, const T
                                                       size_t n, double alpha,
                                                       const double *__restrict__ alpha_dev,
                                                       const double *__restrict__ d2,
-                                                      const double *__restrict__ nrm2, int clamp)
+                                                      const double *__restrict__ nrm2, int clamp,
+                                                      const int *skip)
 {
 #pragma clang fp contract(off)
+    if (skip && *skip) return;
     const double a = alpha_dev ? *alpha_dev : alpha;
     const double ad = a * sqrt(*d2);
     const double nrm = sqrt(*nrm2);
@@ -133,7 +137,7 @@ int launch_sqdiff(const T *a, const T *b, size_t n, TvScratch &s, int slot, hipS
 template <class T, class A>
 int tv_descent(T *u, const T *img_old, T *v, int nx, int ny, int nz, bool three_d, double alpha,
                const double *alpha_dev, int nTVIter, int pos, TvScratch &s, hipStream_t st,
-               bool have_d2)
+               bool have_d2, const int *skip)
 {
     const size_t n = (size_t)nx * ny * nz;
     int rc;
@@ -142,14 +146,16 @@ int tv_descent(T *u, const T *img_old, T *v, int nx, int ny, int nz, bool three_
     for (int it = 0; it < nTVIter; it++) {
         if (three_d)
             tv_grad_kernel<T, A, true><<<gb, RED_THREADS, 0, st>>>(
-                u, v, nx, ny, nz, s.partials, s.counters + TV_SLOT_NRM2, s.scalars + TV_SLOT_NRM2);
+                u, v, nx, ny, nz, s.partials, s.counters + TV_SLOT_NRM2, s.scalars + TV_SLOT_NRM2,
+                skip);
         else
             tv_grad_kernel<T, A, false><<<gb, RED_THREADS, 0, st>>>(
-                u, v, nx, ny, 1, s.partials, s.counters + TV_SLOT_NRM2, s.scalars + TV_SLOT_NRM2);
+                u, v, nx, ny, 1, s.partials, s.counters + TV_SLOT_NRM2, s.scalars + TV_SLOT_NRM2,
+                skip);
         ASD_LAUNCH_CHECK("tv_grad_kernel");
         int clamp = (pos == 1 && it == nTVIter - 1) ? 1 : 0;
         tv_step_kernel<T><<<ub, 256, 0, st>>>(u, v, n, alpha, alpha_dev, s.scalars + TV_SLOT_D2,
-                                              s.scalars + TV_SLOT_NRM2, clamp);
+                                              s.scalars + TV_SLOT_NRM2, clamp, skip);
         ASD_LAUNCH_CHECK("tv_step_kernel");
     }
     return ASD_OK;
@@ -161,22 +167,22 @@ template int launch_sqdiff<double>(const double *, const double *, size_t, TvScr
                                    hipStream_t);
 template int tv_descent<float, float>(float *, const float *, float *, int, int, int, bool,
                                       double, const double *, int, int, TvScratch &,
-                                      hipStream_t, bool);
+                                      hipStream_t, bool, const int *);
 template int tv_descent<float, double>(float *, const float *, float *, int, int, int, bool,
                                        double, const double *, int, int, TvScratch &,
-                                       hipStream_t, bool);
+                                       hipStream_t, bool, const int *);
 template int tv_descent<double, double>(double *, const double *, double *, int, int, int, bool,
                                         double, const double *, int, int, TvScratch &,
-                                        hipStream_t, bool);
+                                        hipStream_t, bool, const int *);
 
 int TvScratch::init()
 {
     int rc;
-    if ((rc = buf.ensure(sizeof(double) * (RED_MAX_BLOCKS + TV_NSLOTS) +
+    if ((rc = buf.ensure(sizeof(double) * (RED_MAX_SUMS * RED_MAX_BLOCKS + TV_NSLOTS) +
                          sizeof(unsigned) * TV_NSLOTS)))
         return rc;
     partials = buf.as<double>();
-    scalars = partials + RED_MAX_BLOCKS;
+    scalars = partials + RED_MAX_SUMS * RED_MAX_BLOCKS;
     counters = reinterpret_cast<unsigned *>(scalars + TV_NSLOTS);
     ASD_HIP(hipMemset(buf.p, 0, buf.bytes));
     return ASD_OK;

@@ -19,7 +19,7 @@ enum : int {
 
 struct TvScratch {
     DeviceBuf buf;
-    double *partials = nullptr;  // RED_MAX_BLOCKS block partials
+    double *partials = nullptr;  // RED_MAX_SUMS * RED_MAX_BLOCKS block partials
     double *scalars = nullptr;   // TV_NSLOTS results
     unsigned *counters = nullptr;  // TV_NSLOTS arrival counters (re-armed by the kernels)
     int init();
@@ -31,10 +31,11 @@ int launch_sqdiff(const T *a, const T *b, size_t n, TvScratch &s, int slot, hipS
 // nTVIter steps of I -= alpha*d*v/||v|| (then positivity when pos == 1), in
 // place on u.  d^2 = ||u - img_old||^2 is computed first unless have_d2, in
 // which case scalars[TV_SLOT_D2] already holds it.  alpha_dev, when not
-// NULL, overrides alpha from device memory.
+// NULL, overrides alpha from device memory; a non-zero *skip (device flag,
+// may be NULL) makes every kernel return at entry.
 template <class T, class A>
 int tv_descent(T *u, const T *img_old, T *v, int nx, int ny, int nz, bool three_d, double alpha,
                const double *alpha_dev, int nTVIter, int pos, TvScratch &s, hipStream_t st,
-               bool have_d2);
+               bool have_d2, const int *skip = nullptr);
 
 }  // namespace asd

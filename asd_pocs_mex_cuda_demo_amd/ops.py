@@ -275,6 +275,9 @@ class Plan:
         return dict(IterationNum=o.IterationNum, dataTolerance=o.dataTolerance, beta=o.beta,
                     alpha=o.alpha), r
 
+    def set_graph(self, on):
+        nat.check(self._L.asd_plan_set_graph(self.h, int(on)))
+
     def set_timing(self, on):
         nat.check(self._L.asd_plan_set_timing(self.h, int(on)))
 

@@ -155,6 +155,9 @@ int asd_plan_sync(asd_plan *plan);
  * 1 = backprojector + update, 2 = TV gradient.  Enabling resets the totals. */
 int asd_plan_set_timing(asd_plan *plan, int enable);
 int asd_plan_get_timing(asd_plan *plan, int stage, double *total_ms, long *count);
+/* 1 (default): replay one outer iteration as a captured hipGraph;
+ * 0: enqueue its kernels one by one (always the case while timing is on). */
+int asd_plan_set_graph(asd_plan *plan, int enable);
 /* Lanes per ray of the forward projector and the plan's hipStream_t. */
 int asd_plan_info(asd_plan *plan, int *lanes_per_ray, void **stream);
 void asd_plan_destroy(asd_plan *plan);

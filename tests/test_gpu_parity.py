@@ -131,3 +131,46 @@ def test_plan_is_deterministic():
     a, _ = ops.ASD_POCS_cuda(P, para)
     b, _ = ops.ASD_POCS_cuda(P, para)
     assert np.array_equal(a, b)
+
+
+def test_graph_replays_match_eager_iterations():
+    """One outer iteration captured as a hipGraph and replayed, with eager
+    kernels and host synchronisation between replays, gives bit-identical
+    images, alpha, beta and RMSE to the same iterations enqueued eagerly."""
+    para, img, P = _small_problem(n=48, nv=12, na=72)
+    para.update(maxnumIter=6, alpha=0.2, numOfgradients=8)
+    eager = ops.Plan(para, para["phi_vec"])
+    graph = ops.Plan(para, para["phi_vec"])
+    eager.set_graph(False)
+    graph.set_graph(True)
+    for pl in (eager, graph):
+        pl.set_projections(P)
+        pl.reset(None, img)
+    for it in range(5):
+        eager.iterate(1)
+        graph.iterate(1)
+        # eager work on the device between replays
+        ops.Ax_cone_gpu(img.astype(np.float32), dict(para, phi=-0.3))
+        a, b = eager.image(), graph.image()
+        assert np.array_equal(a, b), it
+        assert np.array_equal(eager.current(), graph.current()), it
+        oa, ra = eager.out()
+        ob, rb = graph.out()
+        assert oa == ob and np.array_equal(ra, rb)
+    # several replays in one call
+    eager.iterate(3)
+    graph.iterate(3)
+    assert np.array_equal(eager.image(), graph.image())
+    eager.close()
+    graph.close()
+
+
+def test_stop_rule_ends_the_loop():
+    """stopRelChg large: the first iteration already stops; m_res is the
+    post-SART image and IterationNum = 1 (ASD_POCS_cuda.m:146-149)."""
+    para, img, P = _small_problem(n=32, nv=8, na=48)
+    para.update(maxnumIter=5, stopRelChg=1e9)
+    m, out = ops.ASD_POCS_cuda(P, para)
+    sweep, _ = ops.SART_cuda_inline(P, dict(para, **{"lambda": 1.0}))
+    assert out["IterationNum"] == 1 and out["alpha"] == 0.02 and out["beta"] == 1.0
+    np.testing.assert_allclose(m.ravel(order="F"), sweep, rtol=0, atol=0)
