@@ -23,6 +23,8 @@ EXPORTS = [
     "asd_plan_reset", "asd_plan_iterate", "asd_plan_get_image", "asd_plan_get_out",
     "asd_plan_sweep", "asd_plan_get_current", "asd_plan_sync", "asd_plan_set_timing",
     "asd_plan_get_timing", "asd_plan_set_graph", "asd_plan_info", "asd_plan_destroy",
+    "asd_batch_create", "asd_batch_set_case", "asd_batch_iterate", "asd_batch_get_image",
+    "asd_batch_get_out", "asd_batch_set_graph", "asd_batch_sync", "asd_batch_destroy",
 ]
 
 
@@ -105,6 +107,17 @@ def lib():
         fn.argtypes = args
     L.asd_plan_destroy.restype = None
     L.asd_plan_destroy.argtypes = [vp]
+    L.asd_batch_create.restype = i
+    L.asd_batch_create.argtypes = [i, vp, vp, vp, C.POINTER(vp)]
+    for name, args in [("asd_batch_set_case", [vp, i, vp, vp, vp]), ("asd_batch_iterate", [vp, i]),
+                       ("asd_batch_get_image", [vp, i, vp]),
+                       ("asd_batch_get_out", [vp, i, C.POINTER(AsdOut), vp]),
+                       ("asd_batch_set_graph", [vp, i]), ("asd_batch_sync", [vp])]:
+        fn = getattr(L, name)
+        fn.restype = i
+        fn.argtypes = args
+    L.asd_batch_destroy.restype = None
+    L.asd_batch_destroy.argtypes = [vp]
     _ = (fp,)
     _lib = L
     return L

@@ -24,7 +24,7 @@ BUILD = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libasdpocs.so")
 MEXDIR = os.path.join(PKG, "mex")
 
-HIP_SOURCES = ["common.hip", "projectors.hip", "tv.hip", "driver.hip"]
+HIP_SOURCES = ["common.hip", "projectors.hip", "tv.hip", "driver.hip", "batch.hip"]
 MEX_SOURCES = ["Ax_cone_gpu.cpp", "Atx_cone_gpu.cpp", "imgTVGradMin.cpp", "imgTVGradMin3D.cpp",
                "ASD_POCS_gpu.cpp"]
 

@@ -1,0 +1,850 @@
+// Batched ASD-POCS: many independent reconstructions of uneven size and view
+// count advance together, one kernel launch per stage for all of them.
+//
+// Cases are sorted by view count (descending), so at view step j the cases
+// still sweeping are a prefix of that order.  Each stage walks a flat index
+// space over its cases:
+//   ax_batch    rays of view j of every active case (LPR lanes per ray)
+//   atx_batch   256-voxel blocks of every active case (blocks never straddle cases)
+//   tv / post   256-voxel blocks of every case, with per-case FP64 reductions
+//               (the last block of a case folds that case's partials)
+// Per-case alpha, beta, k and stop flag live in device memory, so an outer
+// iteration of the whole batch is one fixed kernel sequence, captured as a
+// hipGraph.  The arithmetic of every case is the single-case driver's.
+#include <hip/hip_fp16.h>
+
+#include <algorithm>
+#include <cstddef>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "projector_kernels.hpp"
+#include "reduce_device.hpp"
+
+namespace asd {
+namespace {
+
+constexpr int B_HIST = 4096;  // RMSE entries kept per case
+
+struct BCtl {
+    double alpha, beta;
+    double dp, dd2, dg, d2, nrm2;
+    int k, stopped;
+};
+
+struct BRules {
+    double alpha_red, beta_red, r_max, epsilon, stopRelChg;
+    int ng;       // TV sub-iterations of this case
+    int three_d;  // 3-D stencil
+};
+
+// One case at one view step.
+struct BItem {
+    ViewGeom g;
+    float *m;
+    const void *P;         // this view of the case's sinogram (float or __half)
+    float *cor;
+    double *res2;
+    RayRec *rt;
+    const float *ydet, *zdet;
+    BCtl *ctl;
+};
+
+// One case for the whole-volume stages.
+struct BCase {
+    int nx, ny, nz;
+    int M;
+    size_t voff, roff;
+    BRules rules;
+    int has_I0;
+};
+
+__device__ __forceinline__ int find_seg(const int *pref, int n, int x)
+{
+    // largest c < n with pref[c] <= x (pref ascending, pref[0] = 0)
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (pref[mid] <= x) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <int LPR, bool KAHAN, class PT>
+__global__ __launch_bounds__(256) void ax_batch_kernel(const BItem *__restrict__ items,
+                                                       const int *__restrict__ rpref, int ncase)
+{
+    constexpr int RPB = 256 / LPR;
+    const int lane = threadIdx.x % LPR;
+    const int gid = blockIdx.x * RPB + threadIdx.x / LPR;
+    const int total = rpref[ncase];
+    bool active = gid < total;
+    int c = active ? find_seg(rpref, ncase, gid) : 0;
+    const BItem &it = items[c];
+    if (active && it.ctl->stopped) active = false;
+    const long id = gid - (active ? rpref[c] : 0);
+    Acc<KAHAN> sa, sl;
+    if (active) {
+        const ViewGeom &g = it.g;
+        int ia = (int)(id % g.na), ib = (int)(id / g.na);
+        AxRay r = ax_ray_setup(g, it.ydet[ia], it.zdet[ib]);
+        int nu = r.xdriven ? g.nx : g.ny;
+        for (int iu = lane; iu < nu; iu += LPR) ax_slab<KAHAN>(r, it.m, iu, g.nx, g.ny, sa, sl);
+    }
+    float acc, accl;
+    if constexpr (LPR > 1 && KAHAN) {
+        double a = (double)sa.s - (double)sa.c, b = (double)sl.s - (double)sl.c;
+#pragma unroll
+        for (int off = LPR / 2; off > 0; off >>= 1) {
+            a += __shfl_xor(a, off, LPR);
+            b += __shfl_xor(b, off, LPR);
+        }
+        acc = (float)a;
+        accl = (float)b;
+    } else {
+        acc = sa.value();
+        accl = sl.value();
+        if constexpr (LPR > 1) {
+#pragma unroll
+            for (int off = LPR / 2; off > 0; off >>= 1) {
+                acc += __shfl_xor(acc, off, LPR);
+                accl += __shfl_xor(accl, off, LPR);
+            }
+        }
+    }
+    if (!active || lane != 0) return;
+    {
+#pragma clang fp contract(off)
+        const ViewGeom &g = it.g;
+        float proj = acc * g.scale, sumr = accl * g.scale;
+        double r = (sumr == 0.0f) ? 1.0 : (double)sumr;
+        double c0 = (double)(float)((const PT *)it.P)[id] - (double)proj;
+        it.res2[id] += c0 * c0;
+        it.cor[id] = (float)(c0 / r);
+        int ia = (int)(id % g.na), ib = (int)(id / g.na);
+        it.rt[id] = make_ray_rec(g, it.ydet[ia], it.zdet[ib]);
+    }
+}
+
+template <bool KAHAN>
+__global__ __launch_bounds__(256) void atx_batch_kernel(const BItem *__restrict__ items,
+                                                        const int *__restrict__ bpref, int ncase)
+{
+#pragma clang fp contract(off)
+    const int b = blockIdx.x;
+    if (b >= bpref[ncase]) return;
+    const int c = find_seg(bpref, ncase, b);
+    const BItem &it = items[c];
+    if (it.ctl->stopped) return;
+    const ViewGeom &g = it.g;
+    const long v = (long)(b - bpref[c]) * 256 + threadIdx.x;
+    const long N = (long)g.nx * g.ny * g.nz;
+    if (v >= N) return;
+    const int ix = (int)(v % g.nx);
+    const long r = v / g.nx;
+    const int iy = (int)(r % g.ny), iz = (int)(r / g.ny);
+    Footprint f = footprint(g, ix, iy, iz);
+    const float sx = g.cos_phi * (-g.SO), sy = g.sin_phi * (-g.SO);
+    const float px = sx - f.xc, py = sy - f.yc, pz = 0.0f - f.zc;
+    const float hz = g.dz / 2.0f;
+    Acc<KAHAN> acc, accl;
+    for (int ib = f.b_lo; ib <= f.b_hi; ib++) {
+        const long row = (long)ib * g.na;
+        for (int ia = f.a_lo; ia <= f.a_hi; ia++) {
+            RayRec R = it.rt[row + ia];
+            float l = find_l_3d_rec(R, px, py, pz, hz, g.dz);
+            acc.add(l * it.cor[row + ia]);
+            accl.add(l);
+        }
+    }
+    float bp = acc.value() * g.scale, sc = accl.value() * g.scale;
+    double cc = (sc == 0.0f) ? 1.0 : (double)sc;
+    double nm = (double)it.m[v] + it.ctl->beta * ((double)bp / cc);
+    it.m[v] = nm < 0 ? 0.0f : (float)nm;
+}
+
+// Per-case sums over the blocks [bpref[c], bpref[c+1]): partials are indexed
+// by global block; the last block of the case folds them.
+template <int K>
+__device__ __forceinline__ bool case_sums(const double (&val)[K], double *partials, int nblocks_all,
+                                          int first, int nblk, unsigned *counter, double (&tot)[K])
+{
+    __shared__ bool last;
+    __shared__ double res[K];
+    double s[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) s[k] = block_sum(val[k]);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < K; k++) partials[(size_t)k * nblocks_all + blockIdx.x] = s[k];
+        unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL,
+                                               __HIP_MEMORY_SCOPE_AGENT);
+        last = (prev == (unsigned)nblk - 1);
+    }
+    __syncthreads();
+    if (!last) return false;
+    __atomic_thread_fence(__ATOMIC_ACQUIRE);
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        double t = 0.0;
+        for (int i = threadIdx.x; i < nblk; i += blockDim.x)
+            t += __hip_atomic_load(&partials[(size_t)k * nblocks_all + first + i],
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        t = block_sum(t);
+        if (threadIdx.x == 0) res[k] = t;
+    }
+    if (threadIdx.x == 0) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; k++) tot[k] = res[k];
+    return true;
+}
+
+template <class T>
+__device__ __forceinline__ float ldz(const T *u, int nx, int ny, int nz, int x, int y, int z)
+{
+    if (x < 0 || x >= nx || y < 0 || y >= ny || z < 0 || z >= nz) return 0.0f;
+    return (float)u[((size_t)z * ny + y) * nx + x];
+}
+
+// TV gradient of every case (FP32, the single-case driver's stencils).
+__global__ __launch_bounds__(256) void tv_grad_batch(const BCase *__restrict__ cs,
+                                                     const int *__restrict__ bpref, int ncase,
+                                                     const float *__restrict__ u_all,
+                                                     float *__restrict__ v_all, BCtl *ctl,
+                                                     double *partials, unsigned *counters,
+                                                     int it)
+{
+#pragma clang fp contract(off)
+    const int b = blockIdx.x;
+    const int c = find_seg(bpref, ncase, b);
+    const BCase &C = cs[c];
+    if (ctl[c].stopped || it >= C.rules.ng) return;
+    const float *u = u_all + C.voff;
+    float *v = v_all + C.voff;
+    const size_t n = (size_t)C.nx * C.ny * C.nz;
+    const size_t i = (size_t)(b - bpref[c]) * 256 + threadIdx.x;
+    double s = 0.0;
+    if (i < n) {
+        const float eps = 1e-8f;
+        int x = (int)(i % C.nx);
+        size_t rr = i / C.nx;
+        int y = (int)(rr % C.ny), z = (int)(rr / C.ny);
+        float g;
+        if (!C.rules.three_d) {
+            const int R = C.nx, Cn = C.ny;
+            float p = ldz(u, R, Cn, 1, x, y, 0);
+            float rm = ldz(u, R, Cn, 1, x - 1, y, 0), cm = ldz(u, R, Cn, 1, x, y - 1, 0);
+            float rp = ldz(u, R, Cn, 1, x + 1, y, 0), rp_cm = ldz(u, R, Cn, 1, x + 1, y - 1, 0);
+            float cp = ldz(u, R, Cn, 1, x, y + 1, 0), rm_cp = ldz(u, R, Cn, 1, x - 1, y + 1, 0);
+            float g1 = ((p - rm) + (p - cm)) / sqrtf(eps + (p - rm) * (p - rm) + (p - cm) * (p - cm));
+            float g2 = -(rp - p) / sqrtf(eps + (rp - p) * (rp - p) + (rp - rp_cm) * (rp - rp_cm));
+            float g3 = -(cp - p) / sqrtf(eps + (cp - p) * (cp - p) + (cp - rm_cp) * (cp - rm_cp));
+            g = g1 + g2 + g3;
+        } else {
+            auto U = [&](int a, int bb, int cc) { return ldz(u, C.nx, C.ny, C.nz, a, bb, cc); };
+            float p = U(x, y, z);
+            float xm = U(x - 1, y, z), ym = U(x, y - 1, z), zm = U(x, y, z - 1);
+            float zp = U(x, y, z + 1), xm_zp = U(x - 1, y, z + 1), ym_zp = U(x, y - 1, z + 1);
+            float yp = U(x, y + 1, z), xm_yp = U(x - 1, y + 1, z), yp_zm = U(x, y + 1, z - 1);
+            float xp = U(x + 1, y, z), xp_ym = U(x + 1, y - 1, z), xp_zm = U(x + 1, y, z - 1);
+            float g1 = (6.0f * p - 2.0f * (xm + ym + zm)) /
+                       sqrtf(eps + (p - xm) * (p - xm) + (p - ym) * (p - ym) + (p - zm) * (p - zm));
+            float g2 = (2.0f * p - 2.0f * zp) /
+                       sqrtf(eps + (zp - xm_zp) * (zp - xm_zp) + (zp - ym_zp) * (zp - ym_zp) + (zp - p) * (zp - p));
+            float g3 = (2.0f * p - 2.0f * yp) /
+                       sqrtf(eps + (yp - xm_yp) * (yp - xm_yp) + (yp - p) * (yp - p) + (yp - yp_zm) * (yp - yp_zm));
+            float g4 = (2.0f * p - 2.0f * xp) /
+                       sqrtf(eps + (xp - p) * (xp - p) + (xp - xp_ym) * (xp - xp_ym) + (xp - xp_zm) * (xp - xp_zm));
+            g = g1 + g2 + g3 + g4;
+        }
+        v[i] = g;
+        s = (double)g * (double)g;
+    }
+    double val[1] = {s}, tot[1];
+    const int nblk = bpref[c + 1] - bpref[c];
+    if (!case_sums<1>(val, partials, bpref[ncase], bpref[c], nblk, counters + c, tot)) return;
+    if (threadIdx.x == 0) ctl[c].nrm2 = tot[0];
+}
+
+__global__ __launch_bounds__(256) void tv_step_batch(const BCase *__restrict__ cs,
+                                                     const int *__restrict__ bpref, int ncase,
+                                                     float *__restrict__ u_all,
+                                                     const float *__restrict__ v_all,
+                                                     const BCtl *ctl, int it)
+{
+#pragma clang fp contract(off)
+    const int b = blockIdx.x;
+    const int c = find_seg(bpref, ncase, b);
+    const BCase &C = cs[c];
+    if (ctl[c].stopped || it >= C.rules.ng) return;
+    const size_t n = (size_t)C.nx * C.ny * C.nz;
+    const size_t i = (size_t)(b - bpref[c]) * 256 + threadIdx.x;
+    if (i >= n) return;
+    const double ad = ctl[c].alpha * sqrt(ctl[c].d2);
+    const double nrm = sqrt(ctl[c].nrm2);
+    double w = (double)u_all[C.voff + i] - ad * (double)v_all[C.voff + i] / nrm;
+    if (it == C.rules.ng - 1 && w < 0) w = 0;  // positivity after the last step
+    u_all[C.voff + i] = (float)w;
+}
+
+struct BBufs {
+    float *m, *mpre, *mk1, *f0, *I0;
+    double *res2, *hist;
+};
+
+__global__ __launch_bounds__(256) void post_sart_batch(const BCase *__restrict__ cs,
+                                                       const int *__restrict__ bpref, int ncase,
+                                                       BBufs bb, BCtl *ctl, double *partials,
+                                                       unsigned *counters, int sweep_only)
+{
+    const int b = blockIdx.x;
+    const int c = find_seg(bpref, ncase, b);
+    const BCase &C = cs[c];
+    if (ctl[c].stopped) return;
+    const size_t n = (size_t)C.nx * C.ny * C.nz;
+    const size_t i = (size_t)(b - bpref[c]) * 256 + threadIdx.x;
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    if (i < n) {
+        const size_t k = C.voff + i;
+        float mi = bb.m[k];
+        double a = (double)mi - (double)bb.mpre[k];
+        double d = (double)mi - (double)bb.mk1[k];
+        v[0] = a * a;
+        v[1] = d * d;
+        if (C.has_I0) {
+            double e = (double)mi - (double)bb.I0[k];
+            v[2] = e * e;
+        }
+        bb.f0[k] = mi;
+    }
+    if (i < (size_t)C.M) {
+        v[3] = bb.res2[C.roff + i];
+        bb.res2[C.roff + i] = 0.0;
+    }
+    double tot[4];
+    const int nblk = bpref[c + 1] - bpref[c];
+    if (!case_sums<4>(v, partials, bpref[ncase], bpref[c], nblk, counters + c, tot)) return;
+    if (threadIdx.x != 0) return;
+    BCtl &K = ctl[c];
+    K.dd2 = tot[3];
+    K.d2 = tot[1];
+    if (sweep_only) return;
+    const int k = K.k + 1;
+    K.k = k;
+    if (C.has_I0 && k - 1 < B_HIST) bb.hist[(size_t)c * B_HIST + k - 1] = sqrt(tot[2] / (double)n);
+    K.dp = sqrt(tot[0]);
+    if (K.dp / (double)n <= C.rules.stopRelChg) K.stopped = 1;
+}
+
+__global__ __launch_bounds__(256) void post_tv_batch(const BCase *__restrict__ cs,
+                                                     const int *__restrict__ bpref, int ncase,
+                                                     BBufs bb, BCtl *ctl, double *partials,
+                                                     unsigned *counters)
+{
+    const int b = blockIdx.x;
+    const int c = find_seg(bpref, ncase, b);
+    const BCase &C = cs[c];
+    if (ctl[c].stopped) return;
+    const size_t n = (size_t)C.nx * C.ny * C.nz;
+    const size_t i = (size_t)(b - bpref[c]) * 256 + threadIdx.x;
+    double v[1] = {0.0};
+    if (i < n) {
+        const size_t k = C.voff + i;
+        float mi = bb.m[k];
+        double e = (double)mi - (double)bb.f0[k];
+        v[0] = e * e;
+        bb.mpre[k] = mi;
+    }
+    double tot[1];
+    const int nblk = bpref[c + 1] - bpref[c];
+    if (!case_sums<1>(v, partials, bpref[ncase], bpref[c], nblk, counters + c, tot)) return;
+    if (threadIdx.x != 0) return;
+    BCtl &K = ctl[c];
+    K.dg = sqrt(tot[0]);
+    if (K.dg > C.rules.r_max * K.dp && sqrt(K.dd2) > C.rules.epsilon)
+        K.alpha = K.alpha * C.rules.alpha_red;
+    K.beta = K.beta * C.rules.beta_red;
+}
+
+__global__ __launch_bounds__(256) void b_copy3(const float *a, float *b, float *c, float *d,
+                                               size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        float x = a[i];
+        b[i] = x;
+        c[i] = x;
+        d[i] = x;
+    }
+}
+
+__global__ __launch_bounds__(256) void b_cvt_f32(const double *a, float *b, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        b[i] = (float)a[i];
+}
+
+__global__ __launch_bounds__(256) void b_cvt_f16(const double *a, __half *b, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        b[i] = __float2half_rn((float)a[i]);
+}
+
+unsigned gridn(size_t n)
+{
+    size_t b = (n + 255) / 256;
+    if (b > 8192) b = 8192;
+    return (unsigned)(b ? b : 1);
+}
+
+}  // namespace
+}  // namespace asd
+
+using namespace asd;
+
+struct asd_batch {
+    int K = 0;                 // cases
+    int precision = 0;
+    std::vector<asd_para> para;
+    std::vector<int> nv;
+    std::vector<int> order;    // cases sorted by nv (descending)
+    std::vector<size_t> voff, roff, poff;
+    size_t Ntot = 0, Mtot = 0, Ptot = 0;
+    int max_nv = 0, max_ng = 0, lpr = 64;
+    std::vector<int> active;   // active cases per view step (prefix of `order`)
+    hipStream_t stream = nullptr;
+    DeviceBuf m, mpre, mk1, f0, I0, v, cor, res2, rt, P, ydet, zdet, ctlb, hist;
+    DeviceBuf items, rpref, bpref, cases, cbpref, partials, counters;
+    std::vector<int> h_rpref_tot, h_bpref_tot;  // per step: total rays / blocks
+    int case_blocks = 0;
+    std::vector<char> have_P, have_I0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool use_graph = true;
+    std::vector<BCtl> h;
+};
+
+namespace {
+
+int b_enqueue_iteration(asd_batch *B, int sweep_only, int iterate)
+{
+    const int K = B->K;
+    // sweep: view steps
+    for (int j = 0; j < B->max_nv; j++) {
+        const int A = B->active[j];
+        if (A == 0) break;
+        const BItem *items = B->items.as<BItem>() + (size_t)j * K;
+        const int *rp = B->rpref.as<int>() + (size_t)j * (K + 1);
+        const int *bp = B->bpref.as<int>() + (size_t)j * (K + 1);
+        const int rays = B->h_rpref_tot[j], blocks = B->h_bpref_tot[j];
+        const unsigned ab = (unsigned)(((long)rays * B->lpr + 255) / 256);
+        if (B->precision == 1) {
+            switch (B->lpr) {
+            case 64: ax_batch_kernel<64, true, __half><<<ab, 256, 0, B->stream>>>(items, rp, A); break;
+            case 16: ax_batch_kernel<16, true, __half><<<ab, 256, 0, B->stream>>>(items, rp, A); break;
+            case 4: ax_batch_kernel<4, true, __half><<<ab, 256, 0, B->stream>>>(items, rp, A); break;
+            default: ax_batch_kernel<1, true, __half><<<ab, 256, 0, B->stream>>>(items, rp, A); break;
+            }
+            ASD_LAUNCH_CHECK("ax_batch_kernel");
+            atx_batch_kernel<true><<<blocks, 256, 0, B->stream>>>(items, bp, A);
+        } else {
+            switch (B->lpr) {
+            case 64: ax_batch_kernel<64, false, float><<<ab, 256, 0, B->stream>>>(items, rp, A); break;
+            case 16: ax_batch_kernel<16, false, float><<<ab, 256, 0, B->stream>>>(items, rp, A); break;
+            case 4: ax_batch_kernel<4, false, float><<<ab, 256, 0, B->stream>>>(items, rp, A); break;
+            default: ax_batch_kernel<1, false, float><<<ab, 256, 0, B->stream>>>(items, rp, A); break;
+            }
+            ASD_LAUNCH_CHECK("ax_batch_kernel");
+            atx_batch_kernel<false><<<blocks, 256, 0, B->stream>>>(items, bp, A);
+        }
+        ASD_LAUNCH_CHECK("atx_batch_kernel");
+    }
+    BBufs bb{B->m.as<float>(), B->mpre.as<float>(), B->mk1.as<float>(), B->f0.as<float>(),
+             B->I0.as<float>(), B->res2.as<double>(), B->hist.as<double>()};
+    const BCase *cs = B->cases.as<BCase>();
+    const int *cbp = B->cbpref.as<int>();
+    BCtl *ctl = B->ctlb.as<BCtl>();
+    post_sart_batch<<<B->case_blocks, 256, 0, B->stream>>>(cs, cbp, K, bb, ctl,
+                                                            B->partials.as<double>(),
+                                                            B->counters.as<unsigned>(),
+                                                            sweep_only);
+    ASD_LAUNCH_CHECK("post_sart_batch");
+    if (!iterate) return ASD_OK;
+    for (int it = 0; it < B->max_ng; it++) {
+        tv_grad_batch<<<B->case_blocks, 256, 0, B->stream>>>(
+            cs, cbp, K, B->m.as<float>(), B->v.as<float>(), ctl, B->partials.as<double>(),
+            B->counters.as<unsigned>() + K, it);
+        ASD_LAUNCH_CHECK("tv_grad_batch");
+        tv_step_batch<<<B->case_blocks, 256, 0, B->stream>>>(cs, cbp, K, B->m.as<float>(),
+                                                             B->v.as<float>(), ctl, it);
+        ASD_LAUNCH_CHECK("tv_step_batch");
+    }
+    post_tv_batch<<<B->case_blocks, 256, 0, B->stream>>>(cs, cbp, K, bb, ctl,
+                                                          B->partials.as<double>(),
+                                                          B->counters.as<unsigned>() + 2 * K);
+    ASD_LAUNCH_CHECK("post_tv_batch");
+    return ASD_OK;
+}
+
+int b_sync(asd_batch *B)
+{
+    ASD_HIP(hipMemcpyAsync(B->h.data(), B->ctlb.p, sizeof(BCtl) * B->K, hipMemcpyDeviceToHost,
+                           B->stream));
+    ASD_HIP(hipStreamSynchronize(B->stream));
+    return ASD_OK;
+}
+
+void b_drop_graph(asd_batch *B)
+{
+    if (B->exec) (void)hipGraphExecDestroy(B->exec);
+    if (B->graph) (void)hipGraphDestroy(B->graph);
+    B->exec = nullptr;
+    B->graph = nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+void asd_batch_destroy(asd_batch *B)
+{
+    if (!B) return;
+    if (B->stream) (void)hipStreamSynchronize(B->stream);
+    b_drop_graph(B);
+    if (B->stream) (void)hipStreamDestroy(B->stream);
+    delete B;
+}
+
+int asd_batch_create(int ncases, const asd_para *paras, const double *const *phi_vecs,
+                     const int *nvs, asd_batch **out)
+{
+    clear_error();
+    ASD_ARG(out && paras && phi_vecs && nvs, "NULL argument");
+    ASD_ARG(ncases > 0, "ncases must be positive");
+    *out = nullptr;
+    for (int c = 0; c < ncases; c++) {
+        const asd_para &p = paras[c];
+        ASD_ARG(p.nx > 0 && p.ny > 0 && p.nz > 0 && p.na > 0 && p.nb > 0,
+                "every case needs positive nx, ny, nz, na, nb");
+        ASD_ARG(nvs[c] > 0 && phi_vecs[c], "every case needs views");
+        ASD_ARG(p.SO > 0 && p.OD > 0 && p.vxlsize > 0 && p.detsize > 0, "bad case geometry");
+        ASD_ARG(p.precision == paras[0].precision, "all cases must share para.precision");
+        ASD_ARG(p.numOfgradients >= 0, "para.numOfgradients must be >= 0");
+    }
+    asd_batch *B = new asd_batch();
+    auto fail = [&](int code) { asd_batch_destroy(B); return code; };
+    const int K = ncases;
+    B->K = K;
+    B->precision = paras[0].precision;
+    B->para.assign(paras, paras + K);
+    B->nv.assign(nvs, nvs + K);
+    B->voff.resize(K);
+    B->roff.resize(K);
+    B->poff.resize(K);
+    long maxrays = 0;
+    int maxu = 0;
+    for (int c = 0; c < K; c++) {
+        const asd_para &p = paras[c];
+        B->voff[c] = B->Ntot;
+        B->roff[c] = B->Mtot;
+        B->poff[c] = B->Ptot;
+        B->Ntot += (size_t)p.nx * p.ny * p.nz;
+        B->Mtot += (size_t)p.na * p.nb;
+        B->Ptot += (size_t)p.na * p.nb * nvs[c];
+        B->max_nv = std::max(B->max_nv, nvs[c]);
+        B->max_ng = std::max(B->max_ng, p.numOfgradients);
+        maxrays = std::max(maxrays, (long)p.na * p.nb);
+        maxu = std::max(maxu, std::max(p.nx, p.ny));
+    }
+    // lanes per ray from the largest case, restricted to the compiled set
+    int l = ax_lanes_per_ray((int)std::min<long>(maxrays, 1L << 30), 1, maxu);
+    B->lpr = l >= 64 ? 64 : l >= 16 ? 16 : l >= 4 ? 4 : 1;
+    if (hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking) != hipSuccess) {
+        B->stream = nullptr;
+        set_error(ASD_ERR_HIP, "hipStreamCreateWithFlags failed");
+        return fail(ASD_ERR_HIP);
+    }
+    B->order.resize(K);
+    std::iota(B->order.begin(), B->order.end(), 0);
+    std::stable_sort(B->order.begin(), B->order.end(),
+                     [&](int a, int b) { return nvs[a] > nvs[b]; });
+    // per-case device buffers
+    const size_t N = B->Ntot, M = B->Mtot;
+    std::vector<size_t> yoff(K), zoff(K);
+    size_t ytot = 0, ztot = 0;
+    for (int c = 0; c < K; c++) {
+        yoff[c] = ytot;
+        zoff[c] = ztot;
+        ytot += (size_t)paras[c].na * nvs[c];
+        ztot += (size_t)paras[c].nb * nvs[c];
+    }
+    int rc;
+    if ((rc = B->m.ensure(N * 4)) || (rc = B->mpre.ensure(N * 4)) || (rc = B->mk1.ensure(N * 4)) ||
+        (rc = B->f0.ensure(N * 4)) || (rc = B->I0.ensure(N * 4)) || (rc = B->v.ensure(N * 4)) ||
+        (rc = B->cor.ensure(M * 4)) || (rc = B->res2.ensure(M * 8)) ||
+        (rc = B->rt.ensure(M * sizeof(RayRec))) ||
+        (rc = B->P.ensure(B->Ptot * (B->precision == 1 ? 2 : 4))) ||
+        (rc = B->ydet.ensure(ytot * 4)) || (rc = B->zdet.ensure(ztot * 4)) ||
+        (rc = B->ctlb.ensure(sizeof(BCtl) * K)) || (rc = B->hist.ensure((size_t)K * B_HIST * 8)))
+        return fail(rc);
+    // geometry tables and per-step work lists
+    std::vector<float> yall(ytot), zall(ztot);
+    std::vector<std::vector<ViewGeom>> geoms(K);
+    for (int c = 0; c < K; c++) {
+        const asd_para &p = paras[c];
+        geoms[c].resize(nvs[c]);
+        for (int j = 0; j < nvs[c]; j++) {
+            ViewGeom &g = geoms[c][j];
+            double theta = phi_vecs[c][j] / 180 * M_PI;
+            float phi = (float)(-1 * theta);
+            float sc = (float)p.vxlsize, det = (float)p.detsize / sc;
+            g.cos_phi = (float)std::cos((double)phi);
+            g.sin_phi = (float)std::sin((double)phi);
+            g.SO = (float)p.SO / sc;
+            g.OD = (float)p.OD / sc;
+            g.scale = sc;
+            g.dz = 1.0f;
+            g.dy_det = g.dz_det = det;
+            g.nx = p.nx; g.ny = p.ny; g.nz = p.nz; g.na = p.na; g.nb = p.nb;
+            for (int i = 0; i < p.na; i++)
+                yall[yoff[c] + (size_t)j * p.na + i] =
+                    (float)(((double)(float)i - (double)(float)p.na / 2.0 + 0.5) * (double)det);
+            for (int i = 0; i < p.nb; i++)
+                zall[zoff[c] + (size_t)j * p.nb + i] =
+                    (float)(((double)(float)i - (double)(float)p.nb / 2.0 + 0.5) * (double)det);
+        }
+    }
+    std::vector<BItem> items((size_t)B->max_nv * K);
+    std::vector<int> rp((size_t)B->max_nv * (K + 1), 0), bp((size_t)B->max_nv * (K + 1), 0);
+    B->active.assign(B->max_nv, 0);
+    B->h_rpref_tot.assign(B->max_nv, 0);
+    B->h_bpref_tot.assign(B->max_nv, 0);
+    BCtl *dctl = B->ctlb.as<BCtl>();
+    const size_t Pe = B->precision == 1 ? 2 : 4;
+    for (int j = 0; j < B->max_nv; j++) {
+        int A = 0, rays = 0, blocks = 0;
+        for (int s = 0; s < K; s++) {
+            int c = B->order[s];
+            if (nvs[c] <= j) break;
+            const asd_para &p = paras[c];
+            BItem &it = items[(size_t)j * K + A];
+            it.g = geoms[c][j];
+            it.m = B->m.as<float>() + B->voff[c];
+            it.P = (const char *)B->P.p + (B->poff[c] + (size_t)j * p.na * p.nb) * Pe;
+            it.cor = B->cor.as<float>() + B->roff[c];
+            it.res2 = B->res2.as<double>() + B->roff[c];
+            it.rt = B->rt.as<RayRec>() + B->roff[c];
+            it.ydet = B->ydet.as<float>() + yoff[c] + (size_t)j * p.na;
+            it.zdet = B->zdet.as<float>() + zoff[c] + (size_t)j * p.nb;
+            it.ctl = dctl + c;
+            rp[(size_t)j * (K + 1) + A] = rays;
+            bp[(size_t)j * (K + 1) + A] = blocks;
+            rays += p.na * p.nb;
+            blocks += (int)(((size_t)p.nx * p.ny * p.nz + 255) / 256);
+            A++;
+        }
+        rp[(size_t)j * (K + 1) + A] = rays;
+        bp[(size_t)j * (K + 1) + A] = blocks;
+        B->active[j] = A;
+        B->h_rpref_tot[j] = rays;
+        B->h_bpref_tot[j] = blocks;
+    }
+    std::vector<BCase> cs(K);
+    std::vector<int> cbp(K + 1, 0);
+    for (int c = 0; c < K; c++) {
+        const asd_para &p = paras[c];
+        cs[c].nx = p.nx; cs[c].ny = p.ny; cs[c].nz = p.nz;
+        cs[c].M = p.na * p.nb;
+        cs[c].voff = B->voff[c];
+        cs[c].roff = B->roff[c];
+        cs[c].rules = BRules{p.alpha_red, p.beta_red, p.r_max, p.epsilon, p.stopRelChg,
+                             p.numOfgradients, p.nz >= 2 ? 1 : 0};
+        cs[c].has_I0 = 0;
+        size_t need = std::max((size_t)p.nx * p.ny * p.nz, (size_t)p.na * p.nb);
+        cbp[c + 1] = cbp[c] + (int)((need + 255) / 256);
+    }
+    B->case_blocks = cbp[K];
+    if ((rc = B->items.ensure(items.size() * sizeof(BItem))) ||
+        (rc = B->rpref.ensure(rp.size() * 4)) || (rc = B->bpref.ensure(bp.size() * 4)) ||
+        (rc = B->cases.ensure(cs.size() * sizeof(BCase))) || (rc = B->cbpref.ensure(cbp.size() * 4)) ||
+        (rc = B->partials.ensure((size_t)4 * B->case_blocks * 8)) ||
+        (rc = B->counters.ensure((size_t)3 * K * 4)))
+        return fail(rc);
+    if (hipMemcpy(B->ydet.p, yall.data(), ytot * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(B->zdet.p, zall.data(), ztot * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(B->items.p, items.data(), items.size() * sizeof(BItem), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(B->rpref.p, rp.data(), rp.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(B->bpref.p, bp.data(), bp.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(B->cases.p, cs.data(), cs.size() * sizeof(BCase), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(B->cbpref.p, cbp.data(), cbp.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemset(B->counters.p, 0, B->counters.bytes) != hipSuccess ||
+        hipMemset(B->m.p, 0, N * 4) != hipSuccess || hipMemset(B->res2.p, 0, M * 8) != hipSuccess ||
+        hipMemset(B->I0.p, 0, N * 4) != hipSuccess || hipMemset(B->P.p, 0, B->P.bytes) != hipSuccess) {
+        set_error(ASD_ERR_HIP, "batch initialisation failed");
+        return fail(ASD_ERR_HIP);
+    }
+    B->have_P.assign(K, 0);
+    B->have_I0.assign(K, 0);
+    B->h.assign(K, BCtl{});
+    for (int c = 0; c < K; c++) {
+        B->h[c].alpha = paras[c].alpha;
+        B->h[c].beta = paras[c].beta;
+    }
+    if (hipMemcpy(B->ctlb.p, B->h.data(), sizeof(BCtl) * K, hipMemcpyHostToDevice) != hipSuccess) {
+        set_error(ASD_ERR_HIP, "batch initialisation failed");
+        return fail(ASD_ERR_HIP);
+    }
+    // start images: zeros
+    b_copy3<<<gridn(N), 256, 0, B->stream>>>(B->m.as<float>(), B->mpre.as<float>(),
+                                             B->mk1.as<float>(), B->f0.as<float>(), N);
+    if (hipStreamSynchronize(B->stream) != hipSuccess) {
+        set_error(ASD_ERR_HIP, "batch initialisation failed");
+        return fail(ASD_ERR_HIP);
+    }
+    *out = B;
+    return ASD_OK;
+}
+
+int asd_batch_set_case(asd_batch *B, int c, const double *P, const double *m0, const double *I0)
+{
+    clear_error();
+    ASD_ARG(B && P, "NULL argument");
+    ASD_ARG(c >= 0 && c < B->K, "case index out of range");
+    const asd_para &p = B->para[c];
+    const size_t np = (size_t)p.na * p.nb * B->nv[c], n = (size_t)p.nx * p.ny * p.nz;
+    int rc;
+    DeviceBuf tmp;
+    if ((rc = tmp.ensure(std::max(np, n) * 8))) return rc;
+    ASD_HIP(hipMemcpyAsync(tmp.p, P, np * 8, hipMemcpyHostToDevice, B->stream));
+    if (B->precision == 1)
+        b_cvt_f16<<<gridn(np), 256, 0, B->stream>>>(tmp.as<double>(), B->P.as<__half>() + B->poff[c], np);
+    else
+        b_cvt_f32<<<gridn(np), 256, 0, B->stream>>>(tmp.as<double>(), B->P.as<float>() + B->poff[c], np);
+    ASD_LAUNCH_CHECK("b_cvt");
+    ASD_HIP(hipStreamSynchronize(B->stream));
+    float *mc = B->m.as<float>() + B->voff[c];
+    if (m0) {
+        ASD_HIP(hipMemcpyAsync(tmp.p, m0, n * 8, hipMemcpyHostToDevice, B->stream));
+        b_cvt_f32<<<gridn(n), 256, 0, B->stream>>>(tmp.as<double>(), mc, n);
+        ASD_LAUNCH_CHECK("b_cvt");
+    } else {
+        ASD_HIP(hipMemsetAsync(mc, 0, n * 4, B->stream));
+    }
+    b_copy3<<<gridn(n), 256, 0, B->stream>>>(mc, B->mpre.as<float>() + B->voff[c],
+                                             B->mk1.as<float>() + B->voff[c],
+                                             B->f0.as<float>() + B->voff[c], n);
+    ASD_LAUNCH_CHECK("b_copy3");
+    int has = 0;
+    if (I0) {
+        ASD_HIP(hipStreamSynchronize(B->stream));
+        ASD_HIP(hipMemcpyAsync(tmp.p, I0, n * 8, hipMemcpyHostToDevice, B->stream));
+        b_cvt_f32<<<gridn(n), 256, 0, B->stream>>>(tmp.as<double>(), B->I0.as<float>() + B->voff[c], n);
+        ASD_LAUNCH_CHECK("b_cvt");
+        has = 1;
+    }
+    ASD_HIP(hipMemcpyAsync((char *)B->cases.p + sizeof(BCase) * c + offsetof(BCase, has_I0), &has,
+                           sizeof(int), hipMemcpyHostToDevice, B->stream));
+    B->h[c] = BCtl{};
+    B->h[c].alpha = p.alpha;
+    B->h[c].beta = p.beta;
+    ASD_HIP(hipMemcpyAsync(B->ctlb.as<BCtl>() + c, &B->h[c], sizeof(BCtl), hipMemcpyHostToDevice,
+                           B->stream));
+    ASD_HIP(hipStreamSynchronize(B->stream));
+    B->have_P[c] = 1;
+    B->have_I0[c] = (char)has;
+    return ASD_OK;
+}
+
+int asd_batch_iterate(asd_batch *B, int n_iter)
+{
+    clear_error();
+    ASD_ARG(B, "NULL batch");
+    ASD_ARG(n_iter >= 0, "n_iter must be >= 0");
+    for (int c = 0; c < B->K; c++)
+        if (!B->have_P[c]) {
+            set_error(ASD_ERR_STATE, "case " + std::to_string(c) + " has no projections");
+            return ASD_ERR_STATE;
+        }
+    int rc;
+    if (B->use_graph) {
+        if (!B->exec) {
+            ASD_HIP(hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal));
+            rc = b_enqueue_iteration(B, 0, 1);
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(B->stream, &g);
+            if (rc) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            if (e != hipSuccess) {
+                set_error(ASD_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+                return ASD_ERR_HIP;
+            }
+            B->graph = g;
+            ASD_HIP(hipGraphInstantiate(&B->exec, g, nullptr, nullptr, 0));
+        }
+        for (int i = 0; i < n_iter; i++) ASD_HIP(hipGraphLaunch(B->exec, B->stream));
+    } else {
+        for (int i = 0; i < n_iter; i++)
+            if ((rc = b_enqueue_iteration(B, 0, 1))) return rc;
+    }
+    return b_sync(B);
+}
+
+int asd_batch_set_graph(asd_batch *B, int enable)
+{
+    clear_error();
+    ASD_ARG(B, "NULL batch");
+    B->use_graph = enable != 0;
+    if (!B->use_graph) b_drop_graph(B);
+    return ASD_OK;
+}
+
+int asd_batch_sync(asd_batch *B)
+{
+    clear_error();
+    ASD_ARG(B, "NULL batch");
+    ASD_HIP(hipStreamSynchronize(B->stream));
+    return ASD_OK;
+}
+
+int asd_batch_get_image(asd_batch *B, int c, double *m_res)
+{
+    clear_error();
+    ASD_ARG(B && m_res, "NULL argument");
+    ASD_ARG(c >= 0 && c < B->K, "case index out of range");
+    const asd_para &p = B->para[c];
+    size_t n = (size_t)p.nx * p.ny * p.nz;
+    std::vector<float> h(n);
+    ASD_HIP(hipMemcpyAsync(h.data(), B->f0.as<float>() + B->voff[c], n * 4, hipMemcpyDeviceToHost,
+                           B->stream));
+    ASD_HIP(hipStreamSynchronize(B->stream));
+    for (size_t i = 0; i < n; i++) m_res[i] = h[i];
+    return ASD_OK;
+}
+
+int asd_batch_get_out(asd_batch *B, int c, asd_out *o, double *rmse)
+{
+    clear_error();
+    ASD_ARG(B && o, "NULL argument");
+    ASD_ARG(c >= 0 && c < B->K, "case index out of range");
+    const BCtl &h = B->h[c];
+    o->IterationNum = h.k;
+    o->dataTolerance = h.dd2;
+    o->alpha = h.alpha;
+    o->beta = h.beta;
+    if (rmse && B->have_I0[c] && h.k > 0) {
+        int n = h.k < B_HIST ? h.k : B_HIST;
+        ASD_HIP(hipMemcpy(rmse, B->hist.as<double>() + (size_t)c * B_HIST, n * 8,
+                          hipMemcpyDeviceToHost));
+    }
+    return ASD_OK;
+}
+
+}  // extern "C"

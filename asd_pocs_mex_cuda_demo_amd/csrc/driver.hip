@@ -15,6 +15,8 @@
 //   post_tv    dg = ||m - f0||; m_pre = m; the alpha and beta rules (m:166-170)
 // alpha, beta, k and the stop flag live in device memory (DevCtl); once the
 // stop rule fires every later kernel returns at entry.
+#include <hip/hip_fp16.h>
+
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -28,7 +30,7 @@ namespace asd {
 namespace {
 
 // Stage timing (events on the plan stream) for the dominant kernels.
-enum : int { ST_AX = 0, ST_ATX = 1, ST_TVGRAD = 2, ST_N = 3 };
+enum : int { ST_AX = 0, ST_ATX = 1, ST_TVGRAD = 2, ST_N = 3 };  // ST_TVGRAD: a whole TV descent
 
 constexpr int HIST_CAP = 1 << 16;  // RMSE history entries kept on the device
 
@@ -165,6 +167,14 @@ __global__ __launch_bounds__(256) void f64_to_f32(const double *__restrict__ a, 
         b[i] = (float)a[i];
 }
 
+__global__ __launch_bounds__(256) void f64_to_f16(const double *__restrict__ a, __half *b,
+                                                  size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        b[i] = __float2half_rn((float)a[i]);
+}
+
 __global__ __launch_bounds__(256) void copy3_f32(const float *__restrict__ a, float *b, float *c,
                                                  float *d, size_t n)
 {
@@ -294,8 +304,11 @@ void collect_timing(asd_plan *p)
     p->ev.clear();
 }
 
-// One SART sweep over all views with relaxation *relax.
-int enqueue_sweep(asd_plan *p, const double *relax)
+// One SART sweep over all views with relaxation *relax.  PT is the sinogram
+// storage type (float, or __half in the precision variant, which also
+// compensates every FP32 sum).
+template <class PT, bool KAHAN>
+int enqueue_sweep_t(asd_plan *p, const double *relax)
 {
     const size_t M = p->M;
     const long N = (long)p->N;
@@ -305,22 +318,29 @@ int enqueue_sweep(asd_plan *p, const double *relax)
         const ViewGeom &g = p->views[iv];
         const float *yd = p->ydet.as<float>() + (size_t)iv * p->para.na;
         const float *zd = p->zdet.as<float>() + (size_t)iv * p->para.nb;
-        EpiSartAx<float> ea{p->P.as<float>() + (size_t)iv * M, p->cor.as<float>(),
-                            p->res2.as<double>(), p->rt.as<RayRec>(), yd, zd, stopped, g};
+        EpiSartAx<PT> ea{p->P.as<PT>() + (size_t)iv * M, p->cor.as<float>(),
+                         p->res2.as<double>(), p->rt.as<RayRec>(), yd, zd, stopped, g};
         int rc;
         {
             StageTimer t(p, ST_AX);
-            if ((rc = dispatch_ax(p->lpr, g, p->m.as<float>(), yd, zd, ea, p->stream))) return rc;
+            if ((rc = dispatch_ax<KAHAN>(p->lpr, g, p->m.as<float>(), yd, zd, ea, p->stream)))
+                return rc;
         }
         {
             StageTimer t(p, ST_ATX);
-            atx_kernel<<<atx_blocks, 256, 0, p->stream>>>(
+            atx_kernel<EpiSartAtx, KAHAN><<<atx_blocks, 256, 0, p->stream>>>(
                 g, p->cor.as<float>(), p->rt.as<RayRec>(),
                 EpiSartAtx{p->m.as<float>(), relax, stopped, g.scale});
             ASD_LAUNCH_CHECK("atx_kernel(sart)");
         }
     }
     return ASD_OK;
+}
+
+int enqueue_sweep(asd_plan *p, const double *relax)
+{
+    if (p->para.precision == 1) return enqueue_sweep_t<__half, true>(p, relax);
+    return enqueue_sweep_t<float, false>(p, relax);
 }
 
 int enqueue_post_sart(asd_plan *p, int sweep_only)
@@ -343,11 +363,14 @@ int enqueue_iteration(asd_plan *p)
     if ((rc = enqueue_post_sart(p, 0))) return rc;
     const asd_para &pa = p->para;
     const bool three_d = pa.nz >= 2;
-    if ((rc = tv_descent<float, float>(p->m.as<float>(), p->mk1.as<float>(), p->v.as<float>(),
-                                       pa.nx, pa.ny, three_d ? pa.nz : 1, three_d, 0.0,
-                                       &ctl->alpha, pa.numOfgradients, 1, p->red, p->stream,
-                                       true, &ctl->stopped)))
-        return rc;
+    {
+        StageTimer t(p, ST_TVGRAD);
+        if ((rc = tv_descent<float, float>(p->m.as<float>(), p->mk1.as<float>(),
+                                           p->v.as<float>(), pa.nx, pa.ny, three_d ? pa.nz : 1,
+                                           three_d, 0.0, &ctl->alpha, pa.numOfgradients, 1,
+                                           p->red, p->stream, true, &ctl->stopped)))
+            return rc;
+    }
     post_tv_kernel<<<red_blocks(p->N), RED_THREADS, 0, p->stream>>>(
         p->m.as<float>(), p->f0.as<float>(), p->mpre.as<float>(), p->N, p->red.partials,
         p->red.counters + TV_SLOT_DG2, p->red.scalars, ctl, p->rules());
@@ -496,7 +519,8 @@ int asd_plan_create(const asd_para *para, const double *phi_vec, int nv, asd_pla
         (rc = p->m.ensure(N * 4)) || (rc = p->mpre.ensure(N * 4)) ||
         (rc = p->mk1.ensure(N * 4)) || (rc = p->f0.ensure(N * 4)) || (rc = p->v.ensure(N * 4)) ||
         (rc = p->cor.ensure(M * 4)) || (rc = p->res2.ensure(M * 8)) ||
-        (rc = p->rt.ensure(M * sizeof(RayRec))) || (rc = p->P.ensure((size_t)nv * M * 4)) ||
+        (rc = p->rt.ensure(M * sizeof(RayRec))) ||
+        (rc = p->P.ensure((size_t)nv * M * (para->precision == 1 ? 2 : 4))) ||
         (rc = p->ctlbuf.ensure(sizeof(DevCtl))) || (rc = p->hist.ensure(HIST_CAP * 8)) ||
         (rc = p->red.init()))
         return fail(rc);
@@ -519,8 +543,18 @@ int asd_plan_set_projections(asd_plan *p, const double *P)
 {
     clear_error();
     ASD_ARG(p && P, "NULL argument");
-    int rc = upload_f64_as_f32(p, p->P, P, (size_t)p->nv * p->M);
-    if (rc) return rc;
+    int rc;
+    if (p->para.precision == 1) {
+        size_t n = (size_t)p->nv * p->M;
+        DeviceBuf tmp;
+        if ((rc = tmp.ensure(n * sizeof(double)))) return rc;
+        ASD_HIP(hipMemcpyAsync(tmp.p, P, n * sizeof(double), hipMemcpyHostToDevice, p->stream));
+        f64_to_f16<<<grid_for(n), 256, 0, p->stream>>>(tmp.as<double>(), p->P.as<__half>(), n);
+        ASD_LAUNCH_CHECK("f64_to_f16");
+        ASD_HIP(hipStreamSynchronize(p->stream));
+    } else if ((rc = upload_f64_as_f32(p, p->P, P, (size_t)p->nv * p->M))) {
+        return rc;
+    }
     p->have_P = true;
     return ASD_OK;
 }

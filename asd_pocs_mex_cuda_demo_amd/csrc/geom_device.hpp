@@ -53,9 +53,31 @@ __device__ __forceinline__ AxRay ax_ray_setup(const ViewGeom &g, float ydet, flo
     return r;
 }
 
+// Running FP32 sum; with KAHAN the rounding error of every addition is
+// carried in `c` (compensated summation; needs no reassociation, which the
+// build guarantees with -ffp-contract=off and no fast-math).
+template <bool KAHAN>
+struct Acc {
+    float s = 0.0f, c = 0.0f;
+    __device__ __forceinline__ void add(float x)
+    {
+#pragma clang fp contract(off)
+        if constexpr (KAHAN) {
+            float y = x - c;
+            float t = s + y;
+            c = (t - s) - y;
+            s = t;
+        } else {
+            s += x;
+        }
+    }
+    __device__ __forceinline__ float value() const { return KAHAN ? s - c : s; }
+};
+
 // Accumulate slab iu of ray r into (acc, accl); X is the volume (x fastest).
+template <bool KAHAN = false>
 __device__ __forceinline__ void ax_slab(const AxRay &r, const float *__restrict__ X, int iu,
-                                        int nx, int ny, float &acc, float &accl)
+                                        int nx, int ny, Acc<KAHAN> &acc, Acc<KAHAN> &accl)
 {
 #pragma clang fp contract(off)
     float ua = (float)(iu - r.nu2), ub = ua + 1;
@@ -94,22 +116,22 @@ __device__ __forceinline__ void ax_slab(const AxRay &r, const float *__restrict_
         size_t idx = r.xdriven ? (size_t)cz1 * sxy + (size_t)ct1 * nx + iu
                                : (size_t)cz1 * sxy + (size_t)iu * nx + ct1;
         float wl = w0 * l;
-        acc += wl * X[idx];
-        accl += wl;
+        acc.add(wl * X[idx]);
+        accl.add(wl);
     }
     if (inm) {
         size_t idx = r.xdriven ? (size_t)czm * sxy + (size_t)ctm * nx + iu
                                : (size_t)czm * sxy + (size_t)iu * nx + ctm;
         float wl = w1 * l;
-        acc += wl * X[idx];
-        accl += wl;
+        acc.add(wl * X[idx]);
+        accl.add(wl);
     }
     if (in_t2 && in_z2) {
         size_t idx = r.xdriven ? (size_t)cz2 * sxy + (size_t)ct2 * nx + iu
                                : (size_t)cz2 * sxy + (size_t)iu * nx + ct2;
         float wl = w2 * l;
-        acc += wl * X[idx];
-        accl += wl;
+        acc.add(wl * X[idx]);
+        accl.add(wl);
     }
 }
 

@@ -13,7 +13,7 @@ namespace asd {
 // (cu:375, :576); with LPR = 64 a whole wave serves one ray, which keeps a 2-D
 // view (a few hundred rays) spread over the chip, and lanes walking along x
 // read consecutive voxels.
-template <int LPR, class Epi>
+template <int LPR, bool KAHAN, class Epi>
 __global__ __launch_bounds__(256) void ax_kernel(ViewGeom g, const float *__restrict__ X,
                                                  const float *__restrict__ y_det,
                                                  const float *__restrict__ z_det, Epi epi)
@@ -24,37 +24,52 @@ __global__ __launch_bounds__(256) void ax_kernel(ViewGeom g, const float *__rest
     const int lane = threadIdx.x % LPR;
     const long id = (long)blockIdx.x * RPB + threadIdx.x / LPR;
     const bool active = id < nrays;
-    float acc = 0.0f, accl = 0.0f;
+    Acc<KAHAN> sa, sl;
     if (active) {
         int ia = (int)(id % g.na), ib = (int)(id / g.na);
         AxRay r = ax_ray_setup(g, y_det[ia], z_det[ib]);
         int nu = r.xdriven ? g.nx : g.ny;
-        for (int iu = lane; iu < nu; iu += LPR) ax_slab(r, X, iu, g.nx, g.ny, acc, accl);
+        for (int iu = lane; iu < nu; iu += LPR) ax_slab<KAHAN>(r, X, iu, g.nx, g.ny, sa, sl);
     }
-    if constexpr (LPR > 1) {
+    float acc, accl;
+    if constexpr (LPR > 1 && KAHAN) {
+        // compensated lane partials are combined in FP64
+        double a = (double)sa.s - (double)sa.c, b = (double)sl.s - (double)sl.c;
 #pragma unroll
         for (int off = LPR / 2; off > 0; off >>= 1) {
-            acc += __shfl_xor(acc, off, LPR);
-            accl += __shfl_xor(accl, off, LPR);
+            a += __shfl_xor(a, off, LPR);
+            b += __shfl_xor(b, off, LPR);
+        }
+        acc = (float)a;
+        accl = (float)b;
+    } else {
+        acc = sa.value();
+        accl = sl.value();
+        if constexpr (LPR > 1) {
+#pragma unroll
+            for (int off = LPR / 2; off > 0; off >>= 1) {
+                acc += __shfl_xor(acc, off, LPR);
+                accl += __shfl_xor(accl, off, LPR);
+            }
         }
     }
     if (active && lane == 0) epi(id, acc, accl);
 }
 
-template <class Epi>
+template <bool KAHAN = false, class Epi>
 int dispatch_ax(int lpr, const ViewGeom &g, const float *x, const float *y_det,
                 const float *z_det, const Epi &epi, hipStream_t s)
 {
     long nrays = (long)g.na * g.nb;
     auto blocks = [&](int l) { return (unsigned)((nrays * l + 255) / 256); };
     switch (lpr) {
-    case 1: ax_kernel<1, Epi><<<blocks(1), 256, 0, s>>>(g, x, y_det, z_det, epi); break;
-    case 2: ax_kernel<2, Epi><<<blocks(2), 256, 0, s>>>(g, x, y_det, z_det, epi); break;
-    case 4: ax_kernel<4, Epi><<<blocks(4), 256, 0, s>>>(g, x, y_det, z_det, epi); break;
-    case 8: ax_kernel<8, Epi><<<blocks(8), 256, 0, s>>>(g, x, y_det, z_det, epi); break;
-    case 16: ax_kernel<16, Epi><<<blocks(16), 256, 0, s>>>(g, x, y_det, z_det, epi); break;
-    case 32: ax_kernel<32, Epi><<<blocks(32), 256, 0, s>>>(g, x, y_det, z_det, epi); break;
-    default: ax_kernel<64, Epi><<<blocks(64), 256, 0, s>>>(g, x, y_det, z_det, epi); break;
+    case 1: ax_kernel<1, KAHAN, Epi><<<blocks(1), 256, 0, s>>>(g, x, y_det, z_det, epi); break;
+    case 2: ax_kernel<2, KAHAN, Epi><<<blocks(2), 256, 0, s>>>(g, x, y_det, z_det, epi); break;
+    case 4: ax_kernel<4, KAHAN, Epi><<<blocks(4), 256, 0, s>>>(g, x, y_det, z_det, epi); break;
+    case 8: ax_kernel<8, KAHAN, Epi><<<blocks(8), 256, 0, s>>>(g, x, y_det, z_det, epi); break;
+    case 16: ax_kernel<16, KAHAN, Epi><<<blocks(16), 256, 0, s>>>(g, x, y_det, z_det, epi); break;
+    case 32: ax_kernel<32, KAHAN, Epi><<<blocks(32), 256, 0, s>>>(g, x, y_det, z_det, epi); break;
+    default: ax_kernel<64, KAHAN, Epi><<<blocks(64), 256, 0, s>>>(g, x, y_det, z_det, epi); break;
     }
     ASD_LAUNCH_CHECK("ax_kernel");
     return ASD_OK;
@@ -64,7 +79,7 @@ int dispatch_ax(int lpr, const ViewGeom &g, const float *x, const float *y_det,
 // footprint, ib outer and ia inner like the reference (cu:255-269), and sums
 // l*y and l in that order.  Blocks are remapped so each XCD owns a contiguous
 // slab of voxels and the rays they share stay in its L2.
-template <class Epi>
+template <class Epi, bool KAHAN = false>
 __global__ __launch_bounds__(256) void atx_kernel(ViewGeom g, const float *__restrict__ Y,
                                                   const RayRec *__restrict__ rt, Epi epi)
 {
@@ -81,17 +96,17 @@ __global__ __launch_bounds__(256) void atx_kernel(ViewGeom g, const float *__res
     const float sx = g.cos_phi * (-g.SO), sy = g.sin_phi * (-g.SO);
     const float px = sx - f.xc, py = sy - f.yc, pz = 0.0f - f.zc;
     const float hz = g.dz / 2.0f;
-    float acc = 0.0f, accl = 0.0f;
+    Acc<KAHAN> acc, accl;
     for (int ib = f.b_lo; ib <= f.b_hi; ib++) {
         const long row = (long)ib * g.na;
         for (int ia = f.a_lo; ia <= f.a_hi; ia++) {
             RayRec R = rt[row + ia];
             float l = find_l_3d_rec(R, px, py, pz, hz, g.dz);
-            acc += l * Y[row + ia];
-            accl += l;
+            acc.add(l * Y[row + ia]);
+            accl.add(l);
         }
     }
-    epi(v, acc, accl);
+    epi(v, acc.value(), accl.value());
 }
 
 }  // namespace asd

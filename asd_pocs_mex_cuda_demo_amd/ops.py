@@ -303,3 +303,69 @@ class Plan:
             self.close()
         except Exception:
             pass
+
+
+class Batch:
+    """Many reconstructions advanced together (asd_batch_* in include/asdpocs.h).
+    `paras` is a list of para dicts (each with its own phi_vec)."""
+
+    def __init__(self, paras):
+        self._L = nat.lib()
+        self.K = len(paras)
+        self.paras = paras
+        arr = (nat.AsdPara * self.K)()
+        self._phis = [np.ascontiguousarray(np.asarray(p["phi_vec"], np.float64).ravel())
+                      for p in paras]
+        for k, p in enumerate(paras):
+            arr[k] = make_para(p)
+        self._arr = arr
+        ptrs = (C.c_void_p * self.K)(*[ph.ctypes.data for ph in self._phis])
+        nvs = (C.c_int * self.K)(*[ph.size for ph in self._phis])
+        h = C.c_void_p()
+        nat.check(self._L.asd_batch_create(self.K, C.cast(arr, C.c_void_p), C.cast(ptrs, C.c_void_p),
+                                           C.cast(nvs, C.c_void_p), C.byref(h)))
+        self.h = h
+
+    def set_case(self, c, P, m0=None, I0=None):
+        p = self._arr[c]
+        P = np.asarray(P, np.float64)
+        M = p.na * p.nb
+        Pv = np.ascontiguousarray(P.T if P.shape[0] == M else P)
+        a = None if m0 is None else np.ascontiguousarray(np.asarray(m0, np.float64).reshape(-1, order="F"))
+        b = None if I0 is None else np.ascontiguousarray(np.asarray(I0, np.float64).reshape(-1, order="F"))
+        nat.check(self._L.asd_batch_set_case(self.h, c, _ptr(Pv), None if a is None else _ptr(a),
+                                             None if b is None else _ptr(b)))
+
+    def iterate(self, n):
+        nat.check(self._L.asd_batch_iterate(self.h, int(n)))
+
+    def set_graph(self, on):
+        nat.check(self._L.asd_batch_set_graph(self.h, int(on)))
+
+    def sync(self):
+        nat.check(self._L.asd_batch_sync(self.h))
+
+    def image(self, c):
+        p = self._arr[c]
+        m = np.zeros(p.nx * p.ny * p.nz)
+        nat.check(self._L.asd_batch_get_image(self.h, c, _ptr(m)))
+        return m
+
+    def out(self, c):
+        o = nat.AsdOut()
+        nat.check(self._L.asd_batch_get_out(self.h, c, C.byref(o), None))
+        r = np.zeros(max(o.IterationNum, 1))
+        nat.check(self._L.asd_batch_get_out(self.h, c, C.byref(o), _ptr(r)))
+        return dict(IterationNum=o.IterationNum, dataTolerance=o.dataTolerance, beta=o.beta,
+                    alpha=o.alpha), r[:o.IterationNum]
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._L.asd_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

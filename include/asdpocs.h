@@ -162,6 +162,29 @@ int asd_plan_set_graph(asd_plan *plan, int enable);
 int asd_plan_info(asd_plan *plan, int *lanes_per_ray, void **stream);
 void asd_plan_destroy(asd_plan *plan);
 
+/* ---------------------------------------------------------------------- */
+/* Batched reconstructions                                                 */
+/* ---------------------------------------------------------------------- */
+
+/* Many independent ASD-POCS reconstructions (each with its own para, size
+ * and view count) advanced together: one kernel launch per stage for all
+ * cases, one hipGraph per outer iteration of the batch.  All cases share
+ * para.precision.  No counterpart in the reference, which runs one
+ * reconstruction per MATLAB call (utilities/ASD_POCS_cuda.m). */
+typedef struct asd_batch asd_batch;
+int asd_batch_create(int ncases, const asd_para *paras, const double *const *phi_vecs,
+                     const int *nvs, asd_batch **batch);
+/* Sinogram of case c ((na*nb) x nv, view contiguous), its start image m0
+ * (NULL: zeros) and reference image I0 for the RMSE (NULL: none). */
+int asd_batch_set_case(asd_batch *batch, int c, const double *P, const double *m0,
+                       const double *I0);
+int asd_batch_iterate(asd_batch *batch, int n_iter);
+int asd_batch_get_image(asd_batch *batch, int c, double *m_res);
+int asd_batch_get_out(asd_batch *batch, int c, asd_out *out, double *rmse);
+int asd_batch_set_graph(asd_batch *batch, int enable);
+int asd_batch_sync(asd_batch *batch);
+void asd_batch_destroy(asd_batch *batch);
+
 #ifdef __cplusplus
 }
 #endif

@@ -174,3 +174,45 @@ def test_stop_rule_ends_the_loop():
     sweep, _ = ops.SART_cuda_inline(P, dict(para, **{"lambda": 1.0}))
     assert out["IterationNum"] == 1 and out["alpha"] == 0.02 and out["beta"] == 1.0
     np.testing.assert_allclose(m.ravel(order="F"), sweep, rtol=0, atol=0)
+
+
+def test_precision_variant_matches_plain_rmse():
+    """FP16-stored sinogram with compensated FP32 sums: the reconstruction
+    RMSE stays within 1e-4 relative of the plain FP32 form on noisy data."""
+    para, img, P = _small_problem(n=64, nv=24, na=96)
+    Pn = phantoms.poisson_log(P, 1e5, seed=7)
+    para.update(maxnumIter=5, alpha=0.2, numOfgradients=10, I0=img)
+    m0, o0 = ops.ASD_POCS_cuda(Pn, para)
+    m1, o1 = ops.ASD_POCS_cuda(Pn, dict(para, precision=1))
+    assert np.all(np.abs(o1["RMSE"] - o0["RMSE"]) <= 1e-4 * o0["RMSE"]), (o0["RMSE"], o1["RMSE"])
+    assert rel_l2(m1, m0) <= 1e-3
+
+
+def test_batch_matches_single_plans():
+    """Cases of different sizes and view counts advanced together give the
+    per-case results of separate plans (to FP64 reduction-order rounding)."""
+    rng = np.random.default_rng(11)
+    paras, Ps, imgs = [], [], []
+    for n, nv in [(32, 10), (48, 7), (40, 13), (32, 4)]:
+        para = dict(nx=n, ny=n, nz=1, SO=500.0, OD=400.0, vxlsize=1.0, detsize=1.3, na=int(1.5 * n),
+                    nb=1, alpha=0.2, numOfgradients=int(rng.integers(3, 8)), maxnumIter=3)
+        para["phi_vec"] = np.sort(rng.uniform(0, 360, nv))
+        img = phantoms.shepp_logan_2d(n)
+        P = np.zeros((para["na"], nv))
+        for i, th in enumerate(para["phi_vec"]):
+            P[:, i] = ops.Ax_cone_gpu(img.astype(np.float32), dict(para, phi=-th / 180 * np.pi))[0]
+        paras.append(para)
+        Ps.append(P)
+        imgs.append(img)
+    B = ops.Batch(paras)
+    for c in range(len(paras)):
+        B.set_case(c, Ps[c], None, imgs[c])
+    B.iterate(3)
+    for c, para in enumerate(paras):
+        m, out = ops.ASD_POCS_cuda(Ps[c], dict(para, I0=imgs[c]))
+        ob, rb = B.out(c)
+        assert ob["IterationNum"] == 3
+        assert rel_l2(B.image(c), m.ravel(order="F")) <= 1e-5, c
+        np.testing.assert_allclose(rb, out["RMSE"], rtol=1e-5)
+        assert ob["beta"] == pytest.approx(out["beta"], rel=1e-12)
+    B.close()
