@@ -22,6 +22,7 @@
 
 #include "projector_kernels.hpp"
 #include "reduce_device.hpp"
+#include "sino_codec.hpp"
 
 namespace asd {
 namespace {
@@ -44,7 +45,8 @@ struct BRules {
 struct BItem {
     ViewGeom g;
     float *m;
-    const void *P;         // this view of the case's sinogram (float or __half)
+    const void *P;         // this view of the case's sinogram (see sino_codec.hpp)
+    double lo, step;       // 16-bit fixed-point scale of this view
     float *cor;
     double *res2;
     RayRec *rt;
@@ -73,6 +75,20 @@ __device__ __forceinline__ int find_seg(const int *pref, int n, int x)
     return lo;
 }
 
+template <class Codec> __device__ __forceinline__ Codec item_codec(const BItem &it);
+template <> __device__ __forceinline__ SinoF32 item_codec<SinoF32>(const BItem &it)
+{
+    return SinoF32{(const float *)it.P};
+}
+template <> __device__ __forceinline__ SinoU16 item_codec<SinoU16>(const BItem &it)
+{
+    return SinoU16{(const uint16_t *)it.P, it.lo, it.step};
+}
+template <> __device__ __forceinline__ SinoF16 item_codec<SinoF16>(const BItem &it)
+{
+    return SinoF16{(const __half *)it.P};
+}
+
 template <int LPR, bool KAHAN, class PT>
 __global__ __launch_bounds__(256) void ax_batch_kernel(const BItem *__restrict__ items,
                                                        const int *__restrict__ rpref, int ncase)
@@ -92,7 +108,8 @@ __global__ __launch_bounds__(256) void ax_batch_kernel(const BItem *__restrict__
         int ia = (int)(id % g.na), ib = (int)(id / g.na);
         AxRay r = ax_ray_setup(g, it.ydet[ia], it.zdet[ib]);
         int nu = r.xdriven ? g.nx : g.ny;
-        for (int iu = lane; iu < nu; iu += LPR) ax_slab<KAHAN>(r, it.m, iu, g.nx, g.ny, sa, sl);
+        for (int iu0 = lane; iu0 < nu; iu0 += 4 * LPR)
+            ax_slabs<4, KAHAN>(r, it.m, iu0, LPR, nu, g.nx, g.ny, sa, sl);
     }
     float acc, accl;
     if constexpr (LPR > 1 && KAHAN) {
@@ -121,7 +138,7 @@ __global__ __launch_bounds__(256) void ax_batch_kernel(const BItem *__restrict__
         const ViewGeom &g = it.g;
         float proj = acc * g.scale, sumr = accl * g.scale;
         double r = (sumr == 0.0f) ? 1.0 : (double)sumr;
-        double c0 = (double)(float)((const PT *)it.P)[id] - (double)proj;
+        double c0 = item_codec<PT>(it).at(id) - (double)proj;
         it.res2[id] += c0 * c0;
         it.cor[id] = (float)(c0 / r);
         int ia = (int)(id % g.na), ib = (int)(id / g.na);
@@ -166,41 +183,24 @@ __global__ __launch_bounds__(256) void atx_batch_kernel(const BItem *__restrict_
     it.m[v] = nm < 0 ? 0.0f : (float)nm;
 }
 
-// Per-case sums over the blocks [bpref[c], bpref[c+1]): partials are indexed
-// by global block; the last block of the case folds them.
+// Per-case folds: block c of a <<<ncase>>> launch sums rows of the partials
+// written by the blocks [bpref[c], bpref[c+1]) of the preceding stage.
 template <int K>
-__device__ __forceinline__ bool case_sums(const double (&val)[K], double *partials, int nblocks_all,
-                                          int first, int nblk, unsigned *counter, double (&tot)[K])
+__device__ __forceinline__ void fold_case(const double *partials, int nblocks_all, int first,
+                                          int nblk, double (&tot)[K])
 {
-    __shared__ bool last;
     __shared__ double res[K];
-    double s[K];
-#pragma unroll
-    for (int k = 0; k < K; k++) s[k] = block_sum(val[k]);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < K; k++) partials[(size_t)k * nblocks_all + blockIdx.x] = s[k];
-        unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL,
-                                               __HIP_MEMORY_SCOPE_AGENT);
-        last = (prev == (unsigned)nblk - 1);
-    }
-    __syncthreads();
-    if (!last) return false;
-    __atomic_thread_fence(__ATOMIC_ACQUIRE);
 #pragma unroll
     for (int k = 0; k < K; k++) {
         double t = 0.0;
         for (int i = threadIdx.x; i < nblk; i += blockDim.x)
-            t += __hip_atomic_load(&partials[(size_t)k * nblocks_all + first + i],
-                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            t += partials[(size_t)k * nblocks_all + first + i];
         t = block_sum(t);
         if (threadIdx.x == 0) res[k] = t;
     }
-    if (threadIdx.x == 0) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < K; k++) tot[k] = res[k];
-    return true;
 }
 
 template <class T>
@@ -214,9 +214,8 @@ __device__ __forceinline__ float ldz(const T *u, int nx, int ny, int nz, int x, 
 __global__ __launch_bounds__(256) void tv_grad_batch(const BCase *__restrict__ cs,
                                                      const int *__restrict__ bpref, int ncase,
                                                      const float *__restrict__ u_all,
-                                                     float *__restrict__ v_all, BCtl *ctl,
-                                                     double *partials, unsigned *counters,
-                                                     int it)
+                                                     float *__restrict__ v_all, const BCtl *ctl,
+                                                     double *partials, int it)
 {
 #pragma clang fp contract(off)
     const int b = blockIdx.x;
@@ -264,9 +263,18 @@ __global__ __launch_bounds__(256) void tv_grad_batch(const BCase *__restrict__ c
         v[i] = g;
         s = (double)g * (double)g;
     }
-    double val[1] = {s}, tot[1];
-    const int nblk = bpref[c + 1] - bpref[c];
-    if (!case_sums<1>(val, partials, bpref[ncase], bpref[c], nblk, counters + c, tot)) return;
+    double val[1] = {s};
+    store_partials<1>(val, partials);
+}
+
+__global__ __launch_bounds__(256) void tv_norm_batch(const BCase *__restrict__ cs,
+                                                     const int *__restrict__ bpref, int ncase,
+                                                     BCtl *ctl, const double *partials, int it)
+{
+    const int c = blockIdx.x;
+    if (ctl[c].stopped || it >= cs[c].rules.ng) return;
+    double tot[1];
+    fold_case<1>(partials, bpref[ncase], bpref[c], bpref[c + 1] - bpref[c], tot);
     if (threadIdx.x == 0) ctl[c].nrm2 = tot[0];
 }
 
@@ -298,8 +306,8 @@ struct BBufs {
 
 __global__ __launch_bounds__(256) void post_sart_batch(const BCase *__restrict__ cs,
                                                        const int *__restrict__ bpref, int ncase,
-                                                       BBufs bb, BCtl *ctl, double *partials,
-                                                       unsigned *counters, int sweep_only)
+                                                       BBufs bb, const BCtl *ctl,
+                                                       double *partials)
 {
     const int b = blockIdx.x;
     const int c = find_seg(bpref, ncase, b);
@@ -325,10 +333,22 @@ __global__ __launch_bounds__(256) void post_sart_batch(const BCase *__restrict__
         v[3] = bb.res2[C.roff + i];
         bb.res2[C.roff + i] = 0.0;
     }
+    store_partials<4>(v, partials);
+}
+
+// One block per case: fold, stop rule, RMSE (ASD_POCS_cuda.m:136-149).
+__global__ __launch_bounds__(256) void ctl_sart_batch(const BCase *__restrict__ cs,
+                                                      const int *__restrict__ bpref, int ncase,
+                                                      BBufs bb, BCtl *ctl, const double *partials,
+                                                      int sweep_only)
+{
+    const int c = blockIdx.x;
+    const BCase &C = cs[c];
+    if (ctl[c].stopped) return;
     double tot[4];
-    const int nblk = bpref[c + 1] - bpref[c];
-    if (!case_sums<4>(v, partials, bpref[ncase], bpref[c], nblk, counters + c, tot)) return;
+    fold_case<4>(partials, bpref[ncase], bpref[c], bpref[c + 1] - bpref[c], tot);
     if (threadIdx.x != 0) return;
+    const size_t n = (size_t)C.nx * C.ny * C.nz;
     BCtl &K = ctl[c];
     K.dd2 = tot[3];
     K.d2 = tot[1];
@@ -342,8 +362,7 @@ __global__ __launch_bounds__(256) void post_sart_batch(const BCase *__restrict__
 
 __global__ __launch_bounds__(256) void post_tv_batch(const BCase *__restrict__ cs,
                                                      const int *__restrict__ bpref, int ncase,
-                                                     BBufs bb, BCtl *ctl, double *partials,
-                                                     unsigned *counters)
+                                                     BBufs bb, const BCtl *ctl, double *partials)
 {
     const int b = blockIdx.x;
     const int c = find_seg(bpref, ncase, b);
@@ -359,9 +378,19 @@ __global__ __launch_bounds__(256) void post_tv_batch(const BCase *__restrict__ c
         v[0] = e * e;
         bb.mpre[k] = mi;
     }
+    store_partials<1>(v, partials);
+}
+
+// One block per case: dg and the alpha / beta rules (ASD_POCS_cuda.m:166-170).
+__global__ __launch_bounds__(256) void ctl_tv_batch(const BCase *__restrict__ cs,
+                                                    const int *__restrict__ bpref, int ncase,
+                                                    BCtl *ctl, const double *partials)
+{
+    const int c = blockIdx.x;
+    const BCase &C = cs[c];
+    if (ctl[c].stopped) return;
     double tot[1];
-    const int nblk = bpref[c + 1] - bpref[c];
-    if (!case_sums<1>(v, partials, bpref[ncase], bpref[c], nblk, counters + c, tot)) return;
+    fold_case<1>(partials, bpref[ncase], bpref[c], bpref[c + 1] - bpref[c], tot);
     if (threadIdx.x != 0) return;
     BCtl &K = ctl[c];
     K.dg = sqrt(tot[0]);
@@ -389,13 +418,6 @@ __global__ __launch_bounds__(256) void b_cvt_f32(const double *a, float *b, size
         b[i] = (float)a[i];
 }
 
-__global__ __launch_bounds__(256) void b_cvt_f16(const double *a, __half *b, size_t n)
-{
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (size_t)gridDim.x * blockDim.x)
-        b[i] = __float2half_rn((float)a[i]);
-}
-
 unsigned gridn(size_t n)
 {
     size_t b = (n + 255) / 256;
@@ -420,7 +442,9 @@ struct asd_batch {
     std::vector<int> active;   // active cases per view step (prefix of `order`)
     hipStream_t stream = nullptr;
     DeviceBuf m, mpre, mk1, f0, I0, v, cor, res2, rt, P, ydet, zdet, ctlb, hist;
-    DeviceBuf items, rpref, bpref, cases, cbpref, partials, counters;
+    DeviceBuf items, rpref, bpref, cases, cbpref, partials;
+    std::vector<BItem> h_items;  // host copy of the per-step work lists
+    std::vector<int> pos;        // position of each case in `order`
     std::vector<int> h_rpref_tot, h_bpref_tot;  // per step: total rays / blocks
     int case_blocks = 0;
     std::vector<char> have_P, have_I0;
@@ -446,19 +470,19 @@ int b_enqueue_iteration(asd_batch *B, int sweep_only, int iterate)
         const unsigned ab = (unsigned)(((long)rays * B->lpr + 255) / 256);
         if (B->precision == 1) {
             switch (B->lpr) {
-            case 64: ax_batch_kernel<64, true, __half><<<ab, 256, 0, B->stream>>>(items, rp, A); break;
-            case 16: ax_batch_kernel<16, true, __half><<<ab, 256, 0, B->stream>>>(items, rp, A); break;
-            case 4: ax_batch_kernel<4, true, __half><<<ab, 256, 0, B->stream>>>(items, rp, A); break;
-            default: ax_batch_kernel<1, true, __half><<<ab, 256, 0, B->stream>>>(items, rp, A); break;
+            case 64: ax_batch_kernel<64, true, SinoU16><<<ab, 256, 0, B->stream>>>(items, rp, A); break;
+            case 16: ax_batch_kernel<16, true, SinoU16><<<ab, 256, 0, B->stream>>>(items, rp, A); break;
+            case 4: ax_batch_kernel<4, true, SinoU16><<<ab, 256, 0, B->stream>>>(items, rp, A); break;
+            default: ax_batch_kernel<1, true, SinoU16><<<ab, 256, 0, B->stream>>>(items, rp, A); break;
             }
             ASD_LAUNCH_CHECK("ax_batch_kernel");
             atx_batch_kernel<true><<<blocks, 256, 0, B->stream>>>(items, bp, A);
         } else {
             switch (B->lpr) {
-            case 64: ax_batch_kernel<64, false, float><<<ab, 256, 0, B->stream>>>(items, rp, A); break;
-            case 16: ax_batch_kernel<16, false, float><<<ab, 256, 0, B->stream>>>(items, rp, A); break;
-            case 4: ax_batch_kernel<4, false, float><<<ab, 256, 0, B->stream>>>(items, rp, A); break;
-            default: ax_batch_kernel<1, false, float><<<ab, 256, 0, B->stream>>>(items, rp, A); break;
+            case 64: ax_batch_kernel<64, false, SinoF32><<<ab, 256, 0, B->stream>>>(items, rp, A); break;
+            case 16: ax_batch_kernel<16, false, SinoF32><<<ab, 256, 0, B->stream>>>(items, rp, A); break;
+            case 4: ax_batch_kernel<4, false, SinoF32><<<ab, 256, 0, B->stream>>>(items, rp, A); break;
+            default: ax_batch_kernel<1, false, SinoF32><<<ab, 256, 0, B->stream>>>(items, rp, A); break;
             }
             ASD_LAUNCH_CHECK("ax_batch_kernel");
             atx_batch_kernel<false><<<blocks, 256, 0, B->stream>>>(items, bp, A);
@@ -470,25 +494,26 @@ int b_enqueue_iteration(asd_batch *B, int sweep_only, int iterate)
     const BCase *cs = B->cases.as<BCase>();
     const int *cbp = B->cbpref.as<int>();
     BCtl *ctl = B->ctlb.as<BCtl>();
-    post_sart_batch<<<B->case_blocks, 256, 0, B->stream>>>(cs, cbp, K, bb, ctl,
-                                                            B->partials.as<double>(),
-                                                            B->counters.as<unsigned>(),
-                                                            sweep_only);
+    double *part = B->partials.as<double>();
+    post_sart_batch<<<B->case_blocks, 256, 0, B->stream>>>(cs, cbp, K, bb, ctl, part);
     ASD_LAUNCH_CHECK("post_sart_batch");
+    ctl_sart_batch<<<K, 256, 0, B->stream>>>(cs, cbp, K, bb, ctl, part, sweep_only);
+    ASD_LAUNCH_CHECK("ctl_sart_batch");
     if (!iterate) return ASD_OK;
     for (int it = 0; it < B->max_ng; it++) {
-        tv_grad_batch<<<B->case_blocks, 256, 0, B->stream>>>(
-            cs, cbp, K, B->m.as<float>(), B->v.as<float>(), ctl, B->partials.as<double>(),
-            B->counters.as<unsigned>() + K, it);
+        tv_grad_batch<<<B->case_blocks, 256, 0, B->stream>>>(cs, cbp, K, B->m.as<float>(),
+                                                             B->v.as<float>(), ctl, part, it);
         ASD_LAUNCH_CHECK("tv_grad_batch");
+        tv_norm_batch<<<K, 256, 0, B->stream>>>(cs, cbp, K, ctl, part, it);
+        ASD_LAUNCH_CHECK("tv_norm_batch");
         tv_step_batch<<<B->case_blocks, 256, 0, B->stream>>>(cs, cbp, K, B->m.as<float>(),
                                                              B->v.as<float>(), ctl, it);
         ASD_LAUNCH_CHECK("tv_step_batch");
     }
-    post_tv_batch<<<B->case_blocks, 256, 0, B->stream>>>(cs, cbp, K, bb, ctl,
-                                                          B->partials.as<double>(),
-                                                          B->counters.as<unsigned>() + 2 * K);
+    post_tv_batch<<<B->case_blocks, 256, 0, B->stream>>>(cs, cbp, K, bb, ctl, part);
     ASD_LAUNCH_CHECK("post_tv_batch");
+    ctl_tv_batch<<<K, 256, 0, B->stream>>>(cs, cbp, K, ctl, part);
+    ASD_LAUNCH_CHECK("ctl_tv_batch");
     return ASD_OK;
 }
 
@@ -535,6 +560,7 @@ int asd_batch_create(int ncases, const asd_para *paras, const double *const *phi
         ASD_ARG(nvs[c] > 0 && phi_vecs[c], "every case needs views");
         ASD_ARG(p.SO > 0 && p.OD > 0 && p.vxlsize > 0 && p.detsize > 0, "bad case geometry");
         ASD_ARG(p.precision == paras[0].precision, "all cases must share para.precision");
+        ASD_ARG(p.precision == 0 || p.precision == 1, "batched para.precision must be 0 or 1");
         ASD_ARG(p.numOfgradients >= 0, "para.numOfgradients must be >= 0");
     }
     asd_batch *B = new asd_batch();
@@ -589,7 +615,7 @@ int asd_batch_create(int ncases, const asd_para *paras, const double *const *phi
         (rc = B->f0.ensure(N * 4)) || (rc = B->I0.ensure(N * 4)) || (rc = B->v.ensure(N * 4)) ||
         (rc = B->cor.ensure(M * 4)) || (rc = B->res2.ensure(M * 8)) ||
         (rc = B->rt.ensure(M * sizeof(RayRec))) ||
-        (rc = B->P.ensure(B->Ptot * (B->precision == 1 ? 2 : 4))) ||
+        (rc = B->P.ensure(B->Ptot * sino_bytes(B->precision))) ||
         (rc = B->ydet.ensure(ytot * 4)) || (rc = B->zdet.ensure(ztot * 4)) ||
         (rc = B->ctlb.ensure(sizeof(BCtl) * K)) || (rc = B->hist.ensure((size_t)K * B_HIST * 8)))
         return fail(rc);
@@ -626,7 +652,7 @@ int asd_batch_create(int ncases, const asd_para *paras, const double *const *phi
     B->h_rpref_tot.assign(B->max_nv, 0);
     B->h_bpref_tot.assign(B->max_nv, 0);
     BCtl *dctl = B->ctlb.as<BCtl>();
-    const size_t Pe = B->precision == 1 ? 2 : 4;
+    const size_t Pe = sino_bytes(B->precision);
     for (int j = 0; j < B->max_nv; j++) {
         int A = 0, rays = 0, blocks = 0;
         for (int s = 0; s < K; s++) {
@@ -643,6 +669,8 @@ int asd_batch_create(int ncases, const asd_para *paras, const double *const *phi
             it.ydet = B->ydet.as<float>() + yoff[c] + (size_t)j * p.na;
             it.zdet = B->zdet.as<float>() + zoff[c] + (size_t)j * p.nb;
             it.ctl = dctl + c;
+            it.lo = 0.0;
+            it.step = 0.0;
             rp[(size_t)j * (K + 1) + A] = rays;
             bp[(size_t)j * (K + 1) + A] = blocks;
             rays += p.na * p.nb;
@@ -670,11 +698,13 @@ int asd_batch_create(int ncases, const asd_para *paras, const double *const *phi
         cbp[c + 1] = cbp[c] + (int)((need + 255) / 256);
     }
     B->case_blocks = cbp[K];
+    B->h_items = items;
+    B->pos.assign(K, 0);
+    for (int s2 = 0; s2 < K; s2++) B->pos[B->order[s2]] = s2;
     if ((rc = B->items.ensure(items.size() * sizeof(BItem))) ||
         (rc = B->rpref.ensure(rp.size() * 4)) || (rc = B->bpref.ensure(bp.size() * 4)) ||
         (rc = B->cases.ensure(cs.size() * sizeof(BCase))) || (rc = B->cbpref.ensure(cbp.size() * 4)) ||
-        (rc = B->partials.ensure((size_t)4 * B->case_blocks * 8)) ||
-        (rc = B->counters.ensure((size_t)3 * K * 4)))
+        (rc = B->partials.ensure((size_t)4 * B->case_blocks * 8)))
         return fail(rc);
     if (hipMemcpy(B->ydet.p, yall.data(), ytot * 4, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(B->zdet.p, zall.data(), ztot * 4, hipMemcpyHostToDevice) != hipSuccess ||
@@ -683,7 +713,6 @@ int asd_batch_create(int ncases, const asd_para *paras, const double *const *phi
         hipMemcpy(B->bpref.p, bp.data(), bp.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(B->cases.p, cs.data(), cs.size() * sizeof(BCase), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(B->cbpref.p, cbp.data(), cbp.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemset(B->counters.p, 0, B->counters.bytes) != hipSuccess ||
         hipMemset(B->m.p, 0, N * 4) != hipSuccess || hipMemset(B->res2.p, 0, M * 8) != hipSuccess ||
         hipMemset(B->I0.p, 0, N * 4) != hipSuccess || hipMemset(B->P.p, 0, B->P.bytes) != hipSuccess) {
         set_error(ASD_ERR_HIP, "batch initialisation failed");
@@ -721,13 +750,39 @@ int asd_batch_set_case(asd_batch *B, int c, const double *P, const double *m0, c
     int rc;
     DeviceBuf tmp;
     if ((rc = tmp.ensure(std::max(np, n) * 8))) return rc;
-    ASD_HIP(hipMemcpyAsync(tmp.p, P, np * 8, hipMemcpyHostToDevice, B->stream));
-    if (B->precision == 1)
-        b_cvt_f16<<<gridn(np), 256, 0, B->stream>>>(tmp.as<double>(), B->P.as<__half>() + B->poff[c], np);
-    else
+    if (B->precision == 1) {
+        // 16-bit fixed point per view (see sino_codec.hpp); the view scales go
+        // into this case's work-list entries
+        const size_t Mc = (size_t)p.na * p.nb;
+        std::vector<uint16_t> q(np);
+        for (int j = 0; j < B->nv[c]; j++) {
+            const double *v = P + (size_t)j * Mc;
+            double lo = v[0], hi = v[0];
+            for (size_t i = 1; i < Mc; i++) {
+                lo = v[i] < lo ? v[i] : lo;
+                hi = v[i] > hi ? v[i] : hi;
+            }
+            const double step = hi > lo ? (hi - lo) / 65535.0 : 0.0;
+            for (size_t i = 0; i < Mc; i++) {
+                double t = step > 0 ? (v[i] - lo) / step : 0.0;
+                long r = std::lround(t);
+                q[(size_t)j * Mc + i] = (uint16_t)(r < 0 ? 0 : r > 65535 ? 65535 : r);
+            }
+            BItem &it = B->h_items[(size_t)j * B->K + B->pos[c]];
+            it.lo = lo;
+            it.step = step;
+            ASD_HIP(hipMemcpyAsync(B->items.as<BItem>() + (size_t)j * B->K + B->pos[c], &it,
+                                   sizeof(BItem), hipMemcpyHostToDevice, B->stream));
+        }
+        ASD_HIP(hipMemcpyAsync(B->P.as<uint16_t>() + B->poff[c], q.data(), np * 2,
+                               hipMemcpyHostToDevice, B->stream));
+        ASD_HIP(hipStreamSynchronize(B->stream));
+    } else {
+        ASD_HIP(hipMemcpyAsync(tmp.p, P, np * 8, hipMemcpyHostToDevice, B->stream));
         b_cvt_f32<<<gridn(np), 256, 0, B->stream>>>(tmp.as<double>(), B->P.as<float>() + B->poff[c], np);
-    ASD_LAUNCH_CHECK("b_cvt");
-    ASD_HIP(hipStreamSynchronize(B->stream));
+        ASD_LAUNCH_CHECK("b_cvt");
+        ASD_HIP(hipStreamSynchronize(B->stream));
+    }
     float *mc = B->m.as<float>() + B->voff[c];
     if (m0) {
         ASD_HIP(hipMemcpyAsync(tmp.p, m0, n * 8, hipMemcpyHostToDevice, B->stream));

@@ -23,6 +23,7 @@
 
 #include "projector_kernels.hpp"
 #include "reduce_device.hpp"
+#include "sino_codec.hpp"
 #include "tv.hpp"
 
 namespace asd {
@@ -47,9 +48,9 @@ struct CtlRules {
     double alpha_red, beta_red, r_max, epsilon, stopRelChg;
 };
 
-template <class PT>
+template <class Codec>
 struct EpiSartAx {
-    const PT *P;          // this view's measured projection
+    Codec P;              // this view's measured projection
     float *cor;           // correction fed to the backprojector
     double *res2;         // per-ray accumulated squared residual of the sweep
     RayRec *rt;           // ray-slope table of this view (for atx_sart)
@@ -62,7 +63,7 @@ struct EpiSartAx {
 #pragma clang fp contract(off)
         float proj = acc * g.scale, sumr = accl * g.scale;
         double r = (sumr == 0.0f) ? 1.0 : (double)sumr;
-        double c0 = (double)(float)P[id] - (double)proj;
+        double c0 = P.at(id) - (double)proj;
         res2[id] += c0 * c0;
         cor[id] = (float)(c0 / r);
         int ia = (int)(id % g.na), ib = (int)(id / g.na);
@@ -86,14 +87,12 @@ struct EpiSartAtx {
     }
 };
 
-// After a sweep: dp^2, d^2, rmse^2 over the image (and f0 = m), dd2 over the
-// rays (and the residual buffer re-zeroed); the last block applies the stop
-// rule and records the RMSE (ASD_POCS_cuda.m:136-149).
+// After a sweep: dp^2, d^2, rmse^2 over the image (and f0 = m) and dd2 over the
+// rays (and the residual buffer re-zeroed), as block partials.
 __global__ __launch_bounds__(256) void post_sart_kernel(
     const float *__restrict__ m, const float *__restrict__ mpre, const float *__restrict__ mk1,
     const float *__restrict__ I0, float *__restrict__ f0, double *__restrict__ res2, size_t N,
-    size_t M, double *partials, unsigned *counter, double *scal, DevCtl *ctl, double *hist,
-    CtlRules rules, int sweep_only)
+    size_t M, double *partials, const DevCtl *ctl)
 {
     if (ctl->stopped) return;
     double v[4] = {0.0, 0.0, 0.0, 0.0};
@@ -114,8 +113,19 @@ __global__ __launch_bounds__(256) void post_sart_kernel(
         v[3] += res2[j];
         res2[j] = 0.0;
     }
+    store_partials<4>(v, partials);
+}
+
+// Folds post_sart's partials; applies the stop rule and records the RMSE
+// (ASD_POCS_cuda.m:136-149).  One block.
+__global__ __launch_bounds__(RED_THREADS) void ctl_after_sart(const double *partials, int nblocks,
+                                                              size_t N, int has_I0, double *scal,
+                                                              DevCtl *ctl, double *hist,
+                                                              CtlRules rules, int sweep_only)
+{
+    if (ctl->stopped) return;
     double tot[4];
-    if (!grid_sums<4>(v, partials, counter, tot)) return;
+    fold_rows<4>(partials, nblocks, tot);
     if (threadIdx.x != 0) return;
     scal[TV_SLOT_DP2] = tot[0];
     scal[TV_SLOT_D2] = tot[1];
@@ -125,19 +135,17 @@ __global__ __launch_bounds__(256) void post_sart_kernel(
     if (sweep_only) return;
     const int k = ctl->k + 1;
     ctl->k = k;
-    if (I0 && k - 1 < HIST_CAP) hist[k - 1] = sqrt(tot[2] / (double)N);
+    if (has_I0 && k - 1 < HIST_CAP) hist[k - 1] = sqrt(tot[2] / (double)N);
     const double dp = sqrt(tot[0]);
     ctl->dp = dp;
     if (dp / (double)N <= rules.stopRelChg) ctl->stopped = 1;
 }
 
-// After TV: dg^2 = ||m - f0||^2, m_pre = m for the next iteration, and the
-// alpha / beta rules (ASD_POCS_cuda.m:166-170).
+// After TV: dg^2 = ||m - f0||^2 partials and m_pre = m for the next iteration.
 __global__ __launch_bounds__(256) void post_tv_kernel(const float *__restrict__ m,
                                                       const float *__restrict__ f0,
                                                       float *__restrict__ mpre, size_t N,
-                                                      double *partials, unsigned *counter,
-                                                      double *scal, DevCtl *ctl, CtlRules rules)
+                                                      double *partials, const DevCtl *ctl)
 {
     if (ctl->stopped) return;
     double v[1] = {0.0};
@@ -148,8 +156,17 @@ __global__ __launch_bounds__(256) void post_tv_kernel(const float *__restrict__ 
         v[0] += e * e;
         mpre[i] = mi;
     }
+    store_partials<1>(v, partials);
+}
+
+// Folds dg^2 and applies the alpha and beta rules (ASD_POCS_cuda.m:166-170).
+__global__ __launch_bounds__(RED_THREADS) void ctl_after_tv(const double *partials, int nblocks,
+                                                            double *scal, DevCtl *ctl,
+                                                            CtlRules rules)
+{
+    if (ctl->stopped) return;
     double tot[1];
-    if (!grid_sums<1>(v, partials, counter, tot)) return;
+    fold_rows<1>(partials, nblocks, tot);
     if (threadIdx.x != 0) return;
     scal[TV_SLOT_DG2] = tot[0];
     const double dg = sqrt(tot[0]);
@@ -167,13 +184,6 @@ __global__ __launch_bounds__(256) void f64_to_f32(const double *__restrict__ a, 
         b[i] = (float)a[i];
 }
 
-__global__ __launch_bounds__(256) void f64_to_f16(const double *__restrict__ a, __half *b,
-                                                  size_t n)
-{
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (size_t)gridDim.x * blockDim.x)
-        b[i] = __float2half_rn((float)a[i]);
-}
 
 __global__ __launch_bounds__(256) void copy3_f32(const float *__restrict__ a, float *b, float *c,
                                                  float *d, size_t n)
@@ -206,6 +216,7 @@ struct asd_plan {
     std::vector<ViewGeom> views;
     hipStream_t stream = nullptr;
     DeviceBuf ydet, zdet, P, m, mpre, mk1, f0, I0, cor, res2, rt, v, ctlbuf, hist;
+    std::vector<double> u16_lo, u16_step;  // per-view scale of the 16-bit fixed-point sinogram
     TvScratch red;
     bool have_P = false, have_I0 = false;
     int lpr = 1;
@@ -266,7 +277,7 @@ int check_para(const asd_para *p, int nv)
     ASD_ARG(p->vxlsize > 0 && p->detsize > 0, "para.vxlsize and para.detsize must be positive");
     ASD_ARG(p->SO > 0 && p->OD > 0, "para.SO and para.OD must be positive");
     ASD_ARG(p->numOfgradients >= 0, "para.numOfgradients must be >= 0");
-    ASD_ARG(p->precision == 0 || p->precision == 1, "para.precision must be 0 or 1");
+    ASD_ARG(p->precision >= 0 && p->precision <= 2, "para.precision must be 0, 1 or 2");
     return ASD_OK;
 }
 
@@ -304,22 +315,37 @@ void collect_timing(asd_plan *p)
     p->ev.clear();
 }
 
-// One SART sweep over all views with relaxation *relax.  PT is the sinogram
-// storage type (float, or __half in the precision variant, which also
-// compensates every FP32 sum).
-template <class PT, bool KAHAN>
+// Codec of view iv of the plan's sinogram.
+template <class Codec>
+Codec view_codec(asd_plan *p, int iv);
+template <>
+SinoF32 view_codec<SinoF32>(asd_plan *p, int iv)
+{
+    return SinoF32{p->P.as<float>() + (size_t)iv * p->M};
+}
+template <>
+SinoU16 view_codec<SinoU16>(asd_plan *p, int iv)
+{
+    return SinoU16{p->P.as<uint16_t>() + (size_t)iv * p->M, p->u16_lo[iv], p->u16_step[iv]};
+}
+template <>
+SinoF16 view_codec<SinoF16>(asd_plan *p, int iv)
+{
+    return SinoF16{p->P.as<__half>() + (size_t)iv * p->M};
+}
+
+// One SART sweep over all views with relaxation *relax.  The precision
+// variant reads a 16-bit sinogram and compensates every FP32 sum.
+template <class Codec, bool KAHAN>
 int enqueue_sweep_t(asd_plan *p, const double *relax)
 {
-    const size_t M = p->M;
-    const long N = (long)p->N;
-    const unsigned atx_blocks = (unsigned)((N + 255) / 256);
     const int *stopped = &p->ctl()->stopped;
     for (int iv = 0; iv < p->nv; iv++) {
         const ViewGeom &g = p->views[iv];
         const float *yd = p->ydet.as<float>() + (size_t)iv * p->para.na;
         const float *zd = p->zdet.as<float>() + (size_t)iv * p->para.nb;
-        EpiSartAx<PT> ea{p->P.as<PT>() + (size_t)iv * M, p->cor.as<float>(),
-                         p->res2.as<double>(), p->rt.as<RayRec>(), yd, zd, stopped, g};
+        EpiSartAx<Codec> ea{view_codec<Codec>(p, iv), p->cor.as<float>(), p->res2.as<double>(),
+                            p->rt.as<RayRec>(), yd, zd, stopped, g};
         int rc;
         {
             StageTimer t(p, ST_AX);
@@ -328,10 +354,14 @@ int enqueue_sweep_t(asd_plan *p, const double *relax)
         }
         {
             StageTimer t(p, ST_ATX);
-            atx_kernel<EpiSartAtx, KAHAN><<<atx_blocks, 256, 0, p->stream>>>(
-                g, p->cor.as<float>(), p->rt.as<RayRec>(),
-                EpiSartAtx{p->m.as<float>(), relax, stopped, g.scale});
-            ASD_LAUNCH_CHECK("atx_kernel(sart)");
+            const EpiSartAtx eb{p->m.as<float>(), relax, stopped, g.scale};
+            if (KAHAN)
+                rc = dispatch_atx_prec<KAHAN>(g, p->cor.as<float>(), yd, zd, p->rt.as<RayRec>(),
+                                              eb, p->stream);
+            else
+                rc = dispatch_atx_tile<KAHAN>(g, p->cor.as<float>(), p->rt.as<RayRec>(), eb,
+                                              p->stream);
+            if (rc) return rc;
         }
     }
     return ASD_OK;
@@ -339,18 +369,23 @@ int enqueue_sweep_t(asd_plan *p, const double *relax)
 
 int enqueue_sweep(asd_plan *p, const double *relax)
 {
-    if (p->para.precision == 1) return enqueue_sweep_t<__half, true>(p, relax);
-    return enqueue_sweep_t<float, false>(p, relax);
+    if (p->para.precision == 1) return enqueue_sweep_t<SinoU16, true>(p, relax);
+    if (p->para.precision == 2) return enqueue_sweep_t<SinoF16, true>(p, relax);
+    return enqueue_sweep_t<SinoF32, false>(p, relax);
 }
 
 int enqueue_post_sart(asd_plan *p, int sweep_only)
 {
-    post_sart_kernel<<<red_blocks(p->N > p->M ? p->N : p->M), RED_THREADS, 0, p->stream>>>(
+    const unsigned nb = red_blocks(p->N > p->M ? p->N : p->M);
+    post_sart_kernel<<<nb, RED_THREADS, 0, p->stream>>>(
         p->m.as<float>(), p->mpre.as<float>(), p->mk1.as<float>(),
         p->have_I0 ? p->I0.as<float>() : nullptr, p->f0.as<float>(), p->res2.as<double>(), p->N,
-        p->M, p->red.partials, p->red.counters + TV_SLOT_DP2, p->red.scalars, p->ctl(),
-        p->hist.as<double>(), p->rules(), sweep_only);
+        p->M, p->red.partials, p->ctl());
     ASD_LAUNCH_CHECK("post_sart_kernel");
+    ctl_after_sart<<<1, RED_THREADS, 0, p->stream>>>(p->red.partials, (int)nb, p->N,
+                                                     p->have_I0 ? 1 : 0, p->red.scalars, p->ctl(),
+                                                     p->hist.as<double>(), p->rules(), sweep_only);
+    ASD_LAUNCH_CHECK("ctl_after_sart");
     return ASD_OK;
 }
 
@@ -371,10 +406,14 @@ int enqueue_iteration(asd_plan *p)
                                            p->red, p->stream, true, &ctl->stopped)))
             return rc;
     }
-    post_tv_kernel<<<red_blocks(p->N), RED_THREADS, 0, p->stream>>>(
-        p->m.as<float>(), p->f0.as<float>(), p->mpre.as<float>(), p->N, p->red.partials,
-        p->red.counters + TV_SLOT_DG2, p->red.scalars, ctl, p->rules());
+    const unsigned nb = red_blocks(p->N);
+    post_tv_kernel<<<nb, RED_THREADS, 0, p->stream>>>(p->m.as<float>(), p->f0.as<float>(),
+                                                      p->mpre.as<float>(), p->N, p->red.partials,
+                                                      ctl);
     ASD_LAUNCH_CHECK("post_tv_kernel");
+    ctl_after_tv<<<1, RED_THREADS, 0, p->stream>>>(p->red.partials, (int)nb, p->red.scalars, ctl,
+                                                   p->rules());
+    ASD_LAUNCH_CHECK("ctl_after_tv");
     return ASD_OK;
 }
 
@@ -520,7 +559,7 @@ int asd_plan_create(const asd_para *para, const double *phi_vec, int nv, asd_pla
         (rc = p->mk1.ensure(N * 4)) || (rc = p->f0.ensure(N * 4)) || (rc = p->v.ensure(N * 4)) ||
         (rc = p->cor.ensure(M * 4)) || (rc = p->res2.ensure(M * 8)) ||
         (rc = p->rt.ensure(M * sizeof(RayRec))) ||
-        (rc = p->P.ensure((size_t)nv * M * (para->precision == 1 ? 2 : 4))) ||
+        (rc = p->P.ensure((size_t)nv * M * sino_bytes(para->precision))) ||
         (rc = p->ctlbuf.ensure(sizeof(DevCtl))) || (rc = p->hist.ensure(HIST_CAP * 8)) ||
         (rc = p->red.init()))
         return fail(rc);
@@ -544,15 +583,35 @@ int asd_plan_set_projections(asd_plan *p, const double *P)
     clear_error();
     ASD_ARG(p && P, "NULL argument");
     int rc;
+    const size_t n = (size_t)p->nv * p->M;
     if (p->para.precision == 1) {
-        size_t n = (size_t)p->nv * p->M;
-        DeviceBuf tmp;
-        if ((rc = tmp.ensure(n * sizeof(double)))) return rc;
-        ASD_HIP(hipMemcpyAsync(tmp.p, P, n * sizeof(double), hipMemcpyHostToDevice, p->stream));
-        f64_to_f16<<<grid_for(n), 256, 0, p->stream>>>(tmp.as<double>(), p->P.as<__half>(), n);
-        ASD_LAUNCH_CHECK("f64_to_f16");
-        ASD_HIP(hipStreamSynchronize(p->stream));
-    } else if ((rc = upload_f64_as_f32(p, p->P, P, (size_t)p->nv * p->M))) {
+        // 16-bit fixed point per view: lo + step * q, q = round((P - lo) / step)
+        std::vector<uint16_t> q(n);
+        p->u16_lo.assign(p->nv, 0.0);
+        p->u16_step.assign(p->nv, 0.0);
+        for (int iv = 0; iv < p->nv; iv++) {
+            const double *v = P + (size_t)iv * p->M;
+            double lo = v[0], hi = v[0];
+            for (size_t i = 1; i < p->M; i++) {
+                lo = v[i] < lo ? v[i] : lo;
+                hi = v[i] > hi ? v[i] : hi;
+            }
+            const double step = hi > lo ? (hi - lo) / 65535.0 : 0.0;
+            p->u16_lo[iv] = lo;
+            p->u16_step[iv] = step;
+            for (size_t i = 0; i < p->M; i++) {
+                double t = step > 0 ? (v[i] - lo) / step : 0.0;
+                long r = std::lround(t);
+                q[(size_t)iv * p->M + i] = (uint16_t)(r < 0 ? 0 : r > 65535 ? 65535 : r);
+            }
+        }
+        ASD_HIP(hipMemcpy(p->P.p, q.data(), n * 2, hipMemcpyHostToDevice));
+        drop_graph(p);  // the view scales are kernel arguments
+    } else if (p->para.precision == 2) {
+        std::vector<__half> h(n);
+        for (size_t i = 0; i < n; i++) h[i] = __float2half_rn((float)P[i]);
+        ASD_HIP(hipMemcpy(p->P.p, h.data(), n * 2, hipMemcpyHostToDevice));
+    } else if ((rc = upload_f64_as_f32(p, p->P, P, n))) {
         return rc;
     }
     p->have_P = true;

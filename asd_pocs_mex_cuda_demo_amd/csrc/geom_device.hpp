@@ -9,7 +9,7 @@
 
 namespace asd {
 
-__device__ __forceinline__ float absf(float a) { return a > 0 ? a : -a; }
+__host__ __device__ __forceinline__ float absf(float a) { return a > 0 ? a : -a; }
 
 // ---------------------------------------------------------------------------
 // Forward projector: one unit slab of a ray (src/AxAtx/Ax_cone_gpu.cu:376-571).
@@ -26,7 +26,7 @@ struct AxRay {
     int xdriven;
 };
 
-__device__ __forceinline__ AxRay ax_ray_setup(const ViewGeom &g, float ydet, float zdet)
+__host__ __device__ __forceinline__ AxRay ax_ray_setup(const ViewGeom &g, float ydet, float zdet)
 {
 #pragma clang fp contract(off)
     AxRay r;
@@ -59,7 +59,7 @@ __device__ __forceinline__ AxRay ax_ray_setup(const ViewGeom &g, float ydet, flo
 template <bool KAHAN>
 struct Acc {
     float s = 0.0f, c = 0.0f;
-    __device__ __forceinline__ void add(float x)
+    __host__ __device__ __forceinline__ void add(float x)
     {
 #pragma clang fp contract(off)
         if constexpr (KAHAN) {
@@ -71,15 +71,25 @@ struct Acc {
             s += x;
         }
     }
-    __device__ __forceinline__ float value() const { return KAHAN ? s - c : s; }
+    __host__ __device__ __forceinline__ float value() const { return KAHAN ? s - c : s; }
 };
 
-// Accumulate slab iu of ray r into (acc, accl); X is the volume (x fastest).
-template <bool KAHAN = false>
-__device__ __forceinline__ void ax_slab(const AxRay &r, const float *__restrict__ X, int iu,
-                                        int nx, int ny, Acc<KAHAN> &acc, Acc<KAHAN> &accl)
+// The (up to) three pieces of slab iu of ray r: voxel index, weight w*l and
+// whether the piece lies in the volume.  Out-of-volume pieces get index 0 (a
+// valid address) so the caller can issue every load before it accumulates.
+struct SlabW {
+    size_t idx[3];
+    float wl[3];
+    bool ok[3];
+};
+
+__host__ __device__ __forceinline__ void ax_slab_geom(const AxRay &r, int iu, int nx, int ny,
+                                                      SlabW &s)
 {
 #pragma clang fp contract(off)
+    s.ok[0] = s.ok[1] = s.ok[2] = false;
+    s.idx[0] = s.idx[1] = s.idx[2] = 0;
+    s.wl[0] = s.wl[1] = s.wl[2] = 0.0f;
     float ua = (float)(iu - r.nu2), ub = ua + 1;
     float ta, tb, za, zb;
     if (r.s1 >= 0) {
@@ -101,38 +111,79 @@ __device__ __forceinline__ void ax_slab(const AxRay &r, const float *__restrict_
     bool in_t1 = ct1 >= 0 && ct1 < r.nt, in_t2 = ct2 >= 0 && ct2 < r.nt;
     bool in_z1 = cz1 >= 0 && cz1 < r.nz, in_z2 = cz2 >= 0 && cz2 < r.nz;
     if (!((in_t1 || in_t2) && (in_z1 || in_z2))) return;
-    float rt = (ct1 == ct2) ? 1.0f : ((float)ct2 - ta) / (tb - ta);
-    float rz = (cz1 == cz2) ? 1.0f : ((float)cz2 - za) / (zb - za);
+    float rt = 1.0f, rz = 1.0f;
+    if (ct1 != ct2) rt = ((float)ct2 - ta) / (tb - ta);
+    if (cz1 != cz2) rz = ((float)cz2 - za) / (zb - za);
     float d = tb - ta;
     float l = sqrtf((d * d + 1) * r.q / r.tmp);
     bool tfirst = !(rt > rz);  // the t crossing comes first (ties: t)
     float lo = tfirst ? rt : rz, hi = tfirst ? rz : rt;
-    // piece 1: (ct1, cz1)
-    float w0 = lo, w1 = hi - lo, w2 = 1.0f - hi;
     int ctm = tfirst ? ct2 : ct1, czm = tfirst ? cz1 : cz2;
-    bool inm = (tfirst ? in_t2 : in_t1) && (tfirst ? in_z1 : in_z2);
     const size_t sxy = (size_t)nx * ny;
-    if (in_t1 && in_z1) {
-        size_t idx = r.xdriven ? (size_t)cz1 * sxy + (size_t)ct1 * nx + iu
-                               : (size_t)cz1 * sxy + (size_t)iu * nx + ct1;
-        float wl = w0 * l;
-        acc.add(wl * X[idx]);
-        accl.add(wl);
+    auto index = [&](int ct, int cz) {
+        return r.xdriven ? (size_t)cz * sxy + (size_t)ct * nx + iu
+                         : (size_t)cz * sxy + (size_t)iu * nx + ct;
+    };
+    s.ok[0] = in_t1 && in_z1;
+    s.ok[1] = (tfirst ? in_t2 : in_t1) && (tfirst ? in_z1 : in_z2);
+    s.ok[2] = in_t2 && in_z2;
+    if (s.ok[0]) s.idx[0] = index(ct1, cz1);
+    if (s.ok[1]) s.idx[1] = index(ctm, czm);
+    if (s.ok[2]) s.idx[2] = index(ct2, cz2);
+    s.wl[0] = lo * l;
+    s.wl[1] = (hi - lo) * l;
+    s.wl[2] = (1.0f - hi) * l;
+}
+
+// Accumulate slab iu of ray r into (acc, accl); X is the volume (x fastest).
+template <bool KAHAN = false>
+__host__ __device__ __forceinline__ void ax_slab(const AxRay &r, const float *__restrict__ X, int iu,
+                                                 int nx, int ny, Acc<KAHAN> &acc,
+                                                 Acc<KAHAN> &accl)
+{
+#pragma clang fp contract(off)
+    SlabW s;
+    ax_slab_geom(r, iu, nx, ny, s);
+    for (int k = 0; k < 3; k++)
+        if (s.ok[k]) {
+            acc.add(s.wl[k] * X[s.idx[k]]);
+            accl.add(s.wl[k]);
+        }
+}
+
+// U consecutive slabs of one lane (iu0, iu0 + stride, ...): all geometry
+// first, then every load, then the sums in slab order.
+template <int U, bool KAHAN>
+__host__ __device__ __forceinline__ void ax_slabs(const AxRay &r, const float *__restrict__ X,
+                                                  int iu0, int stride, int nu, int nx, int ny,
+                                                  Acc<KAHAN> &acc, Acc<KAHAN> &accl)
+{
+#pragma clang fp contract(off)
+    SlabW s[U];
+    float xv[U][3];
+#pragma unroll
+    for (int k = 0; k < U; k++) {
+        const int iu = iu0 + k * stride;
+        if (iu < nu) {
+            ax_slab_geom(r, iu, nx, ny, s[k]);
+        } else {
+            s[k].ok[0] = s[k].ok[1] = s[k].ok[2] = false;
+            s[k].idx[0] = s[k].idx[1] = s[k].idx[2] = 0;
+            s[k].wl[0] = s[k].wl[1] = s[k].wl[2] = 0.0f;
+        }
     }
-    if (inm) {
-        size_t idx = r.xdriven ? (size_t)czm * sxy + (size_t)ctm * nx + iu
-                               : (size_t)czm * sxy + (size_t)iu * nx + ctm;
-        float wl = w1 * l;
-        acc.add(wl * X[idx]);
-        accl.add(wl);
-    }
-    if (in_t2 && in_z2) {
-        size_t idx = r.xdriven ? (size_t)cz2 * sxy + (size_t)ct2 * nx + iu
-                               : (size_t)cz2 * sxy + (size_t)iu * nx + ct2;
-        float wl = w2 * l;
-        acc.add(wl * X[idx]);
-        accl.add(wl);
-    }
+#pragma unroll
+    for (int k = 0; k < U; k++)
+#pragma unroll
+        for (int p = 0; p < 3; p++) xv[k][p] = X[s[k].idx[p]];
+#pragma unroll
+    for (int k = 0; k < U; k++)
+#pragma unroll
+        for (int p = 0; p < 3; p++)
+            if (s[k].ok[p]) {
+                acc.add(s[k].wl[p] * xv[k][p]);
+                accl.add(s[k].wl[p]);
+            }
 }
 
 // ---------------------------------------------------------------------------
@@ -144,7 +195,7 @@ __device__ __forceinline__ void ax_slab(const AxRay &r, const float *__restrict_
 struct Hits {
     float p0x, p0y, p0z, p1x, p1y, p1z;
     int n;
-    __device__ __forceinline__ void take(bool ok, float x, float y, float z)
+    __host__ __device__ __forceinline__ void take(bool ok, float x, float y, float z)
     {
         if (ok) {
             if (n == 0) { p0x = x; p0y = y; p0z = z; }
@@ -156,7 +207,7 @@ struct Hits {
 
 // General 2-D chord (find_l with a' != 0, b' != 0) in plane (u,w):
 // pu = u1_0 - uc, pw = w1_0 - wc, s_wu = b'/a', s_uw = a'/b'.
-__device__ __forceinline__ float chord2(float pu, float pw, float s_wu, float s_uw, float hu,
+__host__ __device__ __forceinline__ float chord2(float pu, float pw, float s_wu, float s_uw, float hu,
                                         float hw, bool u_first)
 {
 #pragma clang fp contract(off)
@@ -183,7 +234,7 @@ __device__ __forceinline__ float chord2(float pu, float pw, float s_wu, float s_
     return sqrtf(ex * ex + ey * ey);
 }
 
-__device__ __forceinline__ float find_l_3d_rec(const RayRec &R, float px, float py, float pz,
+__host__ __device__ __forceinline__ float find_l_3d_rec(const RayRec &R, float px, float py, float pz,
                                                float hz, float dz)
 {
 #pragma clang fp contract(off)
@@ -233,7 +284,7 @@ __device__ __forceinline__ float find_l_3d_rec(const RayRec &R, float px, float 
 }
 
 // Slope record of ray (ia, ib) of view g (the divisions of cu:142-196).
-__device__ __forceinline__ RayRec make_ray_rec(const ViewGeom &g, float ydet, float zdet)
+__host__ __device__ __forceinline__ RayRec make_ray_rec(const ViewGeom &g, float ydet, float zdet)
 {
 #pragma clang fp contract(off)
     float x1 = g.cos_phi * (-g.SO), y1 = g.sin_phi * (-g.SO), z1 = 0.0f;
@@ -266,7 +317,7 @@ struct Footprint {
     float xc, yc, zc;
 };
 
-__device__ __forceinline__ Footprint footprint(const ViewGeom &g, int ix, int iy, int iz)
+__host__ __device__ __forceinline__ Footprint footprint(const ViewGeom &g, int ix, int iy, int iz)
 {
 #pragma clang fp contract(off)
     Footprint f;
@@ -295,7 +346,7 @@ __device__ __forceinline__ Footprint footprint(const ViewGeom &g, int ix, int iy
 // XCD-aware block order: hardware hands consecutive workgroups to the 8 XCDs
 // round-robin; remap so that each XCD walks a contiguous range of tiles and
 // neighbouring tiles share its L2.
-__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nblocks)
+__host__ __device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nblocks)
 {
     constexpr unsigned NXCD = 8;
     if (nblocks < NXCD * 2) return b;

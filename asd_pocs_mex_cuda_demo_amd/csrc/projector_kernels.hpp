@@ -4,6 +4,7 @@
 #pragma once
 
 #include "geom_device.hpp"
+#include "projector_tile.hpp"
 
 namespace asd {
 
@@ -29,7 +30,8 @@ __global__ __launch_bounds__(256) void ax_kernel(ViewGeom g, const float *__rest
         int ia = (int)(id % g.na), ib = (int)(id / g.na);
         AxRay r = ax_ray_setup(g, y_det[ia], z_det[ib]);
         int nu = r.xdriven ? g.nx : g.ny;
-        for (int iu = lane; iu < nu; iu += LPR) ax_slab<KAHAN>(r, X, iu, g.nx, g.ny, sa, sl);
+        for (int iu0 = lane; iu0 < nu; iu0 += 4 * LPR)
+            ax_slabs<4, KAHAN>(r, X, iu0, LPR, nu, g.nx, g.ny, sa, sl);
     }
     float acc, accl;
     if constexpr (LPR > 1 && KAHAN) {
@@ -107,6 +109,136 @@ __global__ __launch_bounds__(256) void atx_kernel(ViewGeom g, const float *__res
         }
     }
     epi(v, acc.value(), accl.value());
+}
+
+// Backprojector on voxel tiles (TX x TY x TZ = 256 voxels per workgroup):
+// the slope records and correction values of the tile's footprint are
+// staged in LDS once (coalesced), then every voxel walks its own footprint,
+// ib outer and ia inner like the reference, from LDS.  Same arithmetic and
+// summation order as atx_kernel, so the result is bit-identical to it.
+template <int TX, int TY, int TZ, bool KAHAN, class Epi>
+__global__ __launch_bounds__(256) void atx_tile_plain_kernel(ViewGeom g,
+                                                             const float *__restrict__ Y,
+                                                             const RayRec *__restrict__ rt,
+                                                             Epi epi)
+{
+#pragma clang fp contract(off)
+    static_assert(TX * TY * TZ == 256, "one voxel per thread");
+    __shared__ RayRec s_rec[TILE_MAXR];
+    __shared__ float s_y[TILE_MAXR];
+    __shared__ int s_rng[4];
+    if (epi.skip()) return;
+    const TileDims td = tile_dims(g, TX, TY, TZ);
+    const unsigned ntiles = (unsigned)td.ntx * td.nty * td.ntz;
+    const unsigned t = xcd_block(blockIdx.x, gridDim.x);
+    if (t >= ntiles) return;
+    const int tile_x = (int)(t % td.ntx), tile_y = (int)((t / td.ntx) % td.nty),
+              tile_z = (int)(t / ((unsigned)td.ntx * td.nty));
+    const int lx = threadIdx.x % TX, ly = (threadIdx.x / TX) % TY, lz = threadIdx.x / (TX * TY);
+    const int ix = tile_x * TX + lx, iy = tile_y * TY + ly, iz = tile_z * TZ + lz;
+    const bool valid = ix < g.nx && iy < g.ny && iz < g.nz;
+    if (threadIdx.x == 0) {
+        s_rng[0] = 1 << 30; s_rng[1] = -(1 << 30);
+        s_rng[2] = 1 << 30; s_rng[3] = -(1 << 30);
+    }
+    __syncthreads();
+    Footprint f;
+    if (valid) {
+        f = footprint(g, ix, iy, iz);
+        if (f.a_lo <= f.a_hi && f.b_lo <= f.b_hi) {
+            atomicMin(&s_rng[0], f.a_lo);
+            atomicMax(&s_rng[1], f.a_hi);
+            atomicMin(&s_rng[2], f.b_lo);
+            atomicMax(&s_rng[3], f.b_hi);
+        }
+    }
+    __syncthreads();
+    const int A0 = s_rng[0], A1 = s_rng[1], B0 = s_rng[2], B1 = s_rng[3];
+    const float sx = g.cos_phi * (-g.SO), sy = g.sin_phi * (-g.SO);
+    float px = 0.0f, py = 0.0f, pz = 0.0f;
+    if (valid) {
+        px = sx - f.xc;
+        py = sy - f.yc;
+        pz = 0.0f - f.zc;
+    }
+    const float hz = g.dz / 2.0f;
+    Acc<KAHAN> acc, accl;
+    if (A0 <= A1 && B0 <= B1) {
+        const int nA = A1 - A0 + 1;
+        if (nA > TILE_MAXR) {
+            // footprint too wide for LDS: read the records directly
+            if (valid)
+                for (int ib = f.b_lo; ib <= f.b_hi; ib++)
+                    for (int ia = f.a_lo; ia <= f.a_hi; ia++) {
+                        const long k = (long)ib * g.na + ia;
+                        float l = find_l_3d_rec(rt[k], px, py, pz, hz, g.dz);
+                        acc.add(l * Y[k]);
+                        accl.add(l);
+                    }
+        } else {
+            const int RB = TILE_MAXR / nA;  // detector rows per LDS chunk
+            for (int b0 = B0; b0 <= B1; b0 += RB) {
+                const int b1 = b0 + RB - 1 < B1 ? b0 + RB - 1 : B1;
+                const int n = nA * (b1 - b0 + 1);
+                __syncthreads();
+                for (int k = threadIdx.x; k < n; k += 256) {
+                    const long gk = (long)(b0 + k / nA) * g.na + A0 + k % nA;
+                    s_rec[k] = rt[gk];
+                    s_y[k] = Y[gk];
+                }
+                __syncthreads();
+                if (valid) {
+                    const int blo = f.b_lo > b0 ? f.b_lo : b0, bhi = f.b_hi < b1 ? f.b_hi : b1;
+                    for (int ib = blo; ib <= bhi; ib++) {
+                        const int row = (ib - b0) * nA - A0;
+                        for (int ia = f.a_lo; ia <= f.a_hi; ia++) {
+                            float l = find_l_3d_rec(s_rec[row + ia], px, py, pz, hz, g.dz);
+                            acc.add(l * s_y[row + ia]);
+                            accl.add(l);
+                        }
+                    }
+                }
+            }
+        }
+    }
+    if (valid) epi((long)((size_t)iz * g.ny + iy) * g.nx + ix, acc.value(), accl.value());
+}
+
+// Tile shape by volume depth: 16x16 slices for 2-D, 8x8x4 bricks for 3-D.
+template <bool KAHAN, class Epi>
+int dispatch_atx_tile(const ViewGeom &g, const float *Y, const RayRec *rt, const Epi &epi,
+                      hipStream_t s)
+{
+    if (g.nz == 1) {
+        const TileDims td = tile_dims(g, 16, 16, 1);
+        atx_tile_plain_kernel<16, 16, 1, KAHAN, Epi>
+            <<<(unsigned)td.ntx * td.nty * td.ntz, 256, 0, s>>>(g, Y, rt, epi);
+    } else {
+        const TileDims td = tile_dims(g, 8, 8, 4);
+        atx_tile_plain_kernel<8, 8, 4, KAHAN, Epi>
+            <<<(unsigned)td.ntx * td.nty * td.ntz, 256, 0, s>>>(g, Y, rt, epi);
+    }
+    ASD_LAUNCH_CHECK("atx_tile_plain_kernel");
+    return ASD_OK;
+}
+
+// Precision-variant backprojector (tile-local FP64 ray setup, see
+// projector_tile.hpp).
+template <bool KAHAN, class Epi>
+int dispatch_atx_prec(const ViewGeom &g, const float *Y, const float *y_det, const float *z_det,
+                      const RayRec *rt, const Epi &epi, hipStream_t s)
+{
+    if (g.nz == 1) {
+        const TileDims td = tile_dims(g, 16, 16, 1);
+        atx_tile_kernel<16, 16, 1, KAHAN, Epi>
+            <<<(unsigned)td.ntx * td.nty * td.ntz, 256, 0, s>>>(g, Y, y_det, z_det, rt, epi);
+    } else {
+        const TileDims td = tile_dims(g, 8, 8, 4);
+        atx_tile_kernel<8, 8, 4, KAHAN, Epi>
+            <<<(unsigned)td.ntx * td.nty * td.ntz, 256, 0, s>>>(g, Y, y_det, z_det, rt, epi);
+    }
+    ASD_LAUNCH_CHECK("atx_tile_kernel");
+    return ASD_OK;
 }
 
 }  // namespace asd

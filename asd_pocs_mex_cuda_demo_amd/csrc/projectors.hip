@@ -78,11 +78,7 @@ int launch_ax(const ViewGeom &g, const float *x, float *y, float *sr, const floa
 int launch_atx(const ViewGeom &g, float *x, const float *y, float *sc, const float *y_det,
                const float *z_det, const RayRec *rt, hipStream_t s)
 {
-    long N = (long)g.nx * g.ny * g.nz;
-    unsigned blocks = (unsigned)((N + 255) / 256);
-    atx_kernel<<<blocks, 256, 0, s>>>(g, y, rt, EpiBackPlain{x, sc, g.scale});
-    ASD_LAUNCH_CHECK("atx_kernel");
-    return ASD_OK;
+    return dispatch_atx_tile<false>(g, y, rt, EpiBackPlain{x, sc, g.scale}, s);
 }
 
 // ---------------------------------------------------------------------------

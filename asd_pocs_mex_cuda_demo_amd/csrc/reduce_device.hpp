@@ -1,6 +1,10 @@
-// Deterministic FP64 reductions: wave butterflies, one partial per block, and
-// the last block to finish folds the partials in a fixed order.  The result
-// does not depend on block scheduling, so repeated runs are bit-identical.
+// Deterministic FP64 reductions: wave butterflies and one partial per block;
+// a separate one-block kernel (fold_partials) folds the partials in a fixed
+// order.  No atomics: a single counter that every block of a large grid
+// increments serialises (~20 ns per block), which cost more than the data
+// pass itself.  Results do not depend on scheduling and are bit-identical
+// run to run.  (grid_sum_to / grid_sums, the last-block variants, remain for
+// grids of a few blocks.)
 #pragma once
 
 #include "asd_internal.hpp"
@@ -109,6 +113,53 @@ __device__ __forceinline__ bool grid_sums(const double (&val)[K], double *partia
 #pragma unroll
     for (int k = 0; k < K; k++) tot[k] = res[k];
     return true;
+}
+
+}  // namespace asd
+
+namespace asd {
+
+// Store the block's K sums at partials[k * gridDim.x + blockIdx.x].
+template <int K>
+__device__ __forceinline__ void store_partials(const double (&val)[K], double *partials)
+{
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        double s = block_sum(val[k]);
+        if (threadIdx.x == 0) partials[(size_t)k * gridDim.x + blockIdx.x] = s;
+    }
+}
+
+// Fold K rows of nblocks partials (one block of RED_THREADS threads); the
+// totals are valid in every thread.
+template <int K>
+__device__ __forceinline__ void fold_rows(const double *partials, int nblocks, double (&tot)[K])
+{
+    __shared__ double res[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        double t = 0.0;
+        for (int i = threadIdx.x; i < nblocks; i += blockDim.x)
+            t += partials[(size_t)k * nblocks + i];
+        t = block_sum(t);
+        if (threadIdx.x == 0) res[k] = t;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; k++) tot[k] = res[k];
+}
+
+// out[k] = sum of row k (one block); skipped when *skip != 0.
+template <int K>
+__global__ __launch_bounds__(RED_THREADS) void fold_partials(const double *partials, int nblocks,
+                                                            double *out, const int *skip)
+{
+    if (skip && *skip) return;
+    double tot[K];
+    fold_rows<K>(partials, nblocks, tot);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < K; k++) out[k] = tot[k];
 }
 
 }  // namespace asd

@@ -60,7 +60,6 @@ __device__ __forceinline__ A tv3_grad(const T *u, int nx, int ny, int nz, int x,
 template <class T, class A, bool THREE_D>
 __global__ __launch_bounds__(256) void tv_grad_kernel(const T *__restrict__ u, T *__restrict__ v,
                                                       int nx, int ny, int nz, double *partials,
-                                                      unsigned *counter, double *nrm2,
                                                       const int *skip)
 {
     if (skip && *skip) return;
@@ -76,7 +75,8 @@ __global__ __launch_bounds__(256) void tv_grad_kernel(const T *__restrict__ u, T
         v[i] = gt;
         s += (double)gt * (double)gt;
     }
-    grid_sum_to(s, partials, counter, nrm2);
+    double val[1] = {s};
+    store_partials<1>(val, partials);
 }
 
 template <class T>
@@ -103,8 +103,7 @@ __global__ __launch_bounds__(256) void tv_step_kernel(T *__restrict__ u, const T
 template <class T>
 __global__ __launch_bounds__(256) void sqdiff_kernel(const T *__restrict__ a,
                                                      const T *__restrict__ b, size_t n,
-                                                     double *partials, unsigned *counter,
-                                                     double *out)
+                                                     double *partials)
 {
     double s = 0.0;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -112,7 +111,8 @@ __global__ __launch_bounds__(256) void sqdiff_kernel(const T *__restrict__ a,
         double e = (double)a[i] - (b ? (double)b[i] : 0.0);
         s += e * e;
     }
-    grid_sum_to(s, partials, counter, out);
+    double val[1] = {s};
+    store_partials<1>(val, partials);
 }
 
 unsigned stream_blocks(size_t n)
@@ -128,9 +128,10 @@ template <class T>
 int launch_sqdiff(const T *a, const T *b, size_t n, TvScratch &s, int slot, hipStream_t st)
 {
     unsigned blocks = red_blocks(n);
-    sqdiff_kernel<T><<<blocks, RED_THREADS, 0, st>>>(a, b, n, s.partials, s.counters + slot,
-                                                    s.scalars + slot);
+    sqdiff_kernel<T><<<blocks, RED_THREADS, 0, st>>>(a, b, n, s.partials);
     ASD_LAUNCH_CHECK("sqdiff_kernel");
+    fold_partials<1><<<1, RED_THREADS, 0, st>>>(s.partials, (int)blocks, s.scalars + slot, nullptr);
+    ASD_LAUNCH_CHECK("fold_partials");
     return ASD_OK;
 }
 
@@ -145,14 +146,15 @@ int tv_descent(T *u, const T *img_old, T *v, int nx, int ny, int nz, bool three_
     const unsigned gb = red_blocks(n), ub = stream_blocks(n);
     for (int it = 0; it < nTVIter; it++) {
         if (three_d)
-            tv_grad_kernel<T, A, true><<<gb, RED_THREADS, 0, st>>>(
-                u, v, nx, ny, nz, s.partials, s.counters + TV_SLOT_NRM2, s.scalars + TV_SLOT_NRM2,
-                skip);
+            tv_grad_kernel<T, A, true><<<gb, RED_THREADS, 0, st>>>(u, v, nx, ny, nz, s.partials,
+                                                                  skip);
         else
-            tv_grad_kernel<T, A, false><<<gb, RED_THREADS, 0, st>>>(
-                u, v, nx, ny, 1, s.partials, s.counters + TV_SLOT_NRM2, s.scalars + TV_SLOT_NRM2,
-                skip);
+            tv_grad_kernel<T, A, false><<<gb, RED_THREADS, 0, st>>>(u, v, nx, ny, 1, s.partials,
+                                                                   skip);
         ASD_LAUNCH_CHECK("tv_grad_kernel");
+        fold_partials<1><<<1, RED_THREADS, 0, st>>>(s.partials, (int)gb,
+                                                    s.scalars + TV_SLOT_NRM2, skip);
+        ASD_LAUNCH_CHECK("fold_partials");
         int clamp = (pos == 1 && it == nTVIter - 1) ? 1 : 0;
         tv_step_kernel<T><<<ub, 256, 0, st>>>(u, v, n, alpha, alpha_dev, s.scalars + TV_SLOT_D2,
                                               s.scalars + TV_SLOT_NRM2, clamp, skip);

@@ -77,15 +77,30 @@ def voxel_boxes(nx, ny, nz):
     return lo, lo + 1.0
 
 
-def system_matrix(cos_phi, sin_phi, SO, OD, ydet, zdet, nx, ny, nz):
+def endpoints(cos_phi, sin_phi, SO, OD, ydet, f32_points=False):
+    """Source (x, y) and bin points (bx[ia], by[ia]).  With f32_points the
+    points are rounded to float32 exactly as the operators form them
+    (cos*(-SO) etc. in FP32, Ax_cone_gpu.cu:259-264), so the ray is the line
+    the operators actually intersect; otherwise float64 products."""
+    if f32_points:
+        c, s = np.float32(cos_phi), np.float32(sin_phi)
+        SOf, ODf = np.float32(SO), np.float32(OD)
+        yd = np.asarray(ydet, np.float32)
+        sx, sy = float(c * (-SOf)), float(s * (-SOf))
+        bx = ((c * ODf) - (s * yd)).astype(np.float64)
+        by = ((s * ODf) + (c * yd)).astype(np.float64)
+        return sx, sy, bx, by
+    c, s = float(cos_phi), float(sin_phi)
+    ydet = np.asarray(ydet, np.float64)
+    return -SO * c, -SO * s, OD * c - s * ydet, OD * s + c * ydet
+
+
+def system_matrix(cos_phi, sin_phi, SO, OD, ydet, zdet, nx, ny, nz, f32_points=False):
     """Exact lengths A[ray, voxel] (rays ib-major, ia fastest; voxels x fastest),
     using the float32 cos/sin the operators receive, promoted to float64."""
-    c, s = float(cos_phi), float(sin_phi)
-    src = np.array([-SO * c, -SO * s, 0.0])
-    ydet = np.asarray(ydet, np.float64)
+    sx, sy, bx, by = endpoints(cos_phi, sin_phi, SO, OD, ydet, f32_points)
+    src = np.array([sx, sy, 0.0])
     zdet = np.asarray(zdet, np.float64)
-    bx = OD * c - s * ydet
-    by = OD * s + c * ydet
     rays = []
     for ib in range(len(zdet)):
         for ia in range(len(ydet)):
@@ -96,12 +111,11 @@ def system_matrix(cos_phi, sin_phi, SO, OD, ydet, zdet, nx, ny, nz):
     return A
 
 
-def system_matrix_2d(cos_phi, sin_phi, SO, OD, ydet, nx, ny):
+def system_matrix_2d(cos_phi, sin_phi, SO, OD, ydet, nx, ny, f32_points=False):
     """2-D exact lengths (rays in the z = 0 plane, pixel boxes in x, y)."""
-    c, s = float(cos_phi), float(sin_phi)
-    src = np.array([-SO * c, -SO * s])
-    ydet = np.asarray(ydet, np.float64)
-    rays = np.stack([OD * c - s * ydet, OD * s + c * ydet], -1)
+    sx, sy, bx, by = endpoints(cos_phi, sin_phi, SO, OD, ydet, f32_points)
+    src = np.array([sx, sy])
+    rays = np.stack([bx, by], -1)
     lo3, hi3 = voxel_boxes(nx, ny, 1)
     return clip_len(src[None, None, :], rays[:, None, :], lo3[None, :, :2], hi3[None, :, :2])
 

@@ -1,0 +1,24 @@
+"""Builds tests/cpu_emul/libemul.so: the projector device functions compiled
+for the host (test infrastructure)."""
+import os
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+
+
+def build():
+    out = os.path.join(HERE, "libemul.so")
+    pkg = os.path.join(ROOT, "asd_pocs_mex_cuda_demo_amd")
+    subprocess.run(["/opt/rocm/bin/hipcc", "--offload-arch=gfx950", "-O2", "-fPIC", "-shared",
+                    "-std=c++17", "-ffp-contract=off", "-fhip-fp32-correctly-rounded-divide-sqrt",
+                    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+                    f"-I{os.path.join(ROOT, 'include')}", f"-I{os.path.join(pkg, 'csrc')}",
+                    os.path.join(HERE, "emul.hip"), "-o", out + ".tmp"],
+                   check=True, capture_output=True)
+    os.replace(out + ".tmp", out)
+    return out
+
+
+if __name__ == "__main__":
+    build()

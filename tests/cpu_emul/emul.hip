@@ -1,0 +1,178 @@
+// Host-side execution of the projector device functions (test infrastructure).
+//
+// The geometry helpers in asd_pocs_mex_cuda_demo_amd/csrc/geom_device.hpp are
+// __host__ __device__: this file runs the exact functions the GPU kernels
+// use, in the kernels' loop order, on the CPU, so tests can compare them with
+// the oracle without a GPU.
+#include <cstddef>
+
+#include "geom_device.hpp"
+#include "projector_tile.hpp"
+
+#include <vector>
+
+using namespace asd;
+
+static ViewGeom mk(float c, float s, float SO, float OD, float scale, float dyd, float dzd,
+                   float dz, int nx, int ny, int nz, int na, int nb)
+{
+    ViewGeom g;
+    g.cos_phi = c; g.sin_phi = s; g.SO = SO; g.OD = OD; g.scale = scale; g.dz = dz;
+    g.dy_det = dyd; g.dz_det = dzd;
+    g.nx = nx; g.ny = ny; g.nz = nz; g.na = na; g.nb = nb;
+    return g;
+}
+
+extern "C" {
+
+// ax_kernel<LPR=lpr>: lane j of a ray takes slabs j, j+lpr, ...; lanes are
+// combined by the same butterfly order as the wave shuffles.
+void emul_ax(const float *X, float *y, float *sr, float c, float s, const float *ydet,
+             const float *zdet, float SO, float OD, float scale, float dz, int nx, int ny, int nz,
+             int na, int nb, int lpr, int kahan)
+{
+    ViewGeom g = mk(c, s, SO, OD, scale, 1.0f, 1.0f, dz, nx, ny, nz, na, nb);
+    float acc[64], accl[64];
+    for (long id = 0; id < (long)na * nb; id++) {
+        int ia = (int)(id % na), ib = (int)(id / na);
+        AxRay r = ax_ray_setup(g, ydet[ia], zdet[ib]);
+        int nu = r.xdriven ? nx : ny;
+        for (int lane = 0; lane < lpr; lane++) {
+            if (kahan) {
+                Acc<true> a, l;
+                for (int iu0 = lane; iu0 < nu; iu0 += 4 * lpr)
+                    ax_slabs<4, true>(r, X, iu0, lpr, nu, nx, ny, a, l);
+                acc[lane] = a.value();
+                accl[lane] = l.value();
+            } else {
+                Acc<false> a, l;
+                for (int iu0 = lane; iu0 < nu; iu0 += 4 * lpr)
+                    ax_slabs<4, false>(r, X, iu0, lpr, nu, nx, ny, a, l);
+                acc[lane] = a.value();
+                accl[lane] = l.value();
+            }
+        }
+        for (int off = lpr / 2; off > 0; off >>= 1) {
+            float na_[64], nl_[64];
+            for (int lane = 0; lane < lpr; lane++) {
+                na_[lane] = acc[lane] + acc[lane ^ off];
+                nl_[lane] = accl[lane] + accl[lane ^ off];
+            }
+            for (int lane = 0; lane < lpr; lane++) {
+                acc[lane] = na_[lane];
+                accl[lane] = nl_[lane];
+            }
+        }
+        y[id] = acc[0] * scale;
+        sr[id] = accl[0] * scale;
+    }
+}
+
+// atx_kernel: one voxel at a time, footprint ib outer / ia inner.
+void emul_atx(float *X, const float *y, float *sc, float c, float s, const float *ydet,
+              const float *zdet, float SO, float OD, float scale, float dyd, float dzd, float dz,
+              int nx, int ny, int nz, int na, int nb)
+{
+    ViewGeom g = mk(c, s, SO, OD, scale, dyd, dzd, dz, nx, ny, nz, na, nb);
+    long M = (long)na * nb;
+    RayRec *rt = new RayRec[M];
+    for (long id = 0; id < M; id++) rt[id] = make_ray_rec(g, ydet[id % na], zdet[id / na]);
+    for (long v = 0; v < (long)nx * ny * nz; v++) {
+        int ix = (int)(v % nx);
+        long r = v / nx;
+        int iy = (int)(r % ny), iz = (int)(r / ny);
+        Footprint f = footprint(g, ix, iy, iz);
+        float sx = g.cos_phi * (-g.SO), sy = g.sin_phi * (-g.SO);
+        float px = sx - f.xc, py = sy - f.yc, pz = 0.0f - f.zc;
+        float hz = g.dz / 2.0f;
+        Acc<false> acc, accl;
+        for (int ib = f.b_lo; ib <= f.b_hi; ib++)
+            for (int ia = f.a_lo; ia <= f.a_hi; ia++) {
+                float l = find_l_3d_rec(rt[(long)ib * na + ia], px, py, pz, hz, g.dz);
+                acc.add(l * y[(long)ib * na + ia]);
+                accl.add(l);
+            }
+        X[v] = acc.value() * scale;
+        sc[v] = accl.value() * scale;
+    }
+    delete[] rt;
+}
+
+// atx_tile_kernel (precision variant) tile by tile: the tile's union
+// footprint, FP64 tile-local ray records, then every voxel ia-outer with the
+// in-plane skip.  tx, ty, tz as the kernel's template arguments.
+void emul_atx_prec(float *X, const float *y, float *sc, float c, float s, const float *ydet,
+                   const float *zdet, float SO, float OD, float scale, float dyd, float dzd,
+                   float dz, int nx, int ny, int nz, int na, int nb, int TX, int TY, int TZ)
+{
+    ViewGeom g = mk(c, s, SO, OD, scale, dyd, dzd, dz, nx, ny, nz, na, nb);
+    long M = (long)na * nb;
+    std::vector<RayRec> rt(M);
+    for (long id = 0; id < M; id++) rt[id] = make_ray_rec(g, ydet[id % na], zdet[id / na]);
+    TileDims td = tile_dims(g, TX, TY, TZ);
+    for (int tzi = 0; tzi < td.ntz; tzi++)
+        for (int tyi = 0; tyi < td.nty; tyi++)
+            for (int txi = 0; txi < td.ntx; txi++) {
+                const int x0 = txi * TX, y0 = tyi * TY, z0 = tzi * TZ;
+                int A0 = 1 << 30, A1 = -(1 << 30), B0 = 1 << 30, B1 = -(1 << 30);
+                for (int lz = 0; lz < TZ; lz++)
+                    for (int ly = 0; ly < TY; ly++)
+                        for (int lx = 0; lx < TX; lx++) {
+                            int ix = x0 + lx, iy = y0 + ly, iz = z0 + lz;
+                            if (ix >= nx || iy >= ny || iz >= nz) continue;
+                            Footprint f = footprint(g, ix, iy, iz);
+                            if (f.a_lo > f.a_hi || f.b_lo > f.b_hi) continue;
+                            A0 = f.a_lo < A0 ? f.a_lo : A0; A1 = f.a_hi > A1 ? f.a_hi : A1;
+                            B0 = f.b_lo < B0 ? f.b_lo : B0; B1 = f.b_hi > B1 ? f.b_hi : B1;
+                        }
+                double xtc, ytc, ztc;
+                tile_centre(g, x0, y0, z0, TX, TY, TZ, xtc, ytc, ztc);
+                int nA = A1 - A0 + 1, nB = B1 - B0 + 1;
+                std::vector<TileRayXY> rxy(nA > 0 ? nA : 0);
+                std::vector<TileRayZ> rz((nA > 0 && nB > 0) ? (size_t)nA * nB : 0);
+                for (int kb = 0; kb < nB; kb++)
+                    for (int ka = 0; ka < nA; ka++) {
+                        TileRayXY a;
+                        tile_ray_setup(g, ydet[A0 + ka], zdet[B0 + kb], xtc, ytc, ztc, a,
+                                       rz[(size_t)kb * nA + ka]);
+                        if (kb == 0) rxy[ka] = a;
+                    }
+                for (int lz = 0; lz < TZ; lz++)
+                    for (int ly = 0; ly < TY; ly++)
+                        for (int lx = 0; lx < TX; lx++) {
+                            int ix = x0 + lx, iy = y0 + ly, iz = z0 + lz;
+                            if (ix >= nx || iy >= ny || iz >= nz) continue;
+                            Footprint f = footprint(g, ix, iy, iz);
+                            float dxo = (float)(lx + 0.5 - 0.5 * TX), dyo = (float)(ly + 0.5 - 0.5 * TY);
+                            float dzo = (float)((lz + 0.5 - 0.5 * TZ) * g.dz), hz = g.dz / 2.0f;
+                            float sx = g.cos_phi * (-g.SO), sy = g.sin_phi * (-g.SO);
+                            Acc<true> acc, accl;
+                            for (int ia = f.a_lo; ia <= f.a_hi; ia++) {
+                                const TileRayXY &r = rxy[ia - A0];
+                                if (r.mode != RAY_X_FIRST && r.mode != RAY_Y_FIRST) {
+                                    float px = sx - f.xc, py = sy - f.yc, pz = 0.0f - f.zc;
+                                    for (int ib = f.b_lo; ib <= f.b_hi; ib++) {
+                                        float l = find_l_3d_rec(rt[(long)ib * na + ia], px, py, pz, hz, g.dz);
+                                        acc.add(l * y[(long)ib * na + ia]);
+                                        accl.add(l);
+                                    }
+                                    continue;
+                                }
+                                float xy, yx;
+                                unsigned m = tile_inplane(r, dxo, dyo, xy, yx);
+                                if (!m) continue;
+                                for (int ib = f.b_lo; ib <= f.b_hi; ib++) {
+                                    float l = tile_find_l(r, rz[(size_t)(ib - B0) * nA + (ia - A0)], m, xy,
+                                                          yx, dxo, dyo, dzo, hz);
+                                    acc.add(l * y[(long)ib * na + ia]);
+                                    accl.add(l);
+                                }
+                            }
+                            long v = ((long)iz * ny + iy) * nx + ix;
+                            X[v] = acc.value() * scale;
+                            sc[v] = accl.value() * scale;
+                        }
+            }
+}
+
+}  // extern "C"

@@ -132,7 +132,7 @@ def test_asd_pocs_gateway_on_gpu(orc):
     outs, _ = host.call("ASD_POCS_gpu", 2, P, para)
     m = host.from_mx(outs[0])
     o = outs[1]
-    assert m.shape == (n, n, 1)
+    assert m.shape == (n, n)  # MATLAB drops the trailing singleton nz = 1
     m_py, out_py = ops.ASD_POCS_cuda(P, para)
     assert np.array_equal(m, m_py)
     assert host.field(o, "IterationNum")[0, 0] == 3
