@@ -292,9 +292,8 @@ __global__ __launch_bounds__(256) void tv_step_batch(const BCase *__restrict__ c
     const size_t n = (size_t)C.nx * C.ny * C.nz;
     const size_t i = (size_t)(b - bpref[c]) * 256 + threadIdx.x;
     if (i >= n) return;
-    const double ad = ctl[c].alpha * sqrt(ctl[c].d2);
-    const double nrm = sqrt(ctl[c].nrm2);
-    double w = (double)u_all[C.voff + i] - ad * (double)v_all[C.voff + i] / nrm;
+    const double st = ctl[c].alpha * sqrt(ctl[c].d2) / sqrt(ctl[c].nrm2);
+    double w = (double)u_all[C.voff + i] - st * (double)v_all[C.voff + i];
     if (it == C.rules.ng - 1 && w < 0) w = 0;  // positivity after the last step
     u_all[C.voff + i] = (float)w;
 }

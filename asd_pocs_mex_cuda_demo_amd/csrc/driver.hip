@@ -215,11 +215,13 @@ struct asd_plan {
     size_t N = 0, M = 0;
     std::vector<ViewGeom> views;
     hipStream_t stream = nullptr;
-    DeviceBuf ydet, zdet, P, m, mpre, mk1, f0, I0, cor, res2, rt, v, ctlbuf, hist;
+    DeviceBuf ydet, zdet, P, m, mpre, mk1, f0, I0, cor, res2, rt, v, u2, v2, ctlbuf, hist;
     std::vector<double> u16_lo, u16_step;  // per-view scale of the 16-bit fixed-point sinogram
     TvScratch red;
     bool have_P = false, have_I0 = false;
     int lpr = 1;
+    int atx_variant = -1;           // backprojector variant picked by the autotuner
+    double atx_us[ATX_NVARIANT];    // its measurements (-1: not tried)
     DevCtl h;  // host copy of the control block after the last sync
     // hipGraph of one outer iteration
     bool use_graph = true;
@@ -359,11 +361,78 @@ int enqueue_sweep_t(asd_plan *p, const double *relax)
                 rc = dispatch_atx_prec<KAHAN>(g, p->cor.as<float>(), yd, zd, p->rt.as<RayRec>(),
                                               eb, p->stream);
             else
-                rc = dispatch_atx_tile<KAHAN>(g, p->cor.as<float>(), p->rt.as<RayRec>(), eb,
-                                              p->stream);
+                rc = dispatch_atx_variant<KAHAN>(p->atx_variant, g, p->cor.as<float>(),
+                                                 p->rt.as<RayRec>(), eb, p->stream);
             if (rc) return rc;
         }
     }
+    return ASD_OK;
+}
+
+// Backprojector epilogue for the autotuner (plain x and sc).
+struct EpiTune {
+    float *x, *sc;
+    float scale;
+    __device__ __forceinline__ bool skip() const { return false; }
+    __device__ __forceinline__ void operator()(long v, float acc, float accl) const
+    {
+        x[v] = acc * scale;
+        sc[v] = accl * scale;
+    }
+};
+
+// Times the bit-identical backprojector variants on view 0 (scratch outputs,
+// before the first sweep) and keeps the fastest.  The choice only changes
+// speed, never results.
+int autotune_atx(asd_plan *p)
+{
+    if (p->atx_variant >= 0) return ASD_OK;
+    for (int i = 0; i < ATX_NVARIANT; i++) p->atx_us[i] = -1.0;
+    if (p->para.precision != 0) {
+        p->atx_variant = ATX_DIRECT;  // the precision variant has its own tiled kernel
+        return ASD_OK;
+    }
+    const ViewGeom &g = p->views[0];
+    int rc;
+    if ((rc = launch_ray_table(g, p->ydet.as<float>(), p->zdet.as<float>(), p->rt.as<RayRec>(),
+                               p->stream)))
+        return rc;
+    ASD_HIP(hipMemsetAsync(p->cor.p, 0, p->M * 4, p->stream));
+    std::vector<int> cand = {ATX_DIRECT, ATX_T16x16x1};
+    if (g.nz > 1) {
+        cand.push_back(ATX_T8x8x4);
+        cand.push_back(ATX_T16x4x4);
+    }
+    hipEvent_t e0, e1;
+    ASD_HIP(hipEventCreate(&e0));
+    ASD_HIP(hipEventCreate(&e1));
+    const EpiTune ep{p->v.as<float>(), p->u2.as<float>(), g.scale};
+    int best = ATX_DIRECT;
+    double best_us = 1e30;
+    for (int v : cand) {
+        float ms_min = 1e30f;
+        for (int rep = 0; rep < 4; rep++) {
+            (void)hipEventRecord(e0, p->stream);
+            rc = dispatch_atx_variant<false>(v, g, p->cor.as<float>(), p->rt.as<RayRec>(), ep,
+                                             p->stream);
+            (void)hipEventRecord(e1, p->stream);
+            if (rc) break;
+            (void)hipEventSynchronize(e1);
+            float ms = 0;
+            if (rep > 0 && hipEventElapsedTime(&ms, e0, e1) == hipSuccess && ms < ms_min)
+                ms_min = ms;
+        }
+        if (rc) break;
+        p->atx_us[v] = ms_min * 1e3;
+        if (ms_min * 1e3 < best_us) {
+            best_us = ms_min * 1e3;
+            best = v;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    p->atx_variant = best;
     return ASD_OK;
 }
 
@@ -401,7 +470,8 @@ int enqueue_iteration(asd_plan *p)
     {
         StageTimer t(p, ST_TVGRAD);
         if ((rc = tv_descent<float, float>(p->m.as<float>(), p->mk1.as<float>(),
-                                           p->v.as<float>(), pa.nx, pa.ny, three_d ? pa.nz : 1,
+                                           p->v.as<float>(), p->u2.as<float>(), p->v2.as<float>(),
+                                           pa.nx, pa.ny, three_d ? pa.nz : 1,
                                            three_d, 0.0, &ctl->alpha, pa.numOfgradients, 1,
                                            p->red, p->stream, true, &ctl->stopped)))
             return rc;
@@ -557,6 +627,7 @@ int asd_plan_create(const asd_para *para, const double *phi_vec, int nv, asd_pla
     if ((rc = p->ydet.ensure(yall.size() * 4)) || (rc = p->zdet.ensure(zall.size() * 4)) ||
         (rc = p->m.ensure(N * 4)) || (rc = p->mpre.ensure(N * 4)) ||
         (rc = p->mk1.ensure(N * 4)) || (rc = p->f0.ensure(N * 4)) || (rc = p->v.ensure(N * 4)) ||
+        (rc = p->u2.ensure(N * 4)) || (rc = p->v2.ensure(N * 4)) ||
         (rc = p->cor.ensure(M * 4)) || (rc = p->res2.ensure(M * 8)) ||
         (rc = p->rt.ensure(M * sizeof(RayRec))) ||
         (rc = p->P.ensure((size_t)nv * M * sino_bytes(para->precision))) ||
@@ -654,6 +725,7 @@ int asd_plan_iterate(asd_plan *p, int n_iter)
     int rc;
     if ((rc = need_projections(p))) return rc;
     if (p->h.stopped || n_iter == 0) return ASD_OK;
+    if ((rc = autotune_atx(p))) return rc;
     if (p->use_graph && !p->timing) {
         if (!p->exec && (rc = build_graph(p))) return rc;
         for (int i = 0; i < n_iter; i++) ASD_HIP(hipGraphLaunch(p->exec, p->stream));
@@ -671,6 +743,7 @@ int asd_plan_sweep(asd_plan *p, double lambda, double *dataTolerance)
     int rc;
     if ((rc = need_projections(p))) return rc;
     ASD_ARG(!p->h.stopped, "plan has stopped; reset it first");
+    if ((rc = autotune_atx(p))) return rc;
     p->h.relax = lambda;
     if ((rc = write_ctl(p))) return rc;
     if ((rc = enqueue_sweep(p, &p->ctl()->relax))) return rc;
@@ -745,6 +818,29 @@ int asd_plan_get_timing(asd_plan *p, int stage, double *total_ms, long *count)
     ASD_ARG(stage >= 0 && stage < ST_N, "stage out of range");
     *total_ms = p->stage_ms[stage];
     *count = p->stage_cnt[stage];
+    return ASD_OK;
+}
+
+int asd_plan_set_atx_variant(asd_plan *p, int atx_variant)
+{
+    clear_error();
+    ASD_ARG(p, "NULL plan");
+    ASD_ARG(atx_variant >= -1 && atx_variant < ATX_NVARIANT, "atx_variant must be -1..3");
+    ASD_ARG(atx_variant <= ATX_T16x16x1 || p->views[0].nz > 1,
+            "the 3-D tile variants need nz > 1");
+    drop_graph(p);
+    p->atx_variant = atx_variant;
+    for (int i = 0; i < ATX_NVARIANT; i++) p->atx_us[i] = -1.0;
+    return ASD_OK;
+}
+
+int asd_plan_tuning(asd_plan *p, int *atx_variant, double *atx_us)
+{
+    clear_error();
+    ASD_ARG(p, "NULL plan");
+    if (atx_variant) *atx_variant = p->atx_variant;
+    if (atx_us)
+        for (int i = 0; i < ATX_NVARIANT; i++) atx_us[i] = p->atx_variant >= 0 ? p->atx_us[i] : -1.0;
     return ASD_OK;
 }
 

@@ -283,6 +283,42 @@ __host__ __device__ __forceinline__ float find_l_3d_rec(const RayRec &R, float p
     }
 }
 
+// Whether the in-plane line of a ray may meet the voxel's square: one of the
+// four in-plane face conditions of find_l_3d holds within a margin of 1e-2
+// voxel.  When none does, the line misses the square by more than every
+// rounding difference between the face tests (~1e-4), so every face test of
+// find_l_3d fails and the length is exactly 0: skipping such a (voxel, ray)
+// pair leaves the sums bit-identical (adding 0 changes nothing).  The
+// a == 0 / b == 0 fallbacks ignore a coordinate and are never skipped.
+__host__ __device__ __forceinline__ bool inplane_may_hit(const RayRec &R, float px, float py)
+{
+#pragma clang fp contract(off)
+    if (R.mode != RAY_X_FIRST && R.mode != RAY_Y_FIRST) return true;
+    const float lim = 0.5f + 1e-2f;
+    const float xy_c = R.s_ba * (-px) + py, xy_o = R.s_ba * 0.5f;
+    const float yx_c = R.s_ab * (-py) + px, yx_o = R.s_ab * 0.5f;
+    return absf(xy_c - xy_o) <= lim || absf(xy_c + xy_o) <= lim || absf(yx_c - yx_o) <= lim ||
+           absf(yx_c + yx_o) <= lim;
+}
+
+// Bit k set when ray a_lo + k may meet the voxel (k < 32; wider footprints
+// keep every ray beyond the first 32).  rec(ia) returns the record of ray ia
+// in any detector row (s_ba, s_ab and mode depend on ia only).
+template <class RecAt>
+__host__ __device__ __forceinline__ unsigned inplane_mask(const RecAt &rec, int a_lo, int a_hi,
+                                                          float px, float py)
+{
+    unsigned m = 0;
+    for (int ia = a_lo; ia <= a_hi && ia < a_lo + 32; ia++)
+        if (inplane_may_hit(rec(ia), px, py)) m |= 1u << (ia - a_lo);
+    return m;
+}
+
+__host__ __device__ __forceinline__ bool mask_keeps(unsigned m, int k)
+{
+    return k >= 32 || ((m >> k) & 1u);
+}
+
 // Slope record of ray (ia, ib) of view g (the divisions of cu:142-196).
 __host__ __device__ __forceinline__ RayRec make_ray_rec(const ViewGeom &g, float ydet, float zdet)
 {

@@ -3,23 +3,28 @@
 // sum becomes (plain y/sr, or the SART correction / image update).
 #pragma once
 
+#include <type_traits>
+
 #include "geom_device.hpp"
 #include "projector_tile.hpp"
 
 namespace asd {
 
 // Forward projector: LPR lanes march one ray together (lane j takes slabs
-// j, j+LPR, ...), then a butterfly over the LPR lanes sums their partials.
-// With LPR = 1 a thread walks its ray in slab order like the reference kernel
-// (cu:375, :576); with LPR = 64 a whole wave serves one ray, which keeps a 2-D
-// view (a few hundred rays) spread over the chip, and lanes walking along x
-// read consecutive voxels.
+// j, j+LPR, ...), then a butterfly over the lanes of a wave and, when a ray
+// spans several waves (LPR = 128, 256), a fixed-order sum of the wave
+// partials through LDS.  With LPR = 1 a thread walks its ray in slab order
+// like the reference kernel (cu:375, :576); a 2-D view (a few hundred rays)
+// gets up to four waves per ray so it still fills the chip, and lanes
+// walking along x read consecutive voxels.
 template <int LPR, bool KAHAN, class Epi>
 __global__ __launch_bounds__(256) void ax_kernel(ViewGeom g, const float *__restrict__ X,
                                                  const float *__restrict__ y_det,
                                                  const float *__restrict__ z_det, Epi epi)
 {
-    constexpr int RPB = 256 / LPR;  // rays per block
+    constexpr int RPB = LPR >= 256 ? 1 : 256 / LPR;  // rays per block
+    constexpr int WL = LPR < 64 ? LPR : 64;          // lanes of a ray inside one wave
+    constexpr int WPR = LPR > 64 ? LPR / 64 : 1;     // waves per ray
     if (epi.skip()) return;
     const long nrays = (long)g.na * g.nb;
     const int lane = threadIdx.x % LPR;
@@ -33,29 +38,36 @@ __global__ __launch_bounds__(256) void ax_kernel(ViewGeom g, const float *__rest
         for (int iu0 = lane; iu0 < nu; iu0 += 4 * LPR)
             ax_slabs<4, KAHAN>(r, X, iu0, LPR, nu, g.nx, g.ny, sa, sl);
     }
-    float acc, accl;
-    if constexpr (LPR > 1 && KAHAN) {
-        // compensated lane partials are combined in FP64
-        double a = (double)sa.s - (double)sa.c, b = (double)sl.s - (double)sl.c;
+    // compensated partials are combined in FP64, plain ones in FP32
+    using R = typename std::conditional<KAHAN, double, float>::type;
+    R a = KAHAN ? (R)sa.s - (R)sa.c : (R)sa.s;
+    R b = KAHAN ? (R)sl.s - (R)sl.c : (R)sl.s;
+    if constexpr (WL > 1) {
 #pragma unroll
-        for (int off = LPR / 2; off > 0; off >>= 1) {
-            a += __shfl_xor(a, off, LPR);
-            b += __shfl_xor(b, off, LPR);
+        for (int off = WL / 2; off > 0; off >>= 1) {
+            a += __shfl_xor(a, off, WL);
+            b += __shfl_xor(b, off, WL);
         }
-        acc = (float)a;
-        accl = (float)b;
-    } else {
-        acc = sa.value();
-        accl = sl.value();
-        if constexpr (LPR > 1) {
+    }
+    if constexpr (WPR > 1) {
+        __shared__ R sp[WPR][2];
+        const int w = threadIdx.x / 64;
+        if ((threadIdx.x & 63) == 0) {
+            sp[w][0] = a;
+            sp[w][1] = b;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            a = sp[0][0];
+            b = sp[0][1];
 #pragma unroll
-            for (int off = LPR / 2; off > 0; off >>= 1) {
-                acc += __shfl_xor(acc, off, LPR);
-                accl += __shfl_xor(accl, off, LPR);
+            for (int k = 1; k < WPR; k++) {
+                a += sp[k][0];
+                b += sp[k][1];
             }
         }
     }
-    if (active && lane == 0) epi(id, acc, accl);
+    if (active && lane == 0) epi(id, (float)a, (float)b);
 }
 
 template <bool KAHAN = false, class Epi>
@@ -71,7 +83,9 @@ int dispatch_ax(int lpr, const ViewGeom &g, const float *x, const float *y_det,
     case 8: ax_kernel<8, KAHAN, Epi><<<blocks(8), 256, 0, s>>>(g, x, y_det, z_det, epi); break;
     case 16: ax_kernel<16, KAHAN, Epi><<<blocks(16), 256, 0, s>>>(g, x, y_det, z_det, epi); break;
     case 32: ax_kernel<32, KAHAN, Epi><<<blocks(32), 256, 0, s>>>(g, x, y_det, z_det, epi); break;
-    default: ax_kernel<64, KAHAN, Epi><<<blocks(64), 256, 0, s>>>(g, x, y_det, z_det, epi); break;
+    case 64: ax_kernel<64, KAHAN, Epi><<<blocks(64), 256, 0, s>>>(g, x, y_det, z_det, epi); break;
+    case 128: ax_kernel<128, KAHAN, Epi><<<blocks(128), 256, 0, s>>>(g, x, y_det, z_det, epi); break;
+    default: ax_kernel<256, KAHAN, Epi><<<blocks(256), 256, 0, s>>>(g, x, y_det, z_det, epi); break;
     }
     ASD_LAUNCH_CHECK("ax_kernel");
     return ASD_OK;
@@ -99,13 +113,19 @@ __global__ __launch_bounds__(256) void atx_kernel(ViewGeom g, const float *__res
     const float px = sx - f.xc, py = sy - f.yc, pz = 0.0f - f.zc;
     const float hz = g.dz / 2.0f;
     Acc<KAHAN> acc, accl;
-    for (int ib = f.b_lo; ib <= f.b_hi; ib++) {
-        const long row = (long)ib * g.na;
-        for (int ia = f.a_lo; ia <= f.a_hi; ia++) {
-            RayRec R = rt[row + ia];
-            float l = find_l_3d_rec(R, px, py, pz, hz, g.dz);
-            acc.add(l * Y[row + ia]);
-            accl.add(l);
+    if (f.a_lo <= f.a_hi && f.b_lo <= f.b_hi) {
+        const RayRec *row0 = rt + (long)f.b_lo * g.na;
+        const unsigned mask =
+            inplane_mask([&](int ia) { return row0[ia]; }, f.a_lo, f.a_hi, px, py);
+        for (int ib = f.b_lo; ib <= f.b_hi; ib++) {
+            const long row = (long)ib * g.na;
+            for (int ia = f.a_lo; ia <= f.a_hi; ia++) {
+                if (!mask_keeps(mask, ia - f.a_lo)) continue;
+                RayRec R = rt[row + ia];
+                float l = find_l_3d_rec(R, px, py, pz, hz, g.dz);
+                acc.add(l * Y[row + ia]);
+                accl.add(l);
+            }
         }
     }
     epi(v, acc.value(), accl.value());
@@ -137,22 +157,9 @@ __global__ __launch_bounds__(256) void atx_tile_plain_kernel(ViewGeom g,
     const int lx = threadIdx.x % TX, ly = (threadIdx.x / TX) % TY, lz = threadIdx.x / (TX * TY);
     const int ix = tile_x * TX + lx, iy = tile_y * TY + ly, iz = tile_z * TZ + lz;
     const bool valid = ix < g.nx && iy < g.ny && iz < g.nz;
-    if (threadIdx.x == 0) {
-        s_rng[0] = 1 << 30; s_rng[1] = -(1 << 30);
-        s_rng[2] = 1 << 30; s_rng[3] = -(1 << 30);
-    }
-    __syncthreads();
     Footprint f;
-    if (valid) {
-        f = footprint(g, ix, iy, iz);
-        if (f.a_lo <= f.a_hi && f.b_lo <= f.b_hi) {
-            atomicMin(&s_rng[0], f.a_lo);
-            atomicMax(&s_rng[1], f.a_hi);
-            atomicMin(&s_rng[2], f.b_lo);
-            atomicMax(&s_rng[3], f.b_hi);
-        }
-    }
-    __syncthreads();
+    if (valid) f = footprint(g, ix, iy, iz);
+    tile_union(valid && f.a_lo <= f.a_hi && f.b_lo <= f.b_hi, f, s_rng);
     const int A0 = s_rng[0], A1 = s_rng[1], B0 = s_rng[2], B1 = s_rng[3];
     const float sx = g.cos_phi * (-g.SO), sy = g.sin_phi * (-g.SO);
     float px = 0.0f, py = 0.0f, pz = 0.0f;
@@ -165,11 +172,17 @@ __global__ __launch_bounds__(256) void atx_tile_plain_kernel(ViewGeom g,
     Acc<KAHAN> acc, accl;
     if (A0 <= A1 && B0 <= B1) {
         const int nA = A1 - A0 + 1;
+        unsigned mask = 0;
+        if (valid && f.a_lo <= f.a_hi && f.b_lo <= f.b_hi) {
+            const RayRec *row0 = rt + (long)f.b_lo * g.na;
+            mask = inplane_mask([&](int ia) { return row0[ia]; }, f.a_lo, f.a_hi, px, py);
+        }
         if (nA > TILE_MAXR) {
             // footprint too wide for LDS: read the records directly
             if (valid)
                 for (int ib = f.b_lo; ib <= f.b_hi; ib++)
                     for (int ia = f.a_lo; ia <= f.a_hi; ia++) {
+                        if (!mask_keeps(mask, ia - f.a_lo)) continue;
                         const long k = (long)ib * g.na + ia;
                         float l = find_l_3d_rec(rt[k], px, py, pz, hz, g.dz);
                         acc.add(l * Y[k]);
@@ -192,6 +205,7 @@ __global__ __launch_bounds__(256) void atx_tile_plain_kernel(ViewGeom g,
                     for (int ib = blo; ib <= bhi; ib++) {
                         const int row = (ib - b0) * nA - A0;
                         for (int ia = f.a_lo; ia <= f.a_hi; ia++) {
+                            if (!mask_keeps(mask, ia - f.a_lo)) continue;
                             float l = find_l_3d_rec(s_rec[row + ia], px, py, pz, hz, g.dz);
                             acc.add(l * s_y[row + ia]);
                             accl.add(l);
@@ -204,22 +218,41 @@ __global__ __launch_bounds__(256) void atx_tile_plain_kernel(ViewGeom g,
     if (valid) epi((long)((size_t)iz * g.ny + iy) * g.nx + ix, acc.value(), accl.value());
 }
 
-// Tile shape by volume depth: 16x16 slices for 2-D, 8x8x4 bricks for 3-D.
-template <bool KAHAN, class Epi>
-int dispatch_atx_tile(const ViewGeom &g, const float *Y, const RayRec *rt, const Epi &epi,
+// Backprojector variants (all bit-identical; the plan's autotuner picks one).
+enum : int {
+    ATX_DIRECT = 0,   // one thread per voxel, records from L1/L2
+    ATX_T16x16x1 = 1, // LDS-staged 16x16 slices
+    ATX_T8x8x4 = 2,   // LDS-staged 8x8x4 bricks
+    ATX_T16x4x4 = 3,  // LDS-staged 16x4x4 bricks
+    ATX_NVARIANT = 4,
+};
+
+template <int TX, int TY, int TZ, bool KAHAN, class Epi>
+int launch_atx_tile_t(const ViewGeom &g, const float *Y, const RayRec *rt, const Epi &epi,
                       hipStream_t s)
 {
-    if (g.nz == 1) {
-        const TileDims td = tile_dims(g, 16, 16, 1);
-        atx_tile_plain_kernel<16, 16, 1, KAHAN, Epi>
-            <<<(unsigned)td.ntx * td.nty * td.ntz, 256, 0, s>>>(g, Y, rt, epi);
-    } else {
-        const TileDims td = tile_dims(g, 8, 8, 4);
-        atx_tile_plain_kernel<8, 8, 4, KAHAN, Epi>
-            <<<(unsigned)td.ntx * td.nty * td.ntz, 256, 0, s>>>(g, Y, rt, epi);
-    }
+    const TileDims td = tile_dims(g, TX, TY, TZ);
+    atx_tile_plain_kernel<TX, TY, TZ, KAHAN, Epi>
+        <<<(unsigned)td.ntx * td.nty * td.ntz, 256, 0, s>>>(g, Y, rt, epi);
     ASD_LAUNCH_CHECK("atx_tile_plain_kernel");
     return ASD_OK;
+}
+
+template <bool KAHAN, class Epi>
+int dispatch_atx_variant(int variant, const ViewGeom &g, const float *Y, const RayRec *rt,
+                         const Epi &epi, hipStream_t s)
+{
+    switch (variant) {
+    case ATX_T16x16x1: return launch_atx_tile_t<16, 16, 1, KAHAN>(g, Y, rt, epi, s);
+    case ATX_T8x8x4: return launch_atx_tile_t<8, 8, 4, KAHAN>(g, Y, rt, epi, s);
+    case ATX_T16x4x4: return launch_atx_tile_t<16, 4, 4, KAHAN>(g, Y, rt, epi, s);
+    default: {
+        const long N = (long)g.nx * g.ny * g.nz;
+        atx_kernel<Epi, KAHAN><<<(unsigned)((N + 255) / 256), 256, 0, s>>>(g, Y, rt, epi);
+        ASD_LAUNCH_CHECK("atx_kernel");
+        return ASD_OK;
+    }
+    }
 }
 
 // Precision-variant backprojector (tile-local FP64 ray setup, see

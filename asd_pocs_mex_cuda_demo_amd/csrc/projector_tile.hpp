@@ -135,6 +135,34 @@ __host__ __device__ __forceinline__ float tile_find_l(const TileRayXY &r, const 
 
 namespace asd {
 
+// Union of the voxels' footprints over the workgroup: wave butterflies, then
+// one LDS atomic per wave and bound (rng = a_lo, a_hi, b_lo, b_hi; empty when
+// lo > hi).  All threads of the block must call it.
+__device__ __forceinline__ void tile_union(bool has, const Footprint &f, int *rng)
+{
+    int v0 = has ? f.a_lo : (1 << 30), v1 = has ? f.a_hi : -(1 << 30);
+    int v2 = has ? f.b_lo : (1 << 30), v3 = has ? f.b_hi : -(1 << 30);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        v0 = min(v0, __shfl_xor(v0, off, 64));
+        v1 = max(v1, __shfl_xor(v1, off, 64));
+        v2 = min(v2, __shfl_xor(v2, off, 64));
+        v3 = max(v3, __shfl_xor(v3, off, 64));
+    }
+    if (threadIdx.x == 0) {
+        rng[0] = 1 << 30; rng[1] = -(1 << 30);
+        rng[2] = 1 << 30; rng[3] = -(1 << 30);
+    }
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) {
+        atomicMin(&rng[0], v0);
+        atomicMax(&rng[1], v1);
+        atomicMin(&rng[2], v2);
+        atomicMax(&rng[3], v3);
+    }
+    __syncthreads();
+}
+
 constexpr int TILE_MAXA = 256;   // in-plane ray records per chunk
 constexpr int TILE_MAXR = 1024;  // (ia, ib) records per chunk
 
@@ -192,22 +220,9 @@ __global__ __launch_bounds__(256) void atx_tile_kernel(ViewGeom g, const float *
     const int lx = threadIdx.x % TX, ly = (threadIdx.x / TX) % TY, lz = threadIdx.x / (TX * TY);
     const int ix = x0 + lx, iy = y0 + ly, iz = z0 + lz;
     const bool valid = ix < g.nx && iy < g.ny && iz < g.nz;
-    if (threadIdx.x == 0) {
-        s_rng[0] = 1 << 30; s_rng[1] = -(1 << 30);
-        s_rng[2] = 1 << 30; s_rng[3] = -(1 << 30);
-    }
-    __syncthreads();
     Footprint f;
-    if (valid) {
-        f = footprint(g, ix, iy, iz);
-        if (f.a_lo <= f.a_hi && f.b_lo <= f.b_hi) {
-            atomicMin(&s_rng[0], f.a_lo);
-            atomicMax(&s_rng[1], f.a_hi);
-            atomicMin(&s_rng[2], f.b_lo);
-            atomicMax(&s_rng[3], f.b_hi);
-        }
-    }
-    __syncthreads();
+    if (valid) f = footprint(g, ix, iy, iz);
+    tile_union(valid && f.a_lo <= f.a_hi && f.b_lo <= f.b_hi, f, s_rng);
     const int A0 = s_rng[0], A1 = s_rng[1], B0 = s_rng[2], B1 = s_rng[3];
     double xtc, ytc, ztc;
     tile_centre(g, x0, y0, z0, TX, TY, TZ, xtc, ytc, ztc);

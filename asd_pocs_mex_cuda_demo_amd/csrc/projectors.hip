@@ -49,11 +49,11 @@ __global__ __launch_bounds__(256) void ray_table_kernel(ViewGeom g,
 int ax_lanes_per_ray(int na, int nb, int nu)
 {
     // Aim for about 2^20 working lanes (4096 waves over 256 CUs) while keeping
-    // at least 4 slabs per lane; a 2-D view (a few hundred rays) gets a full
-    // wave per ray.
+    // at least 2 slabs per lane; a 2-D view (a few hundred rays) gets up to
+    // four waves per ray.
     long rays = (long)na * nb;
     int lpr = 1;
-    while (lpr < 64 && rays * lpr < (1L << 20) && nu / (2 * lpr) >= 4) lpr *= 2;
+    while (lpr < 256 && rays * lpr < (1L << 20) && nu / (2 * lpr) >= 2) lpr *= 2;
     return lpr;
 }
 
@@ -78,7 +78,7 @@ int launch_ax(const ViewGeom &g, const float *x, float *y, float *sr, const floa
 int launch_atx(const ViewGeom &g, float *x, const float *y, float *sc, const float *y_det,
                const float *z_det, const RayRec *rt, hipStream_t s)
 {
-    return dispatch_atx_tile<false>(g, y, rt, EpiBackPlain{x, sc, g.scale}, s);
+    return dispatch_atx_variant<false>(ATX_DIRECT, g, y, rt, EpiBackPlain{x, sc, g.scale}, s);
 }
 
 // ---------------------------------------------------------------------------

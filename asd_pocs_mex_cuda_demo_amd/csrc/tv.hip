@@ -11,23 +11,38 @@ namespace asd {
 
 namespace {
 
-template <class T, class A>
-__device__ __forceinline__ A ld(const T *u, int nx, int ny, int nz, int x, int y, int z)
-{
-    if (x < 0 || x >= nx || y < 0 || y >= ny || z < 0 || z >= nz) return A(0);
-    return A(u[((size_t)z * ny + y) * nx + x]);
-}
+// Image values as the stencil sees them: zero outside the image, and with
+// STEPPED the previous sub-iteration's step applied on the fly,
+// u_i = T(u_{i-1} - s * v_{i-1}), s = alpha * d / ||v_{i-1}|| (the value the
+// unfused step kernel would have stored).
+template <class T, class A, bool STEPPED>
+struct TvSrc {
+    const T *u, *v;
+    double s;
+    int nx, ny, nz;
+    __device__ __forceinline__ T stored(size_t i) const
+    {
+#pragma clang fp contract(off)
+        if constexpr (STEPPED) return T((double)u[i] - s * (double)v[i]);
+        else return u[i];
+    }
+    __device__ __forceinline__ A operator()(int x, int y, int z) const
+    {
+        if (x < 0 || x >= nx || y < 0 || y >= ny || z < 0 || z >= nz) return A(0);
+        return A(stored(((size_t)z * ny + y) * nx + x));
+    }
+};
 
 // 2-D stencil at row r (fastest) and column c (imgTVGradMin.cpp:84-97).
-template <class T, class A>
-__device__ __forceinline__ A tv2_grad(const T *u, int R, int C, int r, int c)
+template <class A, class Src>
+__device__ __forceinline__ A tv2_grad(const Src &U, int r, int c)
 {
 #pragma clang fp contract(off)
     const A eps = A(1e-8);
-    A p = ld<T, A>(u, R, C, 1, r, c, 0);
-    A rm = ld<T, A>(u, R, C, 1, r - 1, c, 0), cm = ld<T, A>(u, R, C, 1, r, c - 1, 0);
-    A rp = ld<T, A>(u, R, C, 1, r + 1, c, 0), rp_cm = ld<T, A>(u, R, C, 1, r + 1, c - 1, 0);
-    A cp = ld<T, A>(u, R, C, 1, r, c + 1, 0), rm_cp = ld<T, A>(u, R, C, 1, r - 1, c + 1, 0);
+    A p = U(r, c, 0);
+    A rm = U(r - 1, c, 0), cm = U(r, c - 1, 0);
+    A rp = U(r + 1, c, 0), rp_cm = U(r + 1, c - 1, 0);
+    A cp = U(r, c + 1, 0), rm_cp = U(r - 1, c + 1, 0);
     A g1 = ((p - rm) + (p - cm)) / sqrt(eps + (p - rm) * (p - rm) + (p - cm) * (p - cm));
     A g2 = -(rp - p) / sqrt(eps + (rp - p) * (rp - p) + (rp - rp_cm) * (rp - rp_cm));
     A g3 = -(cp - p) / sqrt(eps + (cp - p) * (cp - p) + (cp - rm_cp) * (cp - rm_cp));
@@ -35,12 +50,11 @@ __device__ __forceinline__ A tv2_grad(const T *u, int R, int C, int r, int c)
 }
 
 // 3-D stencil at (x,y,z), x fastest (imgTVGradMin3D.cpp:114-140).
-template <class T, class A>
-__device__ __forceinline__ A tv3_grad(const T *u, int nx, int ny, int nz, int x, int y, int z)
+template <class A, class Src>
+__device__ __forceinline__ A tv3_grad(const Src &U, int x, int y, int z)
 {
 #pragma clang fp contract(off)
     const A eps = A(1e-8);
-    auto U = [&](int a, int b, int c) { return ld<T, A>(u, nx, ny, nz, a, b, c); };
     A p = U(x, y, z);
     A xm = U(x - 1, y, z), ym = U(x, y - 1, z), zm = U(x, y, z - 1);
     A zp = U(x, y, z + 1), xm_zp = U(x - 1, y, z + 1), ym_zp = U(x, y - 1, z + 1);
@@ -57,12 +71,31 @@ __device__ __forceinline__ A tv3_grad(const T *u, int nx, int ny, int nz, int x,
     return g1 + g2 + g3 + g4;
 }
 
-template <class T, class A, bool THREE_D>
-__global__ __launch_bounds__(256) void tv_grad_kernel(const T *__restrict__ u, T *__restrict__ v,
-                                                      int nx, int ny, int nz, double *partials,
+// s = alpha * d / ||v|| of the previous sub-iteration (reference: the step
+// I -= alpha*d*v/||v||, imgTVGradMin.cpp:104-107).
+__device__ __forceinline__ double tv_step_size(double alpha, const double *alpha_dev,
+                                               const double *d2, const double *nrm2)
+{
+    const double a = alpha_dev ? *alpha_dev : alpha;
+    return a * sqrt(*d2) / sqrt(*nrm2);
+}
+
+// One TV gradient: v = grad TV(u) with FP64 block partials of ||v||^2.  With
+// STEPPED it first applies the previous step to every value it reads and
+// also stores the stepped image of its own voxel in u_next (the step fused
+// into the next gradient; u_prev / v_prev are read-only here).
+template <class T, class A, bool THREE_D, bool STEPPED>
+__global__ __launch_bounds__(256) void tv_grad_kernel(const T *__restrict__ u,
+                                                      const T *__restrict__ v_prev,
+                                                      T *__restrict__ u_next, T *__restrict__ v,
+                                                      int nx, int ny, int nz, double alpha,
+                                                      const double *alpha_dev, const double *d2,
+                                                      const double *nrm2_prev, double *partials,
                                                       const int *skip)
 {
     if (skip && *skip) return;
+    TvSrc<T, A, STEPPED> U{u, v_prev, 0.0, nx, ny, nz};
+    if constexpr (STEPPED) U.s = tv_step_size(alpha, alpha_dev, d2, nrm2_prev);
     const size_t n = (size_t)nx * ny * nz;
     double s = 0.0;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -70,18 +103,22 @@ __global__ __launch_bounds__(256) void tv_grad_kernel(const T *__restrict__ u, T
         int x = (int)(i % nx);
         size_t r = i / nx;
         int y = (int)(r % ny), z = (int)(r / ny);
-        A g = THREE_D ? tv3_grad<T, A>(u, nx, ny, nz, x, y, z) : tv2_grad<T, A>(u, nx, ny, x, y);
+        A g = THREE_D ? tv3_grad<A>(U, x, y, z) : tv2_grad<A>(U, x, y);
         T gt = T(g);
         v[i] = gt;
+        if constexpr (STEPPED) u_next[i] = U.stored(i);
         s += (double)gt * (double)gt;
     }
     double val[1] = {s};
     store_partials<1>(val, partials);
 }
 
+// The last step: u_out = u - s * v, then positivity when clamp.
 template <class T>
-__global__ __launch_bounds__(256) void tv_step_kernel(T *__restrict__ u, const T *__restrict__ v,
-                                                      size_t n, double alpha,
+__global__ __launch_bounds__(256) void tv_step_kernel(const T *__restrict__ u,
+                                                      const T *__restrict__ v,
+                                                      T *__restrict__ u_out, size_t n,
+                                                      double alpha,
                                                       const double *__restrict__ alpha_dev,
                                                       const double *__restrict__ d2,
                                                       const double *__restrict__ nrm2, int clamp,
@@ -89,14 +126,12 @@ __global__ __launch_bounds__(256) void tv_step_kernel(T *__restrict__ u, const T
 {
 #pragma clang fp contract(off)
     if (skip && *skip) return;
-    const double a = alpha_dev ? *alpha_dev : alpha;
-    const double ad = a * sqrt(*d2);
-    const double nrm = sqrt(*nrm2);
+    const double st = tv_step_size(alpha, alpha_dev, d2, nrm2);
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (size_t)gridDim.x * blockDim.x) {
-        double w = (double)u[i] - ad * (double)v[i] / nrm;
+        double w = (double)u[i] - st * (double)v[i];
         if (clamp && w < 0) w = 0;
-        u[i] = T(w);
+        u_out[i] = T(w);
     }
 }
 
@@ -136,30 +171,50 @@ int launch_sqdiff(const T *a, const T *b, size_t n, TvScratch &s, int slot, hipS
 }
 
 template <class T, class A>
-int tv_descent(T *u, const T *img_old, T *v, int nx, int ny, int nz, bool three_d, double alpha,
-               const double *alpha_dev, int nTVIter, int pos, TvScratch &s, hipStream_t st,
-               bool have_d2, const int *skip)
+int tv_descent(T *u, const T *img_old, T *v, T *u2, T *v2, int nx, int ny, int nz, bool three_d,
+               double alpha, const double *alpha_dev, int nTVIter, int pos, TvScratch &s,
+               hipStream_t st, bool have_d2, const int *skip)
 {
     const size_t n = (size_t)nx * ny * nz;
     int rc;
     if (!have_d2 && (rc = launch_sqdiff<T>(u, img_old, n, s, TV_SLOT_D2, st))) return rc;
+    if (nTVIter <= 0) return ASD_OK;
     const unsigned gb = red_blocks(n), ub = stream_blocks(n);
+    double *d2 = s.scalars + TV_SLOT_D2, *nrm2 = s.scalars + TV_SLOT_NRM2;
+    // (u, v) holds sub-iteration i's image and gradient, alternating with (u2, v2)
+    T *cu = u, *cv = v, *nu = u2, *nv = v2;
     for (int it = 0; it < nTVIter; it++) {
-        if (three_d)
-            tv_grad_kernel<T, A, true><<<gb, RED_THREADS, 0, st>>>(u, v, nx, ny, nz, s.partials,
-                                                                  skip);
-        else
-            tv_grad_kernel<T, A, false><<<gb, RED_THREADS, 0, st>>>(u, v, nx, ny, 1, s.partials,
-                                                                   skip);
+        if (it == 0) {
+            if (three_d)
+                tv_grad_kernel<T, A, true, false><<<gb, RED_THREADS, 0, st>>>(
+                    cu, nullptr, nullptr, cv, nx, ny, nz, alpha, alpha_dev, d2, nrm2, s.partials,
+                    skip);
+            else
+                tv_grad_kernel<T, A, false, false><<<gb, RED_THREADS, 0, st>>>(
+                    cu, nullptr, nullptr, cv, nx, ny, 1, alpha, alpha_dev, d2, nrm2, s.partials,
+                    skip);
+        } else {
+            // step of sub-iteration it-1 fused into gradient it
+            if (three_d)
+                tv_grad_kernel<T, A, true, true><<<gb, RED_THREADS, 0, st>>>(
+                    cu, cv, nu, nv, nx, ny, nz, alpha, alpha_dev, d2, nrm2, s.partials, skip);
+            else
+                tv_grad_kernel<T, A, false, true><<<gb, RED_THREADS, 0, st>>>(
+                    cu, cv, nu, nv, nx, ny, 1, alpha, alpha_dev, d2, nrm2, s.partials, skip);
+            T *t1 = cu, *t2 = cv;
+            cu = nu;
+            cv = nv;
+            nu = t1;
+            nv = t2;
+        }
         ASD_LAUNCH_CHECK("tv_grad_kernel");
-        fold_partials<1><<<1, RED_THREADS, 0, st>>>(s.partials, (int)gb,
-                                                    s.scalars + TV_SLOT_NRM2, skip);
+        fold_partials<1><<<1, RED_THREADS, 0, st>>>(s.partials, (int)gb, nrm2, skip);
         ASD_LAUNCH_CHECK("fold_partials");
-        int clamp = (pos == 1 && it == nTVIter - 1) ? 1 : 0;
-        tv_step_kernel<T><<<ub, 256, 0, st>>>(u, v, n, alpha, alpha_dev, s.scalars + TV_SLOT_D2,
-                                              s.scalars + TV_SLOT_NRM2, clamp, skip);
-        ASD_LAUNCH_CHECK("tv_step_kernel");
     }
+    // last step (with positivity) lands in the caller's buffer u
+    tv_step_kernel<T><<<ub, 256, 0, st>>>(cu, cv, u, n, alpha, alpha_dev, d2, nrm2,
+                                          pos == 1 ? 1 : 0, skip);
+    ASD_LAUNCH_CHECK("tv_step_kernel");
     return ASD_OK;
 }
 
@@ -167,15 +222,12 @@ template int launch_sqdiff<float>(const float *, const float *, size_t, TvScratc
                                   hipStream_t);
 template int launch_sqdiff<double>(const double *, const double *, size_t, TvScratch &, int,
                                    hipStream_t);
-template int tv_descent<float, float>(float *, const float *, float *, int, int, int, bool,
-                                      double, const double *, int, int, TvScratch &,
+template int tv_descent<float, float>(float *, const float *, float *, float *, float *, int, int,
+                                      int, bool, double, const double *, int, int, TvScratch &,
                                       hipStream_t, bool, const int *);
-template int tv_descent<float, double>(float *, const float *, float *, int, int, int, bool,
-                                       double, const double *, int, int, TvScratch &,
-                                       hipStream_t, bool, const int *);
-template int tv_descent<double, double>(double *, const double *, double *, int, int, int, bool,
-                                        double, const double *, int, int, TvScratch &,
-                                        hipStream_t, bool, const int *);
+template int tv_descent<double, double>(double *, const double *, double *, double *, double *,
+                                        int, int, int, bool, double, const double *, int, int,
+                                        TvScratch &, hipStream_t, bool, const int *);
 
 int TvScratch::init()
 {
@@ -198,7 +250,7 @@ namespace {
 struct TvCtx {
     std::mutex mu;
     hipStream_t stream = nullptr;
-    DeviceBuf img, old, v;
+    DeviceBuf img, old, v, u2, v2;
     TvScratch scratch;
     bool ready = false;
     int init()
@@ -229,14 +281,17 @@ int tv_host(const double *img, int nx, int ny, int nz, bool three_d, double alph
     int rc = c.init();
     if (rc) return rc;
     size_t n = (size_t)nx * ny * nz;
-    if ((rc = c.img.ensure(n * 8)) || (rc = c.v.ensure(n * 8))) return rc;
+    if ((rc = c.img.ensure(n * 8)) || (rc = c.v.ensure(n * 8)) || (rc = c.u2.ensure(n * 8)) ||
+        (rc = c.v2.ensure(n * 8)))
+        return rc;
     if (img_old && (rc = c.old.ensure(n * 8))) return rc;
     ASD_HIP(hipMemcpyAsync(c.img.p, img, n * 8, hipMemcpyHostToDevice, c.stream));
     if (img_old)
         ASD_HIP(hipMemcpyAsync(c.old.p, img_old, n * 8, hipMemcpyHostToDevice, c.stream));
     rc = tv_descent<double, double>(c.img.as<double>(), img_old ? c.old.as<double>() : nullptr,
-                                    c.v.as<double>(), nx, ny, nz, three_d, alpha, nullptr,
-                                    nTVIter, pos, c.scratch, c.stream, false);
+                                    c.v.as<double>(), c.u2.as<double>(), c.v2.as<double>(), nx, ny,
+                                    nz, three_d, alpha, nullptr, nTVIter, pos, c.scratch, c.stream,
+                                    false);
     if (rc) return rc;
     ASD_HIP(hipMemcpyAsync(out, c.img.p, n * 8, hipMemcpyDeviceToHost, c.stream));
     ASD_HIP(hipStreamSynchronize(c.stream));
@@ -269,14 +324,15 @@ int asd_tv_min_device(double *img, const double *img_old, int nx, int ny, int nz
     ASD_ARG(img, "NULL image pointer");
     ASD_ARG(nx > 0 && ny > 0 && nz > 0, "image dimensions must be positive");
     static thread_local TvScratch scratch;
-    static thread_local DeviceBuf v;
+    static thread_local DeviceBuf v, u2, v2;
     int rc;
     if (!scratch.partials && (rc = scratch.init())) return rc;
     size_t n = (size_t)nx * ny * nz;
-    if ((rc = v.ensure(n * 8))) return rc;
-    return tv_descent<double, double>(img, img_old, v.as<double>(), nx, ny, three_d ? nz : 1,
-                                      three_d != 0, alpha, nullptr, nTVIter, pos, scratch,
-                                      (hipStream_t)stream, false);
+    if ((rc = v.ensure(n * 8)) || (rc = u2.ensure(n * 8)) || (rc = v2.ensure(n * 8))) return rc;
+    return tv_descent<double, double>(img, img_old, v.as<double>(), u2.as<double>(),
+                                      v2.as<double>(), nx, ny, three_d ? nz : 1, three_d != 0,
+                                      alpha, nullptr, nTVIter, pos, scratch, (hipStream_t)stream,
+                                      false);
 }
 
 }  // extern "C"

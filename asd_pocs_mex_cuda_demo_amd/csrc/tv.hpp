@@ -33,9 +33,11 @@ int launch_sqdiff(const T *a, const T *b, size_t n, TvScratch &s, int slot, hipS
 // which case scalars[TV_SLOT_D2] already holds it.  alpha_dev, when not
 // NULL, overrides alpha from device memory; a non-zero *skip (device flag,
 // may be NULL) makes every kernel return at entry.
+// v, u2, v2: n-element work buffers (u2/v2 hold every other sub-iteration:
+// each step is fused into the next gradient).
 template <class T, class A>
-int tv_descent(T *u, const T *img_old, T *v, int nx, int ny, int nz, bool three_d, double alpha,
-               const double *alpha_dev, int nTVIter, int pos, TvScratch &s, hipStream_t st,
-               bool have_d2, const int *skip = nullptr);
+int tv_descent(T *u, const T *img_old, T *v, T *u2, T *v2, int nx, int ny, int nz, bool three_d,
+               double alpha, const double *alpha_dev, int nTVIter, int pos, TvScratch &s,
+               hipStream_t st, bool have_d2, const int *skip = nullptr);
 
 }  // namespace asd

@@ -287,6 +287,17 @@ class Plan:
         nat.check(self._L.asd_plan_get_timing(self.h, int(stage), C.byref(ms), C.byref(cnt)))
         return ms.value, cnt.value
 
+    def set_atx_variant(self, v):
+        nat.check(self._L.asd_plan_set_atx_variant(self.h, int(v)))
+
+    def tuning(self):
+        v = C.c_int()
+        us = np.zeros(4)
+        nat.check(self._L.asd_plan_tuning(self.h, C.byref(v), _ptr(us)))
+        names = ["direct", "tile16x16x1", "tile8x8x4", "tile16x4x4"]
+        return {"atx_variant": names[v.value] if v.value >= 0 else None,
+                "atx_us": {n: float(t) for n, t in zip(names, us) if t >= 0}}
+
     def info(self):
         lpr = C.c_int()
         st = C.c_void_p()

@@ -279,7 +279,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" if args.precision == 0 else "f32 (fp16-stored sinogram)",
+        "dtype": {0: "f32", 1: "f32 (16-bit fixed-point sinogram, f64 sums)",
+                  2: "f32 (fp16 sinogram, f64 sums)"}[args.precision],
         "data": f"synthetic: modified Shepp-Logan phantom, Poisson noise I0={w['I0']:g} (seeded), log transform",
         "config": {"workload": args.workload, "nx": para["nx"], "ny": para["ny"],
                    "nz": para["nz"], "views": w["nv"], "na": para["na"], "nb": para["nb"],
@@ -288,6 +289,8 @@ def main():
         "stages": per_stage,
         "ms_per_step_eager_with_events": dt_ev / prof_steps * 1e3,
         "rmse_final": float(rmse[-1]) if len(rmse) else None,
+        "ax_lanes_per_ray": plan.info()[0],
+        "atx_tuning": plan.tuning(),
     }
     if not args.no_cpu:
         views = args.cpu_views if args.cpu_views is not None else (0 if para["nz"] == 1 else 2)

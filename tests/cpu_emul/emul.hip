@@ -86,8 +86,14 @@ void emul_atx(float *X, const float *y, float *sc, float c, float s, const float
         float px = sx - f.xc, py = sy - f.yc, pz = 0.0f - f.zc;
         float hz = g.dz / 2.0f;
         Acc<false> acc, accl;
+        unsigned mask = 0;
+        if (f.a_lo <= f.a_hi && f.b_lo <= f.b_hi) {
+            const RayRec *row0 = rt + (long)f.b_lo * na;
+            mask = inplane_mask([&](int ia) { return row0[ia]; }, f.a_lo, f.a_hi, px, py);
+        }
         for (int ib = f.b_lo; ib <= f.b_hi; ib++)
             for (int ia = f.a_lo; ia <= f.a_hi; ia++) {
+                if (!mask_keeps(mask, ia - f.a_lo)) continue;
                 float l = find_l_3d_rec(rt[(long)ib * na + ia], px, py, pz, hz, g.dz);
                 acc.add(l * y[(long)ib * na + ia]);
                 accl.add(l);

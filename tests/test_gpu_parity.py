@@ -167,6 +167,39 @@ def test_graph_replays_match_eager_iterations():
     graph.close()
 
 
+@pytest.mark.parametrize("three_d", [False, True], ids=["fan", "cone"])
+def test_backprojector_variants_are_bit_identical(three_d):
+    """The direct and LDS-tiled backprojectors (the variants the plan's
+    autotuner chooses between) give bit-identical iterations, and the
+    autotuner reports a measured time for every variant it tried."""
+    if three_d:
+        para, img, P = _small_problem(n=24, nv=8, na=40, nz=8, nb=20)
+        variants = [0, 1, 2, 3]
+    else:
+        para, img, P = _small_problem(n=48, nv=12, na=72)
+        variants = [0, 1]
+    para.update(maxnumIter=3, alpha=0.2, numOfgradients=5)
+    ref = None
+    for v in [-1] + variants:
+        pl = ops.Plan(para, para["phi_vec"])
+        pl.set_projections(P)
+        pl.reset(None, img)
+        if v >= 0:
+            pl.set_atx_variant(v)
+        pl.iterate(3)
+        t = pl.tuning()
+        img_v, (o, r) = pl.image(), pl.out()
+        pl.close()
+        if v < 0:
+            assert t["atx_variant"] is not None and len(t["atx_us"]) == len(variants)
+            assert all(us > 0 for us in t["atx_us"].values())
+        if ref is None:
+            ref = (img_v, o, r)
+        else:
+            assert np.array_equal(img_v, ref[0]), v
+            assert o == ref[1] and np.array_equal(r, ref[2]), v
+
+
 def test_stop_rule_ends_the_loop():
     """stopRelChg large: the first iteration already stops; m_res is the
     post-SART image and IterationNum = 1 (ASD_POCS_cuda.m:146-149)."""
@@ -179,15 +212,18 @@ def test_stop_rule_ends_the_loop():
 
 
 def test_precision_variant_matches_plain_rmse():
-    """16-bit fixed-point sinogram with compensated FP32 sums (precision 1):
-    the reconstruction RMSE stays within 1e-4 relative of the plain FP32 form
-    on noisy data.  FP16 storage (precision 2, 11-bit significand) moves it
-    by ~1e-3 and is kept only for comparison."""
+    """16-bit fixed-point sinogram, compensated sums and the tile-local FP64
+    backprojector (precision 1): the reconstruction RMSE stays within 1e-4
+    relative of the plain FP32 form on noisy data.  r_max is raised so the
+    alpha rule (a discrete decision that any perturbation can flip) never
+    fires and the comparison measures arithmetic only.  FP16 storage
+    (precision 2, 11-bit significand) is looser and kept for comparison."""
     para, img, P = _small_problem(n=64, nv=24, na=96)
     Pn = phantoms.poisson_log(P, 1e5, seed=7)
-    para.update(maxnumIter=5, alpha=0.2, numOfgradients=10, I0=img)
+    para.update(maxnumIter=5, alpha=0.2, numOfgradients=10, I0=img, r_max=1e9)
     m0, o0 = ops.ASD_POCS_cuda(Pn, para)
     m1, o1 = ops.ASD_POCS_cuda(Pn, dict(para, precision=1))
+    assert o0["alpha"] == o1["alpha"] == 0.2
     assert np.all(np.abs(o1["RMSE"] - o0["RMSE"]) <= 1e-4 * o0["RMSE"]), (o0["RMSE"], o1["RMSE"])
     assert rel_l2(m1, m0) <= 1e-3
     m2, o2 = ops.ASD_POCS_cuda(Pn, dict(para, precision=2))

@@ -134,7 +134,7 @@ def test_asd_pocs_gateway_on_gpu(orc):
     o = outs[1]
     assert m.shape == (n, n)  # MATLAB drops the trailing singleton nz = 1
     m_py, out_py = ops.ASD_POCS_cuda(P, para)
-    assert np.array_equal(m, m_py)
+    assert np.array_equal(m.ravel(order="F"), m_py.ravel(order="F"))
     assert host.field(o, "IterationNum")[0, 0] == 3
     np.testing.assert_allclose(host.field(o, "RMSE").ravel(), out_py["RMSE"], rtol=1e-12)
     assert host.field(o, "alpha")[0, 0] == out_py["alpha"]
