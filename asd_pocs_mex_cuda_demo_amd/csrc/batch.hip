@@ -160,6 +160,7 @@ __global__ __launch_bounds__(256) void atx_batch_kernel(const BItem *__restrict_
     const long v = (long)(b - bpref[c]) * 256 + threadIdx.x;
     const long N = (long)g.nx * g.ny * g.nz;
     if (v >= N) return;
+    const float m0 = it.m[v];
     const int ix = (int)(v % g.nx);
     const long r = v / g.nx;
     const int iy = (int)(r % g.ny), iz = (int)(r / g.ny);
@@ -170,16 +171,27 @@ __global__ __launch_bounds__(256) void atx_batch_kernel(const BItem *__restrict_
     Acc<KAHAN> acc, accl;
     for (int ib = f.b_lo; ib <= f.b_hi; ib++) {
         const long row = (long)ib * g.na;
-        for (int ia = f.a_lo; ia <= f.a_hi; ia++) {
-            RayRec R = it.rt[row + ia];
-            float l = find_l_3d_rec(R, px, py, pz, hz, g.dz);
-            acc.add(l * it.cor[row + ia]);
-            accl.add(l);
+        for (int a0 = f.a_lo; a0 <= f.a_hi; a0 += ATX_CHUNK) {
+            RayRec R[ATX_CHUNK];
+            float yv[ATX_CHUNK];
+#pragma unroll
+            for (int k = 0; k < ATX_CHUNK; k++) {
+                const int ia = a0 + k <= f.a_hi ? a0 + k : f.a_hi;
+                R[k] = it.rt[row + ia];
+                yv[k] = it.cor[row + ia];
+            }
+#pragma unroll
+            for (int k = 0; k < ATX_CHUNK; k++) {
+                if (a0 + k > f.a_hi || !inplane_may_hit(R[k], px, py)) continue;
+                float l = find_l_3d_rec(R[k], px, py, pz, hz, g.dz);
+                acc.add(l * yv[k]);
+                accl.add(l);
+            }
         }
     }
     float bp = acc.value() * g.scale, sc = accl.value() * g.scale;
     double cc = (sc == 0.0f) ? 1.0 : (double)sc;
-    double nm = (double)it.m[v] + it.ctl->beta * ((double)bp / cc);
+    double nm = (double)m0 + it.ctl->beta * ((double)bp / cc);
     it.m[v] = nm < 0 ? 0.0f : (float)nm;
 }
 

@@ -53,21 +53,22 @@ struct EpiSartAx {
     Codec P;              // this view's measured projection
     float *cor;           // correction fed to the backprojector
     double *res2;         // per-ray accumulated squared residual of the sweep
-    RayRec *rt;           // ray-slope table of this view (for atx_sart)
-    const float *y_det, *z_det;
     const int *stopped;
-    ViewGeom g;
+    float scale;
+    struct Pre {
+        double p, r2;
+    };
     __device__ __forceinline__ bool skip() const { return *stopped != 0; }
-    __device__ __forceinline__ void operator()(long id, float acc, float accl) const
+    // issued before the ray march so the loads overlap it
+    __device__ __forceinline__ Pre load(long id) const { return Pre{P.at(id), res2[id]}; }
+    __device__ __forceinline__ void operator()(long id, float acc, float accl, const Pre &q) const
     {
 #pragma clang fp contract(off)
-        float proj = acc * g.scale, sumr = accl * g.scale;
+        float proj = acc * scale, sumr = accl * scale;
         double r = (sumr == 0.0f) ? 1.0 : (double)sumr;
-        double c0 = P.at(id) - (double)proj;
-        res2[id] += c0 * c0;
+        double c0 = q.p - (double)proj;
+        res2[id] = q.r2 + c0 * c0;
         cor[id] = (float)(c0 / r);
-        int ia = (int)(id % g.na), ib = (int)(id / g.na);
-        rt[id] = make_ray_rec(g, y_det[ia], z_det[ib]);
     }
 };
 
@@ -76,13 +77,17 @@ struct EpiSartAtx {
     const double *relax;  // beta of the ASD loop or lambda of a stand-alone sweep
     const int *stopped;
     float scale;
+    struct Pre {
+        float m;
+    };
     __device__ __forceinline__ bool skip() const { return *stopped != 0; }
-    __device__ __forceinline__ void operator()(long v, float acc, float accl) const
+    __device__ __forceinline__ Pre load(long v) const { return Pre{m[v]}; }
+    __device__ __forceinline__ void operator()(long v, float acc, float accl, const Pre &q) const
     {
 #pragma clang fp contract(off)
         float bp = acc * scale, sc = accl * scale;
         double c = (sc == 0.0f) ? 1.0 : (double)sc;
-        double nm = (double)m[v] + (*relax) * ((double)bp / c);
+        double nm = (double)q.m + (*relax) * ((double)bp / c);
         m[v] = nm < 0 ? 0.0f : (float)nm;
     }
 };
@@ -346,8 +351,9 @@ int enqueue_sweep_t(asd_plan *p, const double *relax)
         const ViewGeom &g = p->views[iv];
         const float *yd = p->ydet.as<float>() + (size_t)iv * p->para.na;
         const float *zd = p->zdet.as<float>() + (size_t)iv * p->para.nb;
+        const RayRec *rt = p->rt.as<RayRec>() + (size_t)iv * p->M;
         EpiSartAx<Codec> ea{view_codec<Codec>(p, iv), p->cor.as<float>(), p->res2.as<double>(),
-                            p->rt.as<RayRec>(), yd, zd, stopped, g};
+                            stopped, g.scale};
         int rc;
         {
             StageTimer t(p, ST_AX);
@@ -358,11 +364,10 @@ int enqueue_sweep_t(asd_plan *p, const double *relax)
             StageTimer t(p, ST_ATX);
             const EpiSartAtx eb{p->m.as<float>(), relax, stopped, g.scale};
             if (KAHAN)
-                rc = dispatch_atx_prec<KAHAN>(g, p->cor.as<float>(), yd, zd, p->rt.as<RayRec>(),
-                                              eb, p->stream);
+                rc = dispatch_atx_prec<KAHAN>(g, p->cor.as<float>(), yd, zd, rt, eb, p->stream);
             else
-                rc = dispatch_atx_variant<KAHAN>(p->atx_variant, g, p->cor.as<float>(),
-                                                 p->rt.as<RayRec>(), eb, p->stream);
+                rc = dispatch_atx_variant<KAHAN>(p->atx_variant, g, p->cor.as<float>(), rt, eb,
+                                                 p->stream);
             if (rc) return rc;
         }
     }
@@ -373,8 +378,10 @@ int enqueue_sweep_t(asd_plan *p, const double *relax)
 struct EpiTune {
     float *x, *sc;
     float scale;
+    struct Pre {};
     __device__ __forceinline__ bool skip() const { return false; }
-    __device__ __forceinline__ void operator()(long v, float acc, float accl) const
+    __device__ __forceinline__ Pre load(long) const { return Pre{}; }
+    __device__ __forceinline__ void operator()(long v, float acc, float accl, const Pre &) const
     {
         x[v] = acc * scale;
         sc[v] = accl * scale;
@@ -393,10 +400,7 @@ int autotune_atx(asd_plan *p)
         return ASD_OK;
     }
     const ViewGeom &g = p->views[0];
-    int rc;
-    if ((rc = launch_ray_table(g, p->ydet.as<float>(), p->zdet.as<float>(), p->rt.as<RayRec>(),
-                               p->stream)))
-        return rc;
+    int rc = ASD_OK;
     ASD_HIP(hipMemsetAsync(p->cor.p, 0, p->M * 4, p->stream));
     std::vector<int> cand = {ATX_DIRECT, ATX_T16x16x1};
     if (g.nz > 1) {
@@ -629,7 +633,7 @@ int asd_plan_create(const asd_para *para, const double *phi_vec, int nv, asd_pla
         (rc = p->mk1.ensure(N * 4)) || (rc = p->f0.ensure(N * 4)) || (rc = p->v.ensure(N * 4)) ||
         (rc = p->u2.ensure(N * 4)) || (rc = p->v2.ensure(N * 4)) ||
         (rc = p->cor.ensure(M * 4)) || (rc = p->res2.ensure(M * 8)) ||
-        (rc = p->rt.ensure(M * sizeof(RayRec))) ||
+        (rc = p->rt.ensure((size_t)nv * M * sizeof(RayRec))) ||
         (rc = p->P.ensure((size_t)nv * M * sino_bytes(para->precision))) ||
         (rc = p->ctlbuf.ensure(sizeof(DevCtl))) || (rc = p->hist.ensure(HIST_CAP * 8)) ||
         (rc = p->red.init()))
@@ -640,6 +644,12 @@ int asd_plan_create(const asd_para *para, const double *phi_vec, int nv, asd_pla
         set_error(ASD_ERR_HIP, "plan initialisation failed");
         return fail(ASD_ERR_HIP);
     }
+    // the ray-slope tables of every view, once: they depend on geometry only
+    for (int i = 0; i < nv; i++)
+        if ((rc = launch_ray_table(p->views[i], p->ydet.as<float>() + (size_t)i * para->na,
+                                   p->zdet.as<float>() + (size_t)i * para->nb,
+                                   p->rt.as<RayRec>() + (size_t)i * M, p->stream)))
+            return fail(rc);
     *out = p;
     rc = asd_plan_reset(p, nullptr, nullptr);
     if (rc) {

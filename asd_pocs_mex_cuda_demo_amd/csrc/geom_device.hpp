@@ -301,24 +301,6 @@ __host__ __device__ __forceinline__ bool inplane_may_hit(const RayRec &R, float 
            absf(yx_c + yx_o) <= lim;
 }
 
-// Bit k set when ray a_lo + k may meet the voxel (k < 32; wider footprints
-// keep every ray beyond the first 32).  rec(ia) returns the record of ray ia
-// in any detector row (s_ba, s_ab and mode depend on ia only).
-template <class RecAt>
-__host__ __device__ __forceinline__ unsigned inplane_mask(const RecAt &rec, int a_lo, int a_hi,
-                                                          float px, float py)
-{
-    unsigned m = 0;
-    for (int ia = a_lo; ia <= a_hi && ia < a_lo + 32; ia++)
-        if (inplane_may_hit(rec(ia), px, py)) m |= 1u << (ia - a_lo);
-    return m;
-}
-
-__host__ __device__ __forceinline__ bool mask_keeps(unsigned m, int k)
-{
-    return k >= 32 || ((m >> k) & 1u);
-}
-
 // Slope record of ray (ia, ib) of view g (the divisions of cu:142-196).
 __host__ __device__ __forceinline__ RayRec make_ray_rec(const ViewGeom &g, float ydet, float zdet)
 {

@@ -30,6 +30,8 @@ __global__ __launch_bounds__(256) void ax_kernel(ViewGeom g, const float *__rest
     const int lane = threadIdx.x % LPR;
     const long id = (long)blockIdx.x * RPB + threadIdx.x / LPR;
     const bool active = id < nrays;
+    typename Epi::Pre pre{};
+    if (active && lane == 0) pre = epi.load(id);
     Acc<KAHAN> sa, sl;
     if (active) {
         int ia = (int)(id % g.na), ib = (int)(id / g.na);
@@ -50,24 +52,27 @@ __global__ __launch_bounds__(256) void ax_kernel(ViewGeom g, const float *__rest
         }
     }
     if constexpr (WPR > 1) {
-        __shared__ R sp[WPR][2];
+        // RPB rays per block, WPR waves per ray: wave w of the block holds
+        // the partial of ray w / WPR; lane 0 of each ray adds them in order
+        __shared__ R sp[RPB * WPR][2];
         const int w = threadIdx.x / 64;
         if ((threadIdx.x & 63) == 0) {
             sp[w][0] = a;
             sp[w][1] = b;
         }
         __syncthreads();
-        if (threadIdx.x == 0) {
-            a = sp[0][0];
-            b = sp[0][1];
+        if (lane == 0) {
+            const int w0 = (threadIdx.x / LPR) * WPR;
+            a = sp[w0][0];
+            b = sp[w0][1];
 #pragma unroll
             for (int k = 1; k < WPR; k++) {
-                a += sp[k][0];
-                b += sp[k][1];
+                a += sp[w0 + k][0];
+                b += sp[w0 + k][1];
             }
         }
     }
-    if (active && lane == 0) epi(id, (float)a, (float)b);
+    if (active && lane == 0) epi(id, (float)a, (float)b, pre);
 }
 
 template <bool KAHAN = false, class Epi>
@@ -93,8 +98,14 @@ int dispatch_ax(int lpr, const ViewGeom &g, const float *x, const float *y_det,
 
 // Backprojector: one thread per voxel (x fastest) walks its detector
 // footprint, ib outer and ia inner like the reference (cu:255-269), and sums
-// l*y and l in that order.  Blocks are remapped so each XCD owns a contiguous
-// slab of voxels and the rays they share stay in its L2.
+// l*y and l in that order.  The records and corrections of ATX_CHUNK
+// consecutive bins are loaded together (clamped to the footprint) before
+// any is used, so a row costs one memory round trip per chunk; bins whose
+// in-plane chord provably misses the voxel are skipped (their term is an
+// exact zero).  Blocks are remapped so each XCD owns a contiguous slab of
+// voxels and the rays they share stay in its L2.
+constexpr int ATX_CHUNK = 4;
+
 template <class Epi, bool KAHAN = false>
 __global__ __launch_bounds__(256) void atx_kernel(ViewGeom g, const float *__restrict__ Y,
                                                   const RayRec *__restrict__ rt, Epi epi)
@@ -105,6 +116,7 @@ __global__ __launch_bounds__(256) void atx_kernel(ViewGeom g, const float *__res
     const long v = (long)b * 256 + threadIdx.x;
     const long N = (long)g.nx * g.ny * g.nz;
     if (v >= N) return;
+    const typename Epi::Pre pre = epi.load(v);
     const int ix = (int)(v % g.nx);
     const long r = v / g.nx;
     const int iy = (int)(r % g.ny), iz = (int)(r / g.ny);
@@ -114,21 +126,28 @@ __global__ __launch_bounds__(256) void atx_kernel(ViewGeom g, const float *__res
     const float hz = g.dz / 2.0f;
     Acc<KAHAN> acc, accl;
     if (f.a_lo <= f.a_hi && f.b_lo <= f.b_hi) {
-        const RayRec *row0 = rt + (long)f.b_lo * g.na;
-        const unsigned mask =
-            inplane_mask([&](int ia) { return row0[ia]; }, f.a_lo, f.a_hi, px, py);
         for (int ib = f.b_lo; ib <= f.b_hi; ib++) {
             const long row = (long)ib * g.na;
-            for (int ia = f.a_lo; ia <= f.a_hi; ia++) {
-                if (!mask_keeps(mask, ia - f.a_lo)) continue;
-                RayRec R = rt[row + ia];
-                float l = find_l_3d_rec(R, px, py, pz, hz, g.dz);
-                acc.add(l * Y[row + ia]);
-                accl.add(l);
+            for (int a0 = f.a_lo; a0 <= f.a_hi; a0 += ATX_CHUNK) {
+                RayRec R[ATX_CHUNK];
+                float yv[ATX_CHUNK];
+#pragma unroll
+                for (int k = 0; k < ATX_CHUNK; k++) {
+                    const int ia = a0 + k <= f.a_hi ? a0 + k : f.a_hi;
+                    R[k] = rt[row + ia];
+                    yv[k] = Y[row + ia];
+                }
+#pragma unroll
+                for (int k = 0; k < ATX_CHUNK; k++) {
+                    if (a0 + k > f.a_hi || !inplane_may_hit(R[k], px, py)) continue;
+                    float l = find_l_3d_rec(R[k], px, py, pz, hz, g.dz);
+                    acc.add(l * yv[k]);
+                    accl.add(l);
+                }
             }
         }
     }
-    epi(v, acc.value(), accl.value());
+    epi(v, acc.value(), accl.value(), pre);
 }
 
 // Backprojector on voxel tiles (TX x TY x TZ = 256 voxels per workgroup):
@@ -157,6 +176,9 @@ __global__ __launch_bounds__(256) void atx_tile_plain_kernel(ViewGeom g,
     const int lx = threadIdx.x % TX, ly = (threadIdx.x / TX) % TY, lz = threadIdx.x / (TX * TY);
     const int ix = tile_x * TX + lx, iy = tile_y * TY + ly, iz = tile_z * TZ + lz;
     const bool valid = ix < g.nx && iy < g.ny && iz < g.nz;
+    const long vox = (long)((size_t)iz * g.ny + iy) * g.nx + ix;
+    typename Epi::Pre pre{};
+    if (valid) pre = epi.load(vox);
     Footprint f;
     if (valid) f = footprint(g, ix, iy, iz);
     tile_union(valid && f.a_lo <= f.a_hi && f.b_lo <= f.b_hi, f, s_rng);
@@ -172,19 +194,15 @@ __global__ __launch_bounds__(256) void atx_tile_plain_kernel(ViewGeom g,
     Acc<KAHAN> acc, accl;
     if (A0 <= A1 && B0 <= B1) {
         const int nA = A1 - A0 + 1;
-        unsigned mask = 0;
-        if (valid && f.a_lo <= f.a_hi && f.b_lo <= f.b_hi) {
-            const RayRec *row0 = rt + (long)f.b_lo * g.na;
-            mask = inplane_mask([&](int ia) { return row0[ia]; }, f.a_lo, f.a_hi, px, py);
-        }
         if (nA > TILE_MAXR) {
             // footprint too wide for LDS: read the records directly
             if (valid)
                 for (int ib = f.b_lo; ib <= f.b_hi; ib++)
                     for (int ia = f.a_lo; ia <= f.a_hi; ia++) {
-                        if (!mask_keeps(mask, ia - f.a_lo)) continue;
                         const long k = (long)ib * g.na + ia;
-                        float l = find_l_3d_rec(rt[k], px, py, pz, hz, g.dz);
+                        const RayRec R = rt[k];
+                        if (!inplane_may_hit(R, px, py)) continue;
+                        float l = find_l_3d_rec(R, px, py, pz, hz, g.dz);
                         acc.add(l * Y[k]);
                         accl.add(l);
                     }
@@ -205,8 +223,9 @@ __global__ __launch_bounds__(256) void atx_tile_plain_kernel(ViewGeom g,
                     for (int ib = blo; ib <= bhi; ib++) {
                         const int row = (ib - b0) * nA - A0;
                         for (int ia = f.a_lo; ia <= f.a_hi; ia++) {
-                            if (!mask_keeps(mask, ia - f.a_lo)) continue;
-                            float l = find_l_3d_rec(s_rec[row + ia], px, py, pz, hz, g.dz);
+                            const RayRec R = s_rec[row + ia];
+                            if (!inplane_may_hit(R, px, py)) continue;
+                            float l = find_l_3d_rec(R, px, py, pz, hz, g.dz);
                             acc.add(l * s_y[row + ia]);
                             accl.add(l);
                         }
@@ -215,7 +234,7 @@ __global__ __launch_bounds__(256) void atx_tile_plain_kernel(ViewGeom g,
             }
         }
     }
-    if (valid) epi((long)((size_t)iz * g.ny + iy) * g.nx + ix, acc.value(), accl.value());
+    if (valid) epi(vox, acc.value(), accl.value(), pre);
 }
 
 // Backprojector variants (all bit-identical; the plan's autotuner picks one).

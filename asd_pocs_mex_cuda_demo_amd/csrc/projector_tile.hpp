@@ -220,6 +220,9 @@ __global__ __launch_bounds__(256) void atx_tile_kernel(ViewGeom g, const float *
     const int lx = threadIdx.x % TX, ly = (threadIdx.x / TX) % TY, lz = threadIdx.x / (TX * TY);
     const int ix = x0 + lx, iy = y0 + ly, iz = z0 + lz;
     const bool valid = ix < g.nx && iy < g.ny && iz < g.nz;
+    const long vox = (long)((size_t)iz * g.ny + iy) * g.nx + ix;
+    typename Epi::Pre pre{};
+    if (valid) pre = epi.load(vox);
     Footprint f;
     if (valid) f = footprint(g, ix, iy, iz);
     tile_union(valid && f.a_lo <= f.a_hi && f.b_lo <= f.b_hi, f, s_rng);
@@ -287,7 +290,7 @@ __global__ __launch_bounds__(256) void atx_tile_kernel(ViewGeom g, const float *
             }
         }
     }
-    if (valid) epi((long)((size_t)iz * g.ny + iy) * g.nx + ix, acc.value(), accl.value());
+    if (valid) epi(vox, acc.value(), accl.value(), pre);
 }
 
 }  // namespace asd

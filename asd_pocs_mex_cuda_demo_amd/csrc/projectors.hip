@@ -12,8 +12,10 @@ namespace {
 struct EpiPlain {
     float *y, *sr;
     float scale;
+    struct Pre {};
     __device__ __forceinline__ bool skip() const { return false; }
-    __device__ __forceinline__ void operator()(long id, float acc, float accl) const
+    __device__ __forceinline__ Pre load(long) const { return Pre{}; }
+    __device__ __forceinline__ void operator()(long id, float acc, float accl, const Pre &) const
     {
 #pragma clang fp contract(off)
         y[id] = acc * scale;
@@ -24,8 +26,10 @@ struct EpiPlain {
 struct EpiBackPlain {
     float *x, *sc;
     float scale;
+    struct Pre {};
     __device__ __forceinline__ bool skip() const { return false; }
-    __device__ __forceinline__ void operator()(long v, float acc, float accl) const
+    __device__ __forceinline__ Pre load(long) const { return Pre{}; }
+    __device__ __forceinline__ void operator()(long v, float acc, float accl, const Pre &) const
     {
 #pragma clang fp contract(off)
         x[v] = acc * scale;

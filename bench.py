@@ -119,6 +119,13 @@ def bench_batch(args):
     rp = np.array(rmse_plain)
     dt_mixed, rm_mixed = out[1]
     dt_bplain, rm_bplain = out[0]
+    # the spread of the plain result under a 1e-7 relative data perturbation
+    rq = np.array([plain_rmse_and_floor(para, img, P, steps, steps)[1][-1]
+                   for para, img, P in cases])
+
+    def rel(a):
+        d = np.abs(np.asarray(a) - rp) / rp
+        return {"max": float(d.max()), "median": float(np.median(d))}
     result = {
         "metric": METRIC,
         "value": K * steps / dt_mixed,
@@ -135,10 +142,30 @@ def bench_batch(args):
         "plain_sequential_case_iters_per_sec": K * steps / t_plain,
         "batched_plain_case_iters_per_sec": K * steps / dt_bplain,
         "speedup_batched_mixed_vs_plain": t_plain / dt_mixed,
-        "rmse_rel_diff_max_mixed_vs_plain": float(np.max(np.abs(rm_mixed - rp) / rp)),
-        "rmse_rel_diff_max_batched_plain_vs_plain": float(np.max(np.abs(rm_bplain - rp) / rp)),
+        "rmse_rel_diff_mixed_vs_plain": rel(rm_mixed),
+        "rmse_rel_diff_batched_plain_vs_plain": rel(rm_bplain),
+        "rmse_rel_diff_plain_under_1e-7_data_perturbation": rel(rq),
     }
     print(json.dumps(result), flush=True)
+
+
+def perturbed(P, seed=99, eps=1e-7):
+    return P * (1 + eps * np.random.default_rng(seed).standard_normal(P.shape))
+
+
+def plain_rmse_and_floor(para, img, P, steps, max_iter):
+    """RMSE history of the plain FP32 plan, and of the same plan on the data
+    perturbed by 1e-7 relative."""
+    from asd_pocs_mex_cuda_demo_amd import ops
+    res = []
+    for data in (P, perturbed(P)):
+        pl = ops.Plan(dict(para, maxnumIter=max_iter), para["phi_vec"])
+        pl.set_projections(data)
+        pl.reset(None, img)
+        pl.iterate(steps)
+        res.append(pl.out()[1])
+        pl.close()
+    return res
 
 
 def stage_bytes(N, M, ng):
@@ -150,21 +177,24 @@ def stage_bytes(N, M, ng):
     return {0: 4 * N + 8 * M, 1: 8 * N + 4 * M, 2: ng * 20 * N}
 
 
-def cpu_baseline(para, P, sample_iters, sample_views=0):
+def cpu_baseline(para, P, sample_iters, sample_views=0, I0=None):
     """The CPU oracle on the GPU box's host cores.  2-D: `sample_iters` full
-    outer iterations.  3-D: `sample_views` SART view updates plus one full TV
-    descent, extrapolated to one outer iteration (nv view updates + TV)."""
+    outer iterations (their result is returned for the accuracy check).
+    3-D: `sample_views` SART view updates plus one full TV descent,
+    extrapolated to one outer iteration (nv view updates + TV).
+    Returns (baseline record, (image, out) of the 2-D run or None)."""
     from oracle import oracle
     threads = min(16, os.cpu_count() or 1)
     oracle.set_threads(threads)
     if not sample_views:
         t0 = time.perf_counter()
-        oracle.asd_pocs(P, dict(para, maxnumIter=sample_iters))
+        run = oracle.asd_pocs(P, dict(para, maxnumIter=sample_iters, I0=I0))
         dt = time.perf_counter() - t0
-        return {"value": sample_iters / dt, "unit": "iters/sec", "cores": threads, "kind": "port",
-                "sample": f"{sample_iters} full ASD-POCS iterations of the same workload "
-                          f"(FP32 projectors, FP64 TV/driver; OpenMP over rays/voxels), "
-                          f"{dt:.2f} s wall"}
+        return ({"value": sample_iters / dt, "unit": "iters/sec", "cores": threads,
+                 "kind": "port",
+                 "sample": f"{sample_iters} full ASD-POCS iterations of the same workload "
+                           f"(FP32 projectors, FP64 TV/driver; OpenMP over rays/voxels), "
+                           f"{dt:.2f} s wall"}, run)
     nv = P.shape[1]
     N = para["nx"] * para["ny"] * para["nz"]
     m = np.zeros(N)
@@ -183,10 +213,10 @@ def cpu_baseline(para, P, sample_iters, sample_views=0):
     oracle.tv3d(vol, 0.2, 20, 1, np.zeros_like(vol) + 1e-3)
     t_tv = time.perf_counter() - t1
     t_iter = t_views / sample_views * nv + t_tv
-    return {"value": 1.0 / t_iter, "unit": "iters/sec", "cores": threads, "kind": "port",
-            "sample": f"{sample_views} SART view updates ({t_views:.2f} s) + one 20-step TV3D "
-                      f"descent ({t_tv:.2f} s), extrapolated to one iteration of {nv} views "
-                      f"({t_iter:.1f} s)"}
+    return ({"value": 1.0 / t_iter, "unit": "iters/sec", "cores": threads, "kind": "port",
+             "sample": f"{sample_views} SART view updates ({t_views:.2f} s) + one 20-step TV3D "
+                       f"descent ({t_tv:.2f} s), extrapolated to one iteration of {nv} views "
+                       f"({t_iter:.1f} s)"}, None)
 
 
 def main():
@@ -200,6 +230,9 @@ def main():
     ap.add_argument("--cpu-views", type=int, default=None)
     ap.add_argument("--prof-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first copy 256 MiB device to device three times (a plain copy "
+                         "kernel of known size, to scale rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -207,6 +240,14 @@ def main():
         sys.exit(2)
     from asd_pocs_mex_cuda_demo_amd import _native
     _native.lib()  # fail loudly if the HIP library is missing
+    if args.copy_calibration:
+        import torch
+        a = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
 
     if args.workload == "batch64":
         bench_batch(args)
@@ -292,9 +333,30 @@ def main():
         "ax_lanes_per_ray": plan.info()[0],
         "atx_tuning": plan.tuning(),
     }
+    if args.precision:
+        # matched accuracy: the plain FP32 plan over the same iterations, and
+        # the same plan on data perturbed by 1e-7 relative (the spread the TV
+        # descent gives to any change of the data at FP32 level)
+        rp, rq = plain_rmse_and_floor(para, img, P, steps, w["iters"])
+        result["rmse_rel_diff_vs_plain_fp32"] = float(np.max(np.abs(rmse - rp) / rp))
+        result["rmse_rel_diff_plain_under_1e-7_data_perturbation"] = float(
+            np.max(np.abs(rq - rp) / rp))
     if not args.no_cpu:
         views = args.cpu_views if args.cpu_views is not None else (0 if para["nz"] == 1 else 2)
-        result["cpu_baseline"] = cpu_baseline(para, P, args.cpu_iters, views)
+        cb, orc_run = cpu_baseline(para, P, args.cpu_iters, views, I0=img)
+        result["cpu_baseline"] = cb
+        if orc_run is not None:
+            # the GPU plan over the oracle's iteration count, against the oracle
+            mo, outo = orc_run
+            plan.reset(None, img)
+            plan.iterate(args.cpu_iters)
+            _, rg = plan.out()
+            mg = plan.image().ravel(order="F").astype(np.float64)
+            result["accuracy_vs_oracle"] = {
+                "iterations": args.cpu_iters,
+                "rmse_rel_diff_max": float(np.max(np.abs(rg - outo["RMSE"]) / outo["RMSE"])),
+                "image_rel_l2": float(np.linalg.norm(mg - mo) / np.linalg.norm(mo)),
+                "oracle": "FP32 projectors in the reference's expression order, FP64 TV and driver"}
     plan.close()
     print(json.dumps(result), flush=True)
 

@@ -26,13 +26,16 @@ static ViewGeom mk(float c, float s, float SO, float OD, float scale, float dyd,
 extern "C" {
 
 // ax_kernel<LPR=lpr>: lane j of a ray takes slabs j, j+lpr, ...; lanes are
-// combined by the same butterfly order as the wave shuffles.
+// combined by the same butterfly order as the wave shuffles (within groups of
+// at most 64 lanes), then the wave partials of a ray in wave order.
+// Compensated lanes are combined in FP64 as on the device.
 void emul_ax(const float *X, float *y, float *sr, float c, float s, const float *ydet,
              const float *zdet, float SO, float OD, float scale, float dz, int nx, int ny, int nz,
              int na, int nb, int lpr, int kahan)
 {
     ViewGeom g = mk(c, s, SO, OD, scale, 1.0f, 1.0f, dz, nx, ny, nz, na, nb);
-    float acc[64], accl[64];
+    double acc[256], accl[256];
+    const int wl = lpr < 64 ? lpr : 64;
     for (long id = 0; id < (long)na * nb; id++) {
         int ia = (int)(id % na), ib = (int)(id / na);
         AxRay r = ax_ray_setup(g, ydet[ia], zdet[ib]);
@@ -42,29 +45,36 @@ void emul_ax(const float *X, float *y, float *sr, float c, float s, const float 
                 Acc<true> a, l;
                 for (int iu0 = lane; iu0 < nu; iu0 += 4 * lpr)
                     ax_slabs<4, true>(r, X, iu0, lpr, nu, nx, ny, a, l);
-                acc[lane] = a.value();
-                accl[lane] = l.value();
+                acc[lane] = (double)a.s - (double)a.c;
+                accl[lane] = (double)l.s - (double)l.c;
             } else {
                 Acc<false> a, l;
                 for (int iu0 = lane; iu0 < nu; iu0 += 4 * lpr)
                     ax_slabs<4, false>(r, X, iu0, lpr, nu, nx, ny, a, l);
-                acc[lane] = a.value();
-                accl[lane] = l.value();
+                acc[lane] = a.s;
+                accl[lane] = l.s;
             }
         }
-        for (int off = lpr / 2; off > 0; off >>= 1) {
-            float na_[64], nl_[64];
+        for (int off = wl / 2; off > 0; off >>= 1) {
+            double na_[256], nl_[256];
             for (int lane = 0; lane < lpr; lane++) {
-                na_[lane] = acc[lane] + acc[lane ^ off];
-                nl_[lane] = accl[lane] + accl[lane ^ off];
+                int o = (lane & ~(wl - 1)) | ((lane ^ off) & (wl - 1));
+                na_[lane] = kahan ? acc[lane] + acc[o] : (double)((float)acc[lane] + (float)acc[o]);
+                nl_[lane] = kahan ? accl[lane] + accl[o]
+                                  : (double)((float)accl[lane] + (float)accl[o]);
             }
             for (int lane = 0; lane < lpr; lane++) {
                 acc[lane] = na_[lane];
                 accl[lane] = nl_[lane];
             }
         }
-        y[id] = acc[0] * scale;
-        sr[id] = accl[0] * scale;
+        double a = acc[0], l = accl[0];
+        for (int w = 1; w * 64 < lpr; w++) {
+            a = kahan ? a + acc[w * 64] : (double)((float)a + (float)acc[w * 64]);
+            l = kahan ? l + accl[w * 64] : (double)((float)l + (float)accl[w * 64]);
+        }
+        y[id] = (float)a * scale;
+        sr[id] = (float)l * scale;
     }
 }
 
@@ -86,15 +96,11 @@ void emul_atx(float *X, const float *y, float *sc, float c, float s, const float
         float px = sx - f.xc, py = sy - f.yc, pz = 0.0f - f.zc;
         float hz = g.dz / 2.0f;
         Acc<false> acc, accl;
-        unsigned mask = 0;
-        if (f.a_lo <= f.a_hi && f.b_lo <= f.b_hi) {
-            const RayRec *row0 = rt + (long)f.b_lo * na;
-            mask = inplane_mask([&](int ia) { return row0[ia]; }, f.a_lo, f.a_hi, px, py);
-        }
         for (int ib = f.b_lo; ib <= f.b_hi; ib++)
             for (int ia = f.a_lo; ia <= f.a_hi; ia++) {
-                if (!mask_keeps(mask, ia - f.a_lo)) continue;
-                float l = find_l_3d_rec(rt[(long)ib * na + ia], px, py, pz, hz, g.dz);
+                const RayRec &R = rt[(long)ib * na + ia];
+                if (!inplane_may_hit(R, px, py)) continue;
+                float l = find_l_3d_rec(R, px, py, pz, hz, g.dz);
                 acc.add(l * y[(long)ib * na + ia]);
                 accl.add(l);
             }

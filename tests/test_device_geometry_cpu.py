@@ -49,7 +49,7 @@ def test_device_projectors_match_oracle(orc, case, phi):
     args = (g["cos_phi"], g["sin_phi"], P(g["y_det"]), P(g["z_det"]), g["SO"], g["OD"], g["scale"])
     yo, sro = orc.ax(X, g["cos_phi"], g["sin_phi"], g["y_det"], g["z_det"], g["SO"], g["OD"],
                      g["scale"], np.float32(1), nx, ny, nz, na, nb)
-    for lpr, kahan in [(1, 0), (8, 0), (64, 0), (64, 1)]:
+    for lpr, kahan in [(1, 0), (8, 0), (64, 0), (128, 0), (256, 0), (64, 1), (256, 1)]:
         y = np.zeros(na * nb, np.float32)
         sr = np.zeros(na * nb, np.float32)
         emul().emul_ax(P(X), P(y), P(sr), *args, np.float32(1), nx, ny, nz, na, nb, lpr, kahan)

@@ -31,6 +31,8 @@ CASES = [
     ("fan64", geom(64, 64, 1, 128, 1, 405.941, 500.31, 0.5, 0.148 * 2, -0.3)),
     ("fan512", geom(512, 512, 1, 768, 1, 541.0, 408.0, 0.5, 0.8, -1.9)),
     ("fan_odd_na", geom(48, 40, 1, 97, 1, 500.0, 500.0, 1.0, 1.0, -2.6)),
+    # 2-D views whose rays span 128 lanes (two rays per block, two waves each)
+    ("fan300", geom(300, 280, 1, 400, 1, 500.0, 450.0, 0.7, 1.0, -1.1)),
     # 3-D cone beam
     ("cone32", geom(32, 32, 16, 48, 24, 500.0, 500.0, 1.0, 1.2, -0.7)),
     ("cone_odd", geom(24, 20, 9, 37, 15, 300.0, 400.0, 1.0, 1.5, -4.0)),
@@ -216,11 +218,16 @@ def test_precision_variant_matches_plain_rmse():
     backprojector (precision 1): the reconstruction RMSE stays within 1e-4
     relative of the plain FP32 form on noisy data.  r_max is raised so the
     alpha rule (a discrete decision that any perturbation can flip) never
-    fires and the comparison measures arithmetic only.  FP16 storage
-    (precision 2, 11-bit significand) is looser and kept for comparison."""
+    fires.  The TV descent with the reference's eps = 1e-8 amplifies any
+    perturbation with every sub-iteration (a 1e-9 relative change of the data
+    reaches ~4e-4 in RMSE after 5 iterations at ng = 10, pinned on the
+    oracle), so the 1e-4 bar is checked where the problem is conditioned:
+    5 iterations at ng = 5, and the first 2 iterations at ng = 10.  FP16
+    storage (precision 2, 11-bit significand) is looser and kept for
+    comparison."""
     para, img, P = _small_problem(n=64, nv=24, na=96)
     Pn = phantoms.poisson_log(P, 1e5, seed=7)
-    para.update(maxnumIter=5, alpha=0.2, numOfgradients=10, I0=img, r_max=1e9)
+    para.update(maxnumIter=5, alpha=0.2, numOfgradients=5, I0=img, r_max=1e9)
     m0, o0 = ops.ASD_POCS_cuda(Pn, para)
     m1, o1 = ops.ASD_POCS_cuda(Pn, dict(para, precision=1))
     assert o0["alpha"] == o1["alpha"] == 0.2
@@ -228,6 +235,10 @@ def test_precision_variant_matches_plain_rmse():
     assert rel_l2(m1, m0) <= 1e-3
     m2, o2 = ops.ASD_POCS_cuda(Pn, dict(para, precision=2))
     assert np.all(np.abs(o2["RMSE"] - o0["RMSE"]) <= 1e-2 * o0["RMSE"])
+    q = dict(para, maxnumIter=2, numOfgradients=10)
+    _, a = ops.ASD_POCS_cuda(Pn, q)
+    _, b = ops.ASD_POCS_cuda(Pn, dict(q, precision=1))
+    assert np.all(np.abs(b["RMSE"] - a["RMSE"]) <= 1e-4 * a["RMSE"]), (a["RMSE"], b["RMSE"])
 
 
 def test_batch_matches_single_plans():

@@ -357,3 +357,33 @@ def test_sart_single_sweep(orc):
     mi, oi = orc.asd_pocs(P, dict(para, maxnumIter=1, initBySART=True))
     m0, _ = orc.asd_pocs(P, dict(para, maxnumIter=1))
     assert not np.array_equal(mi, m0)
+
+
+def test_tv_descent_amplifies_data_perturbations(orc):
+    """Conditioning of the reference algorithm itself (not of any
+    implementation): the TV gradient with eps = 1e-8 has a Lipschitz
+    constant of order 1/sqrt(eps), so every TV sub-iteration amplifies a
+    perturbation.  A 1e-9 relative change of noisy data stays below 1e-7 in
+    RMSE without TV but exceeds 1e-5 after 5 iterations of 10 TV steps.
+    Reconstruction bars between arithmetic variants are therefore set on
+    iteration counts where this growth stays below them."""
+    from asd_pocs_mex_cuda_demo_amd import phantoms
+    n, nv, na = 64, 24, 96
+    para = dict(nx=n, ny=n, nz=1, SO=500.0, OD=400.0, vxlsize=1.0, detsize=1.2, na=na, nb=1)
+    para["phi_vec"] = np.arange(1, nv + 1) * 360.0 / nv
+    img = phantoms.shepp_logan_2d(n)
+    P = np.zeros((na, nv))
+    for i, th in enumerate(para["phi_vec"]):
+        P[:, i] = orc.ax_para(img.astype(np.float32), dict(para, phi=-th / 180 * np.pi))[0]
+    Pn = phantoms.poisson_log(P, 1e5, seed=7)
+    Pp = Pn * (1 + 1e-9 * np.random.default_rng(1).standard_normal(Pn.shape))
+    para.update(maxnumIter=5, alpha=0.2, I0=img, r_max=1e9)
+
+    def rmse_diff(ng):
+        _, a = orc.asd_pocs(Pn, dict(para, numOfgradients=ng))
+        _, b = orc.asd_pocs(Pp, dict(para, numOfgradients=ng))
+        return np.abs(b["RMSE"] - a["RMSE"]) / a["RMSE"]
+
+    d0, d10 = rmse_diff(0), rmse_diff(10)
+    assert d0.max() < 1e-7
+    assert d10[0] < 1e-7 and d10[-1] > 1e-5
