@@ -59,6 +59,7 @@ struct ViewGeom {
     float dz;                // axial voxel pitch (1 in the reference)
     float dy_det, dz_det;    // detector pitch (voxels), used by the backprojector footprint
     int nx, ny, nz, na, nb;
+    float fp_d;              // footprint half-extent sqrt((1 + dz^2) / 2) (geom_finish)
 };
 
 // Per-ray record for the backprojector: the six slope quotients the
@@ -67,7 +68,7 @@ struct ViewGeom {
 struct alignas(32) RayRec {
     float s_ba, s_ab, s_ca, s_cb, s_ac, s_bc;
     int mode;   // RAY_* below
-    float pad;
+    int flat;   // ray in the z = 0 face of a one-slice volume (make_ray_rec)
 };
 
 // find_l_3d dispatch (cu:131-200 and find_l cu:26-113)

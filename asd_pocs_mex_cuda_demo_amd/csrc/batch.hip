@@ -568,6 +568,8 @@ int asd_batch_create(int ncases, const asd_para *paras, const double *const *phi
         const asd_para &p = paras[c];
         ASD_ARG(p.nx > 0 && p.ny > 0 && p.nz > 0 && p.na > 0 && p.nb > 0,
                 "every case needs positive nx, ny, nz, na, nb");
+        ASD_ARG((long)p.nx * p.ny * p.nz < (1L << 32) && (long)p.na * p.nb < (1L << 31),
+                "case volume or detector too large (32-bit voxel and bin indices)");
         ASD_ARG(nvs[c] > 0 && phi_vecs[c], "every case needs views");
         ASD_ARG(p.SO > 0 && p.OD > 0 && p.vxlsize > 0 && p.detsize > 0, "bad case geometry");
         ASD_ARG(p.precision == paras[0].precision, "all cases must share para.precision");
@@ -649,6 +651,7 @@ int asd_batch_create(int ncases, const asd_para *paras, const double *const *phi
             g.dz = 1.0f;
             g.dy_det = g.dz_det = det;
             g.nx = p.nx; g.ny = p.ny; g.nz = p.nz; g.na = p.na; g.nb = p.nb;
+            geom_finish(g);
             for (int i = 0; i < p.na; i++)
                 yall[yoff[c] + (size_t)j * p.na + i] =
                     (float)(((double)(float)i - (double)(float)p.na / 2.0 + 0.5) * (double)det);

@@ -267,6 +267,7 @@ void view_geom_from_para(const asd_para &p, double phi_deg, ViewGeom &g, std::ve
     g.dy_det = det;
     g.dz_det = det;
     g.nx = p.nx; g.ny = p.ny; g.nz = p.nz; g.na = p.na; g.nb = p.nb;
+    geom_finish(g);
     ydet.resize(p.na);
     zdet.resize(p.nb);
     for (int i = 0; i < p.na; i++)
@@ -280,11 +281,13 @@ int check_para(const asd_para *p, int nv)
     ASD_ARG(p, "NULL para");
     ASD_ARG(p->nx > 0 && p->ny > 0 && p->nz > 0, "para.nx, para.ny, para.nz must be positive");
     ASD_ARG(p->na > 0 && p->nb > 0, "para.na, para.nb must be positive");
+    ASD_ARG((long)p->nx * p->ny * p->nz < (1L << 32) && (long)p->na * p->nb < (1L << 31),
+            "volume or detector too large (32-bit voxel and bin indices)");
     ASD_ARG(nv > 0, "phi_vec is empty");
     ASD_ARG(p->vxlsize > 0 && p->detsize > 0, "para.vxlsize and para.detsize must be positive");
     ASD_ARG(p->SO > 0 && p->OD > 0, "para.SO and para.OD must be positive");
     ASD_ARG(p->numOfgradients >= 0, "para.numOfgradients must be >= 0");
-    ASD_ARG(p->precision >= 0 && p->precision <= 2, "para.precision must be 0, 1 or 2");
+    ASD_ARG(p->precision >= 0 && p->precision <= 3, "para.precision must be 0, 1, 2 or 3");
     return ASD_OK;
 }
 
@@ -342,8 +345,9 @@ SinoF16 view_codec<SinoF16>(asd_plan *p, int iv)
 }
 
 // One SART sweep over all views with relaxation *relax.  The precision
-// variant reads a 16-bit sinogram and compensates every FP32 sum.
-template <class Codec, bool KAHAN>
+// variants read a 16-bit sinogram and compensate every FP32 sum; EXACT
+// also sets up the backprojector's rays in FP64 per tile.
+template <class Codec, bool KAHAN, bool EXACT>
 int enqueue_sweep_t(asd_plan *p, const double *relax)
 {
     const int *stopped = &p->ctl()->stopped;
@@ -363,7 +367,7 @@ int enqueue_sweep_t(asd_plan *p, const double *relax)
         {
             StageTimer t(p, ST_ATX);
             const EpiSartAtx eb{p->m.as<float>(), relax, stopped, g.scale};
-            if (KAHAN)
+            if (EXACT)
                 rc = dispatch_atx_prec<KAHAN>(g, p->cor.as<float>(), yd, zd, rt, eb, p->stream);
             else
                 rc = dispatch_atx_variant<KAHAN>(p->atx_variant, g, p->cor.as<float>(), rt, eb,
@@ -391,14 +395,23 @@ struct EpiTune {
 // Times the bit-identical backprojector variants on view 0 (scratch outputs,
 // before the first sweep) and keeps the fastest.  The choice only changes
 // speed, never results.
+template <bool KAHAN>
+int autotune_atx_t(asd_plan *p);
+
 int autotune_atx(asd_plan *p)
 {
     if (p->atx_variant >= 0) return ASD_OK;
     for (int i = 0; i < ATX_NVARIANT; i++) p->atx_us[i] = -1.0;
-    if (p->para.precision != 0) {
-        p->atx_variant = ATX_DIRECT;  // the precision variant has its own tiled kernel
-        return ASD_OK;
+    switch (p->para.precision) {
+    case 0: return autotune_atx_t<false>(p);
+    case 3: p->atx_variant = ATX_DIRECT; return ASD_OK;  // own tiled FP64-geometry kernel
+    default: return autotune_atx_t<true>(p);
     }
+}
+
+template <bool KAHAN>
+int autotune_atx_t(asd_plan *p)
+{
     const ViewGeom &g = p->views[0];
     int rc = ASD_OK;
     ASD_HIP(hipMemsetAsync(p->cor.p, 0, p->M * 4, p->stream));
@@ -417,7 +430,7 @@ int autotune_atx(asd_plan *p)
         float ms_min = 1e30f;
         for (int rep = 0; rep < 4; rep++) {
             (void)hipEventRecord(e0, p->stream);
-            rc = dispatch_atx_variant<false>(v, g, p->cor.as<float>(), p->rt.as<RayRec>(), ep,
+            rc = dispatch_atx_variant<KAHAN>(v, g, p->cor.as<float>(), p->rt.as<RayRec>(), ep,
                                              p->stream);
             (void)hipEventRecord(e1, p->stream);
             if (rc) break;
@@ -442,9 +455,12 @@ int autotune_atx(asd_plan *p)
 
 int enqueue_sweep(asd_plan *p, const double *relax)
 {
-    if (p->para.precision == 1) return enqueue_sweep_t<SinoU16, true>(p, relax);
-    if (p->para.precision == 2) return enqueue_sweep_t<SinoF16, true>(p, relax);
-    return enqueue_sweep_t<SinoF32, false>(p, relax);
+    switch (p->para.precision) {
+    case 1: return enqueue_sweep_t<SinoU16, true, false>(p, relax);
+    case 2: return enqueue_sweep_t<SinoF16, true, false>(p, relax);
+    case 3: return enqueue_sweep_t<SinoU16, true, true>(p, relax);
+    default: return enqueue_sweep_t<SinoF32, false, false>(p, relax);
+    }
 }
 
 int enqueue_post_sart(asd_plan *p, int sweep_only)
@@ -665,7 +681,7 @@ int asd_plan_set_projections(asd_plan *p, const double *P)
     ASD_ARG(p && P, "NULL argument");
     int rc;
     const size_t n = (size_t)p->nv * p->M;
-    if (p->para.precision == 1) {
+    if (p->para.precision == 1 || p->para.precision == 3) {
         // 16-bit fixed point per view: lo + step * q, q = round((P - lo) / step)
         std::vector<uint16_t> q(n);
         p->u16_lo.assign(p->nv, 0.0);

@@ -24,6 +24,10 @@ struct AxRay {
     float u1, t1s, s1, s2, z1, dz, q, tmp;
     int nu2, nt2, nz2, nt, nz;
     int xdriven;
+    // flat: a one-slice volume and a ray in its z = 0 face (z1 = 0, s2 = +-0,
+    // dz = 1), where every piece lies in slice 0 with rz = 1; dz1: dz == 1,
+    // where x / dz == x exactly.  Both only skip work, never change a bit.
+    int flat, dz1;
 };
 
 __host__ __device__ __forceinline__ AxRay ax_ray_setup(const ViewGeom &g, float ydet, float zdet)
@@ -50,7 +54,16 @@ __host__ __device__ __forceinline__ AxRay ax_ray_setup(const ViewGeom &g, float 
     r.nt = r.xdriven ? g.ny : g.nx;
     r.nz = g.nz;
     r.nz2 = g.nz / 2;
+    r.dz1 = g.dz == 1.0f;
+    r.flat = r.dz1 && g.nz == 1 && r.s2 == 0.0f && r.z1 == 0.0f;
     return r;
+}
+
+// Derived per-view constants (host and device give the same FP32 values).
+__host__ __device__ __forceinline__ void geom_finish(ViewGeom &g)
+{
+#pragma clang fp contract(off)
+    g.fp_d = sqrtf((1 + g.dz * g.dz) / 2);
 }
 
 // Running FP32 sum; with KAHAN the rounding error of every addition is
@@ -78,7 +91,7 @@ struct Acc {
 // whether the piece lies in the volume.  Out-of-volume pieces get index 0 (a
 // valid address) so the caller can issue every load before it accumulates.
 struct SlabW {
-    size_t idx[3];
+    unsigned idx[3];
     float wl[3];
     bool ok[3];
 };
@@ -91,7 +104,7 @@ __host__ __device__ __forceinline__ void ax_slab_geom(const AxRay &r, int iu, in
     s.idx[0] = s.idx[1] = s.idx[2] = 0;
     s.wl[0] = s.wl[1] = s.wl[2] = 0.0f;
     float ua = (float)(iu - r.nu2), ub = ua + 1;
-    float ta, tb, za, zb;
+    float ta, tb;
     if (r.s1 >= 0) {
         ta = r.t1s + r.s1 * (ua - r.u1) + r.nt2;
         tb = r.t1s + r.s1 * (ub - r.u1) + r.nt2;
@@ -99,15 +112,28 @@ __host__ __device__ __forceinline__ void ax_slab_geom(const AxRay &r, int iu, in
         ta = r.t1s + r.s1 * (ub - r.u1) + r.nt2;
         tb = r.t1s + r.s1 * (ua - r.u1) + r.nt2;
     }
-    if (r.s2 >= 0) {
-        za = (r.z1 + r.s2 * (ua - r.u1)) / r.dz + r.nz2;
-        zb = (r.z1 + r.s2 * (ub - r.u1)) / r.dz + r.nz2;
-    } else {
-        za = (r.z1 + r.s2 * (ub - r.u1)) / r.dz + r.nz2;
-        zb = (r.z1 + r.s2 * (ua - r.u1)) / r.dz + r.nz2;
-    }
     int ct1 = (int)floorf(ta), ct2 = (int)floorf(tb);
-    int cz1 = (int)floorf(za), cz2 = (int)floorf(zb);
+    int cz1 = 0, cz2 = 0;
+    float za = 0.0f, zb = 0.0f;
+    if (!r.flat) {
+        // (z1 + s2 * (u - u1)) / dz + nz/2; x / 1 == x, so dz == 1 skips the division
+        float pa, pb;
+        if (r.s2 >= 0) {
+            pa = r.z1 + r.s2 * (ua - r.u1);
+            pb = r.z1 + r.s2 * (ub - r.u1);
+        } else {
+            pa = r.z1 + r.s2 * (ub - r.u1);
+            pb = r.z1 + r.s2 * (ua - r.u1);
+        }
+        if (!r.dz1) {
+            pa = pa / r.dz;
+            pb = pb / r.dz;
+        }
+        za = pa + r.nz2;
+        zb = pb + r.nz2;
+        cz1 = (int)floorf(za);
+        cz2 = (int)floorf(zb);
+    }
     bool in_t1 = ct1 >= 0 && ct1 < r.nt, in_t2 = ct2 >= 0 && ct2 < r.nt;
     bool in_z1 = cz1 >= 0 && cz1 < r.nz, in_z2 = cz2 >= 0 && cz2 < r.nz;
     if (!((in_t1 || in_t2) && (in_z1 || in_z2))) return;
@@ -119,10 +145,10 @@ __host__ __device__ __forceinline__ void ax_slab_geom(const AxRay &r, int iu, in
     bool tfirst = !(rt > rz);  // the t crossing comes first (ties: t)
     float lo = tfirst ? rt : rz, hi = tfirst ? rz : rt;
     int ctm = tfirst ? ct2 : ct1, czm = tfirst ? cz1 : cz2;
-    const size_t sxy = (size_t)nx * ny;
+    const unsigned sxy = (unsigned)nx * (unsigned)ny;
     auto index = [&](int ct, int cz) {
-        return r.xdriven ? (size_t)cz * sxy + (size_t)ct * nx + iu
-                         : (size_t)cz * sxy + (size_t)iu * nx + ct;
+        return r.xdriven ? (unsigned)cz * sxy + (unsigned)ct * (unsigned)nx + (unsigned)iu
+                         : (unsigned)cz * sxy + (unsigned)iu * (unsigned)nx + (unsigned)ct;
     };
     s.ok[0] = in_t1 && in_z1;
     s.ok[1] = (tfirst ? in_t2 : in_t1) && (tfirst ? in_z1 : in_z2);
@@ -242,6 +268,12 @@ __host__ __device__ __forceinline__ float find_l_3d_rec(const RayRec &R, float p
     switch (R.mode) {
     case RAY_X_FIRST:
     case RAY_Y_FIRST: {
+        // A flat ray (c == 0 in a one-slice volume, pz = -hz): s_ca = s_cb = +-0
+        // put both hits' z at pz, so every x / y face test's z condition holds
+        // (|pz| == hz) and ez = 0; s_ac, s_bc = +-inf make the z-face tests
+        // NaN / inf comparisons that fail.  What is left is the in-plane
+        // chord, evaluated by the same expressions in the same order.
+        if (R.flat) return chord2(px, py, R.s_ba, R.s_ab, hx, hy, R.mode == RAY_X_FIRST);
         float xy_c = R.s_ba * (-px) + py, xy_o = R.s_ba * hx;
         float xz_c = R.s_ca * (-px) + pz, xz_o = R.s_ca * hx;
         float yx_c = R.s_ab * (-py) + px, yx_o = R.s_ab * hy;
@@ -314,7 +346,7 @@ __host__ __device__ __forceinline__ RayRec make_ray_rec(const ViewGeom &g, float
     R.s_ba = b / a; R.s_ab = a / b;
     R.s_ca = c / a; R.s_cb = c / b;
     R.s_ac = a / c; R.s_bc = b / c;
-    R.pad = 0.0f;
+    R.flat = 0;
     if (a == 0) {
         if (b == 0) R.mode = RAY_A0_B0;
         else if (c == 0) R.mode = RAY_A0_C0;
@@ -324,6 +356,7 @@ __host__ __device__ __forceinline__ RayRec make_ray_rec(const ViewGeom &g, float
         else R.mode = absf(a) > absf(c) ? RAY_B0_XZ_U : RAY_B0_XZ_W;
     } else {
         R.mode = absf(a) > absf(b) ? RAY_X_FIRST : RAY_Y_FIRST;
+        R.flat = c == 0 && g.nz == 1 && g.dz == 1.0f;
     }
     return R;
 }
@@ -342,7 +375,7 @@ __host__ __device__ __forceinline__ Footprint footprint(const ViewGeom &g, int i
     float SD = g.SO + g.OD;
     int na2 = g.na / 2, nb2 = g.nb / 2;
     int nx2 = g.nx / 2, ny2 = g.ny / 2, nz2 = g.nz / 2;
-    float d = sqrtf((1 + g.dz * g.dz) / 2);
+    const float d = g.fp_d;
     f.zc = (float)(iz + 0.5 - nz2) * g.dz;
     f.yc = (float)(iy + 0.5 - ny2);
     f.xc = (float)(ix + 0.5 - nx2);
@@ -351,7 +384,7 @@ __host__ __device__ __forceinline__ Footprint footprint(const ViewGeom &g, int i
     float t = SD / ((xr + g.SO) * g.dy_det);
     int a_hi = (int)floorf((yr + 1) * t + na2);
     int a_lo = (int)floorf((yr - 1) * t + na2);
-    t = SD / ((xr + g.SO) * g.dz_det);
+    if (g.dz_det != g.dy_det) t = SD / ((xr + g.SO) * g.dz_det);
     int b_hi = (int)floorf((f.zc + d) * t + nb2);
     int b_lo = (int)floorf((f.zc - d) * t + nb2);
     f.a_lo = a_lo < 0 ? 0 : a_lo;

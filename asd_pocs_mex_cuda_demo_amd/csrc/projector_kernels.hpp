@@ -37,8 +37,11 @@ __global__ __launch_bounds__(256) void ax_kernel(ViewGeom g, const float *__rest
         int ia = (int)(id % g.na), ib = (int)(id / g.na);
         AxRay r = ax_ray_setup(g, y_det[ia], z_det[ib]);
         int nu = r.xdriven ? g.nx : g.ny;
-        for (int iu0 = lane; iu0 < nu; iu0 += 4 * LPR)
-            ax_slabs<4, KAHAN>(r, X, iu0, LPR, nu, g.nx, g.ny, sa, sl);
+        // slabs per load batch: two when a ray has 128 or more lanes (a
+        // 512-slab ray then needs one batch per lane), else four
+        constexpr int U = LPR >= 128 ? 2 : 4;
+        for (int iu0 = lane; iu0 < nu; iu0 += U * LPR)
+            ax_slabs<U, KAHAN>(r, X, iu0, LPR, nu, g.nx, g.ny, sa, sl);
     }
     // compensated partials are combined in FP64, plain ones in FP32
     using R = typename std::conditional<KAHAN, double, float>::type;
@@ -113,13 +116,13 @@ __global__ __launch_bounds__(256) void atx_kernel(ViewGeom g, const float *__res
 #pragma clang fp contract(off)
     if (epi.skip()) return;
     const unsigned b = xcd_block(blockIdx.x, gridDim.x);
-    const long v = (long)b * 256 + threadIdx.x;
-    const long N = (long)g.nx * g.ny * g.nz;
+    const unsigned v = b * 256u + threadIdx.x;
+    const unsigned N = (unsigned)g.nx * (unsigned)g.ny * (unsigned)g.nz;
     if (v >= N) return;
     const typename Epi::Pre pre = epi.load(v);
-    const int ix = (int)(v % g.nx);
-    const long r = v / g.nx;
-    const int iy = (int)(r % g.ny), iz = (int)(r / g.ny);
+    const int ix = (int)(v % (unsigned)g.nx);
+    const unsigned r = v / (unsigned)g.nx;
+    const int iy = (int)(r % (unsigned)g.ny), iz = (int)(r / (unsigned)g.ny);
     Footprint f = footprint(g, ix, iy, iz);
     const float sx = g.cos_phi * (-g.SO), sy = g.sin_phi * (-g.SO);
     const float px = sx - f.xc, py = sy - f.yc, pz = 0.0f - f.zc;
@@ -127,13 +130,13 @@ __global__ __launch_bounds__(256) void atx_kernel(ViewGeom g, const float *__res
     Acc<KAHAN> acc, accl;
     if (f.a_lo <= f.a_hi && f.b_lo <= f.b_hi) {
         for (int ib = f.b_lo; ib <= f.b_hi; ib++) {
-            const long row = (long)ib * g.na;
+            const unsigned row = (unsigned)ib * (unsigned)g.na;
             for (int a0 = f.a_lo; a0 <= f.a_hi; a0 += ATX_CHUNK) {
                 RayRec R[ATX_CHUNK];
                 float yv[ATX_CHUNK];
 #pragma unroll
                 for (int k = 0; k < ATX_CHUNK; k++) {
-                    const int ia = a0 + k <= f.a_hi ? a0 + k : f.a_hi;
+                    const unsigned ia = (unsigned)(a0 + k <= f.a_hi ? a0 + k : f.a_hi);
                     R[k] = rt[row + ia];
                     yv[k] = Y[row + ia];
                 }

@@ -113,7 +113,8 @@ int check_geom(const ViewGeom &g)
 {
     ASD_ARG(g.nx > 0 && g.ny > 0 && g.nz > 0 && g.na > 0 && g.nb > 0,
             "sizes nx, ny, nz, na, nb must be positive");
-    ASD_ARG((long)g.nx * g.ny * g.nz < (1L << 40), "volume too large");
+    ASD_ARG((long)g.nx * g.ny * g.nz < (1L << 32) && (long)g.na * g.nb < (1L << 31),
+            "volume or detector too large (32-bit voxel and bin indices)");
     return ASD_OK;
 }
 
@@ -124,6 +125,7 @@ ViewGeom make_geom(float cos_phi, float sin_phi, float SO, float OD, float scale
     g.cos_phi = cos_phi; g.sin_phi = sin_phi; g.SO = SO; g.OD = OD; g.scale = scale;
     g.dz = dz; g.dy_det = dy_det; g.dz_det = dz_det;
     g.nx = nx; g.ny = ny; g.nz = nz; g.na = na; g.nb = nb;
+    geom_finish(g);
     return g;
 }
 

@@ -4,6 +4,7 @@
 //   precision 1  16-bit fixed point per view: P = lo + step * q, q in [0, 65535],
 //                lo / step from the view's range (error <= step / 2)
 //   precision 2  FP16 (11-bit significand; kept for comparison)
+//   precision 3  as 1 (the backprojector also sets up its rays in FP64)
 //
 // Each codec reads one value of one view as FP64, the type the SART
 // correction (P - vxl*y) is formed in.

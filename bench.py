@@ -320,8 +320,10 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {0: "f32", 1: "f32 (16-bit fixed-point sinogram, f64 sums)",
-                  2: "f32 (fp16 sinogram, f64 sums)"}[args.precision],
+        "dtype": {0: "f32", 1: "f32 (16-bit fixed-point sinogram, compensated sums)",
+                  2: "f32 (fp16 sinogram, compensated sums)",
+                  3: "f32 (16-bit fixed-point sinogram, compensated sums, "
+                     "f64 tile ray setup)"}[args.precision],
         "data": f"synthetic: modified Shepp-Logan phantom, Poisson noise I0={w['I0']:g} (seeded), log transform",
         "config": {"workload": args.workload, "nx": para["nx"], "ny": para["ny"],
                    "nz": para["nz"], "views": w["nv"], "na": para["na"], "nb": para["nb"],

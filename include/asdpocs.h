@@ -108,7 +108,11 @@ typedef struct asd_para {
     double stopRelChg;      /* default eps */
     int initBySART;         /* 1: m0 = one SART sweep (SART_cuda_inline.m) */
     double lambda;          /* SART relaxation of the warm start, default 1 */
-    int precision;          /* 0: plain FP32 (default); 1: FP16-stored sinogram, compensated sums */
+    int precision;          /* 0: plain FP32 (default);
+                             * 1: 16-bit fixed-point sinogram per view, compensated sums;
+                             * 2: FP16 sinogram, compensated sums;
+                             * 3: as 1, and the backprojector sets up its rays in FP64
+                             *    per voxel tile (lengths within ~1e-6 voxel of exact) */
 } asd_para;
 
 typedef struct asd_out {

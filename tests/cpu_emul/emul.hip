@@ -20,10 +20,31 @@ static ViewGeom mk(float c, float s, float SO, float OD, float scale, float dyd,
     g.cos_phi = c; g.sin_phi = s; g.SO = SO; g.OD = OD; g.scale = scale; g.dz = dz;
     g.dy_det = dyd; g.dz_det = dzd;
     g.nx = nx; g.ny = ny; g.nz = nz; g.na = na; g.nb = nb;
+    geom_finish(g);
     return g;
 }
 
+// When set, the flat / dz == 1 shortcuts are switched off (the general
+// expressions are evaluated), to check that the shortcuts change no bit.
+static int g_general = 0;
+
+static AxRay ray_setup(const ViewGeom &g, float ydet, float zdet)
+{
+    AxRay r = ax_ray_setup(g, ydet, zdet);
+    if (g_general) r.flat = r.dz1 = 0;
+    return r;
+}
+
+static RayRec ray_rec(const ViewGeom &g, float ydet, float zdet)
+{
+    RayRec R = make_ray_rec(g, ydet, zdet);
+    if (g_general) R.flat = 0;
+    return R;
+}
+
 extern "C" {
+
+void emul_set_general(int on) { g_general = on; }
 
 // ax_kernel<LPR=lpr>: lane j of a ray takes slabs j, j+lpr, ...; lanes are
 // combined by the same butterfly order as the wave shuffles (within groups of
@@ -38,7 +59,7 @@ void emul_ax(const float *X, float *y, float *sr, float c, float s, const float 
     const int wl = lpr < 64 ? lpr : 64;
     for (long id = 0; id < (long)na * nb; id++) {
         int ia = (int)(id % na), ib = (int)(id / na);
-        AxRay r = ax_ray_setup(g, ydet[ia], zdet[ib]);
+        AxRay r = ray_setup(g, ydet[ia], zdet[ib]);
         int nu = r.xdriven ? nx : ny;
         for (int lane = 0; lane < lpr; lane++) {
             if (kahan) {
@@ -86,7 +107,7 @@ void emul_atx(float *X, const float *y, float *sc, float c, float s, const float
     ViewGeom g = mk(c, s, SO, OD, scale, dyd, dzd, dz, nx, ny, nz, na, nb);
     long M = (long)na * nb;
     RayRec *rt = new RayRec[M];
-    for (long id = 0; id < M; id++) rt[id] = make_ray_rec(g, ydet[id % na], zdet[id / na]);
+    for (long id = 0; id < M; id++) rt[id] = ray_rec(g, ydet[id % na], zdet[id / na]);
     for (long v = 0; v < (long)nx * ny * nz; v++) {
         int ix = (int)(v % nx);
         long r = v / nx;
@@ -120,7 +141,7 @@ void emul_atx_prec(float *X, const float *y, float *sc, float c, float s, const 
     ViewGeom g = mk(c, s, SO, OD, scale, dyd, dzd, dz, nx, ny, nz, na, nb);
     long M = (long)na * nb;
     std::vector<RayRec> rt(M);
-    for (long id = 0; id < M; id++) rt[id] = make_ray_rec(g, ydet[id % na], zdet[id / na]);
+    for (long id = 0; id < M; id++) rt[id] = ray_rec(g, ydet[id % na], zdet[id / na]);
     TileDims td = tile_dims(g, TX, TY, TZ);
     for (int tzi = 0; tzi < td.ntz; tzi++)
         for (int tyi = 0; tyi < td.nty; tyi++)

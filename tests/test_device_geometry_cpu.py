@@ -151,3 +151,44 @@ def test_backprojector_bit_exact_random_geometries(orc):
                           g["OD"], g["scale"], g["det"], g["det"], np.float32(1), nx, ny, nz,
                           na, nb)
         assert np.array_equal(x, xo) and np.array_equal(sc, sco), (nx, ny, nz, na, nb, SO, OD, det, phi)
+
+
+def test_flat_and_unit_pitch_shortcuts_change_no_bit():
+    """The forward projector's flat-ray and dz == 1 shortcuts and the
+    backprojector's flat-ray chord give bit-identical results to the general
+    expressions on random fan geometries (cardinal angles and odd bin counts
+    included, where the a == 0 / b == 0 fallbacks appear), and on cone
+    geometries (where only the dz == 1 shortcut applies)."""
+    rng = np.random.default_rng(33)
+    lib = emul()
+    lib.emul_set_general.argtypes = [C.c_int]
+    phis = [0.0, -np.pi / 2, -np.pi, float(rng.uniform(-7, 7))] + \
+        [float(v) for v in rng.uniform(-7, 7, 12)]
+    for k, phi in enumerate(phis):
+        cone = k % 4 == 3
+        nx, ny = int(rng.integers(6, 40)), int(rng.integers(6, 40))
+        nz, nb = (int(rng.integers(2, 7)), int(rng.integers(2, 12))) if cone else (1, 1)
+        na = int(rng.integers(5, 70))
+        SO, OD = float(rng.uniform(20, 800)), float(rng.uniform(20, 800))
+        det = float(rng.uniform(0.3, 3.0))
+        from oracle import oracle as orc
+        g = orc.gateway_geometry(phi, SO, OD, 1.0, det, na, nb)
+        X = rng.random(nx * ny * nz).astype(np.float32)
+        Y = rng.standard_normal(na * nb).astype(np.float32)
+        out = []
+        for general in (0, 1):
+            lib.emul_set_general(general)
+            y = np.zeros(na * nb, np.float32)
+            sr = np.zeros_like(y)
+            lib.emul_ax(P(X), P(y), P(sr), g["cos_phi"], g["sin_phi"], P(g["y_det"]),
+                        P(g["z_det"]), g["SO"], g["OD"], g["scale"], np.float32(1), nx, ny, nz,
+                        na, nb, 1, 0)
+            x = np.zeros(nx * ny * nz, np.float32)
+            sc = np.zeros_like(x)
+            lib.emul_atx(P(x), P(Y), P(sc), g["cos_phi"], g["sin_phi"], P(g["y_det"]),
+                         P(g["z_det"]), g["SO"], g["OD"], g["scale"], g["det"], g["det"],
+                         np.float32(1), nx, ny, nz, na, nb)
+            out.append((y, sr, x, sc))
+        lib.emul_set_general(0)
+        for a, b in zip(*out):
+            assert np.array_equal(a, b), (phi, nx, ny, nz, na, nb)
