@@ -11,6 +11,7 @@ produces, in this package directory:
                            process (MATLAB, or the test host in tests/fakemex).
 """
 import os
+import re
 import shutil
 import subprocess
 import sys
@@ -58,7 +59,18 @@ def build(verbose=False, jobs=None):
 
     def obj(src):
         o = os.path.join(BUILD, src.replace(".hip", ".o"))
-        _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", o])
+        r = _run([HIPCC, *HIP_FLAGS, "-Rpass-analysis=kernel-resource-usage", "-c",
+                  os.path.join(CSRC, src), "-o", o])
+        # No kernel may use scratch (private) memory: on gfx950 it goes through
+        # L2 to HBM, costing a store and a load per lane per use.
+        name = None
+        for line in (r.stdout + r.stderr).splitlines():
+            m = re.search(r"Function Name: (\S+)", line)
+            if m:
+                name = m.group(1)
+            m = re.search(r"ScratchSize \[bytes/lane\]: (\d+)", line)
+            if m and int(m.group(1)) > 0:
+                raise RuntimeError(f"{src}: kernel {name} uses {m.group(1)} bytes/lane of scratch")
         return o
 
     with ThreadPoolExecutor(jobs) as ex:

@@ -225,6 +225,8 @@ struct asd_plan {
     TvScratch red;
     bool have_P = false, have_I0 = false;
     int lpr = 1;
+    int *h_stop = nullptr;          // pinned copies of the stop flag (4 checkpoints)
+    hipEvent_t stop_ev[4] = {};
     int atx_variant = -1;           // backprojector variant picked by the autotuner
     double atx_us[ATX_NVARIANT];    // its measurements (-1: not tried)
     DevCtl h;  // host copy of the control block after the last sync
@@ -361,7 +363,9 @@ int enqueue_sweep_t(asd_plan *p, const double *relax)
         int rc;
         {
             StageTimer t(p, ST_AX);
-            if ((rc = dispatch_ax<KAHAN>(p->lpr, g, p->m.as<float>(), yd, zd, ea, p->stream)))
+            // null detector tables: the gateway's grid is computed in the kernel
+            if ((rc = dispatch_ax<KAHAN>(p->lpr, g, p->m.as<float>(), nullptr, nullptr, ea,
+                                         p->stream)))
                 return rc;
         }
         {
@@ -644,6 +648,17 @@ int asd_plan_create(const asd_para *para, const double *phi_vec, int nv, asd_pla
         set_error(ASD_ERR_HIP, "hipStreamCreateWithFlags failed");
         return fail(ASD_ERR_HIP);
     }
+    if (hipHostMalloc((void **)&p->h_stop, 4 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
+        p->h_stop = nullptr;
+        set_error(ASD_ERR_HIP, "hipHostMalloc failed");
+        return fail(ASD_ERR_HIP);
+    }
+    for (hipEvent_t &e : p->stop_ev)
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+            e = nullptr;
+            set_error(ASD_ERR_HIP, "hipEventCreate failed");
+            return fail(ASD_ERR_HIP);
+        }
     if ((rc = p->ydet.ensure(yall.size() * 4)) || (rc = p->zdet.ensure(zall.size() * 4)) ||
         (rc = p->m.ensure(N * 4)) || (rc = p->mpre.ensure(N * 4)) ||
         (rc = p->mk1.ensure(N * 4)) || (rc = p->f0.ensure(N * 4)) || (rc = p->v.ensure(N * 4)) ||
@@ -752,12 +767,29 @@ int asd_plan_iterate(asd_plan *p, int n_iter)
     if ((rc = need_projections(p))) return rc;
     if (p->h.stopped || n_iter == 0) return ASD_OK;
     if ((rc = autotune_atx(p))) return rc;
-    if (p->use_graph && !p->timing) {
-        if (!p->exec && (rc = build_graph(p))) return rc;
-        for (int i = 0; i < n_iter; i++) ASD_HIP(hipGraphLaunch(p->exec, p->stream));
-    } else {
-        for (int i = 0; i < n_iter; i++)
-            if ((rc = enqueue_iteration(p))) return rc;
+    const bool graph = p->use_graph && !p->timing;
+    if (graph && !p->exec && (rc = build_graph(p))) return rc;
+    // The stop flag is copied to pinned host memory after every STOP_EVERY-th
+    // iteration; the host reads the copy two checkpoints later, so once the
+    // loop stops at most ~3 * STOP_EVERY further iterations are enqueued (their
+    // kernels change nothing) while the GPU never waits for the host.
+    constexpr int STOP_EVERY = 4;
+    for (int i = 0; i < n_iter; i++) {
+        if (i >= 2 * STOP_EVERY && i % STOP_EVERY == 0) {
+            const int c = i / STOP_EVERY - 2;
+            ASD_HIP(hipEventSynchronize(p->stop_ev[c % 4]));
+            if (p->h_stop[c % 4]) break;
+        }
+        if (graph)
+            ASD_HIP(hipGraphLaunch(p->exec, p->stream));
+        else if ((rc = enqueue_iteration(p)))
+            return rc;
+        if (i % STOP_EVERY == STOP_EVERY - 1) {
+            const int c = i / STOP_EVERY;
+            ASD_HIP(hipMemcpyAsync(&p->h_stop[c % 4], &p->ctl()->stopped, sizeof(int),
+                                   hipMemcpyDeviceToHost, p->stream));
+            ASD_HIP(hipEventRecord(p->stop_ev[c % 4], p->stream));
+        }
     }
     return sync_ctl(p);
 }
@@ -887,6 +919,9 @@ void asd_plan_destroy(asd_plan *p)
         collect_timing(p);
     }
     drop_graph(p);
+    for (hipEvent_t &e : p->stop_ev)
+        if (e) (void)hipEventDestroy(e);
+    if (p->h_stop) (void)hipHostFree(p->h_stop);
     if (p->stream) (void)hipStreamDestroy(p->stream);
     delete p;
 }

@@ -59,6 +59,15 @@ __host__ __device__ __forceinline__ AxRay ax_ray_setup(const ViewGeom &g, float 
     return r;
 }
 
+// Detector coordinate of bin i of n at pitch det (voxels), as the MEX
+// gateways tabulate it (Ax_main_test.cpp:156-160: FP64 arithmetic rounded
+// to FP32); host and device give the same value.
+__host__ __device__ __forceinline__ float det_coord(int i, int n, float det)
+{
+#pragma clang fp contract(off)
+    return (float)(((double)i - (double)n / 2.0 + 0.5) * (double)det);
+}
+
 // Derived per-view constants (host and device give the same FP32 values).
 __host__ __device__ __forceinline__ void geom_finish(ViewGeom &g)
 {
@@ -217,17 +226,22 @@ __host__ __device__ __forceinline__ void ax_slabs(const AxRay &r, const float *_
 // evaluated from the per-ray slope record.  (px,py,pz) = source - voxel centre.
 // ---------------------------------------------------------------------------
 
-// Take a candidate hit in test order; the first two define the chord.
+// Take a candidate hit in test order; the first two define the chord.  The
+// choice is written as selects: a branch per member lets the compiler merge
+// the stores into one indexed by n, which lives in scratch memory.
 struct Hits {
-    float p0x, p0y, p0z, p1x, p1y, p1z;
-    int n;
+    float p0x = 0.0f, p0y = 0.0f, p0z = 0.0f, p1x = 0.0f, p1y = 0.0f, p1z = 0.0f;
+    int n = 0;
     __host__ __device__ __forceinline__ void take(bool ok, float x, float y, float z)
     {
-        if (ok) {
-            if (n == 0) { p0x = x; p0y = y; p0z = z; }
-            else if (n == 1) { p1x = x; p1y = y; p1z = z; }
-            n++;
-        }
+        const bool t0 = ok && n == 0, t1 = ok && n == 1;
+        p0x = t0 ? x : p0x;
+        p0y = t0 ? y : p0y;
+        p0z = t0 ? z : p0z;
+        p1x = t1 ? x : p1x;
+        p1y = t1 ? y : p1y;
+        p1z = t1 ? z : p1z;
+        n += ok ? 1 : 0;
     }
 };
 

@@ -10,6 +10,11 @@
 
 namespace asd {
 
+// The stop flag of the device-driven loop is read at kernel entry but only
+// consulted before the epilogue's stores, so its load overlaps the work
+// instead of delaying it (a stopped loop's kernels compute and store nothing;
+// the host stops replaying within a few iterations).
+//
 // Forward projector: LPR lanes march one ray together (lane j takes slabs
 // j, j+LPR, ...), then a butterfly over the lanes of a wave and, when a ray
 // spans several waves (LPR = 128, 256), a fixed-order sum of the wave
@@ -25,7 +30,7 @@ __global__ __launch_bounds__(256) void ax_kernel(ViewGeom g, const float *__rest
     constexpr int RPB = LPR >= 256 ? 1 : 256 / LPR;  // rays per block
     constexpr int WL = LPR < 64 ? LPR : 64;          // lanes of a ray inside one wave
     constexpr int WPR = LPR > 64 ? LPR / 64 : 1;     // waves per ray
-    if (epi.skip()) return;
+    const bool stop = epi.skip();
     const long nrays = (long)g.na * g.nb;
     const int lane = threadIdx.x % LPR;
     const long id = (long)blockIdx.x * RPB + threadIdx.x / LPR;
@@ -35,7 +40,10 @@ __global__ __launch_bounds__(256) void ax_kernel(ViewGeom g, const float *__rest
     Acc<KAHAN> sa, sl;
     if (active) {
         int ia = (int)(id % g.na), ib = (int)(id / g.na);
-        AxRay r = ax_ray_setup(g, y_det[ia], z_det[ib]);
+        // null tables: the gateway's regular detector grid, computed in place
+        const float yd = y_det ? y_det[ia] : det_coord(ia, g.na, g.dy_det);
+        const float zd = z_det ? z_det[ib] : det_coord(ib, g.nb, g.dz_det);
+        AxRay r = ax_ray_setup(g, yd, zd);
         int nu = r.xdriven ? g.nx : g.ny;
         // slabs per load batch: two when a ray has 128 or more lanes (a
         // 512-slab ray then needs one batch per lane), else four
@@ -75,7 +83,7 @@ __global__ __launch_bounds__(256) void ax_kernel(ViewGeom g, const float *__rest
             }
         }
     }
-    if (active && lane == 0) epi(id, (float)a, (float)b, pre);
+    if (active && lane == 0 && !stop) epi(id, (float)a, (float)b, pre);
 }
 
 template <bool KAHAN = false, class Epi>
@@ -114,7 +122,7 @@ __global__ __launch_bounds__(256) void atx_kernel(ViewGeom g, const float *__res
                                                   const RayRec *__restrict__ rt, Epi epi)
 {
 #pragma clang fp contract(off)
-    if (epi.skip()) return;
+    const bool stop = epi.skip();
     const unsigned b = xcd_block(blockIdx.x, gridDim.x);
     const unsigned v = b * 256u + threadIdx.x;
     const unsigned N = (unsigned)g.nx * (unsigned)g.ny * (unsigned)g.nz;
@@ -150,7 +158,7 @@ __global__ __launch_bounds__(256) void atx_kernel(ViewGeom g, const float *__res
             }
         }
     }
-    epi(v, acc.value(), accl.value(), pre);
+    if (!stop) epi(v, acc.value(), accl.value(), pre);
 }
 
 // Backprojector on voxel tiles (TX x TY x TZ = 256 voxels per workgroup):
@@ -169,7 +177,7 @@ __global__ __launch_bounds__(256) void atx_tile_plain_kernel(ViewGeom g,
     __shared__ RayRec s_rec[TILE_MAXR];
     __shared__ float s_y[TILE_MAXR];
     __shared__ int s_rng[4];
-    if (epi.skip()) return;
+    const bool stop = epi.skip();
     const TileDims td = tile_dims(g, TX, TY, TZ);
     const unsigned ntiles = (unsigned)td.ntx * td.nty * td.ntz;
     const unsigned t = xcd_block(blockIdx.x, gridDim.x);
@@ -237,7 +245,7 @@ __global__ __launch_bounds__(256) void atx_tile_plain_kernel(ViewGeom g,
             }
         }
     }
-    if (valid) epi(vox, acc.value(), accl.value(), pre);
+    if (valid && !stop) epi(vox, acc.value(), accl.value(), pre);
 }
 
 // Backprojector variants (all bit-identical; the plan's autotuner picks one).

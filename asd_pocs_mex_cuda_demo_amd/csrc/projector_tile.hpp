@@ -209,7 +209,7 @@ __global__ __launch_bounds__(256) void atx_tile_kernel(ViewGeom g, const float *
     __shared__ TileRayZ s_z[TILE_MAXR];
     __shared__ float s_cor[TILE_MAXR];
     __shared__ int s_rng[4];
-    if (epi.skip()) return;
+    const bool stop = epi.skip();
     const TileDims td = tile_dims(g, TX, TY, TZ);
     const unsigned ntiles = (unsigned)td.ntx * td.nty * td.ntz;
     const unsigned t = xcd_block(blockIdx.x, gridDim.x);
@@ -290,7 +290,7 @@ __global__ __launch_bounds__(256) void atx_tile_kernel(ViewGeom g, const float *
             }
         }
     }
-    if (valid) epi(vox, acc.value(), accl.value(), pre);
+    if (valid && !stop) epi(vox, acc.value(), accl.value(), pre);
 }
 
 }  // namespace asd

@@ -72,12 +72,18 @@ __device__ __forceinline__ A tv3_grad(const Src &U, int x, int y, int z)
 }
 
 // s = alpha * d / ||v|| of the previous sub-iteration (reference: the step
-// I -= alpha*d*v/||v||, imgTVGradMin.cpp:104-107).
+// I -= alpha*d*v/||v||, imgTVGradMin.cpp:104-107).  ||v||^2 is folded from
+// the previous kernel's block partials inside every block, in the fixed
+// order of fold_partials, so each block gets the same value and no separate
+// fold kernel runs between sub-iterations.
 __device__ __forceinline__ double tv_step_size(double alpha, const double *alpha_dev,
-                                               const double *d2, const double *nrm2)
+                                               const double *d2, const double *partials_prev,
+                                               int nb_prev)
 {
+    double nrm2[1];
+    fold_rows<1>(partials_prev, nb_prev, nrm2);
     const double a = alpha_dev ? *alpha_dev : alpha;
-    return a * sqrt(*d2) / sqrt(*nrm2);
+    return a * sqrt(*d2) / sqrt(nrm2[0]);
 }
 
 // One TV gradient: v = grad TV(u) with FP64 block partials of ||v||^2.  With
@@ -90,12 +96,12 @@ __global__ __launch_bounds__(256) void tv_grad_kernel(const T *__restrict__ u,
                                                       T *__restrict__ u_next, T *__restrict__ v,
                                                       int nx, int ny, int nz, double alpha,
                                                       const double *alpha_dev, const double *d2,
-                                                      const double *nrm2_prev, double *partials,
-                                                      const int *skip)
+                                                      const double *partials_prev, int nb_prev,
+                                                      double *partials, const int *skip)
 {
     if (skip && *skip) return;
     TvSrc<T, A, STEPPED> U{u, v_prev, 0.0, nx, ny, nz};
-    if constexpr (STEPPED) U.s = tv_step_size(alpha, alpha_dev, d2, nrm2_prev);
+    if constexpr (STEPPED) U.s = tv_step_size(alpha, alpha_dev, d2, partials_prev, nb_prev);
     const size_t n = (size_t)nx * ny * nz;
     double s = 0.0;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -121,12 +127,12 @@ __global__ __launch_bounds__(256) void tv_step_kernel(const T *__restrict__ u,
                                                       double alpha,
                                                       const double *__restrict__ alpha_dev,
                                                       const double *__restrict__ d2,
-                                                      const double *__restrict__ nrm2, int clamp,
-                                                      const int *skip)
+                                                      const double *__restrict__ partials_prev,
+                                                      int nb_prev, int clamp, const int *skip)
 {
 #pragma clang fp contract(off)
     if (skip && *skip) return;
-    const double st = tv_step_size(alpha, alpha_dev, d2, nrm2);
+    const double st = tv_step_size(alpha, alpha_dev, d2, partials_prev, nb_prev);
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (size_t)gridDim.x * blockDim.x) {
         double w = (double)u[i] - st * (double)v[i];
@@ -180,27 +186,32 @@ int tv_descent(T *u, const T *img_old, T *v, T *u2, T *v2, int nx, int ny, int n
     if (!have_d2 && (rc = launch_sqdiff<T>(u, img_old, n, s, TV_SLOT_D2, st))) return rc;
     if (nTVIter <= 0) return ASD_OK;
     const unsigned gb = red_blocks(n), ub = stream_blocks(n);
-    double *d2 = s.scalars + TV_SLOT_D2, *nrm2 = s.scalars + TV_SLOT_NRM2;
+    double *d2 = s.scalars + TV_SLOT_D2;
+    // gradient partials alternate between two rows: kernel i writes one while
+    // reading kernel i-1's in the other
+    double *pp[2] = {s.partials, s.partials + RED_MAX_BLOCKS};
     // (u, v) holds sub-iteration i's image and gradient, alternating with (u2, v2)
     T *cu = u, *cv = v, *nu = u2, *nv = v2;
     for (int it = 0; it < nTVIter; it++) {
+        double *out = pp[it & 1];
+        const double *prev = pp[(it + 1) & 1];
         if (it == 0) {
             if (three_d)
                 tv_grad_kernel<T, A, true, false><<<gb, RED_THREADS, 0, st>>>(
-                    cu, nullptr, nullptr, cv, nx, ny, nz, alpha, alpha_dev, d2, nrm2, s.partials,
+                    cu, nullptr, nullptr, cv, nx, ny, nz, alpha, alpha_dev, d2, nullptr, 0, out,
                     skip);
             else
                 tv_grad_kernel<T, A, false, false><<<gb, RED_THREADS, 0, st>>>(
-                    cu, nullptr, nullptr, cv, nx, ny, 1, alpha, alpha_dev, d2, nrm2, s.partials,
+                    cu, nullptr, nullptr, cv, nx, ny, 1, alpha, alpha_dev, d2, nullptr, 0, out,
                     skip);
         } else {
             // step of sub-iteration it-1 fused into gradient it
             if (three_d)
                 tv_grad_kernel<T, A, true, true><<<gb, RED_THREADS, 0, st>>>(
-                    cu, cv, nu, nv, nx, ny, nz, alpha, alpha_dev, d2, nrm2, s.partials, skip);
+                    cu, cv, nu, nv, nx, ny, nz, alpha, alpha_dev, d2, prev, (int)gb, out, skip);
             else
                 tv_grad_kernel<T, A, false, true><<<gb, RED_THREADS, 0, st>>>(
-                    cu, cv, nu, nv, nx, ny, 1, alpha, alpha_dev, d2, nrm2, s.partials, skip);
+                    cu, cv, nu, nv, nx, ny, 1, alpha, alpha_dev, d2, prev, (int)gb, out, skip);
             T *t1 = cu, *t2 = cv;
             cu = nu;
             cv = nv;
@@ -208,12 +219,11 @@ int tv_descent(T *u, const T *img_old, T *v, T *u2, T *v2, int nx, int ny, int n
             nv = t2;
         }
         ASD_LAUNCH_CHECK("tv_grad_kernel");
-        fold_partials<1><<<1, RED_THREADS, 0, st>>>(s.partials, (int)gb, nrm2, skip);
-        ASD_LAUNCH_CHECK("fold_partials");
     }
     // last step (with positivity) lands in the caller's buffer u
-    tv_step_kernel<T><<<ub, 256, 0, st>>>(cu, cv, u, n, alpha, alpha_dev, d2, nrm2,
-                                          pos == 1 ? 1 : 0, skip);
+    tv_step_kernel<T><<<ub, 256, 0, st>>>(cu, cv, u, n, alpha, alpha_dev, d2,
+                                          pp[(nTVIter - 1) & 1], (int)gb, pos == 1 ? 1 : 0,
+                                          skip);
     ASD_LAUNCH_CHECK("tv_step_kernel");
     return ASD_OK;
 }

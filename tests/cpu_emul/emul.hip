@@ -46,6 +46,9 @@ extern "C" {
 
 void emul_set_general(int on) { g_general = on; }
 
+// det_coord, the detector grid the forward projector computes in place.
+float emul_det_coord(int i, int n, float det) { return det_coord(i, n, det); }
+
 // ax_kernel<LPR=lpr>: lane j of a ray takes slabs j, j+lpr, ...; lanes are
 // combined by the same butterfly order as the wave shuffles (within groups of
 // at most 64 lanes), then the wave partials of a ray in wave order.

@@ -192,3 +192,20 @@ def test_flat_and_unit_pitch_shortcuts_change_no_bit():
         lib.emul_set_general(0)
         for a, b in zip(*out):
             assert np.array_equal(a, b), (phi, nx, ny, nz, na, nb)
+
+
+def test_in_kernel_detector_grid_matches_gateway_tables(orc):
+    """The forward projector computes the gateway's detector coordinates in
+    place (det_coord); they equal the tables the gateways build
+    (Ax_main_test.cpp:156-160), bit for bit, for even and odd bin counts."""
+    lib = emul()
+    lib.emul_det_coord.argtypes = [C.c_int, C.c_int, C.c_float]
+    lib.emul_det_coord.restype = C.c_float
+    rng = np.random.default_rng(8)
+    for _ in range(20):
+        na, nb = int(rng.integers(1, 1600)), int(rng.integers(1, 800))
+        vxl, det = float(rng.uniform(0.1, 2)), float(rng.uniform(0.1, 3))
+        g = orc.gateway_geometry(-0.4, 500.0, 400.0, vxl, det, na, nb)
+        ya = np.array([lib.emul_det_coord(i, na, g["det"]) for i in range(na)], np.float32)
+        zb = np.array([lib.emul_det_coord(j, nb, g["det"]) for j in range(nb)], np.float32)
+        assert np.array_equal(ya, g["y_det"]) and np.array_equal(zb, g["z_det"])
