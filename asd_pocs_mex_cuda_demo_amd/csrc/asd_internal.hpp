@@ -65,8 +65,9 @@ struct ViewGeom {
 // Per-ray record for the backprojector: the six slope quotients the
 // reference's find_l_3d forms per (voxel, ray) (cu:142-196) depend only on the
 // ray, so they are formed once per ray with the same IEEE divisions.
-struct alignas(32) RayRec {
+struct alignas(64) RayRec {
     float s_ba, s_ab, s_ca, s_cb, s_ac, s_bc;
+    float o_ba, o_ab, o_ca, o_cb, o_ac, o_bc;  // face offsets s * h (h = 1/2, 1/2, dz/2)
     int mode;   // RAY_* below
     int flat;   // ray in the z = 0 face of a one-slice volume (make_ray_rec)
 };

@@ -227,37 +227,40 @@ __host__ __device__ __forceinline__ void ax_slabs(const AxRay &r, const float *_
 // ---------------------------------------------------------------------------
 
 // Take a candidate hit in test order; the first two define the chord.  The
-// choice is written as selects: a branch per member lets the compiler merge
-// the stores into one indexed by n, which lives in scratch memory.
+// state is two flags (one / two hits taken): per candidate the flag logic
+// is lane-mask arithmetic and the coordinates are selects, so nothing is
+// stored through an index (which would land in scratch memory).
 struct Hits {
     float p0x = 0.0f, p0y = 0.0f, p0z = 0.0f, p1x = 0.0f, p1y = 0.0f, p1z = 0.0f;
-    int n = 0;
+    bool a0 = false, a1 = false;
     __host__ __device__ __forceinline__ void take(bool ok, float x, float y, float z)
     {
-        const bool t0 = ok && n == 0, t1 = ok && n == 1;
+        const bool t0 = ok && !a0, t1 = ok && a0 && !a1;
         p0x = t0 ? x : p0x;
         p0y = t0 ? y : p0y;
         p0z = t0 ? z : p0z;
         p1x = t1 ? x : p1x;
         p1y = t1 ? y : p1y;
         p1z = t1 ? z : p1z;
-        n += ok ? 1 : 0;
+        a1 = a1 || (ok && a0);
+        a0 = a0 || ok;
     }
+    __host__ __device__ __forceinline__ bool chord() const { return a1; }
 };
 
 // General 2-D chord (find_l with a' != 0, b' != 0) in plane (u,w):
-// pu = u1_0 - uc, pw = w1_0 - wc, s_wu = b'/a', s_uw = a'/b'.
-__host__ __device__ __forceinline__ float chord2(float pu, float pw, float s_wu, float s_uw, float hu,
-                                        float hw, bool u_first)
+// pu = u1_0 - uc, pw = w1_0 - wc, s_wu = b'/a', s_uw = a'/b', and the face
+// offsets ow = s_wu * hu, ou = s_uw * hw (per ray).
+__host__ __device__ __forceinline__ float chord2(float pu, float pw, float s_wu, float s_uw,
+                                                 float ow, float ou, float hu, float hw,
+                                                 bool u_first)
 {
 #pragma clang fp contract(off)
-    float cw = s_wu * (-pu) + pw, ow = s_wu * hu;   // u faces: w coordinate
-    float cu = s_uw * (-pw) + pu, ou = s_uw * hw;   // w faces: u coordinate
+    float cw = s_wu * (-pu) + pw;   // u faces: w coordinate
+    float cu = s_uw * (-pw) + pu;   // w faces: u coordinate
     bool um = absf(cw - ow) <= hw, up = absf(cw + ow) <= hw;
     bool wm = absf(cu - ou) <= hu, wp = absf(cu + ou) <= hu;
     Hits H;
-    H.n = 0;
-    H.p0z = H.p1z = 0.0f;
     if (u_first) {
         H.take(um, -hu, cw - ow, 0.0f);
         H.take(up, hu, cw + ow, 0.0f);
@@ -269,7 +272,7 @@ __host__ __device__ __forceinline__ float chord2(float pu, float pw, float s_wu,
         H.take(um, -hu, cw - ow, 0.0f);
         H.take(up, hu, cw + ow, 0.0f);
     }
-    if (H.n < 2) return 0.0f;
+    if (!H.chord()) return 0.0f;
     float ex = H.p1x - H.p0x, ey = H.p1y - H.p0y;
     return sqrtf(ex * ex + ey * ey);
 }
@@ -287,13 +290,15 @@ __host__ __device__ __forceinline__ float find_l_3d_rec(const RayRec &R, float p
         // (|pz| == hz) and ez = 0; s_ac, s_bc = +-inf make the z-face tests
         // NaN / inf comparisons that fail.  What is left is the in-plane
         // chord, evaluated by the same expressions in the same order.
-        if (R.flat) return chord2(px, py, R.s_ba, R.s_ab, hx, hy, R.mode == RAY_X_FIRST);
-        float xy_c = R.s_ba * (-px) + py, xy_o = R.s_ba * hx;
-        float xz_c = R.s_ca * (-px) + pz, xz_o = R.s_ca * hx;
-        float yx_c = R.s_ab * (-py) + px, yx_o = R.s_ab * hy;
-        float yz_c = R.s_cb * (-py) + pz, yz_o = R.s_cb * hy;
-        float zx_c = R.s_ac * (-pz) + px, zx_o = R.s_ac * hz;
-        float zy_c = R.s_bc * (-pz) + py, zy_o = R.s_bc * hz;
+        if (R.flat)
+            return chord2(px, py, R.s_ba, R.s_ab, R.o_ba, R.o_ab, hx, hy, R.mode == RAY_X_FIRST);
+        // face offsets s * h are per ray (R.o_*), the same FP32 products
+        const float xy_c = R.s_ba * (-px) + py, xy_o = R.o_ba;
+        const float xz_c = R.s_ca * (-px) + pz, xz_o = R.o_ca;
+        const float yx_c = R.s_ab * (-py) + px, yx_o = R.o_ab;
+        const float yz_c = R.s_cb * (-py) + pz, yz_o = R.o_cb;
+        const float zx_c = R.s_ac * (-pz) + px, zx_o = R.o_ac;
+        const float zy_c = R.s_bc * (-pz) + py, zy_o = R.o_bc;
         bool xm = absf(xy_c - xy_o) <= hy && absf(xz_c - xz_o) <= hz;
         bool xp = absf(xy_c + xy_o) <= hy && absf(xz_c + xz_o) <= hz;
         bool ym = absf(yx_c - yx_o) <= hx && absf(yz_c - yz_o) <= hz;
@@ -301,7 +306,6 @@ __host__ __device__ __forceinline__ float find_l_3d_rec(const RayRec &R, float p
         bool zm = absf(zx_c - zx_o) <= hx && absf(zy_c - zy_o) <= hy;
         bool zp = absf(zx_c + zx_o) <= hx && absf(zy_c + zy_o) <= hy;
         Hits H;
-        H.n = 0;
         if (R.mode == RAY_X_FIRST) {
             H.take(xm, -hx, xy_c - xy_o, xz_c - xz_o);
             H.take(xp, hx, xy_c + xy_o, xz_c + xz_o);
@@ -315,17 +319,17 @@ __host__ __device__ __forceinline__ float find_l_3d_rec(const RayRec &R, float p
         }
         H.take(zm, zx_c - zx_o, zy_c - zy_o, -hz);
         H.take(zp, zx_c + zx_o, zy_c + zy_o, hz);
-        if (H.n < 2) return 0.0f;
+        if (!H.chord()) return 0.0f;
         float ex = H.p1x - H.p0x, ey = H.p1y - H.p0y, ez = H.p1z - H.p0z;
         return sqrtf(ex * ex + ey * ey + ez * ez);
     }
     case RAY_A0_B0: return absf(py) <= hy ? dz : 0.0f;
     case RAY_A0_C0: return absf(pz) <= hz ? 1.0f : 0.0f;
-    case RAY_A0_YZ_U: return chord2(py, pz, R.s_cb, R.s_bc, hy, hz, true);
-    case RAY_A0_YZ_W: return chord2(py, pz, R.s_cb, R.s_bc, hy, hz, false);
+    case RAY_A0_YZ_U: return chord2(py, pz, R.s_cb, R.s_bc, R.o_cb, R.o_bc, hy, hz, true);
+    case RAY_A0_YZ_W: return chord2(py, pz, R.s_cb, R.s_bc, R.o_cb, R.o_bc, hy, hz, false);
     case RAY_B0_C0: return absf(pz) <= hz ? 1.0f : 0.0f;
-    case RAY_B0_XZ_U: return chord2(px, pz, R.s_ca, R.s_ac, hx, hz, true);
-    default: return chord2(px, pz, R.s_ca, R.s_ac, hx, hz, false);
+    case RAY_B0_XZ_U: return chord2(px, pz, R.s_ca, R.s_ac, R.o_ca, R.o_ac, hx, hz, true);
+    default: return chord2(px, pz, R.s_ca, R.s_ac, R.o_ca, R.o_ac, hx, hz, false);
     }
 }
 
@@ -341,8 +345,8 @@ __host__ __device__ __forceinline__ bool inplane_may_hit(const RayRec &R, float 
 #pragma clang fp contract(off)
     if (R.mode != RAY_X_FIRST && R.mode != RAY_Y_FIRST) return true;
     const float lim = 0.5f + 1e-2f;
-    const float xy_c = R.s_ba * (-px) + py, xy_o = R.s_ba * 0.5f;
-    const float yx_c = R.s_ab * (-py) + px, yx_o = R.s_ab * 0.5f;
+    const float xy_c = R.s_ba * (-px) + py, xy_o = R.o_ba;
+    const float yx_c = R.s_ab * (-py) + px, yx_o = R.o_ab;
     return absf(xy_c - xy_o) <= lim || absf(xy_c + xy_o) <= lim || absf(yx_c - yx_o) <= lim ||
            absf(yx_c + yx_o) <= lim;
 }
@@ -360,6 +364,11 @@ __host__ __device__ __forceinline__ RayRec make_ray_rec(const ViewGeom &g, float
     R.s_ba = b / a; R.s_ab = a / b;
     R.s_ca = c / a; R.s_cb = c / b;
     R.s_ac = a / c; R.s_bc = b / c;
+    // face offsets of find_l_3d (hx = hy = 1/2, hz = dz/2)
+    const float hx = 0.5f, hy = 0.5f, hz = g.dz / 2.0f;
+    R.o_ba = R.s_ba * hx; R.o_ca = R.s_ca * hx;
+    R.o_ab = R.s_ab * hy; R.o_cb = R.s_cb * hy;
+    R.o_ac = R.s_ac * hz; R.o_bc = R.s_bc * hz;
     R.flat = 0;
     if (a == 0) {
         if (b == 0) R.mode = RAY_A0_B0;

@@ -112,7 +112,6 @@ __host__ __device__ __forceinline__ float tile_find_l(const TileRayXY &r, const 
     const bool zm = absf(zx - zxo) <= hx && absf(zy - zyo) <= hy;
     const bool zp = absf(zx + zxo) <= hx && absf(zy + zyo) <= hy;
     Hits H;
-    H.n = 0;
     if (r.mode == RAY_X_FIRST) {
         H.take(xm, -hx, xy - xyo, xz - xzo);
         H.take(xp, hx, xy + xyo, xz + xzo);
@@ -126,7 +125,7 @@ __host__ __device__ __forceinline__ float tile_find_l(const TileRayXY &r, const 
     }
     H.take(zm, zx - zxo, zy - zyo, -hz);
     H.take(zp, zx + zxo, zy + zyo, hz);
-    if (H.n < 2) return 0.0f;
+    if (!H.chord()) return 0.0f;
     const float ex = H.p1x - H.p0x, ey = H.p1y - H.p0y, ez = H.p1z - H.p0z;
     return sqrtf(ex * ex + ey * ey + ez * ez);
 }
@@ -164,7 +163,7 @@ __device__ __forceinline__ void tile_union(bool has, const Footprint &f, int *rn
 }
 
 constexpr int TILE_MAXA = 256;   // in-plane ray records per chunk
-constexpr int TILE_MAXR = 1024;  // (ia, ib) records per chunk
+constexpr int TILE_MAXR = 512;  // (ia, ib) records per chunk
 
 // Tile geometry shared by the kernel and its host check.
 struct TileDims {

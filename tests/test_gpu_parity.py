@@ -232,7 +232,12 @@ def test_precision_variant_matches_plain_rmse():
     m1, o1 = ops.ASD_POCS_cuda(Pn, dict(para, precision=1))
     assert o0["alpha"] == o1["alpha"] == 0.2
     assert np.all(np.abs(o1["RMSE"] - o0["RMSE"]) <= 1e-4 * o0["RMSE"]), (o0["RMSE"], o1["RMSE"])
-    assert rel_l2(m1, m0) <= 1e-3
+    # pixel by pixel the images differ by what the TV descent makes of any
+    # change of the data: compare with the plain form on data perturbed by 1e-7
+    mp, _ = ops.ASD_POCS_cuda(Pn * (1 + 1e-7 * np.random.default_rng(3).standard_normal(Pn.shape)),
+                              para)
+    floor = rel_l2(mp, m0)
+    assert rel_l2(m1, m0) <= max(3 * floor, 1e-4), (rel_l2(m1, m0), floor)
     m2, o2 = ops.ASD_POCS_cuda(Pn, dict(para, precision=2))
     assert np.all(np.abs(o2["RMSE"] - o0["RMSE"]) <= 1e-2 * o0["RMSE"])
     # precision 3 computes the backprojector's lengths to ~1e-6 voxel where
