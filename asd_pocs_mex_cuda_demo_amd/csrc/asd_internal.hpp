@@ -65,9 +65,8 @@ struct ViewGeom {
 // Per-ray record for the backprojector: the six slope quotients the
 // reference's find_l_3d forms per (voxel, ray) (cu:142-196) depend only on the
 // ray, so they are formed once per ray with the same IEEE divisions.
-struct alignas(64) RayRec {
+struct alignas(32) RayRec {
     float s_ba, s_ab, s_ca, s_cb, s_ac, s_bc;
-    float o_ba, o_ab, o_ca, o_cb, o_ac, o_bc;  // face offsets s * h (h = 1/2, 1/2, dz/2)
     int mode;   // RAY_* below
     int flat;   // ray in the z = 0 face of a one-slice volume (make_ray_rec)
 };
@@ -104,5 +103,21 @@ int launch_atx(const ViewGeom &g, float *x, const float *y, float *sc, const flo
 
 // Lanes cooperating on one ray of the forward projector.
 int ax_lanes_per_ray(int na, int nb, int nu);
+
+// XCD-aware block order: hardware hands consecutive workgroups to the 8 XCDs
+// round-robin; remap so that each XCD walks a contiguous range of tiles and
+// neighbouring tiles share its L2.
+__host__ __device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nblocks)
+{
+    constexpr unsigned NXCD = 8;
+    if (nblocks < NXCD * 2) return b;
+    unsigned per = nblocks / NXCD, rem = nblocks % NXCD;
+    unsigned xcd = b % NXCD, k = b / NXCD;
+    // XCDs < rem own per+1 tiles
+    unsigned start = xcd * per + (xcd < rem ? xcd : rem);
+    unsigned cnt = per + (xcd < rem ? 1u : 0u);
+    if (k < cnt) return start + k;
+    return b;  // unreachable when nblocks is covered exactly
+}
 
 }  // namespace asd

@@ -291,14 +291,14 @@ __host__ __device__ __forceinline__ float find_l_3d_rec(const RayRec &R, float p
         // NaN / inf comparisons that fail.  What is left is the in-plane
         // chord, evaluated by the same expressions in the same order.
         if (R.flat)
-            return chord2(px, py, R.s_ba, R.s_ab, R.o_ba, R.o_ab, hx, hy, R.mode == RAY_X_FIRST);
-        // face offsets s * h are per ray (R.o_*), the same FP32 products
-        const float xy_c = R.s_ba * (-px) + py, xy_o = R.o_ba;
-        const float xz_c = R.s_ca * (-px) + pz, xz_o = R.o_ca;
-        const float yx_c = R.s_ab * (-py) + px, yx_o = R.o_ab;
-        const float yz_c = R.s_cb * (-py) + pz, yz_o = R.o_cb;
-        const float zx_c = R.s_ac * (-pz) + px, zx_o = R.o_ac;
-        const float zy_c = R.s_bc * (-pz) + py, zy_o = R.o_bc;
+            return chord2(px, py, R.s_ba, R.s_ab, R.s_ba * hx, R.s_ab * hy, hx, hy,
+                          R.mode == RAY_X_FIRST);
+        const float xy_c = R.s_ba * (-px) + py, xy_o = R.s_ba * hx;
+        const float xz_c = R.s_ca * (-px) + pz, xz_o = R.s_ca * hx;
+        const float yx_c = R.s_ab * (-py) + px, yx_o = R.s_ab * hy;
+        const float yz_c = R.s_cb * (-py) + pz, yz_o = R.s_cb * hy;
+        const float zx_c = R.s_ac * (-pz) + px, zx_o = R.s_ac * hz;
+        const float zy_c = R.s_bc * (-pz) + py, zy_o = R.s_bc * hz;
         bool xm = absf(xy_c - xy_o) <= hy && absf(xz_c - xz_o) <= hz;
         bool xp = absf(xy_c + xy_o) <= hy && absf(xz_c + xz_o) <= hz;
         bool ym = absf(yx_c - yx_o) <= hx && absf(yz_c - yz_o) <= hz;
@@ -325,11 +325,12 @@ __host__ __device__ __forceinline__ float find_l_3d_rec(const RayRec &R, float p
     }
     case RAY_A0_B0: return absf(py) <= hy ? dz : 0.0f;
     case RAY_A0_C0: return absf(pz) <= hz ? 1.0f : 0.0f;
-    case RAY_A0_YZ_U: return chord2(py, pz, R.s_cb, R.s_bc, R.o_cb, R.o_bc, hy, hz, true);
-    case RAY_A0_YZ_W: return chord2(py, pz, R.s_cb, R.s_bc, R.o_cb, R.o_bc, hy, hz, false);
+    case RAY_A0_YZ_U: return chord2(py, pz, R.s_cb, R.s_bc, R.s_cb * hy, R.s_bc * hz, hy, hz, true);
+    case RAY_A0_YZ_W:
+        return chord2(py, pz, R.s_cb, R.s_bc, R.s_cb * hy, R.s_bc * hz, hy, hz, false);
     case RAY_B0_C0: return absf(pz) <= hz ? 1.0f : 0.0f;
-    case RAY_B0_XZ_U: return chord2(px, pz, R.s_ca, R.s_ac, R.o_ca, R.o_ac, hx, hz, true);
-    default: return chord2(px, pz, R.s_ca, R.s_ac, R.o_ca, R.o_ac, hx, hz, false);
+    case RAY_B0_XZ_U: return chord2(px, pz, R.s_ca, R.s_ac, R.s_ca * hx, R.s_ac * hz, hx, hz, true);
+    default: return chord2(px, pz, R.s_ca, R.s_ac, R.s_ca * hx, R.s_ac * hz, hx, hz, false);
     }
 }
 
@@ -345,8 +346,8 @@ __host__ __device__ __forceinline__ bool inplane_may_hit(const RayRec &R, float 
 #pragma clang fp contract(off)
     if (R.mode != RAY_X_FIRST && R.mode != RAY_Y_FIRST) return true;
     const float lim = 0.5f + 1e-2f;
-    const float xy_c = R.s_ba * (-px) + py, xy_o = R.o_ba;
-    const float yx_c = R.s_ab * (-py) + px, yx_o = R.o_ab;
+    const float xy_c = R.s_ba * (-px) + py, xy_o = R.s_ba * 0.5f;
+    const float yx_c = R.s_ab * (-py) + px, yx_o = R.s_ab * 0.5f;
     return absf(xy_c - xy_o) <= lim || absf(xy_c + xy_o) <= lim || absf(yx_c - yx_o) <= lim ||
            absf(yx_c + yx_o) <= lim;
 }
@@ -364,11 +365,6 @@ __host__ __device__ __forceinline__ RayRec make_ray_rec(const ViewGeom &g, float
     R.s_ba = b / a; R.s_ab = a / b;
     R.s_ca = c / a; R.s_cb = c / b;
     R.s_ac = a / c; R.s_bc = b / c;
-    // face offsets of find_l_3d (hx = hy = 1/2, hz = dz/2)
-    const float hx = 0.5f, hy = 0.5f, hz = g.dz / 2.0f;
-    R.o_ba = R.s_ba * hx; R.o_ca = R.s_ca * hx;
-    R.o_ab = R.s_ab * hy; R.o_cb = R.s_cb * hy;
-    R.o_ac = R.s_ac * hz; R.o_bc = R.s_bc * hz;
     R.flat = 0;
     if (a == 0) {
         if (b == 0) R.mode = RAY_A0_B0;
@@ -415,22 +411,6 @@ __host__ __device__ __forceinline__ Footprint footprint(const ViewGeom &g, int i
     f.a_hi = a_hi > g.na - 1 ? g.na - 1 : a_hi;
     f.b_hi = b_hi > g.nb - 1 ? g.nb - 1 : b_hi;
     return f;
-}
-
-// XCD-aware block order: hardware hands consecutive workgroups to the 8 XCDs
-// round-robin; remap so that each XCD walks a contiguous range of tiles and
-// neighbouring tiles share its L2.
-__host__ __device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nblocks)
-{
-    constexpr unsigned NXCD = 8;
-    if (nblocks < NXCD * 2) return b;
-    unsigned per = nblocks / NXCD, rem = nblocks % NXCD;
-    unsigned xcd = b % NXCD, k = b / NXCD;
-    // XCDs < rem own per+1 tiles
-    unsigned start = xcd * per + (xcd < rem ? xcd : rem);
-    unsigned cnt = per + (xcd < rem ? 1u : 0u);
-    if (k < cnt) return start + k;
-    return b;  // unreachable when nblocks is covered exactly
 }
 
 }  // namespace asd

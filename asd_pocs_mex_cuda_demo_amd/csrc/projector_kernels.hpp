@@ -33,7 +33,9 @@ __global__ __launch_bounds__(256) void ax_kernel(ViewGeom g, const float *__rest
     const bool stop = epi.skip();
     const long nrays = (long)g.na * g.nb;
     const int lane = threadIdx.x % LPR;
-    const long id = (long)blockIdx.x * RPB + threadIdx.x / LPR;
+    // XCD-contiguous rays: neighbouring rays of a fan cross the same voxels,
+    // so each XCD's L2 holds one wedge of the volume instead of all of it
+    const long id = (long)xcd_block(blockIdx.x, gridDim.x) * RPB + threadIdx.x / LPR;
     const bool active = id < nrays;
     typename Epi::Pre pre{};
     if (active && lane == 0) pre = epi.load(id);
@@ -45,9 +47,9 @@ __global__ __launch_bounds__(256) void ax_kernel(ViewGeom g, const float *__rest
         const float zd = z_det ? z_det[ib] : det_coord(ib, g.nb, g.dz_det);
         AxRay r = ax_ray_setup(g, yd, zd);
         int nu = r.xdriven ? g.nx : g.ny;
-        // slabs per load batch: two when a ray has 128 or more lanes (a
-        // 512-slab ray then needs one batch per lane), else four
-        constexpr int U = LPR >= 128 ? 2 : 4;
+        // slabs per load batch: two when a ray has 256 lanes, else four (a
+        // lane then walks its slabs in one or two batches)
+        constexpr int U = LPR >= 256 ? 2 : 4;
         for (int iu0 = lane; iu0 < nu; iu0 += U * LPR)
             ax_slabs<U, KAHAN>(r, X, iu0, LPR, nu, g.nx, g.ny, sa, sl);
     }

@@ -163,7 +163,7 @@ __device__ __forceinline__ void tile_union(bool has, const Footprint &f, int *rn
 }
 
 constexpr int TILE_MAXA = 256;   // in-plane ray records per chunk
-constexpr int TILE_MAXR = 512;  // (ia, ib) records per chunk
+constexpr int TILE_MAXR = 1024;  // (ia, ib) records per chunk
 
 // Tile geometry shared by the kernel and its host check.
 struct TileDims {

@@ -53,11 +53,12 @@ __global__ __launch_bounds__(256) void ray_table_kernel(ViewGeom g,
 int ax_lanes_per_ray(int na, int nb, int nu)
 {
     // Aim for about 2^20 working lanes (4096 waves over 256 CUs) while keeping
-    // at least 2 slabs per lane; a 2-D view (a few hundred rays) gets up to
-    // four waves per ray.
+    // at least 4 slabs per lane (the per-lane ray setup and the lane reduction
+    // are paid per lane); a 2-D view (a few hundred rays) gets up to four
+    // waves per ray.
     long rays = (long)na * nb;
     int lpr = 1;
-    while (lpr < 256 && rays * lpr < (1L << 20) && nu / (2 * lpr) >= 2) lpr *= 2;
+    while (lpr < 256 && rays * lpr < (1L << 20) && nu / (2 * lpr) >= 4) lpr *= 2;
     return lpr;
 }
 

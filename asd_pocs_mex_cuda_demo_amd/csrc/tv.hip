@@ -104,7 +104,8 @@ __global__ __launch_bounds__(256) void tv_grad_kernel(const T *__restrict__ u,
     if constexpr (STEPPED) U.s = tv_step_size(alpha, alpha_dev, d2, partials_prev, nb_prev);
     const size_t n = (size_t)nx * ny * nz;
     double s = 0.0;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+    // XCD-contiguous blocks: a stencil's neighbour rows stay in one XCD's L2
+    for (size_t i = (size_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; i < n;
          i += (size_t)gridDim.x * blockDim.x) {
         int x = (int)(i % nx);
         size_t r = i / nx;
