@@ -160,6 +160,25 @@ def bench_batch(args):
     print(json.dumps(result), flush=True)
 
 
+ATX_KERNELS = {"direct": "atx_kernel<EpiSartAtx, false>",
+               "tile16x16x1": "atx_tile_plain_kernel<16, 16, 1, false, EpiSartAtx>",
+               "tile8x8x4": "atx_tile_plain_kernel<8, 8, 4, false, EpiSartAtx>",
+               "tile16x4x4": "atx_tile_plain_kernel<16, 4, 4, false, EpiSartAtx>"}
+
+
+def profiled_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary of this workload (FETCH_SIZE + WRITE_SIZE, each scaled by a
+    256 MiB copy), or None when that summary has no row for it."""
+    path = os.path.join(ROOT, "profiles", f"{workload}_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("| `") and f"`{kernel}`" in line:
+            return int(line.strip().strip("|").split("|")[-1])
+    return None
+
+
 def perturbed(P, seed=99, eps=1e-7):
     return P * (1 + eps * np.random.default_rng(seed).standard_normal(P.shape))
 
@@ -316,9 +335,16 @@ def main():
     dom = max(stage, key=lambda s: stage[s][0])
     ms_d, cnt_d = stage[dom]
     achieved = nbytes[dom] / (ms_d / cnt_d * 1e-3) / 1e9 if cnt_d else 0.0
-    roof = {"bound": "hbm", "kernel": names[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-            "algorithmic_bytes_per_launch": nbytes[dom]}
+    tuning = plan.tuning()
+    lpr = plan.info()[0]
+    kernel = None
+    if args.precision == 0:
+        kernel = {0: f"ax_kernel<{lpr}, false, EpiSartAx<SinoF32> >",
+                  1: ATX_KERNELS.get(tuning["atx_variant"])}.get(dom)
+    traffic = profiled_traffic(args.workload, kernel) if kernel else None
+    roof = {"bound": "hbm", "kernel": kernel or names[dom], "achieved": achieved,
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic, "algorithmic_bytes_per_launch": nbytes[dom]}
 
     result = {
         "metric": METRIC,
@@ -343,8 +369,8 @@ def main():
         "stages": per_stage,
         "ms_per_step_eager_with_events": dt_ev / prof_steps * 1e3,
         "rmse_final": float(rmse[-1]) if len(rmse) else None,
-        "ax_lanes_per_ray": plan.info()[0],
-        "atx_tuning": plan.tuning(),
+        "ax_lanes_per_ray": lpr,
+        "atx_tuning": tuning,
     }
     if args.precision:
         # matched accuracy: the plain FP32 plan over the same iterations, and
