@@ -60,6 +60,7 @@ struct ViewGeom {
     float dy_det, dz_det;    // detector pitch (voxels), used by the backprojector footprint
     int nx, ny, nz, na, nb;
     float fp_d;              // footprint half-extent sqrt((1 + dz^2) / 2) (geom_finish)
+    int tight;               // footprints may be cut to the voxel's shadow (view_tight)
 };
 
 // Per-ray record for the backprojector: the six slope quotients the

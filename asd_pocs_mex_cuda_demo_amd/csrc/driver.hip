@@ -276,6 +276,7 @@ void view_geom_from_para(const asd_para &p, double phi_deg, ViewGeom &g, std::ve
         ydet[i] = (float)(((double)(float)i - (double)(float)p.na / 2.0 + 0.5) * (double)det);
     for (int j = 0; j < p.nb; j++)
         zdet[j] = (float)(((double)(float)j - (double)(float)p.nb / 2.0 + 0.5) * (double)det);
+    g.tight = view_tight(g, ydet.data());
 }
 
 int check_para(const asd_para *p, int nv)

@@ -69,10 +69,38 @@ __host__ __device__ __forceinline__ float det_coord(int i, int n, float det)
 }
 
 // Derived per-view constants (host and device give the same FP32 values).
+// Footprint cutting stays off until view_tight has checked the view's rays.
 __host__ __device__ __forceinline__ void geom_finish(ViewGeom &g)
 {
 #pragma clang fp contract(off)
     g.fp_d = sqrtf((1 + g.dz * g.dz) / 2);
+    g.tight = 0;
+}
+
+// Whether the backprojector footprint may be cut to the voxel's shadow: no
+// ray of the view takes find_l_3d's a == 0 / b == 0 fallbacks (cu:131-135),
+// which ignore a coordinate and give non-zero lengths outside the shadow.
+// a and b are formed exactly as make_ray_rec forms them.
+inline int view_tight(const ViewGeom &g, const float *y_det)
+{
+#pragma clang fp contract(off)
+    const float x1 = g.cos_phi * (-g.SO), y1 = g.sin_phi * (-g.SO);
+    for (int ia = 0; ia < g.na; ia++) {
+        const float x2 = g.cos_phi * g.OD - g.sin_phi * y_det[ia];
+        const float y2 = g.sin_phi * g.OD + g.cos_phi * y_det[ia];
+        if (x2 - x1 == 0.0f || y2 - y1 == 0.0f) return 0;
+    }
+    return 1;
+}
+
+// Reciprocal for conservative bounds only (one ulp on the device).
+__host__ __device__ __forceinline__ float bound_rcp(float x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_rcpf(x);
+#else
+    return 1.0f / x;
+#endif
 }
 
 // Running FP32 sum; with KAHAN the rounding error of every addition is
@@ -387,6 +415,54 @@ struct Footprint {
     float xc, yc, zc;
 };
 
+// Cuts the footprint to the bins whose ray passes within SHADOW_MARGIN
+// voxel of the voxel: the shadow of its square through the source (the
+// extreme projections of its four corners) and of its z extent over the
+// square's depth, widened by the margin.  The reference's footprint is about
+// twice as wide.  A ray outside the widened shadow misses the voxel by more
+// than the margin, far beyond the ~1e-4 voxel rounding of find_l_3d's face
+// tests, so every test fails and its length is exactly 0: the sums are
+// unchanged (bit-identical when uncompensated).  (xr, yr) is the voxel
+// centre in the view frame (x' towards the detector).
+constexpr float SHADOW_MARGIN = 0.05f;
+
+__host__ __device__ __forceinline__ void shadow_cut(const ViewGeom &g, float xr, float yr,
+                                                    Footprint &f)
+{
+    const float SD = g.SO + g.OD;
+    const float hc = 0.5f * g.cos_phi, hs = 0.5f * g.sin_phi;
+    float umin = 3.0e38f, umax = -3.0e38f, xmin = 3.0e38f, xmax = -3.0e38f;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const float sx = (k & 1) ? 0.5f : -0.5f, sy = (k & 2) ? 0.5f : -0.5f;
+        const float xk = xr + 2.0f * (sx * hc + sy * hs);
+        const float yk = yr + 2.0f * (sy * hc - sx * hs);
+        const float u = yk * SD * bound_rcp(g.SO + xk);
+        umin = fminf(umin, u);
+        umax = fmaxf(umax, u);
+        xmin = fminf(xmin, xk);
+        xmax = fmaxf(xmax, xk);
+    }
+    if (g.SO + xmin <= 0.0f) return;  // voxel reaches behind the source: keep all
+    const float m_near = SD * bound_rcp(g.SO + xmin), m_far = SD * bound_rcp(g.SO + xmax);
+    const float eps = SHADOW_MARGIN * m_near;
+    // bins i with centre (i - n/2 + 1/2) * det inside [lo - eps, hi + eps]
+    const float ia_off = 0.5f * (float)g.na - 0.5f, ib_off = 0.5f * (float)g.nb - 0.5f;
+    const float rdy = bound_rcp(g.dy_det), rdz = bound_rcp(g.dz_det);
+    const int a_lo = (int)ceilf((umin - eps) * rdy + ia_off);
+    const int a_hi = (int)floorf((umax + eps) * rdy + ia_off);
+    const float hz = 0.5f * g.dz;
+    const float z0 = f.zc - hz, z1 = f.zc + hz;
+    const float vmin = z0 * (z0 < 0.0f ? m_near : m_far);
+    const float vmax = z1 * (z1 > 0.0f ? m_near : m_far);
+    const int b_lo = (int)ceilf((vmin - eps) * rdz + ib_off);
+    const int b_hi = (int)floorf((vmax + eps) * rdz + ib_off);
+    f.a_lo = a_lo > f.a_lo ? a_lo : f.a_lo;
+    f.a_hi = a_hi < f.a_hi ? a_hi : f.a_hi;
+    f.b_lo = b_lo > f.b_lo ? b_lo : f.b_lo;
+    f.b_hi = b_hi < f.b_hi ? b_hi : f.b_hi;
+}
+
 __host__ __device__ __forceinline__ Footprint footprint(const ViewGeom &g, int ix, int iy, int iz)
 {
 #pragma clang fp contract(off)
@@ -410,6 +486,7 @@ __host__ __device__ __forceinline__ Footprint footprint(const ViewGeom &g, int i
     f.b_lo = b_lo < 0 ? 0 : b_lo;
     f.a_hi = a_hi > g.na - 1 ? g.na - 1 : a_hi;
     f.b_hi = b_hi > g.nb - 1 ? g.nb - 1 : b_hi;
+    if (g.tight) shadow_cut(g, xr, yr, f);
     return f;
 }
 

@@ -53,12 +53,13 @@ __global__ __launch_bounds__(256) void ray_table_kernel(ViewGeom g,
 int ax_lanes_per_ray(int na, int nb, int nu)
 {
     // Aim for about 2^20 working lanes (4096 waves over 256 CUs) while keeping
-    // at least 4 slabs per lane (the per-lane ray setup and the lane reduction
-    // are paid per lane); a 2-D view (a few hundred rays) gets up to four
-    // waves per ray.
+    // at least 2 slabs per lane; a 2-D view (a few hundred rays) gets up to
+    // four waves per ray.  (A 512-slab ray at 128 lanes puts 1.5 blocks of a
+    // 768-ray view on each CU, which leaves half the SIMDs with one wave
+    // more than the rest; at 256 lanes every SIMD gets three.)
     long rays = (long)na * nb;
     int lpr = 1;
-    while (lpr < 256 && rays * lpr < (1L << 20) && nu / (2 * lpr) >= 4) lpr *= 2;
+    while (lpr < 256 && rays * lpr < (1L << 20) && nu / (2 * lpr) >= 2) lpr *= 2;
     return lpr;
 }
 
@@ -176,7 +177,9 @@ int atx_host(float *X, const float *y, float *sc, const ViewGeom &g, const float
     if ((rc = launch_ray_table(g, d.ydet.as<float>(), d.zdet.as<float>(), d.rt.as<RayRec>(),
                                d.stream)))
         return rc;
-    if ((rc = launch_atx(g, d.vol.as<float>(), d.proj.as<float>(), d.vol2.as<float>(),
+    ViewGeom gt = g;
+    gt.tight = view_tight(g, y_det);  // host table: the rays' a and b can be checked here
+    if ((rc = launch_atx(gt, d.vol.as<float>(), d.proj.as<float>(), d.vol2.as<float>(),
                          d.ydet.as<float>(), d.zdet.as<float>(), d.rt.as<RayRec>(), d.stream)))
         return rc;
     ASD_HIP(hipMemcpyAsync(X, d.vol.p, N * 4, hipMemcpyDeviceToHost, d.stream));

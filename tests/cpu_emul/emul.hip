@@ -24,8 +24,9 @@ static ViewGeom mk(float c, float s, float SO, float OD, float scale, float dyd,
     return g;
 }
 
-// When set, the flat / dz == 1 shortcuts are switched off (the general
-// expressions are evaluated), to check that the shortcuts change no bit.
+// When set, the flat / dz == 1 shortcuts and the footprint cut are switched
+// off (the general expressions over the reference's footprint are
+// evaluated), to check that the shortcuts change no bit.
 static int g_general = 0;
 
 static AxRay ray_setup(const ViewGeom &g, float ydet, float zdet)
@@ -45,6 +46,25 @@ static RayRec ray_rec(const ViewGeom &g, float ydet, float zdet)
 extern "C" {
 
 void emul_set_general(int on) { g_general = on; }
+
+// Total (voxel, bin) pairs the backprojector visits for one view: with the
+// footprint cut (cut = 1, where the view allows it) or the reference's
+// footprint (cut = 0).
+long emul_footprint_pairs(float c, float s, const float *ydet, float SO, float OD, float dyd,
+                          float dzd, int nx, int ny, int nz, int na, int nb, int cut)
+{
+    ViewGeom g = mk(c, s, SO, OD, 1.0f, dyd, dzd, 1.0f, nx, ny, nz, na, nb);
+    if (cut) g.tight = view_tight(g, ydet);
+    long n = 0;
+    for (int iz = 0; iz < nz; iz++)
+        for (int iy = 0; iy < ny; iy++)
+            for (int ix = 0; ix < nx; ix++) {
+                Footprint f = footprint(g, ix, iy, iz);
+                if (f.a_lo <= f.a_hi && f.b_lo <= f.b_hi)
+                    n += (long)(f.a_hi - f.a_lo + 1) * (f.b_hi - f.b_lo + 1);
+            }
+    return n;
+}
 
 // det_coord, the detector grid the forward projector computes in place.
 float emul_det_coord(int i, int n, float det) { return det_coord(i, n, det); }
@@ -108,6 +128,7 @@ void emul_atx(float *X, const float *y, float *sc, float c, float s, const float
               int nx, int ny, int nz, int na, int nb)
 {
     ViewGeom g = mk(c, s, SO, OD, scale, dyd, dzd, dz, nx, ny, nz, na, nb);
+    if (!g_general) g.tight = view_tight(g, ydet);
     long M = (long)na * nb;
     RayRec *rt = new RayRec[M];
     for (long id = 0; id < M; id++) rt[id] = ray_rec(g, ydet[id % na], zdet[id / na]);
@@ -142,6 +163,7 @@ void emul_atx_prec(float *X, const float *y, float *sc, float c, float s, const 
                    float dz, int nx, int ny, int nz, int na, int nb, int TX, int TY, int TZ)
 {
     ViewGeom g = mk(c, s, SO, OD, scale, dyd, dzd, dz, nx, ny, nz, na, nb);
+    if (!g_general) g.tight = view_tight(g, ydet);
     long M = (long)na * nb;
     std::vector<RayRec> rt(M);
     for (long id = 0; id < M; id++) rt[id] = ray_rec(g, ydet[id % na], zdet[id / na]);

@@ -31,8 +31,9 @@ CASES = [
     ("fan64", geom(64, 64, 1, 128, 1, 405.941, 500.31, 0.5, 0.148 * 2, -0.3)),
     ("fan512", geom(512, 512, 1, 768, 1, 541.0, 408.0, 0.5, 0.8, -1.9)),
     ("fan_odd_na", geom(48, 40, 1, 97, 1, 500.0, 500.0, 1.0, 1.0, -2.6)),
-    # 2-D views whose rays span 256 lanes (four waves per ray; the 512^2 view
-    # above spans 128: two rays per block, two waves each)
+    # 2-D views whose rays span 128 lanes (two rays per block, two waves
+    # each) and 256 lanes (four waves per ray, like the 512^2 view above)
+    ("fan300", geom(300, 280, 1, 400, 1, 500.0, 450.0, 0.7, 1.0, -1.1)),
     ("fan1040", geom(1040, 200, 1, 64, 1, 800.0, 500.0, 0.5, 1.0, -0.2)),
     # 3-D cone beam
     ("cone32", geom(32, 32, 16, 48, 24, 500.0, 500.0, 1.0, 1.2, -0.7)),
