@@ -182,7 +182,7 @@ __global__ __launch_bounds__(256) void atx_batch_kernel(const BItem *__restrict_
             }
 #pragma unroll
             for (int k = 0; k < ATX_CHUNK; k++) {
-                if (a0 + k > f.a_hi || !inplane_may_hit(R[k], px, py)) continue;
+                if (a0 + k > f.a_hi || (!g.tight && !inplane_may_hit(R[k], px, py))) continue;
                 float l = find_l_3d_rec(R[k], px, py, pz, hz, g.dz);
                 acc.add(l * yv[k]);
                 accl.add(l);

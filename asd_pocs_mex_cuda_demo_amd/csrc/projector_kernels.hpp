@@ -111,12 +111,14 @@ int dispatch_ax(int lpr, const ViewGeom &g, const float *x, const float *y_det,
 
 // Backprojector: one thread per voxel (x fastest) walks its detector
 // footprint, ib outer and ia inner like the reference (cu:255-269), and sums
-// l*y and l in that order.  The records and corrections of ATX_CHUNK
-// consecutive bins are loaded together (clamped to the footprint) before
-// any is used, so a row costs one memory round trip per chunk; bins whose
-// in-plane chord provably misses the voxel are skipped (their term is an
-// exact zero).  Blocks are remapped so each XCD owns a contiguous slab of
-// voxels and the rays they share stay in its L2.
+// l*y and l in that order.  The footprint's (ib, ia) pairs are taken in that
+// order in chunks of ATX_CHUNK whose records and corrections are loaded
+// together (clamped to the last pair) before any is used, so a voxel whose
+// cut footprint holds a handful of pairs costs one memory round trip.  On
+// views whose footprints are not cut, bins whose in-plane chord provably
+// misses the voxel are skipped (their term is an exact zero).  Blocks are
+// remapped so each XCD owns a contiguous slab of voxels and the rays they
+// share stay in its L2.
 constexpr int ATX_CHUNK = 4;
 
 template <class Epi, bool KAHAN = false>
@@ -139,24 +141,30 @@ __global__ __launch_bounds__(256) void atx_kernel(ViewGeom g, const float *__res
     const float hz = g.dz / 2.0f;
     Acc<KAHAN> acc, accl;
     if (f.a_lo <= f.a_hi && f.b_lo <= f.b_hi) {
-        for (int ib = f.b_lo; ib <= f.b_hi; ib++) {
-            const unsigned row = (unsigned)ib * (unsigned)g.na;
-            for (int a0 = f.a_lo; a0 <= f.a_hi; a0 += ATX_CHUNK) {
-                RayRec R[ATX_CHUNK];
-                float yv[ATX_CHUNK];
+        const int npairs = (f.a_hi - f.a_lo + 1) * (f.b_hi - f.b_lo + 1);
+        int ia = f.a_lo, ib = f.b_lo;  // cursor: the next pair in reference order
+        for (int k0 = 0; k0 < npairs; k0 += ATX_CHUNK) {
+            RayRec R[ATX_CHUNK];
+            float yv[ATX_CHUNK];
 #pragma unroll
-                for (int k = 0; k < ATX_CHUNK; k++) {
-                    const unsigned ia = (unsigned)(a0 + k <= f.a_hi ? a0 + k : f.a_hi);
-                    R[k] = rt[row + ia];
-                    yv[k] = Y[row + ia];
+            for (int k = 0; k < ATX_CHUNK; k++) {
+                const unsigned idx = (unsigned)ib * (unsigned)g.na + (unsigned)ia;
+                R[k] = rt[idx];
+                yv[k] = Y[idx];
+                // advance unless this is the last pair (the rest of the chunk
+                // re-reads it and is not used)
+                if (k0 + k + 1 < npairs) {
+                    const bool wrap = ia == f.a_hi;
+                    ia = wrap ? f.a_lo : ia + 1;
+                    ib = wrap ? ib + 1 : ib;
                 }
+            }
 #pragma unroll
-                for (int k = 0; k < ATX_CHUNK; k++) {
-                    if (a0 + k > f.a_hi || !inplane_may_hit(R[k], px, py)) continue;
-                    float l = find_l_3d_rec(R[k], px, py, pz, hz, g.dz);
-                    acc.add(l * yv[k]);
-                    accl.add(l);
-                }
+            for (int k = 0; k < ATX_CHUNK; k++) {
+                if (k0 + k >= npairs || (!g.tight && !inplane_may_hit(R[k], px, py))) continue;
+                float l = find_l_3d_rec(R[k], px, py, pz, hz, g.dz);
+                acc.add(l * yv[k]);
+                accl.add(l);
             }
         }
     }
@@ -237,7 +245,7 @@ __global__ __launch_bounds__(256) void atx_tile_plain_kernel(ViewGeom g,
                         const int row = (ib - b0) * nA - A0;
                         for (int ia = f.a_lo; ia <= f.a_hi; ia++) {
                             const RayRec R = s_rec[row + ia];
-                            if (!inplane_may_hit(R, px, py)) continue;
+                            if (!g.tight && !inplane_may_hit(R, px, py)) continue;
                             float l = find_l_3d_rec(R, px, py, pz, hz, g.dz);
                             acc.add(l * s_y[row + ia]);
                             accl.add(l);

@@ -260,6 +260,10 @@ def main():
     ap.add_argument("--cpu-views", type=int, default=None)
     ap.add_argument("--prof-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--atx-variant", default=None,
+                    choices=["direct", "tile16x16x1", "tile8x8x4", "tile16x4x4"],
+                    help="force the backprojector variant instead of the autotuner's pick "
+                         "(profiling runs use this to profile one variant)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 256 MiB device to device three times (a plain copy "
                          "kernel of known size, to scale rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -291,6 +295,8 @@ def main():
         plan = ops.Plan(dict(para, maxnumIter=w["iters"], precision=args.precision),
                         para["phi_vec"])
     plan.set_projections(P)
+    if args.atx_variant:
+        plan.set_atx_variant(list(ATX_KERNELS).index(args.atx_variant))
     N = para["nx"] * para["ny"] * para["nz"]
     M = para["na"] * para["nb"]
 

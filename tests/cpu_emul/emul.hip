@@ -144,7 +144,7 @@ void emul_atx(float *X, const float *y, float *sc, float c, float s, const float
         for (int ib = f.b_lo; ib <= f.b_hi; ib++)
             for (int ia = f.a_lo; ia <= f.a_hi; ia++) {
                 const RayRec &R = rt[(long)ib * na + ia];
-                if (!inplane_may_hit(R, px, py)) continue;
+                if (!g.tight && !inplane_may_hit(R, px, py)) continue;
                 float l = find_l_3d_rec(R, px, py, pz, hz, g.dz);
                 acc.add(l * y[(long)ib * na + ia]);
                 accl.add(l);
