@@ -121,6 +121,44 @@ int dispatch_ax(int lpr, const ViewGeom &g, const float *x, const float *y_det,
 // share stay in its L2.
 constexpr int ATX_CHUNK = 4;
 
+// The footprint walk of one voxel (shared by atx_kernel and the fallback of
+// atx_pairs_kernel).
+template <bool KAHAN>
+__device__ __forceinline__ void atx_voxel_walk(const ViewGeom &g, const float *__restrict__ Y,
+                                               const RayRec *__restrict__ rt, const Footprint &f,
+                                               float px, float py, float pz, float hz,
+                                               Acc<KAHAN> &acc, Acc<KAHAN> &accl)
+{
+#pragma clang fp contract(off)
+    if (!(f.a_lo <= f.a_hi && f.b_lo <= f.b_hi)) return;
+    const int npairs = (f.a_hi - f.a_lo + 1) * (f.b_hi - f.b_lo + 1);
+    int ia = f.a_lo, ib = f.b_lo;  // cursor: the next pair in reference order
+    for (int k0 = 0; k0 < npairs; k0 += ATX_CHUNK) {
+        RayRec R[ATX_CHUNK];
+        float yv[ATX_CHUNK];
+#pragma unroll
+        for (int k = 0; k < ATX_CHUNK; k++) {
+            const unsigned idx = (unsigned)ib * (unsigned)g.na + (unsigned)ia;
+            R[k] = rt[idx];
+            yv[k] = Y[idx];
+            // advance unless this is the last pair (the rest of the chunk
+            // re-reads it and is not used)
+            if (k0 + k + 1 < npairs) {
+                const bool wrap = ia == f.a_hi;
+                ia = wrap ? f.a_lo : ia + 1;
+                ib = wrap ? ib + 1 : ib;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < ATX_CHUNK; k++) {
+            if (k0 + k >= npairs || (!g.tight && !inplane_may_hit(R[k], px, py))) continue;
+            float l = find_l_3d_rec(R[k], px, py, pz, hz, g.dz);
+            acc.add(l * yv[k]);
+            accl.add(l);
+        }
+    }
+}
+
 template <class Epi, bool KAHAN = false>
 __global__ __launch_bounds__(256) void atx_kernel(ViewGeom g, const float *__restrict__ Y,
                                                   const RayRec *__restrict__ rt, Epi epi)
@@ -138,37 +176,115 @@ __global__ __launch_bounds__(256) void atx_kernel(ViewGeom g, const float *__res
     Footprint f = footprint(g, ix, iy, iz);
     const float sx = g.cos_phi * (-g.SO), sy = g.sin_phi * (-g.SO);
     const float px = sx - f.xc, py = sy - f.yc, pz = 0.0f - f.zc;
+    Acc<KAHAN> acc, accl;
+    atx_voxel_walk<KAHAN>(g, Y, rt, f, px, py, pz, g.dz / 2.0f, acc, accl);
+    if (!stop) epi(v, acc.value(), accl.value(), pre);
+}
+
+// Backprojector with the work spread over (voxel, bin) pairs: a workgroup
+// lists the pairs of its 256 voxels' cut footprints (each voxel's pairs in
+// reference order), evaluates their lengths four per thread with all loads
+// issued first, and parks the terms l*y and l in LDS; then every voxel adds
+// its own terms in order.  The sums are the same FP32 additions in the same
+// order as atx_kernel (bit-identical), but a wave no longer runs as many
+// length evaluations as its largest footprint needs.  Blocks with more than
+// PAIRS_MAX pairs, and views without the footprint cut, walk per voxel.
+constexpr int PAIRS_MAX = 2048;
+constexpr int PAIRS_PER_THREAD = 4;
+
+template <class Epi, bool KAHAN = false>
+__global__ __launch_bounds__(256) void atx_pairs_kernel(ViewGeom g, const float *__restrict__ Y,
+                                                        const RayRec *__restrict__ rt, Epi epi)
+{
+#pragma clang fp contract(off)
+    __shared__ float s_t[PAIRS_MAX], s_l[PAIRS_MAX];
+    __shared__ unsigned s_bin[PAIRS_MAX];
+    __shared__ unsigned char s_own[PAIRS_MAX];
+    __shared__ float s_p[3][256];
+    __shared__ int s_wave[4];
+    const bool stop = epi.skip();
+    const unsigned b = xcd_block(blockIdx.x, gridDim.x);
+    const unsigned v = b * 256u + threadIdx.x;
+    const unsigned N = (unsigned)g.nx * (unsigned)g.ny * (unsigned)g.nz;
+    const bool valid = v < N;
+    typename Epi::Pre pre{};
+    if (valid) pre = epi.load(v);
+    Footprint f;
+    f.a_lo = f.b_lo = 0;
+    f.a_hi = f.b_hi = -1;
+    float px = 0.0f, py = 0.0f, pz = 0.0f;
+    if (valid) {
+        const int ix = (int)(v % (unsigned)g.nx);
+        const unsigned r = v / (unsigned)g.nx;
+        f = footprint(g, ix, (int)(r % (unsigned)g.ny), (int)(r / (unsigned)g.ny));
+        px = g.cos_phi * (-g.SO) - f.xc;
+        py = g.sin_phi * (-g.SO) - f.yc;
+        pz = 0.0f - f.zc;
+    }
+    const int np = (f.a_lo <= f.a_hi && f.b_lo <= f.b_hi)
+                       ? (f.a_hi - f.a_lo + 1) * (f.b_hi - f.b_lo + 1) : 0;
+    // exclusive scan of np over the workgroup
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int incl = np;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int t = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += t;
+    }
+    if (lane == 63) s_wave[w] = incl;
+    __syncthreads();
+    int base = 0, total = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        base += k < w ? s_wave[k] : 0;
+        total += s_wave[k];
+    }
+    const int off = base + incl - np;
     const float hz = g.dz / 2.0f;
     Acc<KAHAN> acc, accl;
-    if (f.a_lo <= f.a_hi && f.b_lo <= f.b_hi) {
-        const int npairs = (f.a_hi - f.a_lo + 1) * (f.b_hi - f.b_lo + 1);
-        int ia = f.a_lo, ib = f.b_lo;  // cursor: the next pair in reference order
-        for (int k0 = 0; k0 < npairs; k0 += ATX_CHUNK) {
-            RayRec R[ATX_CHUNK];
-            float yv[ATX_CHUNK];
+    if (!g.tight || total > PAIRS_MAX) {
+        if (valid) atx_voxel_walk<KAHAN>(g, Y, rt, f, px, py, pz, hz, acc, accl);
+    } else {
+        // list this voxel's pairs (reference order) and its position
+        s_p[0][threadIdx.x] = px;
+        s_p[1][threadIdx.x] = py;
+        s_p[2][threadIdx.x] = pz;
+        int ia = f.a_lo, ib = f.b_lo;
+        for (int k = 0; k < np; k++) {
+            s_bin[off + k] = (unsigned)ib * (unsigned)g.na + (unsigned)ia;
+            s_own[off + k] = (unsigned char)threadIdx.x;
+            const bool wrap = ia == f.a_hi;
+            ia = wrap ? f.a_lo : ia + 1;
+            ib = wrap ? ib + 1 : ib;
+        }
+        __syncthreads();
+        for (int p0 = threadIdx.x; p0 < total; p0 += 256 * PAIRS_PER_THREAD) {
+            RayRec R[PAIRS_PER_THREAD];
+            float yv[PAIRS_PER_THREAD];
 #pragma unroll
-            for (int k = 0; k < ATX_CHUNK; k++) {
-                const unsigned idx = (unsigned)ib * (unsigned)g.na + (unsigned)ia;
-                R[k] = rt[idx];
-                yv[k] = Y[idx];
-                // advance unless this is the last pair (the rest of the chunk
-                // re-reads it and is not used)
-                if (k0 + k + 1 < npairs) {
-                    const bool wrap = ia == f.a_hi;
-                    ia = wrap ? f.a_lo : ia + 1;
-                    ib = wrap ? ib + 1 : ib;
-                }
+            for (int j = 0; j < PAIRS_PER_THREAD; j++) {
+                const int p = p0 + 256 * j;
+                const unsigned idx = s_bin[p < total ? p : p0];
+                R[j] = rt[idx];
+                yv[j] = Y[idx];
             }
 #pragma unroll
-            for (int k = 0; k < ATX_CHUNK; k++) {
-                if (k0 + k >= npairs || (!g.tight && !inplane_may_hit(R[k], px, py))) continue;
-                float l = find_l_3d_rec(R[k], px, py, pz, hz, g.dz);
-                acc.add(l * yv[k]);
-                accl.add(l);
+            for (int j = 0; j < PAIRS_PER_THREAD; j++) {
+                const int p = p0 + 256 * j;
+                if (p >= total) continue;
+                const int o = s_own[p];
+                const float l = find_l_3d_rec(R[j], s_p[0][o], s_p[1][o], s_p[2][o], hz, g.dz);
+                s_t[p] = l * yv[j];
+                s_l[p] = l;
             }
         }
+        __syncthreads();
+        for (int k = 0; k < np; k++) {
+            acc.add(s_t[off + k]);
+            accl.add(s_l[off + k]);
+        }
     }
-    if (!stop) epi(v, acc.value(), accl.value(), pre);
+    if (valid && !stop) epi(v, acc.value(), accl.value(), pre);
 }
 
 // Backprojector on voxel tiles (TX x TY x TZ = 256 voxels per workgroup):
@@ -264,7 +380,8 @@ enum : int {
     ATX_T16x16x1 = 1, // LDS-staged 16x16 slices
     ATX_T8x8x4 = 2,   // LDS-staged 8x8x4 bricks
     ATX_T16x4x4 = 3,  // LDS-staged 16x4x4 bricks
-    ATX_NVARIANT = 4,
+    ATX_PAIRS = 4,    // (voxel, bin) pairs spread over the workgroup
+    ATX_NVARIANT = 5,
 };
 
 template <int TX, int TY, int TZ, bool KAHAN, class Epi>
@@ -286,6 +403,12 @@ int dispatch_atx_variant(int variant, const ViewGeom &g, const float *Y, const R
     case ATX_T16x16x1: return launch_atx_tile_t<16, 16, 1, KAHAN>(g, Y, rt, epi, s);
     case ATX_T8x8x4: return launch_atx_tile_t<8, 8, 4, KAHAN>(g, Y, rt, epi, s);
     case ATX_T16x4x4: return launch_atx_tile_t<16, 4, 4, KAHAN>(g, Y, rt, epi, s);
+    case ATX_PAIRS: {
+        const long N = (long)g.nx * g.ny * g.nz;
+        atx_pairs_kernel<Epi, KAHAN><<<(unsigned)((N + 255) / 256), 256, 0, s>>>(g, Y, rt, epi);
+        ASD_LAUNCH_CHECK("atx_pairs_kernel");
+        return ASD_OK;
+    }
     default: {
         const long N = (long)g.nx * g.ny * g.nz;
         atx_kernel<Epi, KAHAN><<<(unsigned)((N + 255) / 256), 256, 0, s>>>(g, Y, rt, epi);

@@ -163,7 +163,8 @@ def bench_batch(args):
 ATX_KERNELS = {"direct": "atx_kernel<EpiSartAtx, false>",
                "tile16x16x1": "atx_tile_plain_kernel<16, 16, 1, false, EpiSartAtx>",
                "tile8x8x4": "atx_tile_plain_kernel<8, 8, 4, false, EpiSartAtx>",
-               "tile16x4x4": "atx_tile_plain_kernel<16, 4, 4, false, EpiSartAtx>"}
+               "tile16x4x4": "atx_tile_plain_kernel<16, 4, 4, false, EpiSartAtx>",
+               "pairs": "atx_pairs_kernel<EpiSartAtx, false>"}
 
 
 def profiled_traffic(workload, kernel):
@@ -261,7 +262,7 @@ def main():
     ap.add_argument("--prof-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--atx-variant", default=None,
-                    choices=["direct", "tile16x16x1", "tile8x8x4", "tile16x4x4"],
+                    choices=["direct", "tile16x16x1", "tile8x8x4", "tile16x4x4", "pairs"],
                     help="force the backprojector variant instead of the autotuner's pick "
                          "(profiling runs use this to profile one variant)")
     ap.add_argument("--copy-calibration", action="store_true",
