@@ -60,6 +60,8 @@ struct ViewGeom {
     float dy_det, dz_det;    // detector pitch (voxels), used by the backprojector footprint
     int nx, ny, nz, na, nb;
     float fp_d;              // footprint half-extent sqrt((1 + dz^2) / 2) (geom_finish)
+    float fp_cx[4], fp_cy[4];  // voxel corner offsets in the view frame (bounds only)
+    float fp_rdy, fp_rdz;    // 1 / dy_det, 1 / dz_det (bounds only)
     int tight;               // footprints may be cut to the voxel's shadow (view_tight)
 };
 

@@ -420,7 +420,7 @@ int autotune_atx_t(asd_plan *p)
     const ViewGeom &g = p->views[0];
     int rc = ASD_OK;
     ASD_HIP(hipMemsetAsync(p->cor.p, 0, p->M * 4, p->stream));
-    std::vector<int> cand = {ATX_DIRECT, ATX_T16x16x1, ATX_PAIRS};
+    std::vector<int> cand = {ATX_DIRECT, ATX_DIRECT2, ATX_T16x16x1, ATX_PAIRS};
     if (g.nz > 1) {
         cand.push_back(ATX_T8x8x4);
         cand.push_back(ATX_T16x4x4);
@@ -884,7 +884,7 @@ int asd_plan_set_atx_variant(asd_plan *p, int atx_variant)
 {
     clear_error();
     ASD_ARG(p, "NULL plan");
-    ASD_ARG(atx_variant >= -1 && atx_variant < ATX_NVARIANT, "atx_variant must be -1..4");
+    ASD_ARG(atx_variant >= -1 && atx_variant < ATX_NVARIANT, "atx_variant must be -1..5");
     ASD_ARG((atx_variant != ATX_T8x8x4 && atx_variant != ATX_T16x4x4) || p->views[0].nz > 1,
             "the 3-D tile variants need nz > 1");
     drop_graph(p);

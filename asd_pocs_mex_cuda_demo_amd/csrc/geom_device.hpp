@@ -74,6 +74,14 @@ __host__ __device__ __forceinline__ void geom_finish(ViewGeom &g)
 {
 #pragma clang fp contract(off)
     g.fp_d = sqrtf((1 + g.dz * g.dz) / 2);
+    // corner offsets of a voxel in the view frame (x' = c x + s y, y' = -s x + c y)
+    for (int k = 0; k < 4; k++) {
+        const float sx = (k & 1) ? 0.5f : -0.5f, sy = (k & 2) ? 0.5f : -0.5f;
+        g.fp_cx[k] = sx * g.cos_phi + sy * g.sin_phi;
+        g.fp_cy[k] = sy * g.cos_phi - sx * g.sin_phi;
+    }
+    g.fp_rdy = 1.0f / g.dy_det;
+    g.fp_rdz = 1.0f / g.dz_det;
     g.tight = 0;
 }
 
@@ -429,34 +437,34 @@ constexpr float SHADOW_MARGIN = 0.05f;
 __host__ __device__ __forceinline__ void shadow_cut(const ViewGeom &g, float xr, float yr,
                                                     Footprint &f)
 {
+    // the detector coordinate y' * SD / (SO + x') of the four corners (offsets
+    // fp_cx/fp_cy in the view frame); the corners also give the square's
+    // nearest and farthest depth, hence the largest and smallest magnification
     const float SD = g.SO + g.OD;
-    const float hc = 0.5f * g.cos_phi, hs = 0.5f * g.sin_phi;
-    float umin = 3.0e38f, umax = -3.0e38f, xmin = 3.0e38f, xmax = -3.0e38f;
+    float umin = 3.0e38f, umax = -3.0e38f, m_near = 0.0f, m_far = 3.0e38f, dmin = 3.0e38f;
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-        const float sx = (k & 1) ? 0.5f : -0.5f, sy = (k & 2) ? 0.5f : -0.5f;
-        const float xk = xr + 2.0f * (sx * hc + sy * hs);
-        const float yk = yr + 2.0f * (sy * hc - sx * hs);
-        const float u = yk * SD * bound_rcp(g.SO + xk);
+        const float depth = g.SO + (xr + g.fp_cx[k]);
+        const float m = SD * bound_rcp(depth);
+        const float u = (yr + g.fp_cy[k]) * m;
         umin = fminf(umin, u);
         umax = fmaxf(umax, u);
-        xmin = fminf(xmin, xk);
-        xmax = fmaxf(xmax, xk);
+        m_near = fmaxf(m_near, m);
+        m_far = fminf(m_far, m);
+        dmin = fminf(dmin, depth);
     }
-    if (g.SO + xmin <= 0.0f) return;  // voxel reaches behind the source: keep all
-    const float m_near = SD * bound_rcp(g.SO + xmin), m_far = SD * bound_rcp(g.SO + xmax);
+    if (dmin <= 0.0f) return;  // voxel reaches behind the source: keep all
     const float eps = SHADOW_MARGIN * m_near;
     // bins i with centre (i - n/2 + 1/2) * det inside [lo - eps, hi + eps]
     const float ia_off = 0.5f * (float)g.na - 0.5f, ib_off = 0.5f * (float)g.nb - 0.5f;
-    const float rdy = bound_rcp(g.dy_det), rdz = bound_rcp(g.dz_det);
-    const int a_lo = (int)ceilf((umin - eps) * rdy + ia_off);
-    const int a_hi = (int)floorf((umax + eps) * rdy + ia_off);
+    const int a_lo = (int)ceilf((umin - eps) * g.fp_rdy + ia_off);
+    const int a_hi = (int)floorf((umax + eps) * g.fp_rdy + ia_off);
     const float hz = 0.5f * g.dz;
     const float z0 = f.zc - hz, z1 = f.zc + hz;
     const float vmin = z0 * (z0 < 0.0f ? m_near : m_far);
     const float vmax = z1 * (z1 > 0.0f ? m_near : m_far);
-    const int b_lo = (int)ceilf((vmin - eps) * rdz + ib_off);
-    const int b_hi = (int)floorf((vmax + eps) * rdz + ib_off);
+    const int b_lo = (int)ceilf((vmin - eps) * g.fp_rdz + ib_off);
+    const int b_hi = (int)floorf((vmax + eps) * g.fp_rdz + ib_off);
     f.a_lo = a_lo > f.a_lo ? a_lo : f.a_lo;
     f.a_hi = a_hi < f.a_hi ? a_hi : f.a_hi;
     f.b_lo = b_lo > f.b_lo ? b_lo : f.b_lo;
@@ -471,9 +479,11 @@ __host__ __device__ __forceinline__ Footprint footprint(const ViewGeom &g, int i
     int na2 = g.na / 2, nb2 = g.nb / 2;
     int nx2 = g.nx / 2, ny2 = g.ny / 2, nz2 = g.nz / 2;
     const float d = g.fp_d;
-    f.zc = (float)(iz + 0.5 - nz2) * g.dz;
-    f.yc = (float)(iy + 0.5 - ny2);
-    f.xc = (float)(ix + 0.5 - nx2);
+    // i + 0.5 - n/2 (the reference forms it in FP64 and rounds) is exact in
+    // FP32 for |i - n/2| < 2^22
+    f.zc = ((float)(iz - nz2) + 0.5f) * g.dz;
+    f.yc = (float)(iy - ny2) + 0.5f;
+    f.xc = (float)(ix - nx2) + 0.5f;
     float xr = g.cos_phi * f.xc + g.sin_phi * f.yc;
     float yr = -g.sin_phi * f.xc + g.cos_phi * f.yc;
     float t = SD / ((xr + g.SO) * g.dy_det);

@@ -122,8 +122,8 @@ int dispatch_ax(int lpr, const ViewGeom &g, const float *x, const float *y_det,
 constexpr int ATX_CHUNK = 4;
 
 // The footprint walk of one voxel (shared by atx_kernel and the fallback of
-// atx_pairs_kernel).
-template <bool KAHAN>
+// atx_pairs_kernel), CHUNK pairs per load batch.
+template <bool KAHAN, int CHUNK = ATX_CHUNK>
 __device__ __forceinline__ void atx_voxel_walk(const ViewGeom &g, const float *__restrict__ Y,
                                                const RayRec *__restrict__ rt, const Footprint &f,
                                                float px, float py, float pz, float hz,
@@ -133,11 +133,11 @@ __device__ __forceinline__ void atx_voxel_walk(const ViewGeom &g, const float *_
     if (!(f.a_lo <= f.a_hi && f.b_lo <= f.b_hi)) return;
     const int npairs = (f.a_hi - f.a_lo + 1) * (f.b_hi - f.b_lo + 1);
     int ia = f.a_lo, ib = f.b_lo;  // cursor: the next pair in reference order
-    for (int k0 = 0; k0 < npairs; k0 += ATX_CHUNK) {
-        RayRec R[ATX_CHUNK];
-        float yv[ATX_CHUNK];
+    for (int k0 = 0; k0 < npairs; k0 += CHUNK) {
+        RayRec R[CHUNK];
+        float yv[CHUNK];
 #pragma unroll
-        for (int k = 0; k < ATX_CHUNK; k++) {
+        for (int k = 0; k < CHUNK; k++) {
             const unsigned idx = (unsigned)ib * (unsigned)g.na + (unsigned)ia;
             R[k] = rt[idx];
             yv[k] = Y[idx];
@@ -150,7 +150,7 @@ __device__ __forceinline__ void atx_voxel_walk(const ViewGeom &g, const float *_
             }
         }
 #pragma unroll
-        for (int k = 0; k < ATX_CHUNK; k++) {
+        for (int k = 0; k < CHUNK; k++) {
             if (k0 + k >= npairs || (!g.tight && !inplane_may_hit(R[k], px, py))) continue;
             float l = find_l_3d_rec(R[k], px, py, pz, hz, g.dz);
             acc.add(l * yv[k]);
@@ -159,7 +159,7 @@ __device__ __forceinline__ void atx_voxel_walk(const ViewGeom &g, const float *_
     }
 }
 
-template <class Epi, bool KAHAN = false>
+template <class Epi, bool KAHAN = false, int CHUNK = ATX_CHUNK>
 __global__ __launch_bounds__(256) void atx_kernel(ViewGeom g, const float *__restrict__ Y,
                                                   const RayRec *__restrict__ rt, Epi epi)
 {
@@ -177,7 +177,7 @@ __global__ __launch_bounds__(256) void atx_kernel(ViewGeom g, const float *__res
     const float sx = g.cos_phi * (-g.SO), sy = g.sin_phi * (-g.SO);
     const float px = sx - f.xc, py = sy - f.yc, pz = 0.0f - f.zc;
     Acc<KAHAN> acc, accl;
-    atx_voxel_walk<KAHAN>(g, Y, rt, f, px, py, pz, g.dz / 2.0f, acc, accl);
+    atx_voxel_walk<KAHAN, CHUNK>(g, Y, rt, f, px, py, pz, g.dz / 2.0f, acc, accl);
     if (!stop) epi(v, acc.value(), accl.value(), pre);
 }
 
@@ -381,7 +381,8 @@ enum : int {
     ATX_T8x8x4 = 2,   // LDS-staged 8x8x4 bricks
     ATX_T16x4x4 = 3,  // LDS-staged 16x4x4 bricks
     ATX_PAIRS = 4,    // (voxel, bin) pairs spread over the workgroup
-    ATX_NVARIANT = 5,
+    ATX_DIRECT2 = 5,  // as ATX_DIRECT, two pairs per load batch
+    ATX_NVARIANT = 6,
 };
 
 template <int TX, int TY, int TZ, bool KAHAN, class Epi>
@@ -403,6 +404,12 @@ int dispatch_atx_variant(int variant, const ViewGeom &g, const float *Y, const R
     case ATX_T16x16x1: return launch_atx_tile_t<16, 16, 1, KAHAN>(g, Y, rt, epi, s);
     case ATX_T8x8x4: return launch_atx_tile_t<8, 8, 4, KAHAN>(g, Y, rt, epi, s);
     case ATX_T16x4x4: return launch_atx_tile_t<16, 4, 4, KAHAN>(g, Y, rt, epi, s);
+    case ATX_DIRECT2: {
+        const long N = (long)g.nx * g.ny * g.nz;
+        atx_kernel<Epi, KAHAN, 2><<<(unsigned)((N + 255) / 256), 256, 0, s>>>(g, Y, rt, epi);
+        ASD_LAUNCH_CHECK("atx_kernel");
+        return ASD_OK;
+    }
     case ATX_PAIRS: {
         const long N = (long)g.nx * g.ny * g.nz;
         atx_pairs_kernel<Epi, KAHAN><<<(unsigned)((N + 255) / 256), 256, 0, s>>>(g, Y, rt, epi);

@@ -164,7 +164,8 @@ ATX_KERNELS = {"direct": "atx_kernel<EpiSartAtx, false>",
                "tile16x16x1": "atx_tile_plain_kernel<16, 16, 1, false, EpiSartAtx>",
                "tile8x8x4": "atx_tile_plain_kernel<8, 8, 4, false, EpiSartAtx>",
                "tile16x4x4": "atx_tile_plain_kernel<16, 4, 4, false, EpiSartAtx>",
-               "pairs": "atx_pairs_kernel<EpiSartAtx, false>"}
+               "pairs": "atx_pairs_kernel<EpiSartAtx, false>",
+               "direct2": "atx_kernel<EpiSartAtx, false, 2>"}
 
 
 def profiled_traffic(workload, kernel):
@@ -262,7 +263,8 @@ def main():
     ap.add_argument("--prof-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--atx-variant", default=None,
-                    choices=["direct", "tile16x16x1", "tile8x8x4", "tile16x4x4", "pairs"],
+                    choices=["direct", "tile16x16x1", "tile8x8x4", "tile16x4x4", "pairs",
+                             "direct2"],
                     help="force the backprojector variant instead of the autotuner's pick "
                          "(profiling runs use this to profile one variant)")
     ap.add_argument("--copy-calibration", action="store_true",
