@@ -262,55 +262,125 @@ __host__ __device__ __forceinline__ void ax_slabs(const AxRay &r, const float *_
 // evaluated from the per-ray slope record.  (px,py,pz) = source - voxel centre.
 // ---------------------------------------------------------------------------
 
-// Take a candidate hit in test order; the first two define the chord.  The
-// state is two flags (one / two hits taken): per candidate the flag logic
-// is lane-mask arithmetic and the coordinates are selects, so nothing is
-// stored through an index (which would land in scratch memory).
-struct Hits {
-    float p0x = 0.0f, p0y = 0.0f, p0z = 0.0f, p1x = 0.0f, p1y = 0.0f, p1z = 0.0f;
-    bool a0 = false, a1 = false;
-    __host__ __device__ __forceinline__ void take(bool ok, float x, float y, float z)
-    {
-        const bool t0 = ok && !a0, t1 = ok && a0 && !a1;
-        p0x = t0 ? x : p0x;
-        p0y = t0 ? y : p0y;
-        p0z = t0 ? z : p0z;
-        p1x = t1 ? x : p1x;
-        p1y = t1 ? y : p1y;
-        p1z = t1 ? z : p1z;
-        a1 = a1 || (ok && a0);
-        a0 = a0 || ok;
-    }
-    __host__ __device__ __forceinline__ bool chord() const { return a1; }
+// The chord between the first two candidate hits in test order (find_l_3d's
+// rule, cu:144-206).  The first hit is a select chain over the candidates;
+// the second is the chain over "hit with an earlier hit": the chain
+// conditions are prefix masks (scalar work per wave), so a candidate costs
+// one vector select per coordinate and point, and nothing is stored through
+// an index (which would land in scratch memory).
+template <int K>
+struct Cands {
+    bool f[K];
+    float x[K], y[K], z[K];
 };
+
+template <int K>
+__host__ __device__ __forceinline__ bool chord_points(const Cands<K> &c, float &ex, float &ey,
+                                                      float &ez)
+{
+#pragma clang fp contract(off)
+    bool g[K];  // candidate k is a hit and an earlier one is too
+    bool any = c.f[0];
+    g[0] = false;
+#pragma unroll
+    for (int k = 1; k < K; k++) {
+        g[k] = c.f[k] && any;
+        any = any || c.f[k];
+    }
+    float p0x = c.x[K - 1], p0y = c.y[K - 1], p0z = c.z[K - 1];
+    float p1x = c.x[K - 1], p1y = c.y[K - 1], p1z = c.z[K - 1];
+#pragma unroll
+    for (int k = K - 2; k >= 0; k--) {
+        p0x = c.f[k] ? c.x[k] : p0x;
+        p0y = c.f[k] ? c.y[k] : p0y;
+        p0z = c.f[k] ? c.z[k] : p0z;
+    }
+#pragma unroll
+    for (int k = K - 2; k >= 1; k--) {
+        p1x = g[k] ? c.x[k] : p1x;
+        p1y = g[k] ? c.y[k] : p1y;
+        p1z = g[k] ? c.z[k] : p1z;
+    }
+    bool two = false;
+#pragma unroll
+    for (int k = 1; k < K; k++) two = two || g[k];
+    ex = p1x - p0x;
+    ey = p1y - p0y;
+    ez = p1z - p0z;
+    return two;
+}
 
 // General 2-D chord (find_l with a' != 0, b' != 0) in plane (u,w):
 // pu = u1_0 - uc, pw = w1_0 - wc, s_wu = b'/a', s_uw = a'/b', and the face
 // offsets ow = s_wu * hu, ou = s_uw * hw (per ray).
+// Candidate list of a 2-D chord in test order (u faces first when U_FIRST).
+template <bool U_FIRST>
+__host__ __device__ __forceinline__ bool chord2_points(float cw, float ow, float cu, float ou,
+                                                       float hu, float hw, float &ex, float &ey)
+{
+#pragma clang fp contract(off)
+    const bool um = absf(cw - ow) <= hw, up = absf(cw + ow) <= hw;
+    const bool wm = absf(cu - ou) <= hu, wp = absf(cu + ou) <= hu;
+    constexpr int iu = U_FIRST ? 0 : 2, iw = U_FIRST ? 2 : 0;
+    Cands<4> c;
+    c.f[iu] = um; c.x[iu] = -hu; c.y[iu] = cw - ow; c.z[iu] = 0.0f;
+    c.f[iu + 1] = up; c.x[iu + 1] = hu; c.y[iu + 1] = cw + ow; c.z[iu + 1] = 0.0f;
+    c.f[iw] = wm; c.x[iw] = cu - ou; c.y[iw] = -hw; c.z[iw] = 0.0f;
+    c.f[iw + 1] = wp; c.x[iw + 1] = cu + ou; c.y[iw + 1] = hw; c.z[iw + 1] = 0.0f;
+    float ez;
+    return chord_points<4>(c, ex, ey, ez);
+}
+
 __host__ __device__ __forceinline__ float chord2(float pu, float pw, float s_wu, float s_uw,
                                                  float ow, float ou, float hu, float hw,
                                                  bool u_first)
 {
 #pragma clang fp contract(off)
-    float cw = s_wu * (-pu) + pw;   // u faces: w coordinate
-    float cu = s_uw * (-pw) + pu;   // w faces: u coordinate
-    bool um = absf(cw - ow) <= hw, up = absf(cw + ow) <= hw;
-    bool wm = absf(cu - ou) <= hu, wp = absf(cu + ou) <= hu;
-    Hits H;
-    if (u_first) {
-        H.take(um, -hu, cw - ow, 0.0f);
-        H.take(up, hu, cw + ow, 0.0f);
-        H.take(wm, cu - ou, -hw, 0.0f);
-        H.take(wp, cu + ou, hw, 0.0f);
-    } else {
-        H.take(wm, cu - ou, -hw, 0.0f);
-        H.take(wp, cu + ou, hw, 0.0f);
-        H.take(um, -hu, cw - ow, 0.0f);
-        H.take(up, hu, cw + ow, 0.0f);
-    }
-    if (!H.chord()) return 0.0f;
-    float ex = H.p1x - H.p0x, ey = H.p1y - H.p0y;
+    const float cw = s_wu * (-pu) + pw;   // u faces: w coordinate
+    const float cu = s_uw * (-pw) + pu;   // w faces: u coordinate
+    float ex, ey;
+    const bool two = u_first ? chord2_points<true>(cw, ow, cu, ou, hu, hw, ex, ey)
+                             : chord2_points<false>(cw, ow, cu, ou, hu, hw, ex, ey);
+    if (!two) return 0.0f;
     return sqrtf(ex * ex + ey * ey);
+}
+
+// The chord of find_l_3d from its six face candidates, given in face order
+// (x-, x+, y-, y+, z-, z+): tested x faces then y faces (X_FIRST) or y faces
+// then x faces, then the z faces.
+template <bool X_FIRST>
+__host__ __device__ __forceinline__ bool chord6(const bool (&f)[6], const float (&x)[6],
+                                                const float (&y)[6], const float (&z)[6],
+                                                float &ex, float &ey, float &ez)
+{
+    Cands<6> c;
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        const int o = X_FIRST ? k : (k < 2 ? k + 2 : k < 4 ? k - 2 : k);
+        c.f[k] = f[o];
+        c.x[k] = x[o];
+        c.y[k] = y[o];
+        c.z[k] = z[o];
+    }
+    return chord_points<6>(c, ex, ey, ez);
+}
+
+__host__ __device__ __forceinline__ bool chord3_points(
+    bool x_first, float xy_c, float xy_o, float xz_c, float xz_o, float yx_c, float yx_o,
+    float yz_c, float yz_o, float zx_c, float zx_o, float zy_c, float zy_o, float hx, float hy,
+    float hz, float &ex, float &ey, float &ez)
+{
+#pragma clang fp contract(off)
+    const bool f[6] = {absf(xy_c - xy_o) <= hy && absf(xz_c - xz_o) <= hz,
+                       absf(xy_c + xy_o) <= hy && absf(xz_c + xz_o) <= hz,
+                       absf(yx_c - yx_o) <= hx && absf(yz_c - yz_o) <= hz,
+                       absf(yx_c + yx_o) <= hx && absf(yz_c + yz_o) <= hz,
+                       absf(zx_c - zx_o) <= hx && absf(zy_c - zy_o) <= hy,
+                       absf(zx_c + zx_o) <= hx && absf(zy_c + zy_o) <= hy};
+    const float x[6] = {-hx, hx, yx_c - yx_o, yx_c + yx_o, zx_c - zx_o, zx_c + zx_o};
+    const float y[6] = {xy_c - xy_o, xy_c + xy_o, -hy, hy, zy_c - zy_o, zy_c + zy_o};
+    const float z[6] = {xz_c - xz_o, xz_c + xz_o, yz_c - yz_o, yz_c + yz_o, -hz, hz};
+    return x_first ? chord6<true>(f, x, y, z, ex, ey, ez) : chord6<false>(f, x, y, z, ex, ey, ez);
 }
 
 __host__ __device__ __forceinline__ float find_l_3d_rec(const RayRec &R, float px, float py, float pz,
@@ -335,28 +405,11 @@ __host__ __device__ __forceinline__ float find_l_3d_rec(const RayRec &R, float p
         const float yz_c = R.s_cb * (-py) + pz, yz_o = R.s_cb * hy;
         const float zx_c = R.s_ac * (-pz) + px, zx_o = R.s_ac * hz;
         const float zy_c = R.s_bc * (-pz) + py, zy_o = R.s_bc * hz;
-        bool xm = absf(xy_c - xy_o) <= hy && absf(xz_c - xz_o) <= hz;
-        bool xp = absf(xy_c + xy_o) <= hy && absf(xz_c + xz_o) <= hz;
-        bool ym = absf(yx_c - yx_o) <= hx && absf(yz_c - yz_o) <= hz;
-        bool yp = absf(yx_c + yx_o) <= hx && absf(yz_c + yz_o) <= hz;
-        bool zm = absf(zx_c - zx_o) <= hx && absf(zy_c - zy_o) <= hy;
-        bool zp = absf(zx_c + zx_o) <= hx && absf(zy_c + zy_o) <= hy;
-        Hits H;
-        if (R.mode == RAY_X_FIRST) {
-            H.take(xm, -hx, xy_c - xy_o, xz_c - xz_o);
-            H.take(xp, hx, xy_c + xy_o, xz_c + xz_o);
-            H.take(ym, yx_c - yx_o, -hy, yz_c - yz_o);
-            H.take(yp, yx_c + yx_o, hy, yz_c + yz_o);
-        } else {
-            H.take(ym, yx_c - yx_o, -hy, yz_c - yz_o);
-            H.take(yp, yx_c + yx_o, hy, yz_c + yz_o);
-            H.take(xm, -hx, xy_c - xy_o, xz_c - xz_o);
-            H.take(xp, hx, xy_c + xy_o, xz_c + xz_o);
-        }
-        H.take(zm, zx_c - zx_o, zy_c - zy_o, -hz);
-        H.take(zp, zx_c + zx_o, zy_c + zy_o, hz);
-        if (!H.chord()) return 0.0f;
-        float ex = H.p1x - H.p0x, ey = H.p1y - H.p0y, ez = H.p1z - H.p0z;
+        float ex, ey, ez;
+        const bool two = chord3_points(R.mode == RAY_X_FIRST, xy_c, xy_o, xz_c, xz_o, yx_c, yx_o,
+                                       yz_c, yz_o, zx_c, zx_o, zy_c, zy_o, hx, hy, hz, ex, ey,
+                                       ez);
+        if (!two) return 0.0f;
         return sqrtf(ex * ex + ey * ey + ez * ez);
     }
     case RAY_A0_B0: return absf(py) <= hy ? dz : 0.0f;

@@ -105,28 +105,19 @@ __host__ __device__ __forceinline__ float tile_find_l(const TileRayXY &r, const 
     const float zx = z.e_xz + z.s_ac * dz - dx, zxo = z.s_ac * hz;
     const float zy = z.e_yz + z.s_bc * dz - dy, zyo = z.s_bc * hz;
     const float xyo = r.s_ba * hx, yxo = r.s_ab * hy;
-    const bool xm = (m & 1u) && absf(xz - xzo) <= hz;
-    const bool xp = (m & 2u) && absf(xz + xzo) <= hz;
-    const bool ym = (m & 4u) && absf(yz - yzo) <= hz;
-    const bool yp = (m & 8u) && absf(yz + yzo) <= hz;
-    const bool zm = absf(zx - zxo) <= hx && absf(zy - zyo) <= hy;
-    const bool zp = absf(zx + zxo) <= hx && absf(zy + zyo) <= hy;
-    Hits H;
-    if (r.mode == RAY_X_FIRST) {
-        H.take(xm, -hx, xy - xyo, xz - xzo);
-        H.take(xp, hx, xy + xyo, xz + xzo);
-        H.take(ym, yx - yxo, -hy, yz - yzo);
-        H.take(yp, yx + yxo, hy, yz + yzo);
-    } else {
-        H.take(ym, yx - yxo, -hy, yz - yzo);
-        H.take(yp, yx + yxo, hy, yz + yzo);
-        H.take(xm, -hx, xy - xyo, xz - xzo);
-        H.take(xp, hx, xy + xyo, xz + xzo);
-    }
-    H.take(zm, zx - zxo, zy - zyo, -hz);
-    H.take(zp, zx + zxo, zy + zyo, hz);
-    if (!H.chord()) return 0.0f;
-    const float ex = H.p1x - H.p0x, ey = H.p1y - H.p0y, ez = H.p1z - H.p0z;
+    const bool f[6] = {(m & 1u) && absf(xz - xzo) <= hz,
+                       (m & 2u) && absf(xz + xzo) <= hz,
+                       (m & 4u) && absf(yz - yzo) <= hz,
+                       (m & 8u) && absf(yz + yzo) <= hz,
+                       absf(zx - zxo) <= hx && absf(zy - zyo) <= hy,
+                       absf(zx + zxo) <= hx && absf(zy + zyo) <= hy};
+    const float cx[6] = {-hx, hx, yx - yxo, yx + yxo, zx - zxo, zx + zxo};
+    const float cy[6] = {xy - xyo, xy + xyo, -hy, hy, zy - zyo, zy + zyo};
+    const float cz[6] = {xz - xzo, xz + xzo, yz - yzo, yz + yzo, -hz, hz};
+    float ex, ey, ez;
+    const bool two = r.mode == RAY_X_FIRST ? chord6<true>(f, cx, cy, cz, ex, ey, ez)
+                                           : chord6<false>(f, cx, cy, cz, ex, ey, ez);
+    if (!two) return 0.0f;
     return sqrtf(ex * ex + ey * ey + ez * ez);
 }
 
