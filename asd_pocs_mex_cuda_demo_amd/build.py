@@ -25,7 +25,8 @@ BUILD = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libasdpocs.so")
 MEXDIR = os.path.join(PKG, "mex")
 
-HIP_SOURCES = ["common.hip", "projectors.hip", "tv.hip", "driver.hip", "batch.hip"]
+HIP_SOURCES = ["common.hip", "projectors.hip", "tv.hip", "driver.hip", "batch.hip",
+               "simulate.hip"]
 MEX_SOURCES = ["Ax_cone_gpu.cpp", "Atx_cone_gpu.cpp", "imgTVGradMin.cpp", "imgTVGradMin3D.cpp",
                "ASD_POCS_gpu.cpp"]
 

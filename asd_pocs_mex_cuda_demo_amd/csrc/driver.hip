@@ -21,6 +21,7 @@
 #include <cstring>
 #include <vector>
 
+#include "plan.hpp"
 #include "projector_kernels.hpp"
 #include "reduce_device.hpp"
 #include "sino_codec.hpp"
@@ -30,23 +31,6 @@ namespace asd {
 
 namespace {
 
-// Stage timing (events on the plan stream) for the dominant kernels.
-enum : int { ST_AX = 0, ST_ATX = 1, ST_TVGRAD = 2, ST_N = 3 };  // ST_TVGRAD: a whole TV descent
-
-constexpr int HIST_CAP = 1 << 16;  // RMSE history entries kept on the device
-
-struct DevCtl {
-    double alpha;    // current TV step factor
-    double beta;     // current SART relaxation of the ASD loop
-    double relax;    // relaxation of a stand-alone sweep (SART_cuda_inline's lambda)
-    double dp, dd2, dg;
-    int k;           // outer iterations completed
-    int stopped;     // the stop rule fired
-};
-
-struct CtlRules {
-    double alpha_red, beta_red, r_max, epsilon, stopRelChg;
-};
 
 template <class Codec>
 struct EpiSartAx {
@@ -214,40 +198,6 @@ unsigned grid_for(size_t n)
 
 using namespace asd;
 
-struct asd_plan {
-    asd_para para;
-    int nv = 0;
-    size_t N = 0, M = 0;
-    std::vector<ViewGeom> views;
-    hipStream_t stream = nullptr;
-    DeviceBuf ydet, zdet, P, m, mpre, mk1, f0, I0, cor, res2, rt, v, u2, v2, ctlbuf, hist;
-    std::vector<double> u16_lo, u16_step;  // per-view scale of the 16-bit fixed-point sinogram
-    TvScratch red;
-    bool have_P = false, have_I0 = false;
-    int lpr = 1;
-    int *h_stop = nullptr;          // pinned copies of the stop flag (4 checkpoints)
-    hipEvent_t stop_ev[4] = {};
-    int atx_variant = -1;           // backprojector variant picked by the autotuner
-    double atx_us[ATX_NVARIANT];    // its measurements (-1: not tried)
-    DevCtl h;  // host copy of the control block after the last sync
-    // hipGraph of one outer iteration
-    bool use_graph = true;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    // optional stage timing (eager path only)
-    bool timing = false;
-    bool capturing = false;
-    struct Ev { hipEvent_t a, b; int stage; };
-    std::vector<Ev> ev;
-    double stage_ms[ST_N] = {0, 0, 0};
-    long stage_cnt[ST_N] = {0, 0, 0};
-
-    DevCtl *ctl() { return ctlbuf.as<DevCtl>(); }
-    CtlRules rules() const
-    {
-        return CtlRules{para.alpha_red, para.beta_red, para.r_max, para.epsilon, para.stopRelChg};
-    }
-};
 
 namespace {
 
@@ -599,6 +549,8 @@ int need_projections(asd_plan *p)
 
 }  // namespace
 
+void asd::plan_drop_graph(asd_plan *p) { drop_graph(p); }
+
 extern "C" {
 
 void asd_para_defaults(asd_para *p)
@@ -746,8 +698,9 @@ int asd_plan_reset(asd_plan *p, const double *m0, const double *I0)
         if ((rc = p->I0.ensure(p->N * 4))) return rc;
         if ((rc = upload_f64_as_f32(p, p->I0, I0, p->N))) return rc;
         p->have_I0 = true;
+        p->I0_sim = false;
     } else {
-        p->have_I0 = false;
+        p->have_I0 = p->I0_sim;  // a simulated phantom stays the reference
     }
     if (had_I0 != p->have_I0) drop_graph(p);  // the graph bakes in the I0 pointer
     ASD_HIP(hipMemsetAsync(p->res2.p, 0, p->M * 8, p->stream));

@@ -248,6 +248,25 @@ class Plan:
     def iterate(self, n):
         nat.check(self._L.asd_plan_iterate(self.h, int(n)))
 
+    def simulate(self, table, scale=0.02, I0=0.0, seed=0):
+        """Phantom (rows A, a, b, c, x0, y0, z0, phi_deg), its projections
+        and optional Poisson noise, made on the device (asd_plan_simulate)."""
+        t = np.ascontiguousarray(np.asarray(table, np.float64).reshape(-1, 8))
+        nat.check(self._L.asd_plan_simulate(self.h, _ptr(t), t.shape[0], float(scale),
+                                            float(I0), int(seed)))
+
+    def projections(self):
+        """The stored sinogram, decoded, as (na*nb) x nv."""
+        P = np.zeros((len(self.phi), self.M))
+        nat.check(self._L.asd_plan_get_projections(self.h, _ptr(P)))
+        return P.T
+
+    def reference(self):
+        """The RMSE reference image (flat, x fastest)."""
+        r = np.zeros(self.N)
+        nat.check(self._L.asd_plan_get_reference(self.h, _ptr(r)))
+        return r
+
     def sweep(self, lam=1.0):
         tol = C.c_double()
         nat.check(self._L.asd_plan_sweep(self.h, float(lam), C.byref(tol)))

@@ -33,7 +33,39 @@ WORKLOADS = {
                       SDD=949.0, I0=1e5, iters=50),
     "cone3d_256": dict(n=256, nz=256, vxl=1.0, nv=180, na=384, nb=384, det=1.2, SO=500.0,
                        SDD=1000.0, I0=5e4, iters=30),
+    # 512^3 at 0.5 mm, Shepp-Logan plus 200 seeded spheres (radius 1-4 mm,
+    # attenuation 0.01-0.03 /mm), 720 views, 1024 x 768 bins at 0.6 mm, SID
+    # 600 mm, SDD 1100 mm, FP16-stored projections (precision 2), simulated on
+    # the device; a bench run times 2 of its 20 iterations by default
+    "cone3d_512": dict(n=512, nz=512, vxl=0.5, nv=720, na=1024, nb=768, det=0.6, SO=600.0,
+                       SDD=1100.0, I0=0.0, iters=20, steps=2, precision=2, sim="gpu",
+                       spheres=200),
 }
+
+
+def sphere_table(k, seed, width_mm):
+    """k seeded spheres (radius 1-4 mm, attenuation 0.01-0.03 /mm, centres in
+    the inner 60% of the phantom) as phantom rows, values relative to the
+    phantom's scale of 0.02 /mm, coordinates in units of half the width."""
+    rng = np.random.default_rng(seed)
+    r = rng.uniform(1.0, 4.0, k) / (width_mm / 2)
+    mu = rng.uniform(0.01, 0.03, k) / 0.02
+    c = rng.uniform(-0.6, 0.6, (k, 3)) * np.array([0.69, 0.92, 0.81])
+    return np.column_stack([mu, r, r, r, c, np.zeros(k)])
+
+
+def make_sim_problem(w, precision, seed=20241203):
+    """The workload simulated on the device: phantom, projections, noise."""
+    from asd_pocs_mex_cuda_demo_amd import ops, phantoms
+    n, nz = w["n"], w["nz"]
+    para = dict(nx=n, ny=n, nz=nz, SO=w["SO"], OD=w["SDD"] - w["SO"], vxlsize=w["vxl"],
+                detsize=w["det"], na=w["na"], nb=w["nb"], alpha=0.2, numOfgradients=20)
+    para["phi_vec"] = np.arange(1, w["nv"] + 1) * 360.0 / w["nv"]
+    table = np.vstack([phantoms.ellipses(),
+                       sphere_table(w.get("spheres", 0), seed, n * w["vxl"])])
+    plan = ops.Plan(dict(para, maxnumIter=w["iters"], precision=precision), para["phi_vec"])
+    plan.simulate(table, scale=0.02, I0=w["I0"], seed=seed)
+    return para, plan
 
 
 def make_problem(w, seed=20241203):
@@ -200,16 +232,16 @@ def plain_rmse_and_floor(para, img, P, steps, max_iter):
     return res
 
 
-def stage_bytes(N, M, ng):
+def stage_bytes(N, M, ng, sino=4):
     """Algorithmic (compulsory) HBM bytes per timed unit of each stage:
-    forward projector launch reads m (4N) and P_v (4M) and writes cor (4M);
-    backprojector+update launch reads cor (4M) and reads/writes m (8N); a TV
-    descent runs ng x (gradient: read u, write v = 8N; step: read u, v,
-    write u = 12N)."""
-    return {0: 4 * N + 8 * M, 1: 8 * N + 4 * M, 2: ng * 20 * N}
+    forward projector launch reads m (4N) and P_v (`sino` bytes per value)
+    and writes cor (4M); backprojector+update launch reads cor (4M) and
+    reads/writes m (8N); a TV descent runs ng x (gradient: read u, write v =
+    8N; step: read u, v, write u = 12N)."""
+    return {0: 4 * N + (sino + 4) * M, 1: 8 * N + 4 * M, 2: ng * 20 * N}
 
 
-def cpu_baseline(para, P, sample_iters, sample_views=0, I0=None):
+def cpu_baseline(para, P, sample_iters, sample_views=0, I0=None, tv_iters=20):
     """The CPU oracle on the GPU box's host cores.  2-D: `sample_iters` full
     outer iterations (their result is returned for the accuracy check).
     3-D: `sample_views` SART view updates plus one full TV descent,
@@ -242,13 +274,13 @@ def cpu_baseline(para, P, sample_iters, sample_views=0, I0=None):
     t_views = time.perf_counter() - t0
     vol = m.reshape(para["nx"], para["ny"], para["nz"], order="F")
     t1 = time.perf_counter()
-    oracle.tv3d(vol, 0.2, 20, 1, np.zeros_like(vol) + 1e-3)
-    t_tv = time.perf_counter() - t1
+    oracle.tv3d(vol, 0.2, tv_iters, 1, np.zeros_like(vol) + 1e-3)
+    t_tv = (time.perf_counter() - t1) * 20 / tv_iters
     t_iter = t_views / sample_views * nv + t_tv
     return ({"value": 1.0 / t_iter, "unit": "iters/sec", "cores": threads, "kind": "port",
-             "sample": f"{sample_views} SART view updates ({t_views:.2f} s) + one 20-step TV3D "
-                       f"descent ({t_tv:.2f} s), extrapolated to one iteration of {nv} views "
-                       f"({t_iter:.1f} s)"}, None)
+             "sample": f"{sample_views} SART view updates ({t_views:.2f} s) + {tv_iters} TV3D "
+                       f"steps (scaled to 20: {t_tv:.2f} s), extrapolated to one iteration of "
+                       f"{nv} views ({t_iter:.1f} s)"}, None)
 
 
 def main():
@@ -257,7 +289,10 @@ def main():
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="fan2d_512", choices=sorted(WORKLOADS) + ["batch64"])
-    ap.add_argument("--precision", type=int, default=0)
+    ap.add_argument("--cpu-tv-iters", type=int, default=None,
+                    help="TV sub-iterations the CPU sample times (scaled to ng)")
+    ap.add_argument("--precision", type=int, default=None,
+                    help="sinogram storage / arithmetic variant (default: the workload's)")
     ap.add_argument("--cpu-iters", type=int, default=5)
     ap.add_argument("--cpu-views", type=int, default=None)
     ap.add_argument("--prof-steps", type=int, default=10)
@@ -290,14 +325,21 @@ def main():
         bench_batch(args)
         return
     w = WORKLOADS[args.workload]
-    steps = args.steps if args.steps is not None else w["iters"]
-    para, img, P, plan = make_problem(w)
-    if args.precision:
-        plan.close()
-        from asd_pocs_mex_cuda_demo_amd import ops
-        plan = ops.Plan(dict(para, maxnumIter=w["iters"], precision=args.precision),
-                        para["phi_vec"])
-    plan.set_projections(P)
+    steps = args.steps if args.steps is not None else w.get("steps", w["iters"])
+    sim = w.get("sim") == "gpu"
+    if args.precision is None:
+        args.precision = w.get("precision", 0)
+    if sim:
+        para, plan = make_sim_problem(w, args.precision)
+        img = P = None  # the plan holds the phantom (RMSE reference) and sinogram
+    else:
+        para, img, P, plan = make_problem(w)
+        if args.precision:
+            plan.close()
+            from asd_pocs_mex_cuda_demo_amd import ops
+            plan = ops.Plan(dict(para, maxnumIter=w["iters"], precision=args.precision),
+                            para["phi_vec"])
+        plan.set_projections(P)
     if args.atx_variant:
         plan.set_atx_variant(list(ATX_KERNELS).index(args.atx_variant))
     N = para["nx"] * para["ny"] * para["nz"]
@@ -331,7 +373,7 @@ def main():
     stage = {s: plan.timing(s) for s in (0, 1, 2)}
     plan.set_timing(False)
 
-    nbytes = stage_bytes(N, M, 20)
+    nbytes = stage_bytes(N, M, 20, 4 if args.precision == 0 else 2)
     names = {0: "ax_kernel (fused SART forward step)", 1: "atx_kernel (fused backprojection + update)",
              2: "TV descent (20 x tv_grad_kernel + tv_step_kernel)"}
     per_stage = {}
@@ -370,7 +412,12 @@ def main():
                   2: "f32 (fp16 sinogram, compensated sums)",
                   3: "f32 (16-bit fixed-point sinogram, compensated sums, "
                      "f64 tile ray setup)"}[args.precision],
-        "data": f"synthetic: modified Shepp-Logan phantom, Poisson noise I0={w['I0']:g} (seeded), log transform",
+        "data": ("synthetic, simulated on the device: modified Shepp-Logan phantom"
+                 + (f" + {w['spheres']} seeded spheres" if w.get("spheres") else "")
+                 + (f", Poisson noise I0={w['I0']:g} (seeded), log transform" if w["I0"]
+                    else ", noiseless projections")) if sim else
+                f"synthetic: modified Shepp-Logan phantom, Poisson noise I0={w['I0']:g} (seeded), "
+                f"log transform",
         "config": {"workload": args.workload, "nx": para["nx"], "ny": para["ny"],
                    "nz": para["nz"], "views": w["nv"], "na": para["na"], "nb": para["nb"],
                    "iterations": steps, "ng": 20, "precision": args.precision},
@@ -381,7 +428,7 @@ def main():
         "ax_lanes_per_ray": lpr,
         "atx_tuning": tuning,
     }
-    if args.precision:
+    if args.precision and not sim:
         # matched accuracy: the plain FP32 plan over the same iterations, and
         # the same plan on data perturbed by 1e-7 relative (the spread the TV
         # descent gives to any change of the data at FP32 level)
@@ -390,8 +437,12 @@ def main():
         result["rmse_rel_diff_plain_under_1e-7_data_perturbation"] = float(
             np.max(np.abs(rq - rp) / rp))
     if not args.no_cpu:
-        views = args.cpu_views if args.cpu_views is not None else (0 if para["nz"] == 1 else 2)
-        cb, orc_run = cpu_baseline(para, P, args.cpu_iters, views, I0=img)
+        views = args.cpu_views if args.cpu_views is not None else (
+            0 if para["nz"] == 1 else 1 if sim else 2)
+        if P is None:
+            P = plan.projections()
+        tv_iters = args.cpu_tv_iters if args.cpu_tv_iters is not None else (1 if sim else 20)
+        cb, orc_run = cpu_baseline(para, P, args.cpu_iters, views, I0=img, tv_iters=tv_iters)
         result["cpu_baseline"] = cb
         if orc_run is not None:
             # the GPU plan over the oracle's iteration count, against the oracle

@@ -155,6 +155,23 @@ int asd_plan_sweep(asd_plan *plan, double lambda, double *dataTolerance);
 int asd_plan_get_current(asd_plan *plan, double *m_host);
 /* Synchronise the plan's stream (for timing). */
 int asd_plan_sync(asd_plan *plan);
+/* Fills the plan with simulated data on the device.  No counterpart in the
+ * reference, whose demos build their inputs in MATLAB (demo_test.m:35-52).
+ *  - The ellipsoid phantom (n_ell rows of A, a, b, c, x0, y0, z0, phi in
+ *    degrees; coordinates in [-1, 1] across the grid, rotation about z;
+ *    values summed in row order, times scale) becomes the plan's RMSE
+ *    reference; a later asd_plan_reset with I0 == NULL keeps it.
+ *  - Every view is forward projected.  With I0_photons > 0 each projection
+ *    value p becomes -ln(max(k, 1) / I0_photons), k ~ Poisson(I0_photons *
+ *    exp(-p)), from a Philox4x32-10 stream per value keyed by seed.
+ *  - The result is encoded into the plan's sinogram storage as
+ *    asd_plan_set_projections would encode it. */
+int asd_plan_simulate(asd_plan *plan, const double *ellipsoids, int n_ell, double scale,
+                      double I0_photons, unsigned long long seed);
+/* The plan's sinogram decoded to FP64, views contiguous (nv x na*nb). */
+int asd_plan_get_projections(asd_plan *plan, double *P_host);
+/* The plan's RMSE reference image (FP32 values as FP64). */
+int asd_plan_get_reference(asd_plan *plan, double *I0_host);
 /* Per-kernel event timing on the plan stream: stage 0 = forward projector,
  * 1 = backprojector + update, 2 = TV gradient.  Enabling resets the totals. */
 int asd_plan_set_timing(asd_plan *plan, int enable);
