@@ -89,24 +89,32 @@ template <> __device__ __forceinline__ SinoF16 item_codec<SinoF16>(const BItem &
     return SinoF16{(const __half *)it.P};
 }
 
+// Rays of one view step of every active case.  Each case's rays fill whole
+// blocks (abpref: block prefix over the active cases), so the case, its work
+// item and its stop flag are block-uniform (scalar loads).  The detector
+// coordinates are the gateway grid computed in place; the ray tables were
+// built once per case and view.
 template <int LPR, bool KAHAN, class PT>
 __global__ __launch_bounds__(256) void ax_batch_kernel(const BItem *__restrict__ items,
-                                                       const int *__restrict__ rpref, int ncase)
+                                                       const int *__restrict__ abpref, int ncase)
 {
     constexpr int RPB = 256 / LPR;
     const int lane = threadIdx.x % LPR;
-    const int gid = blockIdx.x * RPB + threadIdx.x / LPR;
-    const int total = rpref[ncase];
-    bool active = gid < total;
-    int c = active ? find_seg(rpref, ncase, gid) : 0;
+    if ((int)blockIdx.x >= abpref[ncase]) return;
+    const int c = find_seg(abpref, ncase, (int)blockIdx.x);
     const BItem &it = items[c];
-    if (active && it.ctl->stopped) active = false;
-    const long id = gid - (active ? rpref[c] : 0);
+    const ViewGeom &g = it.g;
+    const long id = (long)((int)blockIdx.x - abpref[c]) * RPB + threadIdx.x / LPR;
+    const bool active = id < (long)g.na * g.nb && !it.ctl->stopped;
+    double pv = 0.0, r2 = 0.0;
+    if (active && lane == 0) {  // epilogue operands, loaded before the march
+        pv = item_codec<PT>(it).at(id);
+        r2 = it.res2[id];
+    }
     Acc<KAHAN> sa, sl;
     if (active) {
-        const ViewGeom &g = it.g;
         int ia = (int)(id % g.na), ib = (int)(id / g.na);
-        AxRay r = ax_ray_setup(g, it.ydet[ia], it.zdet[ib]);
+        AxRay r = ax_ray_setup(g, det_coord(ia, g.na, g.dy_det), det_coord(ib, g.nb, g.dz_det));
         int nu = r.xdriven ? g.nx : g.ny;
         for (int iu0 = lane; iu0 < nu; iu0 += 4 * LPR)
             ax_slabs<4, KAHAN>(r, it.m, iu0, LPR, nu, g.nx, g.ny, sa, sl);
@@ -135,14 +143,11 @@ __global__ __launch_bounds__(256) void ax_batch_kernel(const BItem *__restrict__
     if (!active || lane != 0) return;
     {
 #pragma clang fp contract(off)
-        const ViewGeom &g = it.g;
         float proj = acc * g.scale, sumr = accl * g.scale;
         double r = (sumr == 0.0f) ? 1.0 : (double)sumr;
-        double c0 = item_codec<PT>(it).at(id) - (double)proj;
-        it.res2[id] += c0 * c0;
+        double c0 = pv - (double)proj;
+        it.res2[id] = r2 + c0 * c0;
         it.cor[id] = (float)(c0 / r);
-        int ia = (int)(id % g.na), ib = (int)(id / g.na);
-        it.rt[id] = make_ray_rec(g, it.ydet[ia], it.zdet[ib]);
     }
 }
 
@@ -167,28 +172,8 @@ __global__ __launch_bounds__(256) void atx_batch_kernel(const BItem *__restrict_
     Footprint f = footprint(g, ix, iy, iz);
     const float sx = g.cos_phi * (-g.SO), sy = g.sin_phi * (-g.SO);
     const float px = sx - f.xc, py = sy - f.yc, pz = 0.0f - f.zc;
-    const float hz = g.dz / 2.0f;
     Acc<KAHAN> acc, accl;
-    for (int ib = f.b_lo; ib <= f.b_hi; ib++) {
-        const long row = (long)ib * g.na;
-        for (int a0 = f.a_lo; a0 <= f.a_hi; a0 += ATX_CHUNK) {
-            RayRec R[ATX_CHUNK];
-            float yv[ATX_CHUNK];
-#pragma unroll
-            for (int k = 0; k < ATX_CHUNK; k++) {
-                const int ia = a0 + k <= f.a_hi ? a0 + k : f.a_hi;
-                R[k] = it.rt[row + ia];
-                yv[k] = it.cor[row + ia];
-            }
-#pragma unroll
-            for (int k = 0; k < ATX_CHUNK; k++) {
-                if (a0 + k > f.a_hi || (!g.tight && !inplane_may_hit(R[k], px, py))) continue;
-                float l = find_l_3d_rec(R[k], px, py, pz, hz, g.dz);
-                acc.add(l * yv[k]);
-                accl.add(l);
-            }
-        }
-    }
+    atx_voxel_walk<KAHAN, 2>(g, it.cor, it.rt, f, px, py, pz, g.dz / 2.0f, acc, accl);
     float bp = acc.value() * g.scale, sc = accl.value() * g.scale;
     double cc = (sc == 0.0f) ? 1.0 : (double)sc;
     double nm = (double)m0 + it.ctl->beta * ((double)bp / cc);
@@ -477,8 +462,8 @@ int b_enqueue_iteration(asd_batch *B, int sweep_only, int iterate)
         const BItem *items = B->items.as<BItem>() + (size_t)j * K;
         const int *rp = B->rpref.as<int>() + (size_t)j * (K + 1);
         const int *bp = B->bpref.as<int>() + (size_t)j * (K + 1);
-        const int rays = B->h_rpref_tot[j], blocks = B->h_bpref_tot[j];
-        const unsigned ab = (unsigned)(((long)rays * B->lpr + 255) / 256);
+        const int blocks = B->h_bpref_tot[j];
+        const unsigned ab = (unsigned)B->h_rpref_tot[j];  // forward-projector blocks
         if (B->precision == 1) {
             switch (B->lpr) {
             case 64: ax_batch_kernel<64, true, SinoU16><<<ab, 256, 0, B->stream>>>(items, rp, A); break;
@@ -627,7 +612,7 @@ int asd_batch_create(int ncases, const asd_para *paras, const double *const *phi
     if ((rc = B->m.ensure(N * 4)) || (rc = B->mpre.ensure(N * 4)) || (rc = B->mk1.ensure(N * 4)) ||
         (rc = B->f0.ensure(N * 4)) || (rc = B->I0.ensure(N * 4)) || (rc = B->v.ensure(N * 4)) ||
         (rc = B->cor.ensure(M * 4)) || (rc = B->res2.ensure(M * 8)) ||
-        (rc = B->rt.ensure(M * sizeof(RayRec))) ||
+        (rc = B->rt.ensure(B->Ptot * sizeof(RayRec))) ||
         (rc = B->P.ensure(B->Ptot * sino_bytes(B->precision))) ||
         (rc = B->ydet.ensure(ytot * 4)) || (rc = B->zdet.ensure(ztot * 4)) ||
         (rc = B->ctlb.ensure(sizeof(BCtl) * K)) || (rc = B->hist.ensure((size_t)K * B_HIST * 8)))
@@ -668,6 +653,7 @@ int asd_batch_create(int ncases, const asd_para *paras, const double *const *phi
     B->h_bpref_tot.assign(B->max_nv, 0);
     BCtl *dctl = B->ctlb.as<BCtl>();
     const size_t Pe = sino_bytes(B->precision);
+    const int rpb = 256 / B->lpr;  // rays per forward-projector block
     for (int j = 0; j < B->max_nv; j++) {
         int A = 0, rays = 0, blocks = 0;
         for (int s = 0; s < K; s++) {
@@ -680,7 +666,7 @@ int asd_batch_create(int ncases, const asd_para *paras, const double *const *phi
             it.P = (const char *)B->P.p + (B->poff[c] + (size_t)j * p.na * p.nb) * Pe;
             it.cor = B->cor.as<float>() + B->roff[c];
             it.res2 = B->res2.as<double>() + B->roff[c];
-            it.rt = B->rt.as<RayRec>() + B->roff[c];
+            it.rt = B->rt.as<RayRec>() + B->poff[c] + (size_t)j * p.na * p.nb;
             it.ydet = B->ydet.as<float>() + yoff[c] + (size_t)j * p.na;
             it.zdet = B->zdet.as<float>() + zoff[c] + (size_t)j * p.nb;
             it.ctl = dctl + c;
@@ -688,7 +674,7 @@ int asd_batch_create(int ncases, const asd_para *paras, const double *const *phi
             it.step = 0.0;
             rp[(size_t)j * (K + 1) + A] = rays;
             bp[(size_t)j * (K + 1) + A] = blocks;
-            rays += p.na * p.nb;
+            rays += (p.na * p.nb + rpb - 1) / rpb;  // whole blocks per case
             blocks += (int)(((size_t)p.nx * p.ny * p.nz + 255) / 256);
             A++;
         }
@@ -744,6 +730,16 @@ int asd_batch_create(int ncases, const asd_para *paras, const double *const *phi
         set_error(ASD_ERR_HIP, "batch initialisation failed");
         return fail(ASD_ERR_HIP);
     }
+    // the ray-slope tables of every case and view, once
+    for (int c = 0; c < K; c++)
+        for (int j = 0; j < nvs[c]; j++) {
+            const asd_para &p = paras[c];
+            if ((rc = launch_ray_table(geoms[c][j], B->ydet.as<float>() + yoff[c] + (size_t)j * p.na,
+                                       B->zdet.as<float>() + zoff[c] + (size_t)j * p.nb,
+                                       B->rt.as<RayRec>() + B->poff[c] + (size_t)j * p.na * p.nb,
+                                       B->stream)))
+                return fail(rc);
+        }
     // start images: zeros
     b_copy3<<<gridn(N), 256, 0, B->stream>>>(B->m.as<float>(), B->mpre.as<float>(),
                                              B->mk1.as<float>(), B->f0.as<float>(), N);
