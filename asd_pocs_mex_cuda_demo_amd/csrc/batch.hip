@@ -23,6 +23,7 @@
 #include "projector_kernels.hpp"
 #include "reduce_device.hpp"
 #include "sino_codec.hpp"
+#include "tv_stencil.hpp"
 
 namespace asd {
 namespace {
@@ -200,99 +201,76 @@ __device__ __forceinline__ void fold_case(const double *partials, int nblocks_al
     for (int k = 0; k < K; k++) tot[k] = res[k];
 }
 
-template <class T>
-__device__ __forceinline__ float ldz(const T *u, int nx, int ny, int nz, int x, int y, int z)
-{
-    if (x < 0 || x >= nx || y < 0 || y >= ny || z < 0 || z >= nz) return 0.0f;
-    return (float)u[((size_t)z * ny + y) * nx + x];
-}
-
-// TV gradient of every case (FP32, the single-case driver's stencils).
-__global__ __launch_bounds__(256) void tv_grad_batch(const BCase *__restrict__ cs,
-                                                     const int *__restrict__ bpref, int ncase,
-                                                     const float *__restrict__ u_all,
-                                                     float *__restrict__ v_all, const BCtl *ctl,
-                                                     double *partials, int it)
+// One TV sub-iteration `it` of every case, the step of sub-iteration it-1
+// fused into gradient it as in the single-case descent (tv.hip):
+//   it == 0          gradient of the case's image, FP64 partials of ||v||^2
+//   0 < it < ng      fold the case's previous partials (in every block, fixed
+//                    order) into s = alpha * d / ||v||, apply that step to
+//                    every value read, store this voxel's stepped value and
+//                    the new gradient
+//   it == ng         the last step into the image, with positivity
+// Blocks never straddle cases, so the case is block-uniform.
+__global__ __launch_bounds__(256) void tv_sub_batch(const BCase *__restrict__ cs,
+                                                    const int *__restrict__ bpref, int ncase,
+                                                    float *__restrict__ u_all, float *__restrict__ v_all,
+                                                    float *__restrict__ u2_all,
+                                                    float *__restrict__ v2_all, const BCtl *ctl,
+                                                    double *partials, int it)
 {
 #pragma clang fp contract(off)
     const int b = blockIdx.x;
     const int c = find_seg(bpref, ncase, b);
     const BCase &C = cs[c];
-    if (ctl[c].stopped || it >= C.rules.ng) return;
-    const float *u = u_all + C.voff;
-    float *v = v_all + C.voff;
+    const int ng = C.rules.ng;
+    if (ctl[c].stopped || ng == 0 || it > ng) return;
+    const int nb_all = bpref[ncase];
     const size_t n = (size_t)C.nx * C.ny * C.nz;
     const size_t i = (size_t)(b - bpref[c]) * 256 + threadIdx.x;
+    // sub-iteration it-1's image and gradient are (u, v) when it-1 is even,
+    // (u2, v2) when odd; sub-iteration it writes the other pair
+    const bool odd = it & 1;
+    const float *u = (odd ? u_all : u2_all) + C.voff;
+    const float *vp = (odd ? v_all : v2_all) + C.voff;
+    float *un = (odd ? u2_all : u_all) + C.voff;
+    float *vn = (odd ? v2_all : v_all) + C.voff;
+    double *prow = partials + (size_t)(it & 1) * nb_all;
+    const double *prev = partials + (size_t)((it + 1) & 1) * nb_all;
+    double st = 0.0;
+    if (it > 0) {
+        double nrm2[1];
+        fold_case<1>(prev, 0, bpref[c], bpref[c + 1] - bpref[c], nrm2);
+        st = ctl[c].alpha * sqrt(ctl[c].d2) / sqrt(nrm2[0]);
+    }
+    if (it == ng) {
+        // last step (imgTVGradMin.cpp:104-116): into the case's image
+        if (i < n) {
+            double w = (double)u[i] - st * (double)vp[i];
+            if (w < 0) w = 0;
+            u_all[C.voff + i] = (float)w;
+        }
+        return;
+    }
+    // image and gradient of this sub-iteration: (u, vp) stepped when it > 0
+    const float *src_u = it == 0 ? u_all + C.voff : u;
     double s = 0.0;
     if (i < n) {
-        const float eps = 1e-8f;
-        int x = (int)(i % C.nx);
-        size_t rr = i / C.nx;
-        int y = (int)(rr % C.ny), z = (int)(rr / C.ny);
+        const int x = (int)(i % C.nx);
+        const size_t rr = i / C.nx;
+        const int y = (int)(rr % C.ny), z = (int)(rr / C.ny);
         float g;
-        if (!C.rules.three_d) {
-            const int R = C.nx, Cn = C.ny;
-            float p = ldz(u, R, Cn, 1, x, y, 0);
-            float rm = ldz(u, R, Cn, 1, x - 1, y, 0), cm = ldz(u, R, Cn, 1, x, y - 1, 0);
-            float rp = ldz(u, R, Cn, 1, x + 1, y, 0), rp_cm = ldz(u, R, Cn, 1, x + 1, y - 1, 0);
-            float cp = ldz(u, R, Cn, 1, x, y + 1, 0), rm_cp = ldz(u, R, Cn, 1, x - 1, y + 1, 0);
-            float g1 = ((p - rm) + (p - cm)) / sqrtf(eps + (p - rm) * (p - rm) + (p - cm) * (p - cm));
-            float g2 = -(rp - p) / sqrtf(eps + (rp - p) * (rp - p) + (rp - rp_cm) * (rp - rp_cm));
-            float g3 = -(cp - p) / sqrtf(eps + (cp - p) * (cp - p) + (cp - rm_cp) * (cp - rm_cp));
-            g = g1 + g2 + g3;
+        if (it == 0) {
+            const TvSrc<float, float, false> U{src_u, nullptr, 0.0, C.nx, C.ny, C.nz};
+            g = C.rules.three_d ? tv3_grad<float>(U, x, y, z) : tv2_grad<float>(U, x, y);
         } else {
-            auto U = [&](int a, int bb, int cc) { return ldz(u, C.nx, C.ny, C.nz, a, bb, cc); };
-            float p = U(x, y, z);
-            float xm = U(x - 1, y, z), ym = U(x, y - 1, z), zm = U(x, y, z - 1);
-            float zp = U(x, y, z + 1), xm_zp = U(x - 1, y, z + 1), ym_zp = U(x, y - 1, z + 1);
-            float yp = U(x, y + 1, z), xm_yp = U(x - 1, y + 1, z), yp_zm = U(x, y + 1, z - 1);
-            float xp = U(x + 1, y, z), xp_ym = U(x + 1, y - 1, z), xp_zm = U(x + 1, y, z - 1);
-            float g1 = (6.0f * p - 2.0f * (xm + ym + zm)) /
-                       sqrtf(eps + (p - xm) * (p - xm) + (p - ym) * (p - ym) + (p - zm) * (p - zm));
-            float g2 = (2.0f * p - 2.0f * zp) /
-                       sqrtf(eps + (zp - xm_zp) * (zp - xm_zp) + (zp - ym_zp) * (zp - ym_zp) + (zp - p) * (zp - p));
-            float g3 = (2.0f * p - 2.0f * yp) /
-                       sqrtf(eps + (yp - xm_yp) * (yp - xm_yp) + (yp - p) * (yp - p) + (yp - yp_zm) * (yp - yp_zm));
-            float g4 = (2.0f * p - 2.0f * xp) /
-                       sqrtf(eps + (xp - p) * (xp - p) + (xp - xp_ym) * (xp - xp_ym) + (xp - xp_zm) * (xp - xp_zm));
-            g = g1 + g2 + g3 + g4;
+            const TvSrc<float, float, true> U{u, vp, st, C.nx, C.ny, C.nz};
+            g = C.rules.three_d ? tv3_grad<float>(U, x, y, z) : tv2_grad<float>(U, x, y);
+            un[i] = U.stored(i);
         }
-        v[i] = g;
+        (it == 0 ? v_all + C.voff : vn)[i] = g;
         s = (double)g * (double)g;
     }
-    double val[1] = {s};
-    store_partials<1>(val, partials);
-}
-
-__global__ __launch_bounds__(256) void tv_norm_batch(const BCase *__restrict__ cs,
-                                                     const int *__restrict__ bpref, int ncase,
-                                                     BCtl *ctl, const double *partials, int it)
-{
-    const int c = blockIdx.x;
-    if (ctl[c].stopped || it >= cs[c].rules.ng) return;
-    double tot[1];
-    fold_case<1>(partials, bpref[ncase], bpref[c], bpref[c + 1] - bpref[c], tot);
-    if (threadIdx.x == 0) ctl[c].nrm2 = tot[0];
-}
-
-__global__ __launch_bounds__(256) void tv_step_batch(const BCase *__restrict__ cs,
-                                                     const int *__restrict__ bpref, int ncase,
-                                                     float *__restrict__ u_all,
-                                                     const float *__restrict__ v_all,
-                                                     const BCtl *ctl, int it)
-{
-#pragma clang fp contract(off)
-    const int b = blockIdx.x;
-    const int c = find_seg(bpref, ncase, b);
-    const BCase &C = cs[c];
-    if (ctl[c].stopped || it >= C.rules.ng) return;
-    const size_t n = (size_t)C.nx * C.ny * C.nz;
-    const size_t i = (size_t)(b - bpref[c]) * 256 + threadIdx.x;
-    if (i >= n) return;
-    const double st = ctl[c].alpha * sqrt(ctl[c].d2) / sqrt(ctl[c].nrm2);
-    double w = (double)u_all[C.voff + i] - st * (double)v_all[C.voff + i];
-    if (it == C.rules.ng - 1 && w < 0) w = 0;  // positivity after the last step
-    u_all[C.voff + i] = (float)w;
+    s = block_sum(s);
+    if (threadIdx.x == 0) prow[b] = s;
 }
 
 struct BBufs {
@@ -437,7 +415,7 @@ struct asd_batch {
     int max_nv = 0, max_ng = 0, lpr = 64;
     std::vector<int> active;   // active cases per view step (prefix of `order`)
     hipStream_t stream = nullptr;
-    DeviceBuf m, mpre, mk1, f0, I0, v, cor, res2, rt, P, ydet, zdet, ctlb, hist;
+    DeviceBuf m, mpre, mk1, f0, I0, v, u2, v2, cor, res2, rt, P, ydet, zdet, ctlb, hist;
     DeviceBuf items, rpref, bpref, cases, cbpref, partials;
     std::vector<BItem> h_items;  // host copy of the per-step work lists
     std::vector<int> pos;        // position of each case in `order`
@@ -496,15 +474,11 @@ int b_enqueue_iteration(asd_batch *B, int sweep_only, int iterate)
     ctl_sart_batch<<<K, 256, 0, B->stream>>>(cs, cbp, K, bb, ctl, part, sweep_only);
     ASD_LAUNCH_CHECK("ctl_sart_batch");
     if (!iterate) return ASD_OK;
-    for (int it = 0; it < B->max_ng; it++) {
-        tv_grad_batch<<<B->case_blocks, 256, 0, B->stream>>>(cs, cbp, K, B->m.as<float>(),
-                                                             B->v.as<float>(), ctl, part, it);
-        ASD_LAUNCH_CHECK("tv_grad_batch");
-        tv_norm_batch<<<K, 256, 0, B->stream>>>(cs, cbp, K, ctl, part, it);
-        ASD_LAUNCH_CHECK("tv_norm_batch");
-        tv_step_batch<<<B->case_blocks, 256, 0, B->stream>>>(cs, cbp, K, B->m.as<float>(),
-                                                             B->v.as<float>(), ctl, it);
-        ASD_LAUNCH_CHECK("tv_step_batch");
+    for (int it = 0; it <= B->max_ng; it++) {
+        tv_sub_batch<<<B->case_blocks, 256, 0, B->stream>>>(cs, cbp, K, B->m.as<float>(),
+                                                            B->v.as<float>(), B->u2.as<float>(),
+                                                            B->v2.as<float>(), ctl, part, it);
+        ASD_LAUNCH_CHECK("tv_sub_batch");
     }
     post_tv_batch<<<B->case_blocks, 256, 0, B->stream>>>(cs, cbp, K, bb, ctl, part);
     ASD_LAUNCH_CHECK("post_tv_batch");
@@ -611,6 +585,7 @@ int asd_batch_create(int ncases, const asd_para *paras, const double *const *phi
     int rc;
     if ((rc = B->m.ensure(N * 4)) || (rc = B->mpre.ensure(N * 4)) || (rc = B->mk1.ensure(N * 4)) ||
         (rc = B->f0.ensure(N * 4)) || (rc = B->I0.ensure(N * 4)) || (rc = B->v.ensure(N * 4)) ||
+        (rc = B->u2.ensure(N * 4)) || (rc = B->v2.ensure(N * 4)) ||
         (rc = B->cor.ensure(M * 4)) || (rc = B->res2.ensure(M * 8)) ||
         (rc = B->rt.ensure(B->Ptot * sizeof(RayRec))) ||
         (rc = B->P.ensure(B->Ptot * sino_bytes(B->precision))) ||
