@@ -119,7 +119,7 @@ def bench_batch(args):
     # plain: one plan per case, sequentially
     t_plain = 0.0
     rmse_plain, alpha_plain = [], []
-    for para, img, P in cases:
+    for para, img, P in ([] if args.batch_only else cases):
         pl = ops.Plan(dict(para, maxnumIter=steps), para["phi_vec"])
         pl.set_projections(P)
         pl.reset(None, img)
@@ -151,9 +151,15 @@ def bench_batch(args):
         al = np.array([B.out(c)[0]["alpha"] for c in range(K)])
         out[prec] = (dt, rm, al)
         B.close()
-    rp = np.array(rmse_plain)
     dt_mixed, rm_mixed, al_mixed = out[1]
     dt_bplain, rm_bplain, al_bplain = out[0]
+    if args.batch_only:
+        print(json.dumps({"workload": "batch64_fan2d", "cases": K, "iterations": steps,
+                          "batched_mixed_case_iters_per_sec": K * steps / dt_mixed,
+                          "batched_plain_case_iters_per_sec": K * steps / dt_bplain}),
+              flush=True)
+        return
+    rp = np.array(rmse_plain)
     ap = np.array(alpha_plain)
     # the spread of the plain result under a 1e-7 relative data perturbation
     rq = np.array([plain_rmse_and_floor(para, img, P, steps, steps)[1][-1]
@@ -302,6 +308,9 @@ def main():
                              "direct2"],
                     help="force the backprojector variant instead of the autotuner's pick "
                          "(profiling runs use this to profile one variant)")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="batch64: time only the batched forms (skips the sequential plain "
+                         "plans and the perturbation floor; for kernel traces of the batch)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 256 MiB device to device three times (a plain copy "
                          "kernel of known size, to scale rocprofv3 FETCH_SIZE/WRITE_SIZE)")
