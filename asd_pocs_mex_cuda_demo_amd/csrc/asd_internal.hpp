@@ -2,6 +2,7 @@
 // the device-side geometry helpers shared by the projector and driver kernels.
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
@@ -48,6 +49,37 @@ int last_status();
         }                                                                          \
     } while (0)
 
+// Unsigned division by a runtime constant d >= 1 as a multiply-high, an add
+// and shifts (T. Granlund and P. Montgomery, "Division by invariant integers
+// using multiplication", PLDI 1994, Fig. 4.1): exact for every 32-bit
+// dividend, and a handful of instructions instead of a general division.
+struct FastDiv {
+    unsigned m;
+    int s1, s2;
+};
+
+__host__ __device__ inline FastDiv make_fastdiv(unsigned d)
+{
+    if (d == 0) d = 1;  // sizes are validated separately; never divide by zero here
+    int l = 0;          // ceil(log2 d)
+    while ((1ull << l) < d) l++;
+    FastDiv f;
+    f.m = (unsigned)((((1ull << 32) * ((1ull << l) - d)) / d) + 1);
+    f.s1 = l < 1 ? l : 1;
+    f.s2 = l > 1 ? l - 1 : 0;
+    return f;
+}
+
+__host__ __device__ __forceinline__ unsigned fastdiv(unsigned n, const FastDiv &f)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    const unsigned t = __umulhi(f.m, n);
+#else
+    const unsigned t = (unsigned)(((unsigned long long)f.m * n) >> 32);
+#endif
+    return (t + ((n - t) >> f.s1)) >> f.s2;
+}
+
 // ---------------------------------------------------------------------------
 // Geometry of one view in voxel units (what the reference gateways compute,
 // src/AxAtx/Ax_main_test.cpp:138-161).
@@ -63,6 +95,8 @@ struct ViewGeom {
     float fp_cx[4], fp_cy[4];  // voxel corner offsets in the view frame (bounds only)
     float fp_rdy, fp_rdz;    // 1 / dy_det, 1 / dz_det (bounds only)
     int tight;               // footprints may be cut to the voxel's shadow (view_tight)
+    FastDiv div_nx, div_ny;  // voxel index -> (x, row) and row -> (y, z) (geom_finish)
+    FastDiv div_na;          // bin index -> (ia, ib) (geom_finish)
 };
 
 // Per-ray record for the backprojector: the six slope quotients the
@@ -121,6 +155,60 @@ __host__ __device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nblo
     unsigned cnt = per + (xcd < rem ? 1u : 0u);
     if (k < cnt) return start + k;
     return b;  // unreachable when nblocks is covered exactly
+}
+
+// Index sizes the kernels assume: voxel and bin indices are 32-bit, and the
+// forward projector forms a voxel index as (z * ny + y) * nx + x and the
+// backprojector a bin index as ib * na + ia with 24-bit multiply-adds
+// (mad24), so each factor must stay below 2^24.
+inline bool index_fits(long nx, long ny, long nz, long na, long nb)
+{
+    constexpr long L24 = 1L << 24;
+    return nx * ny * nz < (1L << 32) && na * nb < (1L << 31) && nx < L24 && ny * nz < L24 &&
+           na < L24 && nb < L24;
+}
+#define ASD_INDEX_LIMITS                                                                  \
+    "volume or detector too large (32-bit voxel and bin indices; nx, na, nb and ny*nz " \
+    "below 2^24)"
+
+// a * b + c for a, b < 2^24 (exact while the result is below 2^32): a
+// full-rate 24-bit multiply-add instead of a quarter-rate 32-bit multiply.
+// The empty asm keeps the result opaque: when only its low 24 bits feed
+// another mad24, the optimiser would otherwise rewrite this one as a 32-bit
+// multiply (same low bits, four times the cost).
+__host__ __device__ __forceinline__ unsigned mad24(unsigned a, unsigned b, unsigned c)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    unsigned p = __umul24(a, b) + c;
+    asm("" : "+v"(p));
+    return p;
+#else
+    return a * b + c;
+#endif
+}
+
+// Kernel timing by the dispatch itself.  While a LaunchTiming is installed on
+// the calling thread, klaunch hands its events to hipExtLaunchKernel, which
+// stamps `start` with the first kernel's start and `stop` with the end of
+// the last one: the kernels' own execution time, without the launch and
+// queue gaps that events recorded around a launch also measure.
+struct LaunchTiming {
+    hipEvent_t start, stop;
+    int launches;
+};
+inline thread_local LaunchTiming *t_launch_timing = nullptr;
+
+template <typename... KArgs, typename... Args>
+inline void klaunch(void (*kernel)(KArgs...), dim3 grid, dim3 block, unsigned shmem,
+                    hipStream_t s, Args... args)
+{
+    if (LaunchTiming *t = t_launch_timing) {
+        hipExtLaunchKernelGGL(kernel, grid, block, shmem, s, t->launches == 0 ? t->start : nullptr,
+                              t->stop, 0u, KArgs(args)...);
+        t->launches++;
+    } else {
+        kernel<<<grid, block, shmem, s>>>(KArgs(args)...);
+    }
 }
 
 }  // namespace asd

@@ -114,7 +114,8 @@ __global__ __launch_bounds__(256) void ax_batch_kernel(const BItem *__restrict__
     }
     Acc<KAHAN> sa, sl;
     if (active) {
-        int ia = (int)(id % g.na), ib = (int)(id / g.na);
+        const unsigned ub = fastdiv((unsigned)id, g.div_na);
+        const int ia = (int)((unsigned)id - mad24(ub, (unsigned)g.na, 0u)), ib = (int)ub;
         AxRay r = ax_ray_setup(g, det_coord(ia, g.na, g.dy_det), det_coord(ib, g.nb, g.dz_det));
         int nu = r.xdriven ? g.nx : g.ny;
         for (int iu0 = lane; iu0 < nu; iu0 += 4 * LPR)
@@ -167,9 +168,9 @@ __global__ __launch_bounds__(256) void atx_batch_kernel(const BItem *__restrict_
     const long N = (long)g.nx * g.ny * g.nz;
     if (v >= N) return;
     const float m0 = it.m[v];
-    const int ix = (int)(v % g.nx);
-    const long r = v / g.nx;
-    const int iy = (int)(r % g.ny), iz = (int)(r / g.ny);
+    const unsigned r = fastdiv((unsigned)v, g.div_nx), uz = fastdiv(r, g.div_ny);
+    const int ix = (int)((unsigned)v - mad24(r, (unsigned)g.nx, 0u));
+    const int iy = (int)(r - mad24(uz, (unsigned)g.ny, 0u)), iz = (int)uz;
     Footprint f = footprint(g, ix, iy, iz);
     const float sx = g.cos_phi * (-g.SO), sy = g.sin_phi * (-g.SO);
     const float px = sx - f.xc, py = sy - f.yc, pz = 0.0f - f.zc;
@@ -254,16 +255,17 @@ __global__ __launch_bounds__(256) void tv_sub_batch(const BCase *__restrict__ cs
     const float *src_u = it == 0 ? u_all + C.voff : u;
     double s = 0.0;
     if (i < n) {
-        const int x = (int)(i % C.nx);
-        const size_t rr = i / C.nx;
-        const int y = (int)(rr % C.ny), z = (int)(rr / C.ny);
+        const VoxelCursor cur(i, C.nx, C.ny);
+        const int x = cur.x, y = cur.y, z = cur.z;
         float g;
         if (it == 0) {
             const TvSrc<float, float, false> U{src_u, nullptr, 0.0, C.nx, C.ny, C.nz};
-            g = C.rules.three_d ? tv3_grad<float>(U, x, y, z) : tv2_grad<float>(U, x, y);
+            const TvAt<float, float, false> at{U, i, x, y, z};
+            g = C.rules.three_d ? tv3_grad<float>(at, x, y, z) : tv2_grad<float>(at, x, y);
         } else {
             const TvSrc<float, float, true> U{u, vp, st, C.nx, C.ny, C.nz};
-            g = C.rules.three_d ? tv3_grad<float>(U, x, y, z) : tv2_grad<float>(U, x, y);
+            const TvAt<float, float, true> at{U, i, x, y, z};
+            g = C.rules.three_d ? tv3_grad<float>(at, x, y, z) : tv2_grad<float>(at, x, y);
             un[i] = U.stored(i);
         }
         (it == 0 ? v_all + C.voff : vn)[i] = g;
@@ -527,8 +529,7 @@ int asd_batch_create(int ncases, const asd_para *paras, const double *const *phi
         const asd_para &p = paras[c];
         ASD_ARG(p.nx > 0 && p.ny > 0 && p.nz > 0 && p.na > 0 && p.nb > 0,
                 "every case needs positive nx, ny, nz, na, nb");
-        ASD_ARG((long)p.nx * p.ny * p.nz < (1L << 32) && (long)p.na * p.nb < (1L << 31),
-                "case volume or detector too large (32-bit voxel and bin indices)");
+        ASD_ARG(index_fits(p.nx, p.ny, p.nz, p.na, p.nb), "case " ASD_INDEX_LIMITS);
         ASD_ARG(nvs[c] > 0 && phi_vecs[c], "every case needs views");
         ASD_ARG(p.SO > 0 && p.OD > 0 && p.vxlsize > 0 && p.detsize > 0, "bad case geometry");
         ASD_ARG(p.precision == paras[0].precision, "all cases must share para.precision");

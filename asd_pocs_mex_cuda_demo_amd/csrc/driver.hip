@@ -234,8 +234,7 @@ int check_para(const asd_para *p, int nv)
     ASD_ARG(p, "NULL para");
     ASD_ARG(p->nx > 0 && p->ny > 0 && p->nz > 0, "para.nx, para.ny, para.nz must be positive");
     ASD_ARG(p->na > 0 && p->nb > 0, "para.na, para.nb must be positive");
-    ASD_ARG((long)p->nx * p->ny * p->nz < (1L << 32) && (long)p->na * p->nb < (1L << 31),
-            "volume or detector too large (32-bit voxel and bin indices)");
+    ASD_ARG(index_fits(p->nx, p->ny, p->nz, p->na, p->nb), ASD_INDEX_LIMITS);
     ASD_ARG(nv > 0, "phi_vec is empty");
     ASD_ARG(p->vxlsize > 0 && p->detsize > 0, "para.vxlsize and para.detsize must be positive");
     ASD_ARG(p->SO > 0 && p->OD > 0, "para.SO and para.OD must be positive");
@@ -244,22 +243,41 @@ int check_para(const asd_para *p, int nv)
     return ASD_OK;
 }
 
-// Timed launch helper: events bracket the kernel on the plan stream.
+// Stage timing on the plan stream.  A one-kernel stage (a projector launch)
+// gets its events from the dispatch itself (klaunch), so it measures the
+// kernel's execution alone; a multi-kernel stage (a TV descent) is bracketed
+// by events recorded before and after it.
 struct StageTimer {
     asd_plan *p;
     int stage;
+    bool one_kernel;
     hipEvent_t a = nullptr, b = nullptr;
-    StageTimer(asd_plan *pp, int st) : p(pp), stage(st)
+    LaunchTiming lt{};
+    StageTimer(asd_plan *pp, int st, bool one) : p(pp), stage(st), one_kernel(one)
     {
         if (!p->timing || p->capturing) return;
         if (hipEventCreate(&a) != hipSuccess) { a = nullptr; return; }
         if (hipEventCreate(&b) != hipSuccess) { (void)hipEventDestroy(a); a = nullptr; return; }
-        (void)hipEventRecord(a, p->stream);
+        if (one_kernel) {
+            lt = LaunchTiming{a, b, 0};
+            t_launch_timing = &lt;
+        } else {
+            (void)hipEventRecord(a, p->stream);
+        }
     }
     ~StageTimer()
     {
         if (!a) return;
-        (void)hipEventRecord(b, p->stream);
+        if (one_kernel) {
+            t_launch_timing = nullptr;
+            if (lt.launches != 1) {  // nothing (or more than one kernel) was timed
+                (void)hipEventDestroy(a);
+                (void)hipEventDestroy(b);
+                return;
+            }
+        } else {
+            (void)hipEventRecord(b, p->stream);
+        }
         p->ev.push_back({a, b, stage});
     }
 };
@@ -313,14 +331,14 @@ int enqueue_sweep_t(asd_plan *p, const double *relax)
                             stopped, g.scale};
         int rc;
         {
-            StageTimer t(p, ST_AX);
+            StageTimer t(p, ST_AX, true);
             // null detector tables: the gateway's grid is computed in the kernel
             if ((rc = dispatch_ax<KAHAN>(p->lpr, g, p->m.as<float>(), nullptr, nullptr, ea,
                                          p->stream)))
                 return rc;
         }
         {
-            StageTimer t(p, ST_ATX);
+            StageTimer t(p, ST_ATX, true);
             const EpiSartAtx eb{p->m.as<float>(), relax, stopped, g.scale};
             if (EXACT)
                 rc = dispatch_atx_prec<KAHAN>(g, p->cor.as<float>(), yd, zd, rt, eb, p->stream);
@@ -370,7 +388,7 @@ int autotune_atx_t(asd_plan *p)
     const ViewGeom &g = p->views[0];
     int rc = ASD_OK;
     ASD_HIP(hipMemsetAsync(p->cor.p, 0, p->M * 4, p->stream));
-    std::vector<int> cand = {ATX_DIRECT, ATX_DIRECT2, ATX_T16x16x1, ATX_PAIRS};
+    std::vector<int> cand = {ATX_DIRECT, ATX_DIRECT2, ATX_DIRECT1, ATX_T16x16x1, ATX_PAIRS};
     if (g.nz > 1) {
         cand.push_back(ATX_T8x8x4);
         cand.push_back(ATX_T16x4x4);
@@ -384,10 +402,12 @@ int autotune_atx_t(asd_plan *p)
     for (int v : cand) {
         float ms_min = 1e30f;
         for (int rep = 0; rep < 4; rep++) {
-            (void)hipEventRecord(e0, p->stream);
+            // the kernel's own execution time (events stamped by the dispatch)
+            LaunchTiming lt{e0, e1, 0};
+            t_launch_timing = &lt;
             rc = dispatch_atx_variant<KAHAN>(v, g, p->cor.as<float>(), p->rt.as<RayRec>(), ep,
                                              p->stream);
-            (void)hipEventRecord(e1, p->stream);
+            t_launch_timing = nullptr;
             if (rc) break;
             (void)hipEventSynchronize(e1);
             float ms = 0;
@@ -443,7 +463,7 @@ int enqueue_iteration(asd_plan *p)
     const asd_para &pa = p->para;
     const bool three_d = pa.nz >= 2;
     {
-        StageTimer t(p, ST_TVGRAD);
+        StageTimer t(p, ST_TVGRAD, false);
         if ((rc = tv_descent<float, float>(p->m.as<float>(), p->mk1.as<float>(),
                                            p->v.as<float>(), p->u2.as<float>(), p->v2.as<float>(),
                                            pa.nx, pa.ny, three_d ? pa.nz : 1,
@@ -837,7 +857,7 @@ int asd_plan_set_atx_variant(asd_plan *p, int atx_variant)
 {
     clear_error();
     ASD_ARG(p, "NULL plan");
-    ASD_ARG(atx_variant >= -1 && atx_variant < ATX_NVARIANT, "atx_variant must be -1..5");
+    ASD_ARG(atx_variant >= -1 && atx_variant < ATX_NVARIANT, "atx_variant must be -1..6");
     ASD_ARG((atx_variant != ATX_T8x8x4 && atx_variant != ATX_T16x4x4) || p->views[0].nz > 1,
             "the 3-D tile variants need nz > 1");
     drop_graph(p);

@@ -21,14 +21,64 @@ __host__ __device__ __forceinline__ float absf(float a) { return a > 0 ? a : -a;
 // This one rule covers the reference's sixteen labelled cases 11..44.
 // ---------------------------------------------------------------------------
 struct AxRay {
-    float u1, t1s, s1, s2, z1, dz, q, tmp;
+    float u1, t1s, s1, s2, dz, q, tmp;
+    // rtmp = 1 / tmp correctly rounded; mk: tmp and q lie where the division
+    // by tmp may be finished from rtmp (div_by_tmp)
+    float rtmp;
+    int mk;
     int nu2, nt2, nz2, nt, nz;
     int xdriven;
-    // flat: a one-slice volume and a ray in its z = 0 face (z1 = 0, s2 = +-0,
-    // dz = 1), where every piece lies in slice 0 with rz = 1; dz1: dz == 1,
-    // where x / dz == x exactly.  Both only skip work, never change a bit.
+    // flat: a one-slice volume and a ray in its z = 0 face (s2 = +-0, dz = 1),
+    // where every piece lies in slice 0 with rz = 1; dz1: dz == 1, where
+    // x / dz == x exactly.  Both only skip work, never change a bit.
     int flat, dz1;
 };
+
+// The source lies in the plane z = 0 (cu:362-367), so the reference's axial
+// coordinate z1 + s2 * (u - u1) adds z1 = 0.  Leaving that addition out
+// changes only the sign of a zero product, and the slab's axial position
+// p + nz/2 (or p / dz + nz/2) rounds -0 and +0 to the same value, so no
+// result bit depends on it.
+
+// x / tmp for one slab of the ray (the length factor of cu:402), finished
+// from the ray's correctly rounded reciprocal by Markstein's correction:
+// with y = RN(1/b) and q0 = RN(a*y) within one ulp of a/b, the residual
+// r = a - b*q0 is exact in an FMA and RN(q0 + r*y) = RN(a/b) (P. Markstein,
+// IBM J. Res. Dev. 34(1), 1990; Muller et al., Handbook of Floating-Point
+// Arithmetic, the FMA-based division theorems).  The result is the IEEE
+// quotient bit for bit whenever no step underflows or overflows, which
+// mk guarantees (2^-60 <= tmp <= q <= 2^60 and 1 <= x / q <= ~2); other rays
+// divide.
+__host__ __device__ __forceinline__ float div_by_tmp(const AxRay &r, float x)
+{
+#pragma clang fp contract(off)
+    if (!r.mk) return x / r.tmp;
+    const float q0 = x * r.rtmp;
+    const float res = fmaf(-r.tmp, q0, x);
+    return fmaf(res, r.rtmp, q0);
+}
+
+// Correctly rounded square root of x >= 2^-96 (or +-0, +inf, NaN): the
+// compiler's IEEE expansion of sqrtf without its rescaling of tiny
+// arguments, which is inactive in that range, so the result is the same
+// bits as sqrtf.  The approximation s is corrected by one ulp up or down
+// from the signs of the FMA residuals x - (s -+ ulp) * s.
+__host__ __device__ __forceinline__ float sqrt_ge_tiny(float x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float sd = __int_as_float(__float_as_int(s) - 1);
+    const float su = __int_as_float(__float_as_int(s) + 1);
+    const float rd = __builtin_fmaf(-sd, s, x);
+    const float ru = __builtin_fmaf(-su, s, x);
+    float t = rd <= 0.0f ? sd : s;
+    t = ru > 0.0f ? su : t;
+    // +-0 and +inf are their own roots (class mask: -0, +0, +inf)
+    return __builtin_amdgcn_classf(x, 0x020 | 0x040 | 0x200) ? x : t;
+#else
+    return sqrtf(x);
+#endif
+}
 
 __host__ __device__ __forceinline__ AxRay ax_ray_setup(const ViewGeom &g, float ydet, float zdet)
 {
@@ -47,15 +97,16 @@ __host__ __device__ __forceinline__ AxRay ax_ray_setup(const ViewGeom &g, float 
     float dt = r.xdriven ? (y2 - y1) : (x2 - x1);
     r.s1 = dt / du;
     r.s2 = (z2 - z1) / du;
-    r.z1 = z1;
     r.dz = g.dz;
+    r.rtmp = 1.0f / r.tmp;
+    r.mk = r.tmp >= 0x1p-60f && r.q <= 0x1p60f;
     r.nu2 = r.xdriven ? g.nx / 2 : g.ny / 2;
     r.nt2 = r.xdriven ? g.ny / 2 : g.nx / 2;
     r.nt = r.xdriven ? g.ny : g.nx;
     r.nz = g.nz;
     r.nz2 = g.nz / 2;
     r.dz1 = g.dz == 1.0f;
-    r.flat = r.dz1 && g.nz == 1 && r.s2 == 0.0f && r.z1 == 0.0f;
+    r.flat = r.dz1 && g.nz == 1 && r.s2 == 0.0f;
     return r;
 }
 
@@ -83,6 +134,9 @@ __host__ __device__ __forceinline__ void geom_finish(ViewGeom &g)
     g.fp_rdy = 1.0f / g.dy_det;
     g.fp_rdz = 1.0f / g.dz_det;
     g.tight = 0;
+    g.div_nx = make_fastdiv((unsigned)g.nx);
+    g.div_ny = make_fastdiv((unsigned)g.ny);
+    g.div_na = make_fastdiv((unsigned)g.na);
 }
 
 // Whether the backprojector footprint may be cut to the voxel's shadow: no
@@ -161,14 +215,15 @@ __host__ __device__ __forceinline__ void ax_slab_geom(const AxRay &r, int iu, in
     int cz1 = 0, cz2 = 0;
     float za = 0.0f, zb = 0.0f;
     if (!r.flat) {
-        // (z1 + s2 * (u - u1)) / dz + nz/2; x / 1 == x, so dz == 1 skips the division
+        // (z1 + s2 * (u - u1)) / dz + nz/2 with z1 = 0 (see above); x / 1 == x,
+        // so dz == 1 skips the division
         float pa, pb;
         if (r.s2 >= 0) {
-            pa = r.z1 + r.s2 * (ua - r.u1);
-            pb = r.z1 + r.s2 * (ub - r.u1);
+            pa = r.s2 * (ua - r.u1);
+            pb = r.s2 * (ub - r.u1);
         } else {
-            pa = r.z1 + r.s2 * (ub - r.u1);
-            pb = r.z1 + r.s2 * (ua - r.u1);
+            pa = r.s2 * (ub - r.u1);
+            pb = r.s2 * (ua - r.u1);
         }
         if (!r.dz1) {
             pa = pa / r.dz;
@@ -186,40 +241,28 @@ __host__ __device__ __forceinline__ void ax_slab_geom(const AxRay &r, int iu, in
     if (ct1 != ct2) rt = ((float)ct2 - ta) / (tb - ta);
     if (cz1 != cz2) rz = ((float)cz2 - za) / (zb - za);
     float d = tb - ta;
-    float l = sqrtf((d * d + 1) * r.q / r.tmp);
+    // (d*d + 1) * q >= q >= tmp, so the root's argument is >= 1
+    float l = sqrt_ge_tiny(div_by_tmp(r, (d * d + 1) * r.q));
     bool tfirst = !(rt > rz);  // the t crossing comes first (ties: t)
     float lo = tfirst ? rt : rz, hi = tfirst ? rz : rt;
     int ctm = tfirst ? ct2 : ct1, czm = tfirst ? cz1 : cz2;
-    const unsigned sxy = (unsigned)nx * (unsigned)ny;
+    // (z * ny + y) * nx + x in 24-bit multiply-adds (index_fits); pieces out
+    // of the volume get index 0
     auto index = [&](int ct, int cz) {
-        return r.xdriven ? (unsigned)cz * sxy + (unsigned)ct * (unsigned)nx + (unsigned)iu
-                         : (unsigned)cz * sxy + (unsigned)iu * (unsigned)nx + (unsigned)ct;
+        const unsigned y = r.xdriven ? (unsigned)ct : (unsigned)iu;
+        const unsigned x = r.xdriven ? (unsigned)iu : (unsigned)ct;
+        return mad24(mad24((unsigned)cz, (unsigned)ny, y), (unsigned)nx, x);
     };
     s.ok[0] = in_t1 && in_z1;
-    s.ok[1] = (tfirst ? in_t2 : in_t1) && (tfirst ? in_z1 : in_z2);
+    // (written as mask logic, which stays in scalar lane masks)
+    s.ok[1] = (tfirst && in_t2 && in_z1) || (!tfirst && in_t1 && in_z2);
     s.ok[2] = in_t2 && in_z2;
-    if (s.ok[0]) s.idx[0] = index(ct1, cz1);
-    if (s.ok[1]) s.idx[1] = index(ctm, czm);
-    if (s.ok[2]) s.idx[2] = index(ct2, cz2);
+    s.idx[0] = s.ok[0] ? index(ct1, cz1) : 0u;
+    s.idx[1] = s.ok[1] ? index(ctm, czm) : 0u;
+    s.idx[2] = s.ok[2] ? index(ct2, cz2) : 0u;
     s.wl[0] = lo * l;
     s.wl[1] = (hi - lo) * l;
     s.wl[2] = (1.0f - hi) * l;
-}
-
-// Accumulate slab iu of ray r into (acc, accl); X is the volume (x fastest).
-template <bool KAHAN = false>
-__host__ __device__ __forceinline__ void ax_slab(const AxRay &r, const float *__restrict__ X, int iu,
-                                                 int nx, int ny, Acc<KAHAN> &acc,
-                                                 Acc<KAHAN> &accl)
-{
-#pragma clang fp contract(off)
-    SlabW s;
-    ax_slab_geom(r, iu, nx, ny, s);
-    for (int k = 0; k < 3; k++)
-        if (s.ok[k]) {
-            acc.add(s.wl[k] * X[s.idx[k]]);
-            accl.add(s.wl[k]);
-        }
 }
 
 // U consecutive slabs of one lane (iu0, iu0 + stride, ...): all geometry
@@ -388,9 +431,9 @@ __host__ __device__ __forceinline__ float find_l_3d_rec(const RayRec &R, float p
 {
 #pragma clang fp contract(off)
     const float hx = 0.5f, hy = 0.5f;
-    switch (R.mode) {
-    case RAY_X_FIRST:
-    case RAY_Y_FIRST: {
+    // the general rays first, behind one test (the other modes are the
+    // a == 0 / b == 0 fallbacks, which only views without a footprint cut have)
+    if (R.mode <= RAY_Y_FIRST) {
         // A flat ray (c == 0 in a one-slice volume, pz = -hz): s_ca = s_cb = +-0
         // put both hits' z at pz, so every x / y face test's z condition holds
         // (|pz| == hz) and ez = 0; s_ac, s_bc = +-inf make the z-face tests
@@ -412,6 +455,7 @@ __host__ __device__ __forceinline__ float find_l_3d_rec(const RayRec &R, float p
         if (!two) return 0.0f;
         return sqrtf(ex * ex + ey * ey + ez * ez);
     }
+    switch (R.mode) {
     case RAY_A0_B0: return absf(py) <= hy ? dz : 0.0f;
     case RAY_A0_C0: return absf(pz) <= hz ? 1.0f : 0.0f;
     case RAY_A0_YZ_U: return chord2(py, pz, R.s_cb, R.s_bc, R.s_cb * hy, R.s_bc * hz, hy, hz, true);

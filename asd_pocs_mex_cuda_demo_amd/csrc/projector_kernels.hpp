@@ -41,7 +41,9 @@ __global__ __launch_bounds__(256) void ax_kernel(ViewGeom g, const float *__rest
     if (active && lane == 0) pre = epi.load(id);
     Acc<KAHAN> sa, sl;
     if (active) {
-        int ia = (int)(id % g.na), ib = (int)(id / g.na);
+        // id < na * nb < 2^31 and ib < nb < 2^24 (index_fits)
+        const unsigned ub = fastdiv((unsigned)id, g.div_na);
+        const int ia = (int)((unsigned)id - mad24(ub, (unsigned)g.na, 0u)), ib = (int)ub;
         // null tables: the gateway's regular detector grid, computed in place
         const float yd = y_det ? y_det[ia] : det_coord(ia, g.na, g.dy_det);
         const float zd = z_det ? z_det[ib] : det_coord(ib, g.nb, g.dz_det);
@@ -95,15 +97,15 @@ int dispatch_ax(int lpr, const ViewGeom &g, const float *x, const float *y_det,
     long nrays = (long)g.na * g.nb;
     auto blocks = [&](int l) { return (unsigned)((nrays * l + 255) / 256); };
     switch (lpr) {
-    case 1: ax_kernel<1, KAHAN, Epi><<<blocks(1), 256, 0, s>>>(g, x, y_det, z_det, epi); break;
-    case 2: ax_kernel<2, KAHAN, Epi><<<blocks(2), 256, 0, s>>>(g, x, y_det, z_det, epi); break;
-    case 4: ax_kernel<4, KAHAN, Epi><<<blocks(4), 256, 0, s>>>(g, x, y_det, z_det, epi); break;
-    case 8: ax_kernel<8, KAHAN, Epi><<<blocks(8), 256, 0, s>>>(g, x, y_det, z_det, epi); break;
-    case 16: ax_kernel<16, KAHAN, Epi><<<blocks(16), 256, 0, s>>>(g, x, y_det, z_det, epi); break;
-    case 32: ax_kernel<32, KAHAN, Epi><<<blocks(32), 256, 0, s>>>(g, x, y_det, z_det, epi); break;
-    case 64: ax_kernel<64, KAHAN, Epi><<<blocks(64), 256, 0, s>>>(g, x, y_det, z_det, epi); break;
-    case 128: ax_kernel<128, KAHAN, Epi><<<blocks(128), 256, 0, s>>>(g, x, y_det, z_det, epi); break;
-    default: ax_kernel<256, KAHAN, Epi><<<blocks(256), 256, 0, s>>>(g, x, y_det, z_det, epi); break;
+    case 1: klaunch(ax_kernel<1, KAHAN, Epi>, blocks(1), 256, 0, s, g, x, y_det, z_det, epi); break;
+    case 2: klaunch(ax_kernel<2, KAHAN, Epi>, blocks(2), 256, 0, s, g, x, y_det, z_det, epi); break;
+    case 4: klaunch(ax_kernel<4, KAHAN, Epi>, blocks(4), 256, 0, s, g, x, y_det, z_det, epi); break;
+    case 8: klaunch(ax_kernel<8, KAHAN, Epi>, blocks(8), 256, 0, s, g, x, y_det, z_det, epi); break;
+    case 16: klaunch(ax_kernel<16, KAHAN, Epi>, blocks(16), 256, 0, s, g, x, y_det, z_det, epi); break;
+    case 32: klaunch(ax_kernel<32, KAHAN, Epi>, blocks(32), 256, 0, s, g, x, y_det, z_det, epi); break;
+    case 64: klaunch(ax_kernel<64, KAHAN, Epi>, blocks(64), 256, 0, s, g, x, y_det, z_det, epi); break;
+    case 128: klaunch(ax_kernel<128, KAHAN, Epi>, blocks(128), 256, 0, s, g, x, y_det, z_det, epi); break;
+    default: klaunch(ax_kernel<256, KAHAN, Epi>, blocks(256), 256, 0, s, g, x, y_det, z_det, epi); break;
     }
     ASD_LAUNCH_CHECK("ax_kernel");
     return ASD_OK;
@@ -132,13 +134,16 @@ __device__ __forceinline__ void atx_voxel_walk(const ViewGeom &g, const float *_
 #pragma clang fp contract(off)
     if (!(f.a_lo <= f.a_hi && f.b_lo <= f.b_hi)) return;
     const int npairs = (f.a_hi - f.a_lo + 1) * (f.b_hi - f.b_lo + 1);
-    int ia = f.a_lo, ib = f.b_lo;  // cursor: the next pair in reference order
+    // cursor: the next pair in reference order, as its column ia and its bin
+    // index ib * na + ia (a row wrap jumps to column a_lo of the next row)
+    int ia = f.a_lo;
+    unsigned idx = mad24((unsigned)f.b_lo, (unsigned)g.na, (unsigned)f.a_lo);
+    const unsigned jump = (unsigned)(g.na - (f.a_hi - f.a_lo));
     for (int k0 = 0; k0 < npairs; k0 += CHUNK) {
         RayRec R[CHUNK];
         float yv[CHUNK];
 #pragma unroll
         for (int k = 0; k < CHUNK; k++) {
-            const unsigned idx = (unsigned)ib * (unsigned)g.na + (unsigned)ia;
             R[k] = rt[idx];
             yv[k] = Y[idx];
             // advance unless this is the last pair (the rest of the chunk
@@ -146,7 +151,7 @@ __device__ __forceinline__ void atx_voxel_walk(const ViewGeom &g, const float *_
             if (k0 + k + 1 < npairs) {
                 const bool wrap = ia == f.a_hi;
                 ia = wrap ? f.a_lo : ia + 1;
-                ib = wrap ? ib + 1 : ib;
+                idx += wrap ? jump : 1u;
             }
         }
 #pragma unroll
@@ -170,9 +175,10 @@ __global__ __launch_bounds__(256) void atx_kernel(ViewGeom g, const float *__res
     const unsigned N = (unsigned)g.nx * (unsigned)g.ny * (unsigned)g.nz;
     if (v >= N) return;
     const typename Epi::Pre pre = epi.load(v);
-    const int ix = (int)(v % (unsigned)g.nx);
-    const unsigned r = v / (unsigned)g.nx;
-    const int iy = (int)(r % (unsigned)g.ny), iz = (int)(r / (unsigned)g.ny);
+    // r < ny*nz < 2^24 and iz < nz (index_fits): 24-bit products
+    const unsigned r = fastdiv(v, g.div_nx), iz = fastdiv(r, g.div_ny);
+    const int ix = (int)(v - mad24(r, (unsigned)g.nx, 0u));
+    const int iy = (int)(r - mad24(iz, (unsigned)g.ny, 0u));
     Footprint f = footprint(g, ix, iy, iz);
     const float sx = g.cos_phi * (-g.SO), sy = g.sin_phi * (-g.SO);
     const float px = sx - f.xc, py = sy - f.yc, pz = 0.0f - f.zc;
@@ -214,9 +220,9 @@ __global__ __launch_bounds__(256) void atx_pairs_kernel(ViewGeom g, const float 
     f.a_hi = f.b_hi = -1;
     float px = 0.0f, py = 0.0f, pz = 0.0f;
     if (valid) {
-        const int ix = (int)(v % (unsigned)g.nx);
-        const unsigned r = v / (unsigned)g.nx;
-        f = footprint(g, ix, (int)(r % (unsigned)g.ny), (int)(r / (unsigned)g.ny));
+        const unsigned r = fastdiv(v, g.div_nx), iz = fastdiv(r, g.div_ny);
+        f = footprint(g, (int)(v - mad24(r, (unsigned)g.nx, 0u)),
+                      (int)(r - mad24(iz, (unsigned)g.ny, 0u)), (int)iz);
         px = g.cos_phi * (-g.SO) - f.xc;
         py = g.sin_phi * (-g.SO) - f.yc;
         pz = 0.0f - f.zc;
@@ -382,7 +388,8 @@ enum : int {
     ATX_T16x4x4 = 3,  // LDS-staged 16x4x4 bricks
     ATX_PAIRS = 4,    // (voxel, bin) pairs spread over the workgroup
     ATX_DIRECT2 = 5,  // as ATX_DIRECT, two pairs per load batch
-    ATX_NVARIANT = 6,
+    ATX_DIRECT1 = 6,  // as ATX_DIRECT, one pair at a time
+    ATX_NVARIANT = 7,
 };
 
 template <int TX, int TY, int TZ, bool KAHAN, class Epi>
@@ -390,8 +397,8 @@ int launch_atx_tile_t(const ViewGeom &g, const float *Y, const RayRec *rt, const
                       hipStream_t s)
 {
     const TileDims td = tile_dims(g, TX, TY, TZ);
-    atx_tile_plain_kernel<TX, TY, TZ, KAHAN, Epi>
-        <<<(unsigned)td.ntx * td.nty * td.ntz, 256, 0, s>>>(g, Y, rt, epi);
+    klaunch(atx_tile_plain_kernel<TX, TY, TZ, KAHAN, Epi>, (unsigned)td.ntx * td.nty * td.ntz, 256,
+            0, s, g, Y, rt, epi);
     ASD_LAUNCH_CHECK("atx_tile_plain_kernel");
     return ASD_OK;
 }
@@ -406,19 +413,25 @@ int dispatch_atx_variant(int variant, const ViewGeom &g, const float *Y, const R
     case ATX_T16x4x4: return launch_atx_tile_t<16, 4, 4, KAHAN>(g, Y, rt, epi, s);
     case ATX_DIRECT2: {
         const long N = (long)g.nx * g.ny * g.nz;
-        atx_kernel<Epi, KAHAN, 2><<<(unsigned)((N + 255) / 256), 256, 0, s>>>(g, Y, rt, epi);
+        klaunch(atx_kernel<Epi, KAHAN, 2>, (unsigned)((N + 255) / 256), 256, 0, s, g, Y, rt, epi);
+        ASD_LAUNCH_CHECK("atx_kernel");
+        return ASD_OK;
+    }
+    case ATX_DIRECT1: {
+        const long N = (long)g.nx * g.ny * g.nz;
+        klaunch(atx_kernel<Epi, KAHAN, 1>, (unsigned)((N + 255) / 256), 256, 0, s, g, Y, rt, epi);
         ASD_LAUNCH_CHECK("atx_kernel");
         return ASD_OK;
     }
     case ATX_PAIRS: {
         const long N = (long)g.nx * g.ny * g.nz;
-        atx_pairs_kernel<Epi, KAHAN><<<(unsigned)((N + 255) / 256), 256, 0, s>>>(g, Y, rt, epi);
+        klaunch(atx_pairs_kernel<Epi, KAHAN>, (unsigned)((N + 255) / 256), 256, 0, s, g, Y, rt, epi);
         ASD_LAUNCH_CHECK("atx_pairs_kernel");
         return ASD_OK;
     }
     default: {
         const long N = (long)g.nx * g.ny * g.nz;
-        atx_kernel<Epi, KAHAN><<<(unsigned)((N + 255) / 256), 256, 0, s>>>(g, Y, rt, epi);
+        klaunch(atx_kernel<Epi, KAHAN>, (unsigned)((N + 255) / 256), 256, 0, s, g, Y, rt, epi);
         ASD_LAUNCH_CHECK("atx_kernel");
         return ASD_OK;
     }
@@ -433,12 +446,12 @@ int dispatch_atx_prec(const ViewGeom &g, const float *Y, const float *y_det, con
 {
     if (g.nz == 1) {
         const TileDims td = tile_dims(g, 16, 16, 1);
-        atx_tile_kernel<16, 16, 1, KAHAN, Epi>
-            <<<(unsigned)td.ntx * td.nty * td.ntz, 256, 0, s>>>(g, Y, y_det, z_det, rt, epi);
+        klaunch(atx_tile_kernel<16, 16, 1, KAHAN, Epi>, (unsigned)td.ntx * td.nty * td.ntz, 256, 0,
+                s, g, Y, y_det, z_det, rt, epi);
     } else {
         const TileDims td = tile_dims(g, 8, 8, 4);
-        atx_tile_kernel<8, 8, 4, KAHAN, Epi>
-            <<<(unsigned)td.ntx * td.nty * td.ntz, 256, 0, s>>>(g, Y, y_det, z_det, rt, epi);
+        klaunch(atx_tile_kernel<8, 8, 4, KAHAN, Epi>, (unsigned)td.ntx * td.nty * td.ntz, 256, 0, s,
+                g, Y, y_det, z_det, rt, epi);
     }
     ASD_LAUNCH_CHECK("atx_tile_kernel");
     return ASD_OK;

@@ -115,8 +115,7 @@ int check_geom(const ViewGeom &g)
 {
     ASD_ARG(g.nx > 0 && g.ny > 0 && g.nz > 0 && g.na > 0 && g.nb > 0,
             "sizes nx, ny, nz, na, nb must be positive");
-    ASD_ARG((long)g.nx * g.ny * g.nz < (1L << 32) && (long)g.na * g.nb < (1L << 31),
-            "volume or detector too large (32-bit voxel and bin indices)");
+    ASD_ARG(index_fits(g.nx, g.ny, g.nz, g.na, g.nb), ASD_INDEX_LIMITS);
     return ASD_OK;
 }
 

@@ -1,10 +1,9 @@
 // Deterministic FP64 reductions: wave butterflies and one partial per block;
-// a separate one-block kernel (fold_partials) folds the partials in a fixed
-// order.  No atomics: a single counter that every block of a large grid
-// increments serialises (~20 ns per block), which cost more than the data
-// pass itself.  Results do not depend on scheduling and are bit-identical
-// run to run.  (grid_sum_to / grid_sums, the last-block variants, remain for
-// grids of a few blocks.)
+// the partials are folded in a fixed order, by a one-block kernel
+// (fold_partials) or inside every block of the consumer kernel (fold_rows).
+// No atomics: a single counter that every block of a large grid increments
+// serialises (~20 ns per block), which cost more than the data pass itself.
+// Results do not depend on scheduling and are bit-identical run to run.
 #pragma once
 
 #include "asd_internal.hpp"
@@ -13,7 +12,7 @@ namespace asd {
 
 constexpr int RED_THREADS = 256;
 constexpr int RED_MAX_BLOCKS = 2048;  // grid cap of the grid-stride reductions
-constexpr int RED_MAX_SUMS = 4;       // values reduced together by grid_sums
+constexpr int RED_MAX_SUMS = 4;       // rows of partials a scratch buffer holds
 
 __device__ __forceinline__ double wave_sum(double v)
 {
@@ -39,80 +38,12 @@ __device__ __forceinline__ double block_sum(double val)
     return s;
 }
 
-// Each block contributes `val`; the last block writes the total to *out and
-// re-arms *counter.  partials must hold gridDim.x doubles.
-__device__ __forceinline__ void grid_sum_to(double val, double *partials, unsigned *counter,
-                                            double *out)
-{
-    __shared__ bool last;
-    double s = block_sum(val);
-    if (threadIdx.x == 0) {
-        partials[blockIdx.x] = s;
-        unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL,
-                                               __HIP_MEMORY_SCOPE_AGENT);
-        last = (prev == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!last) return;
-    __atomic_thread_fence(__ATOMIC_ACQUIRE);
-    double t = 0.0;
-    for (unsigned i = threadIdx.x; i < gridDim.x; i += blockDim.x)
-        t += __hip_atomic_load(&partials[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    t = block_sum(t);
-    if (threadIdx.x == 0) {
-        *out = t;
-        __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-}
-
 inline unsigned red_blocks(size_t n)
 {
     size_t b = (n + RED_THREADS - 1) / RED_THREADS;
     if (b > RED_MAX_BLOCKS) b = RED_MAX_BLOCKS;
     if (b == 0) b = 1;
     return (unsigned)b;
-}
-
-}  // namespace asd
-
-namespace asd {
-
-// K simultaneous sums.  Every block stores K partials; the last block to
-// arrive folds them (fixed order) into tot[0..K-1], visible to all its
-// threads, and returns true.  partials must hold K * gridDim.x doubles.
-template <int K>
-__device__ __forceinline__ bool grid_sums(const double (&val)[K], double *partials,
-                                          unsigned *counter, double (&tot)[K])
-{
-    __shared__ bool last;
-    __shared__ double res[K];
-    double s[K];
-#pragma unroll
-    for (int k = 0; k < K; k++) s[k] = block_sum(val[k]);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < K; k++) partials[(size_t)k * gridDim.x + blockIdx.x] = s[k];
-        unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL,
-                                               __HIP_MEMORY_SCOPE_AGENT);
-        last = (prev == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!last) return false;
-    __atomic_thread_fence(__ATOMIC_ACQUIRE);
-#pragma unroll
-    for (int k = 0; k < K; k++) {
-        double t = 0.0;
-        for (unsigned i = threadIdx.x; i < gridDim.x; i += blockDim.x)
-            t += __hip_atomic_load(&partials[(size_t)k * gridDim.x + i], __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-        t = block_sum(t);
-        if (threadIdx.x == 0) res[k] = t;
-    }
-    if (threadIdx.x == 0) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < K; k++) tot[k] = res[k];
-    return true;
 }
 
 }  // namespace asd

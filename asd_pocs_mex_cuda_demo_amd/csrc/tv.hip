@@ -45,13 +45,18 @@ __global__ __launch_bounds__(256) void tv_grad_kernel(const T *__restrict__ u,
     if constexpr (STEPPED) U.s = tv_step_size(alpha, alpha_dev, d2, partials_prev, nb_prev);
     const size_t n = (size_t)nx * ny * nz;
     double s = 0.0;
-    // XCD-contiguous blocks: a stencil's neighbour rows stay in one XCD's L2
-    for (size_t i = (size_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; i < n;
-         i += (size_t)gridDim.x * blockDim.x) {
-        int x = (int)(i % nx);
-        size_t r = i / nx;
-        int y = (int)(r % ny), z = (int)(r / ny);
-        A g = THREE_D ? tv3_grad<A>(U, x, y, z) : tv2_grad<A>(U, x, y);
+    // Each block walks one contiguous chunk of the image, and each XCD owns a
+    // contiguous run of chunks (an eighth of the volume).  The chunks one
+    // plane up and down belong to blocks of the same XCD that run at the same
+    // time, so a stencil's z neighbours are read from that XCD's L2 while they
+    // are still there instead of again from HBM.
+    const size_t chunk = (n + (size_t)gridDim.x * 256 - 1) / ((size_t)gridDim.x * 256) * 256;
+    const size_t i0 = (size_t)xcd_block(blockIdx.x, gridDim.x) * chunk;
+    const size_t i1 = i0 + chunk < n ? i0 + chunk : n;
+    VoxelCursor cur(i0 + threadIdx.x, nx, ny);
+    for (size_t i = i0 + threadIdx.x; i < i1; i += blockDim.x, cur.advance(256)) {
+        const TvAt<T, A, STEPPED> at{U, i, cur.x, cur.y, cur.z};
+        A g = THREE_D ? tv3_grad<A>(at, cur.x, cur.y, cur.z) : tv2_grad<A>(at, cur.x, cur.y);
         T gt = T(g);
         v[i] = gt;
         if constexpr (STEPPED) u_next[i] = U.stored(i);
@@ -184,12 +189,10 @@ template int tv_descent<double, double>(double *, const double *, double *, doub
 int TvScratch::init()
 {
     int rc;
-    if ((rc = buf.ensure(sizeof(double) * (RED_MAX_SUMS * RED_MAX_BLOCKS + TV_NSLOTS) +
-                         sizeof(unsigned) * TV_NSLOTS)))
+    if ((rc = buf.ensure(sizeof(double) * (RED_MAX_SUMS * RED_MAX_BLOCKS + TV_NSLOTS))))
         return rc;
     partials = buf.as<double>();
     scalars = partials + RED_MAX_SUMS * RED_MAX_BLOCKS;
-    counters = reinterpret_cast<unsigned *>(scalars + TV_NSLOTS);
     ASD_HIP(hipMemset(buf.p, 0, buf.bytes));
     return ASD_OK;
 }

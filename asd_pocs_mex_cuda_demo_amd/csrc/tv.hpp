@@ -21,7 +21,6 @@ struct TvScratch {
     DeviceBuf buf;
     double *partials = nullptr;  // RED_MAX_SUMS * RED_MAX_BLOCKS block partials
     double *scalars = nullptr;   // TV_NSLOTS results
-    unsigned *counters = nullptr;  // TV_NSLOTS arrival counters (re-armed by the kernels)
     int init();
 };
 

@@ -29,6 +29,60 @@ struct TvSrc {
     }
 };
 
+// The same values addressed relative to one voxel (x, y, z) at linear index
+// i: the stencils ask for fixed offsets from it, so after inlining every
+// neighbour is i plus a constant multiple of the row and plane strides, and
+// no point forms its own index from coordinates.  A point outside the image
+// reads the voxel itself and is replaced by 0 afterwards: every load is
+// unconditional, so all of a stencil's loads are in flight together instead
+// of one bounds-checked branch (and one memory round trip) after another.
+template <class T, class A, bool STEPPED>
+struct TvAt {
+    const TvSrc<T, A, STEPPED> &S;
+    size_t i;
+    int x, y, z;
+    __device__ __forceinline__ A operator()(int xx, int yy, int zz) const
+    {
+        const bool ok = (xx >= 0) & (xx < S.nx) & (yy >= 0) & (yy < S.ny) & (zz >= 0) &
+                        (zz < S.nz);
+        const long off = (long)(xx - x) + (long)(yy - y) * (long)S.nx +
+                         (long)(zz - z) * ((long)S.nx * (long)S.ny);
+        const A val = A(S.stored(i + (ok ? off : 0L)));
+        return ok ? val : A(0);
+    }
+};
+
+// Coordinates of consecutive linear indices i, i + step, ... (x fastest),
+// advanced without a division per voxel.
+struct VoxelCursor {
+    int x, y, z, nx, ny;
+    __host__ __device__ __forceinline__ VoxelCursor(size_t i, int nx_, int ny_) : nx(nx_), ny(ny_)
+    {
+        if (i <= 0xffffffffu) {  // 32-bit division where the index allows it
+            const unsigned u = (unsigned)i, r = u / (unsigned)nx;
+            x = (int)(u - r * (unsigned)nx);
+            y = (int)(r % (unsigned)ny);
+            z = (int)(r / (unsigned)ny);
+        } else {
+            x = (int)(i % (size_t)nx);
+            const size_t r = i / (size_t)nx;
+            y = (int)(r % (size_t)ny);
+            z = (int)(r / (size_t)ny);
+        }
+    }
+    __host__ __device__ __forceinline__ void advance(int step)
+    {
+        x += step;
+        while (x >= nx) {
+            x -= nx;
+            if (++y == ny) {
+                y = 0;
+                ++z;
+            }
+        }
+    }
+};
+
 // 2-D stencil at row r (fastest) and column c (imgTVGradMin.cpp:84-97).
 template <class A, class Src>
 __device__ __forceinline__ A tv2_grad(const Src &U, int r, int c)

@@ -137,6 +137,8 @@ def bench_batch(args):
     out = {}
     for prec in (0, 1):
         B = ops.Batch([dict(p, maxnumIter=steps, precision=prec) for p, _, _ in cases])
+        if args.eager:
+            B.set_graph(False)
         for c, (para, img, P) in enumerate(cases):
             B.set_case(c, P, None, img)
         B.iterate(1)  # graph capture + warm-up
@@ -203,7 +205,8 @@ ATX_KERNELS = {"direct": "atx_kernel<EpiSartAtx, false>",
                "tile8x8x4": "atx_tile_plain_kernel<8, 8, 4, false, EpiSartAtx>",
                "tile16x4x4": "atx_tile_plain_kernel<16, 4, 4, false, EpiSartAtx>",
                "pairs": "atx_pairs_kernel<EpiSartAtx, false>",
-               "direct2": "atx_kernel<EpiSartAtx, false, 2>"}
+               "direct2": "atx_kernel<EpiSartAtx, false, 2>",
+               "direct1": "atx_kernel<EpiSartAtx, false, 1>"}
 
 
 def profiled_traffic(workload, kernel):
@@ -305,12 +308,16 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--atx-variant", default=None,
                     choices=["direct", "tile16x16x1", "tile8x8x4", "tile16x4x4", "pairs",
-                             "direct2"],
+                             "direct2", "direct1"],
                     help="force the backprojector variant instead of the autotuner's pick "
                          "(profiling runs use this to profile one variant)")
     ap.add_argument("--batch-only", action="store_true",
                     help="batch64: time only the batched forms (skips the sequential plain "
                          "plans and the perturbation floor; for kernel traces of the batch)")
+    ap.add_argument("--eager", action="store_true",
+                    help="batch64: launch the batched iterations eagerly instead of replaying "
+                         "their hipGraph (same kernels; for tracers that cannot follow graph "
+                         "launches)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 256 MiB device to device three times (a plain copy "
                          "kernel of known size, to scale rocprofv3 FETCH_SIZE/WRITE_SIZE)")

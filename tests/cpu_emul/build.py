@@ -7,6 +7,17 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 
 
+def stale(out):
+    """Whether `out` is missing or older than a source it is built from."""
+    if not os.path.exists(out):
+        return True
+    csrc = os.path.join(ROOT, "asd_pocs_mex_cuda_demo_amd", "csrc")
+    srcs = [os.path.join(HERE, "emul.hip"), os.path.join(ROOT, "include", "asdpocs.h")]
+    srcs += [os.path.join(csrc, f) for f in os.listdir(csrc) if f.endswith(".hpp")]
+    t = os.path.getmtime(out)
+    return any(os.path.getmtime(f) > t for f in srcs)
+
+
 def build():
     out = os.path.join(HERE, "libemul.so")
     pkg = os.path.join(ROOT, "asd_pocs_mex_cuda_demo_amd")

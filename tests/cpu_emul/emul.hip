@@ -47,6 +47,13 @@ extern "C" {
 
 void emul_set_general(int on) { g_general = on; }
 
+// n / d by the kernels' multiply-high division (FastDiv)
+void emul_fastdiv(unsigned d, const unsigned *n, unsigned *q, long count)
+{
+    const FastDiv f = make_fastdiv(d);
+    for (long i = 0; i < count; i++) q[i] = fastdiv(n[i], f);
+}
+
 // Total (voxel, bin) pairs the backprojector visits for one view: with the
 // footprint cut (cut = 1, where the view allows it) or the reference's
 // footprint (cut = 0).

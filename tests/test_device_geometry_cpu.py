@@ -15,9 +15,9 @@ _lib = None
 def emul():
     global _lib
     if _lib is None:
+        from tests.cpu_emul import build
         path = os.path.join(HERE, "cpu_emul", "libemul.so")
-        if not os.path.exists(path):
-            from tests.cpu_emul import build
+        if build.stale(path):
             build.build()
         _lib = C.CDLL(path)
         f, i, vp = C.c_float, C.c_int, C.c_void_p
@@ -257,3 +257,25 @@ def test_footprint_cut_is_exact_and_halves_the_work(orc):
         cut_pairs += lib.emul_footprint_pairs(*args, 1)
         full_pairs += lib.emul_footprint_pairs(*args, 0)
     assert cut_pairs <= 0.67 * full_pairs, (cut_pairs, full_pairs)
+
+
+def test_fastdiv_is_exact():
+    """The multiply-high division the backprojectors use to split a voxel
+    index equals integer division for every divisor up to 2^12, for powers
+    of two and their neighbours up to 2^24, and for random divisors, on
+    dividends at the 32-bit extremes, around multiples of the divisor and at
+    random."""
+    lib = emul()
+    lib.emul_fastdiv.argtypes = [C.c_uint, C.c_void_p, C.c_void_p, C.c_long]
+    rng = np.random.default_rng(11)
+    ds = list(range(1, 4097))
+    ds += [d for k in range(12, 25) for d in ((1 << k) - 1, 1 << k, (1 << k) + 1) if d < (1 << 24)]
+    ds += [int(d) for d in rng.integers(1, 1 << 24, 300)]
+    for d in ds:
+        n = np.concatenate([
+            np.array([0, 1, d - 1, d, d + 1, 2**32 - 1, 2**32 - 2, 2**31, 2**31 - 1], np.uint64),
+            (np.arange(1, 200, dtype=np.uint64) * d + np.uint64(d - 1)) % np.uint64(2**32),
+            rng.integers(0, 2**32, 2000, dtype=np.uint64)]).astype(np.uint32)
+        q = np.zeros_like(n)
+        lib.emul_fastdiv(d, P(n), P(q), n.size)
+        assert np.array_equal(q, n // np.uint32(d)), d

@@ -178,10 +178,10 @@ def test_backprojector_variants_are_bit_identical(three_d):
     autotuner reports a measured time for every variant it tried."""
     if three_d:
         para, img, P = _small_problem(n=24, nv=8, na=40, nz=8, nb=20)
-        variants = [0, 1, 2, 3, 4, 5]
+        variants = [0, 1, 2, 3, 4, 5, 6]
     else:
         para, img, P = _small_problem(n=48, nv=12, na=72)
-        variants = [0, 1, 4, 5]
+        variants = [0, 1, 4, 5, 6]
     para.update(maxnumIter=3, alpha=0.2, numOfgradients=5)
     ref = None
     for v in [-1] + variants:
