@@ -187,6 +187,38 @@ __host__ __device__ __forceinline__ unsigned mad24(unsigned a, unsigned b, unsig
 #endif
 }
 
+// Sum over each group of W consecutive lanes (W a power of two, at most 64),
+// returned in every lane of the group.  Within a 16-lane row the partners
+// are exchanged by DPP lane moves fused into the adds (quad swaps, then the
+// half-row and row mirrors), which cost no more than the adds themselves;
+// only the exchanges across rows go through ds_bpermute.
+template <int CTRL>
+__device__ __forceinline__ float dpp_move(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double((long long)(unsigned)lo | ((long long)hi << 32));
+}
+template <int W, class R>
+__device__ __forceinline__ R group_sum(R v)
+{
+    static_assert(W >= 1 && W <= 64 && (W & (W - 1)) == 0, "lane groups of 2^k <= 64");
+    constexpr int QUAD_XOR1 = 0xB1, QUAD_XOR2 = 0x4E, ROW_HALF_MIRROR = 0x141, ROW_MIRROR = 0x140;
+    if constexpr (W >= 2) v += dpp_move<QUAD_XOR1>(v);
+    if constexpr (W >= 4) v += dpp_move<QUAD_XOR2>(v);
+    if constexpr (W >= 8) v += dpp_move<ROW_HALF_MIRROR>(v);
+    if constexpr (W >= 16) v += dpp_move<ROW_MIRROR>(v);
+    if constexpr (W >= 32) v += __shfl_xor(v, 16, 64);
+    if constexpr (W >= 64) v += __shfl_xor(v, 32, 64);
+    return v;
+}
+
 // Kernel timing by the dispatch itself.  While a LaunchTiming is installed on
 // the calling thread, klaunch hands its events to hipExtLaunchKernel, which
 // stamps `start` with the first kernel's start and `stop` with the end of

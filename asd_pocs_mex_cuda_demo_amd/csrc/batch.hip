@@ -123,23 +123,16 @@ __global__ __launch_bounds__(256) void ax_batch_kernel(const BItem *__restrict__
     }
     float acc, accl;
     if constexpr (LPR > 1 && KAHAN) {
-        double a = (double)sa.s - (double)sa.c, b = (double)sl.s - (double)sl.c;
-#pragma unroll
-        for (int off = LPR / 2; off > 0; off >>= 1) {
-            a += __shfl_xor(a, off, LPR);
-            b += __shfl_xor(b, off, LPR);
-        }
+        const double a = group_sum<LPR>((double)sa.s - (double)sa.c);
+        const double b = group_sum<LPR>((double)sl.s - (double)sl.c);
         acc = (float)a;
         accl = (float)b;
     } else {
         acc = sa.value();
         accl = sl.value();
         if constexpr (LPR > 1) {
-#pragma unroll
-            for (int off = LPR / 2; off > 0; off >>= 1) {
-                acc += __shfl_xor(acc, off, LPR);
-                accl += __shfl_xor(accl, off, LPR);
-            }
+            acc = group_sum<LPR>(acc);
+            accl = group_sum<LPR>(accl);
         }
     }
     if (!active || lane != 0) return;

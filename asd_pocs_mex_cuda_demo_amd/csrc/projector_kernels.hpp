@@ -60,11 +60,8 @@ __global__ __launch_bounds__(256) void ax_kernel(ViewGeom g, const float *__rest
     R a = KAHAN ? (R)sa.s - (R)sa.c : (R)sa.s;
     R b = KAHAN ? (R)sl.s - (R)sl.c : (R)sl.s;
     if constexpr (WL > 1) {
-#pragma unroll
-        for (int off = WL / 2; off > 0; off >>= 1) {
-            a += __shfl_xor(a, off, WL);
-            b += __shfl_xor(b, off, WL);
-        }
+        a = group_sum<WL>(a);
+        b = group_sum<WL>(b);
     }
     if constexpr (WPR > 1) {
         // RPB rays per block, WPR waves per ray: wave w of the block holds

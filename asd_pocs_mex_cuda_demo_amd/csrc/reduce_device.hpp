@@ -14,12 +14,7 @@ constexpr int RED_THREADS = 256;
 constexpr int RED_MAX_BLOCKS = 2048;  // grid cap of the grid-stride reductions
 constexpr int RED_MAX_SUMS = 4;       // rows of partials a scratch buffer holds
 
-__device__ __forceinline__ double wave_sum(double v)
-{
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
+__device__ __forceinline__ double wave_sum(double v) { return group_sum<64>(v); }
 
 // Sum of `val` over the block (256 threads); valid in thread 0.
 __device__ __forceinline__ double block_sum(double val)
