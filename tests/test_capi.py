@@ -59,6 +59,12 @@ def test_bad_arguments_are_rejected_before_the_device():
     p.precision = 7
     rc = L.asd_plan_create(C.byref(p), phi.ctypes.data_as(C.c_void_p), 4, C.byref(h))
     assert rc == nat.ASD_ERR_ARG
+    p.precision = 0
+    # ny * nz beyond the kernels' 24-bit index factors (the volume itself fits 32 bits)
+    p.ny, p.nz = 256, 1 << 17
+    rc = L.asd_plan_create(C.byref(p), phi.ctypes.data_as(C.c_void_p), 4, C.byref(h))
+    assert rc == nat.ASD_ERR_ARG and b"2^24" in L.asd_last_error()
+    p.ny, p.nz = 8, 1
     assert L.asd_tv_min_2d(None, 4, 4, 0.1, 1, 1, None, None) == nat.ASD_ERR_ARG
     assert L.asd_tv_min_3d(None, 4, 4, 4, 0.1, 1, 1, None, None) == nat.ASD_ERR_ARG
     x = np.zeros(4, np.float32)
