@@ -77,8 +77,9 @@ long emul_footprint_pairs(float c, float s, const float *ydet, float SO, float O
 float emul_det_coord(int i, int n, float det) { return det_coord(i, n, det); }
 
 // ax_kernel<LPR=lpr>: lane j of a ray takes slabs j, j+lpr, ...; lanes are
-// combined by the same butterfly order as the wave shuffles (within groups of
-// at most 64 lanes), then the wave partials of a ray in wave order.
+// combined in the order of group_sum (quad swaps, half-row and row mirrors,
+// then the 16- and 32-lane swaps, within groups of at most 64 lanes), then
+// the wave partials of a ray in wave order.
 // Compensated lanes are combined in FP64 as on the device.
 void emul_ax(const float *X, float *y, float *sr, float c, float s, const float *ydet,
              const float *zdet, float SO, float OD, float scale, float dz, int nx, int ny, int nz,
@@ -90,7 +91,7 @@ void emul_ax(const float *X, float *y, float *sr, float c, float s, const float 
     for (long id = 0; id < (long)na * nb; id++) {
         int ia = (int)(id % na), ib = (int)(id / na);
         AxRay r = ray_setup(g, ydet[ia], zdet[ib]);
-        int nu = r.xdriven ? nx : ny;
+        const int nu = r.xdriven ? nx : ny;
         for (int lane = 0; lane < lpr; lane++) {
             if (kahan) {
                 Acc<true> a, l;
@@ -106,10 +107,12 @@ void emul_ax(const float *X, float *y, float *sr, float c, float s, const float 
                 accl[lane] = l.s;
             }
         }
-        for (int off = wl / 2; off > 0; off >>= 1) {
+        for (int w = 2; w <= wl; w *= 2) {
             double na_[256], nl_[256];
             for (int lane = 0; lane < lpr; lane++) {
-                int o = (lane & ~(wl - 1)) | ((lane ^ off) & (wl - 1));
+                const int o = w == 8    ? (lane & ~7) | (7 - (lane & 7))
+                              : w == 16 ? (lane & ~15) | (15 - (lane & 15))
+                                        : lane ^ (w / 2);
                 na_[lane] = kahan ? acc[lane] + acc[o] : (double)((float)acc[lane] + (float)acc[o]);
                 nl_[lane] = kahan ? accl[lane] + accl[o]
                                   : (double)((float)accl[lane] + (float)accl[o]);
