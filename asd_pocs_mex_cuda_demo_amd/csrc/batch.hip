@@ -184,9 +184,7 @@ __device__ __forceinline__ void fold_case(const double *partials, int nblocks_al
     __shared__ double res[K];
 #pragma unroll
     for (int k = 0; k < K; k++) {
-        double t = 0.0;
-        for (int i = threadIdx.x; i < nblk; i += blockDim.x)
-            t += partials[(size_t)k * nblocks_all + first + i];
+        double t = thread_row_sum(partials + (size_t)k * nblocks_all + first, nblk);
         t = block_sum(t);
         if (threadIdx.x == 0) res[k] = t;
     }

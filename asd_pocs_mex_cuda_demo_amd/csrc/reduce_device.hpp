@@ -56,6 +56,26 @@ __device__ __forceinline__ void store_partials(const double (&val)[K], double *p
     }
 }
 
+// This thread's share of row[0..n): row[t], row[t + 256], ... added in index
+// order.  The first RED_MAX_BLOCKS / RED_THREADS of them are loaded together,
+// so a row of up to RED_MAX_BLOCKS partials costs one memory round trip.
+__device__ __forceinline__ double thread_row_sum(const double *row, int n)
+{
+    constexpr int PER = RED_MAX_BLOCKS / RED_THREADS;
+    double val[PER];
+#pragma unroll
+    for (int j = 0; j < PER; j++) {
+        const int i = (int)threadIdx.x + j * RED_THREADS;
+        val[j] = i < n ? row[i] : 0.0;
+    }
+    double t = 0.0;
+#pragma unroll
+    for (int j = 0; j < PER; j++)
+        if ((int)threadIdx.x + j * RED_THREADS < n) t += val[j];
+    for (int i = (int)threadIdx.x + PER * RED_THREADS; i < n; i += RED_THREADS) t += row[i];
+    return t;
+}
+
 // Fold K rows of nblocks partials (one block of RED_THREADS threads); the
 // totals are valid in every thread.
 template <int K>
@@ -64,9 +84,7 @@ __device__ __forceinline__ void fold_rows(const double *partials, int nblocks, d
     __shared__ double res[K];
 #pragma unroll
     for (int k = 0; k < K; k++) {
-        double t = 0.0;
-        for (int i = threadIdx.x; i < nblocks; i += blockDim.x)
-            t += partials[(size_t)k * nblocks + i];
+        double t = thread_row_sum(partials + (size_t)k * nblocks, nblocks);
         t = block_sum(t);
         if (threadIdx.x == 0) res[k] = t;
     }

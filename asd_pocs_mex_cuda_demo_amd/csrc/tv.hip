@@ -42,7 +42,6 @@ __global__ __launch_bounds__(256) void tv_grad_kernel(const T *__restrict__ u,
 {
     if (skip && *skip) return;
     TvSrc<T, A, STEPPED> U{u, v_prev, 0.0, nx, ny, nz};
-    if constexpr (STEPPED) U.s = tv_step_size(alpha, alpha_dev, d2, partials_prev, nb_prev);
     const size_t n = (size_t)nx * ny * nz;
     double s = 0.0;
     // Each block walks one contiguous chunk of the image, and each XCD owns a
@@ -54,7 +53,35 @@ __global__ __launch_bounds__(256) void tv_grad_kernel(const T *__restrict__ u,
     const size_t i0 = (size_t)xcd_block(blockIdx.x, gridDim.x) * chunk;
     const size_t i1 = i0 + chunk < n ? i0 + chunk : n;
     VoxelCursor cur(i0 + threadIdx.x, nx, ny);
-    for (size_t i = i0 + threadIdx.x; i < i1; i += blockDim.x, cur.advance(256)) {
+    size_t i = i0 + threadIdx.x;
+    if constexpr (STEPPED) {
+        // the first voxel's stencil is loaded before the previous norm is
+        // folded, so the two memory round trips overlap
+        constexpr int K = THREE_D ? 13 : 7;
+        TvRaw<T, K> raw;
+        if (i < i1) {
+            const TvGather<T, A, K> gat{u, v_prev, i, cur.x, cur.y, cur.z, nx, ny, nz, raw, 0};
+            (void)(THREE_D ? tv3_grad<A>(gat, cur.x, cur.y, cur.z)
+                           : tv2_grad<A>(gat, cur.x, cur.y));
+        }
+        U.s = tv_step_size(alpha, alpha_dev, d2, partials_prev, nb_prev);
+        // the gathered values are used only after the fold: the empty asm ties
+        // them to the step size, which keeps the compiler from converting them
+        // (and so waiting for their loads) before the fold starts
+#pragma unroll
+        for (int k = 0; k < K; k++) asm volatile("" : "+v"(raw.u[k]), "+v"(raw.v[k]) : "v"(U.s));
+        if (i < i1) {
+            const TvReplay<T, A, K> rep{raw, U.s, 0};
+            A g = THREE_D ? tv3_grad<A>(rep, cur.x, cur.y, cur.z) : tv2_grad<A>(rep, cur.x, cur.y);
+            T gt = T(g);
+            v[i] = gt;
+            u_next[i] = U.stored(i);
+            s += (double)gt * (double)gt;
+        }
+        i += blockDim.x;
+        cur.advance(256);
+    }
+    for (; i < i1; i += blockDim.x, cur.advance(256)) {
         const TvAt<T, A, STEPPED> at{U, i, cur.x, cur.y, cur.z};
         A g = THREE_D ? tv3_grad<A>(at, cur.x, cur.y, cur.z) : tv2_grad<A>(at, cur.x, cur.y);
         T gt = T(g);

@@ -52,6 +52,54 @@ struct TvAt {
     }
 };
 
+// The stencil's points, gathered ahead: TvGather loads the raw (u, v) of
+// every point the stencil asks for, in the order it asks, into a TvRaw; a
+// later evaluation through TvReplay applies the step to them in the same
+// order.  The loads can then be issued before the step size is known (the
+// fused kernel folds the previous norm meanwhile).  The point indices are
+// compile-time constants after inlining, so the values stay in registers.
+template <class T, int K>
+struct TvRaw {
+    T u[K], v[K];
+    bool ok[K];
+};
+
+template <class T, class A, int K>
+struct TvGather {
+    const T *u, *v;
+    size_t i;
+    int x, y, z, nx, ny, nz;
+    TvRaw<T, K> &raw;
+    mutable int k;
+    __device__ __forceinline__ A operator()(int xx, int yy, int zz) const
+    {
+        const bool ok = (xx >= 0) & (xx < nx) & (yy >= 0) & (yy < ny) & (zz >= 0) & (zz < nz);
+        const long off = (long)(xx - x) + (long)(yy - y) * (long)nx +
+                         (long)(zz - z) * ((long)nx * (long)ny);
+        const size_t j = i + (ok ? off : 0L);
+        raw.u[k] = u[j];
+        raw.v[k] = v[j];
+        raw.ok[k] = ok;
+        k++;
+        return A(0);
+    }
+};
+
+template <class T, class A, int K>
+struct TvReplay {
+    const TvRaw<T, K> &raw;
+    double s;
+    mutable int k;
+    __device__ __forceinline__ A operator()(int, int, int) const
+    {
+#pragma clang fp contract(off)
+        const int j = k++;
+        // the value TvSrc<T, A, true>::stored gives for the same point
+        const A val = A(T((double)raw.u[j] - s * (double)raw.v[j]));
+        return raw.ok[j] ? val : A(0);
+    }
+};
+
 // Coordinates of consecutive linear indices i, i + step, ... (x fastest),
 // advanced without a division per voxel.
 struct VoxelCursor {

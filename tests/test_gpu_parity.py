@@ -290,3 +290,31 @@ def test_batch_matches_single_plans():
         np.testing.assert_allclose(rb, out["RMSE"], rtol=1e-5)
         assert ob["beta"] == pytest.approx(out["beta"], rel=1e-12)
     B.close()
+
+
+def test_stage_timing_counts_every_launch_and_changes_nothing():
+    """Per-stage timing (kernels launched with dispatch-stamped events) records
+    one forward and one backprojector launch per view and one TV descent per
+    iteration, each with a positive duration, and leaves the iterations
+    bit-identical to an untimed plan."""
+    para, img, P = _small_problem(n=48, nv=12, na=72)
+    para.update(maxnumIter=2, alpha=0.2, numOfgradients=5)
+    res = []
+    for timed in (False, True):
+        pl = ops.Plan(para, para["phi_vec"])
+        pl.set_projections(P)
+        pl.reset(None, img)
+        pl.set_atx_variant(5)  # same kernels in both runs (no autotuning launches)
+        if timed:
+            pl.set_timing(True)
+        pl.iterate(2)
+        pl.sync()
+        if timed:
+            for stage, expect in ((0, 2 * 12), (1, 2 * 12), (2, 2)):
+                ms, cnt = pl.timing(stage)
+                assert cnt == expect, (stage, cnt)
+                assert ms > 0.0, (stage, ms)
+        res.append((pl.image(), pl.out()))
+        pl.close()
+    assert np.array_equal(res[0][0], res[1][0])
+    assert res[0][1][0] == res[1][1][0] and np.array_equal(res[0][1][1], res[1][1][1])
