@@ -96,7 +96,10 @@ __host__ __device__ __forceinline__ AxRay ax_ray_setup(const ViewGeom &g, float 
     float du = r.xdriven ? (x2 - x1) : (y2 - y1);
     float dt = r.xdriven ? (y2 - y1) : (x2 - x1);
     r.s1 = dt / du;
-    r.s2 = (z2 - z1) / du;
+    // a ray in the source's plane (every ray of a 2-D view) has s2 = +-0
+    // without a division; the sign of that zero reaches no result (s2 >= 0
+    // holds for both, and s2 * (u - u1) + nz/2 rounds both to the same value)
+    r.s2 = z2 == z1 ? 0.0f : (z2 - z1) / du;
     r.dz = g.dz;
     r.rtmp = 1.0f / r.tmp;
     r.mk = r.tmp >= 0x1p-60f && r.q <= 0x1p60f;
