@@ -392,6 +392,8 @@ int autotune_atx_t(asd_plan *p)
     if (g.nz > 1) {
         cand.push_back(ATX_T8x8x4);
         cand.push_back(ATX_T16x4x4);
+        cand.push_back(ATX_SORTED16);
+        cand.push_back(ATX_SORTED8);
     }
     hipEvent_t e0, e1;
     ASD_HIP(hipEventCreate(&e0));
@@ -857,8 +859,9 @@ int asd_plan_set_atx_variant(asd_plan *p, int atx_variant)
 {
     clear_error();
     ASD_ARG(p, "NULL plan");
-    ASD_ARG(atx_variant >= -1 && atx_variant < ATX_NVARIANT, "atx_variant must be -1..6");
-    ASD_ARG((atx_variant != ATX_T8x8x4 && atx_variant != ATX_T16x4x4) || p->views[0].nz > 1,
+    ASD_ARG(atx_variant >= -1 && atx_variant < ATX_NVARIANT, "atx_variant must be -1..8");
+    ASD_ARG((atx_variant != ATX_T8x8x4 && atx_variant != ATX_T16x4x4 &&
+             atx_variant != ATX_SORTED8) || p->views[0].nz > 1,
             "the 3-D tile variants need nz > 1");
     drop_graph(p);
     p->atx_variant = atx_variant;

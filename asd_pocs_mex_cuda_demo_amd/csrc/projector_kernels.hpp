@@ -290,6 +290,93 @@ __global__ __launch_bounds__(256) void atx_pairs_kernel(ViewGeom g, const float 
     if (valid && !stop) epi(v, acc.value(), accl.value(), pre);
 }
 
+// Backprojector with the voxels of a tile regrouped by footprint size.  A
+// wave runs as many pair evaluations as its largest footprint needs; cut 3-D
+// footprints hold 2 to 12 pairs, so a wave of neighbouring voxels runs about
+// 1.5 times the pairs its voxels have.  Here a workgroup owns a TX x TY x TZ
+// tile, computes every footprint, counting-sorts its voxels by pair
+// count (one LDS atomic per voxel, largest first), and thread t then walks
+// the voxel in sorted slot t, so each wave's voxels have similar footprints
+// while the tile's rays stay few (the tile is compact on the detector).
+// Each voxel's walk and sum are those of atx_kernel (bit-identical); only
+// the lane a voxel runs on changes.
+constexpr int SORT_CLASSES = 32;  // pair counts 0..30, and 31 for "31 or more"
+
+template <int TX, int TY, int TZ, class Epi, bool KAHAN = false, int CHUNK = 2>
+__global__ __launch_bounds__(TX *TY *TZ) void atx_sorted_kernel(ViewGeom g,
+                                                                 const float *__restrict__ Y,
+                                                                 const RayRec *__restrict__ rt,
+                                                                 Epi epi)
+{
+#pragma clang fp contract(off)
+    constexpr int T = TX * TY * TZ;
+    __shared__ int s_cnt[SORT_CLASSES], s_off[SORT_CLASSES];
+    __shared__ unsigned s_v[T];
+    __shared__ int s_f[4][T];
+    __shared__ float s_p[3][T];
+    const bool stop = epi.skip();
+    const TileDims td = tile_dims(g, TX, TY, TZ);
+    const unsigned t = xcd_block(blockIdx.x, gridDim.x);
+    const int tile_x = (int)(t % td.ntx), tile_y = (int)((t / td.ntx) % td.nty),
+              tile_z = (int)(t / ((unsigned)td.ntx * td.nty));
+    const int ix = tile_x * TX + (int)(threadIdx.x % TX),
+              iy = tile_y * TY + (int)((threadIdx.x / TX) % TY),
+              iz = tile_z * TZ + (int)(threadIdx.x / (TX * TY));
+    const bool valid = ix < g.nx && iy < g.ny && iz < g.nz;
+    Footprint f;
+    f.a_lo = f.b_lo = 0;
+    f.a_hi = f.b_hi = -1;
+    float px = 0.0f, py = 0.0f, pz = 0.0f;
+    if (valid) {
+        f = footprint(g, ix, iy, iz);
+        px = g.cos_phi * (-g.SO) - f.xc;
+        py = g.sin_phi * (-g.SO) - f.yc;
+        pz = 0.0f - f.zc;
+    }
+    const int np = (f.a_lo <= f.a_hi && f.b_lo <= f.b_hi)
+                       ? (f.a_hi - f.a_lo + 1) * (f.b_hi - f.b_lo + 1) : 0;
+    const int cls = np < SORT_CLASSES - 1 ? np : SORT_CLASSES - 1;
+    if (threadIdx.x < SORT_CLASSES) s_cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const int pos = atomicAdd(&s_cnt[cls], 1);
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        // exclusive offsets of the classes, largest class first
+        const int lane = threadIdx.x, c = SORT_CLASSES - 1 - lane;
+        const int n = lane < SORT_CLASSES ? s_cnt[c] : 0;
+        int incl = n;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int up = __shfl_up(incl, d, 64);
+            if (lane >= d) incl += up;
+        }
+        if (lane < SORT_CLASSES) s_off[c] = incl - n;
+    }
+    __syncthreads();
+    const int slot = s_off[cls] + pos;
+    s_v[slot] = valid ? (unsigned)(((size_t)iz * g.ny + iy) * g.nx + ix) : ~0u;
+    s_f[0][slot] = f.a_lo;
+    s_f[1][slot] = f.a_hi;
+    s_f[2][slot] = f.b_lo;
+    s_f[3][slot] = f.b_hi;
+    s_p[0][slot] = px;
+    s_p[1][slot] = py;
+    s_p[2][slot] = pz;
+    __syncthreads();
+    const unsigned vv = s_v[threadIdx.x];
+    if (vv == ~0u) return;
+    const typename Epi::Pre pre = epi.load(vv);
+    Footprint fs;
+    fs.a_lo = s_f[0][threadIdx.x];
+    fs.a_hi = s_f[1][threadIdx.x];
+    fs.b_lo = s_f[2][threadIdx.x];
+    fs.b_hi = s_f[3][threadIdx.x];
+    Acc<KAHAN> acc, accl;
+    atx_voxel_walk<KAHAN, CHUNK>(g, Y, rt, fs, s_p[0][threadIdx.x], s_p[1][threadIdx.x],
+                                 s_p[2][threadIdx.x], g.dz / 2.0f, acc, accl);
+    if (!stop) epi(vv, acc.value(), accl.value(), pre);
+}
+
 // Backprojector on voxel tiles (TX x TY x TZ = 256 voxels per workgroup):
 // the slope records and correction values of the tile's footprint are
 // staged in LDS once (coalesced), then every voxel walks its own footprint,
@@ -386,7 +473,9 @@ enum : int {
     ATX_PAIRS = 4,    // (voxel, bin) pairs spread over the workgroup
     ATX_DIRECT2 = 5,  // as ATX_DIRECT, two pairs per load batch
     ATX_DIRECT1 = 6,  // as ATX_DIRECT, one pair at a time
-    ATX_NVARIANT = 7,
+    ATX_SORTED16 = 7, // two-pair walks, 16x16x1 tiles regrouped by footprint size
+    ATX_SORTED8 = 8,  // the same on 8x8x4 tiles
+    ATX_NVARIANT = 9,
 };
 
 template <int TX, int TY, int TZ, bool KAHAN, class Epi>
@@ -412,6 +501,18 @@ int dispatch_atx_variant(int variant, const ViewGeom &g, const float *Y, const R
         const long N = (long)g.nx * g.ny * g.nz;
         klaunch(atx_kernel<Epi, KAHAN, 2>, (unsigned)((N + 255) / 256), 256, 0, s, g, Y, rt, epi);
         ASD_LAUNCH_CHECK("atx_kernel");
+        return ASD_OK;
+    }
+    case ATX_SORTED16:
+    case ATX_SORTED8: {
+        const bool t16 = variant == ATX_SORTED16;
+        const TileDims td = t16 ? tile_dims(g, 16, 16, 1) : tile_dims(g, 8, 8, 4);
+        const unsigned nt = (unsigned)td.ntx * td.nty * td.ntz;
+        if (t16)
+            klaunch(atx_sorted_kernel<16, 16, 1, Epi, KAHAN, 2>, nt, 256, 0, s, g, Y, rt, epi);
+        else
+            klaunch(atx_sorted_kernel<8, 8, 4, Epi, KAHAN, 2>, nt, 256, 0, s, g, Y, rt, epi);
+        ASD_LAUNCH_CHECK("atx_sorted_kernel");
         return ASD_OK;
     }
     case ATX_DIRECT1: {

@@ -206,7 +206,9 @@ ATX_KERNELS = {"direct": "atx_kernel<EpiSartAtx, false>",
                "tile16x4x4": "atx_tile_plain_kernel<16, 4, 4, false, EpiSartAtx>",
                "pairs": "atx_pairs_kernel<EpiSartAtx, false>",
                "direct2": "atx_kernel<EpiSartAtx, false, 2>",
-               "direct1": "atx_kernel<EpiSartAtx, false, 1>"}
+               "direct1": "atx_kernel<EpiSartAtx, false, 1>",
+               "sorted16": "atx_sorted_kernel<16, 16, 1, EpiSartAtx, false, 2>",
+               "sorted8x8x4": "atx_sorted_kernel<8, 8, 4, EpiSartAtx, false, 2>"}
 
 
 def profiled_traffic(workload, kernel):
@@ -308,7 +310,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--atx-variant", default=None,
                     choices=["direct", "tile16x16x1", "tile8x8x4", "tile16x4x4", "pairs",
-                             "direct2", "direct1"],
+                             "direct2", "direct1", "sorted16", "sorted8x8x4"],
                     help="force the backprojector variant instead of the autotuner's pick "
                          "(profiling runs use this to profile one variant)")
     ap.add_argument("--batch-only", action="store_true",

@@ -178,10 +178,12 @@ def test_backprojector_variants_are_bit_identical(three_d):
     autotuner reports a measured time for every variant it tried."""
     if three_d:
         para, img, P = _small_problem(n=24, nv=8, na=40, nz=8, nb=20)
-        variants = [0, 1, 2, 3, 4, 5, 6]
+        variants = [0, 1, 2, 3, 4, 5, 6, 7, 8]
+        tuned = variants
     else:
         para, img, P = _small_problem(n=48, nv=12, na=72)
-        variants = [0, 1, 4, 5, 6]
+        variants = [0, 1, 4, 5, 6, 7]
+        tuned = [0, 1, 4, 5, 6]  # the sorted tiles are tried on 3-D views only
     para.update(maxnumIter=3, alpha=0.2, numOfgradients=5)
     ref = None
     for v in [-1] + variants:
@@ -195,7 +197,7 @@ def test_backprojector_variants_are_bit_identical(three_d):
         img_v, (o, r) = pl.image(), pl.out()
         pl.close()
         if v < 0:
-            assert t["atx_variant"] is not None and len(t["atx_us"]) == len(variants)
+            assert t["atx_variant"] is not None and len(t["atx_us"]) == len(tuned)
             assert all(us > 0 for us in t["atx_us"].values())
         if ref is None:
             ref = (img_v, o, r)
