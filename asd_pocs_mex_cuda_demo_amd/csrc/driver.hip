@@ -394,6 +394,7 @@ int autotune_atx_t(asd_plan *p)
         cand.push_back(ATX_T16x4x4);
         cand.push_back(ATX_SORTED16);
         cand.push_back(ATX_SORTED8);
+        cand.push_back(ATX_SORTED8Z);
     }
     hipEvent_t e0, e1;
     ASD_HIP(hipEventCreate(&e0));
@@ -859,9 +860,9 @@ int asd_plan_set_atx_variant(asd_plan *p, int atx_variant)
 {
     clear_error();
     ASD_ARG(p, "NULL plan");
-    ASD_ARG(atx_variant >= -1 && atx_variant < ATX_NVARIANT, "atx_variant must be -1..8");
+    ASD_ARG(atx_variant >= -1 && atx_variant < ATX_NVARIANT, "atx_variant must be -1..9");
     ASD_ARG((atx_variant != ATX_T8x8x4 && atx_variant != ATX_T16x4x4 &&
-             atx_variant != ATX_SORTED8) || p->views[0].nz > 1,
+             atx_variant != ATX_SORTED8 && atx_variant != ATX_SORTED8Z) || p->views[0].nz > 1,
             "the 3-D tile variants need nz > 1");
     drop_graph(p);
     p->atx_variant = atx_variant;

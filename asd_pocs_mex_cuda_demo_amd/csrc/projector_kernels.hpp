@@ -475,7 +475,8 @@ enum : int {
     ATX_DIRECT1 = 6,  // as ATX_DIRECT, one pair at a time
     ATX_SORTED16 = 7, // two-pair walks, 16x16x1 tiles regrouped by footprint size
     ATX_SORTED8 = 8,  // the same on 8x8x4 tiles
-    ATX_NVARIANT = 9,
+    ATX_SORTED8Z = 9, // the same on 8x4x8 tiles
+    ATX_NVARIANT = 10,
 };
 
 template <int TX, int TY, int TZ, bool KAHAN, class Epi>
@@ -512,6 +513,13 @@ int dispatch_atx_variant(int variant, const ViewGeom &g, const float *Y, const R
             klaunch(atx_sorted_kernel<16, 16, 1, Epi, KAHAN, 2>, nt, 256, 0, s, g, Y, rt, epi);
         else
             klaunch(atx_sorted_kernel<8, 8, 4, Epi, KAHAN, 2>, nt, 256, 0, s, g, Y, rt, epi);
+        ASD_LAUNCH_CHECK("atx_sorted_kernel");
+        return ASD_OK;
+    }
+    case ATX_SORTED8Z: {
+        const TileDims td = tile_dims(g, 8, 4, 8);
+        klaunch(atx_sorted_kernel<8, 4, 8, Epi, KAHAN, 2>, (unsigned)td.ntx * td.nty * td.ntz, 256,
+                0, s, g, Y, rt, epi);
         ASD_LAUNCH_CHECK("atx_sorted_kernel");
         return ASD_OK;
     }

@@ -311,10 +311,10 @@ class Plan:
 
     def tuning(self):
         v = C.c_int()
-        us = np.zeros(9)
+        us = np.zeros(10)
         nat.check(self._L.asd_plan_tuning(self.h, C.byref(v), _ptr(us)))
         names = ["direct", "tile16x16x1", "tile8x8x4", "tile16x4x4", "pairs", "direct2",
-                 "direct1", "sorted16", "sorted8x8x4"]
+                 "direct1", "sorted16", "sorted8x8x4", "sorted8x4x8"]
         return {"atx_variant": names[v.value] if v.value >= 0 else None,
                 "atx_us": {n: float(t) for n, t in zip(names, us) if t >= 0}}
 

@@ -208,7 +208,8 @@ ATX_KERNELS = {"direct": "atx_kernel<EpiSartAtx, false>",
                "direct2": "atx_kernel<EpiSartAtx, false, 2>",
                "direct1": "atx_kernel<EpiSartAtx, false, 1>",
                "sorted16": "atx_sorted_kernel<16, 16, 1, EpiSartAtx, false, 2>",
-               "sorted8x8x4": "atx_sorted_kernel<8, 8, 4, EpiSartAtx, false, 2>"}
+               "sorted8x8x4": "atx_sorted_kernel<8, 8, 4, EpiSartAtx, false, 2>",
+               "sorted8x4x8": "atx_sorted_kernel<8, 4, 8, EpiSartAtx, false, 2>"}
 
 
 def profiled_traffic(workload, kernel):
@@ -310,7 +311,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--atx-variant", default=None,
                     choices=["direct", "tile16x16x1", "tile8x8x4", "tile16x4x4", "pairs",
-                             "direct2", "direct1", "sorted16", "sorted8x8x4"],
+                             "direct2", "direct1", "sorted16", "sorted8x8x4",
+                             "sorted8x4x8"],
                     help="force the backprojector variant instead of the autotuner's pick "
                          "(profiling runs use this to profile one variant)")
     ap.add_argument("--batch-only", action="store_true",

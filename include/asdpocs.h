@@ -181,12 +181,12 @@ int asd_plan_get_timing(asd_plan *plan, int stage, double *total_ms, long *count
 int asd_plan_set_graph(asd_plan *plan, int enable);
 /* Backprojector variant the plan's autotuner picked on its first sweep
  * (0 direct, 1 16x16x1 tiles, 2 8x8x4, 3 16x4x4, 4 voxel-bin pairs, 5 direct
- * with two-pair load batches, 6 direct one pair at a time, 7 and 8 two-pair
- * walks of 16x16x1 / 8x8x4 tiles regrouped by footprint size; -1 before the
- * first sweep) and the measured time of each (atx_us[9], microseconds, -1 =
- * not tried).  All variants give bit-identical results. */
+ * with two-pair load batches, 6 direct one pair at a time, 7 to 9 two-pair
+ * walks of 16x16x1 / 8x8x4 / 8x4x8 tiles regrouped by footprint size; -1
+ * before the first sweep) and the measured time of each (atx_us[10],
+ * microseconds, -1 = not tried).  All variants give bit-identical results. */
 int asd_plan_tuning(asd_plan *plan, int *atx_variant, double *atx_us);
-/* Forces the backprojector variant (0..8, as above; the tiles with four
+/* Forces the backprojector variant (0..9, as above; the tiles with four
  * slices need nz > 1) or, with -1, asks for a new measurement on the next
  * sweep. */
 int asd_plan_set_atx_variant(asd_plan *plan, int atx_variant);

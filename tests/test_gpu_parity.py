@@ -178,7 +178,7 @@ def test_backprojector_variants_are_bit_identical(three_d):
     autotuner reports a measured time for every variant it tried."""
     if three_d:
         para, img, P = _small_problem(n=24, nv=8, na=40, nz=8, nb=20)
-        variants = [0, 1, 2, 3, 4, 5, 6, 7, 8]
+        variants = [0, 1, 2, 3, 4, 5, 6, 7, 8, 9]
         tuned = variants
     else:
         para, img, P = _small_problem(n=48, nv=12, na=72)
