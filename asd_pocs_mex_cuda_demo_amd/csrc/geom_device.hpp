@@ -26,6 +26,8 @@ struct AxRay {
     // by tmp may be finished from rtmp (div_by_tmp)
     float rtmp;
     int mk;
+    // fast_t / fast_z: the slab's crossing fractions may use div_cell (below)
+    int fast_t, fast_z;
     int nu2, nt2, nz2, nt, nz;
     int xdriven;
     // flat: a one-slice volume and a ray in its z = 0 face (s2 = +-0, dz = 1),
@@ -56,6 +58,31 @@ __host__ __device__ __forceinline__ float div_by_tmp(const AxRay &r, float x)
     const float q0 = x * r.rtmp;
     const float res = fmaf(-r.tmp, q0, x);
     return fmaf(res, r.rtmp, q0);
+}
+
+// (c - a) / (b - a) for a slab's cell crossing, bit for bit the IEEE
+// quotient: the compiler's expansion of the FP32 division without its
+// operand scaling and special-value fixup, which leave the result unchanged
+// when no step under- or overflows.  That holds for these quotients when the
+// cell grid has at least two cells along the axis (so the position of a
+// slab end is zero or at least 2^-24 away from an integer, by Sterbenz's
+// lemma on p + n/2 with n/2 >= 1) and the coordinates stay below 2^22
+// (ax_ray_setup sets fast_t / fast_z only then): numerator and denominator
+// lie in [2^-24, 2^23] and every residual is a normal number.
+__host__ __device__ __forceinline__ float div_cell(float num, float den)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    float y = __builtin_amdgcn_rcpf(den);
+    const float e = __builtin_fmaf(-den, y, 1.0f);
+    y = __builtin_fmaf(e, y, y);
+    float q = num * y;
+    const float r0 = __builtin_fmaf(-den, q, num);
+    q = __builtin_fmaf(r0, y, q);
+    const float r1 = __builtin_fmaf(-den, q, num);
+    return __builtin_fmaf(r1, y, q);
+#else
+    return num / den;
+#endif
 }
 
 // Correctly rounded square root of x >= 2^-96 (or +-0, +inf, NaN): the
@@ -109,6 +136,10 @@ __host__ __device__ __forceinline__ AxRay ax_ray_setup(const ViewGeom &g, float 
     r.nz = g.nz;
     r.nz2 = g.nz / 2;
     r.dz1 = g.dz == 1.0f;
+    // coordinates below 2^22 and at least two cells along each axis (div_cell)
+    const bool small = g.SO + g.OD + (float)g.nx + (float)g.ny + (float)g.nz < 0x1p21f;
+    r.fast_t = small && g.nx >= 2 && g.ny >= 2;
+    r.fast_z = small && g.nz >= 2;
     r.flat = r.dz1 && g.nz == 1 && r.s2 == 0.0f;
     return r;
 }
@@ -241,8 +272,8 @@ __host__ __device__ __forceinline__ void ax_slab_geom(const AxRay &r, int iu, in
     bool in_z1 = cz1 >= 0 && cz1 < r.nz, in_z2 = cz2 >= 0 && cz2 < r.nz;
     if (!((in_t1 || in_t2) && (in_z1 || in_z2))) return;
     float rt = 1.0f, rz = 1.0f;
-    if (ct1 != ct2) rt = ((float)ct2 - ta) / (tb - ta);
-    if (cz1 != cz2) rz = ((float)cz2 - za) / (zb - za);
+    if (ct1 != ct2) rt = r.fast_t ? div_cell((float)ct2 - ta, tb - ta) : ((float)ct2 - ta) / (tb - ta);
+    if (cz1 != cz2) rz = r.fast_z ? div_cell((float)cz2 - za, zb - za) : ((float)cz2 - za) / (zb - za);
     float d = tb - ta;
     // (d*d + 1) * q >= q >= tmp, so the root's argument is >= 1
     float l = sqrt_ge_tiny(div_by_tmp(r, (d * d + 1) * r.q));
