@@ -561,17 +561,46 @@ struct Footprint {
 // twice as wide.  A ray outside the widened shadow misses the voxel by more
 // than the margin, far beyond the ~1e-4 voxel rounding of find_l_3d's face
 // tests, so every test fails and its length is exactly 0: the sums are
-// unchanged (bit-identical when uncompensated).  (xr, yr) is the voxel
-// centre in the view frame (x' towards the detector).
+// unchanged (bit-identical when uncompensated).
 constexpr float SHADOW_MARGIN = 0.05f;
 
-__host__ __device__ __forceinline__ void shadow_cut(const ViewGeom &g, float xr, float yr,
-                                                    Footprint &f)
+// The part of a voxel's footprint its whole column (ix, iy) shares: the
+// in-plane bin range (reference bounds, clamped and cut to the square's
+// shadow) and what the axial range needs from the column.
+struct FootCol {
+    float xc, yc;
+    float tz;        // SD / ((x' + SO) * dz_det), the reference's axial scale (cu:247-253)
+    int a_lo, a_hi;
+    float m_near, m_far, eps;  // magnification range over the square, cut margin
+    int cut;         // the axial cut applies (tight view, square in front of the source)
+};
+
+__host__ __device__ __forceinline__ FootCol footprint_col(const ViewGeom &g, int ix, int iy)
 {
-    // the detector coordinate y' * SD / (SO + x') of the four corners (offsets
-    // fp_cx/fp_cy in the view frame); the corners also give the square's
-    // nearest and farthest depth, hence the largest and smallest magnification
+#pragma clang fp contract(off)
+    FootCol c;
     const float SD = g.SO + g.OD;
+    const int na2 = g.na / 2, nx2 = g.nx / 2, ny2 = g.ny / 2;
+    // i + 0.5 - n/2 (the reference forms it in FP64 and rounds) is exact in
+    // FP32 for |i - n/2| < 2^22
+    c.yc = (float)(iy - ny2) + 0.5f;
+    c.xc = (float)(ix - nx2) + 0.5f;
+    // (xr, yr): the voxel centre in the view frame (x' towards the detector)
+    const float xr = g.cos_phi * c.xc + g.sin_phi * c.yc;
+    const float yr = -g.sin_phi * c.xc + g.cos_phi * c.yc;
+    const float t = SD / ((xr + g.SO) * g.dy_det);
+    const int a_hi = (int)floorf((yr + 1) * t + na2);
+    const int a_lo = (int)floorf((yr - 1) * t + na2);
+    c.tz = g.dz_det != g.dy_det ? SD / ((xr + g.SO) * g.dz_det) : t;
+    c.a_lo = a_lo < 0 ? 0 : a_lo;
+    c.a_hi = a_hi > g.na - 1 ? g.na - 1 : a_hi;
+    c.cut = 0;
+    c.m_near = c.m_far = c.eps = 0.0f;
+    if (!g.tight) return c;
+    // the detector coordinate y' * SD / (SO + x') of the square's four
+    // corners (offsets fp_cx/fp_cy in the view frame); the corners also give
+    // its nearest and farthest depth, hence the largest and smallest
+    // magnification
     float umin = 3.0e38f, umax = -3.0e38f, m_near = 0.0f, m_far = 3.0e38f, dmin = 3.0e38f;
 #pragma unroll
     for (int k = 0; k < 4; k++) {
@@ -584,51 +613,55 @@ __host__ __device__ __forceinline__ void shadow_cut(const ViewGeom &g, float xr,
         m_far = fminf(m_far, m);
         dmin = fminf(dmin, depth);
     }
-    if (dmin <= 0.0f) return;  // voxel reaches behind the source: keep all
-    const float eps = SHADOW_MARGIN * m_near;
+    if (dmin <= 0.0f) return c;  // the square reaches behind the source: keep all
+    c.cut = 1;
+    c.m_near = m_near;
+    c.m_far = m_far;
+    c.eps = SHADOW_MARGIN * m_near;
     // bins i with centre (i - n/2 + 1/2) * det inside [lo - eps, hi + eps]
-    const float ia_off = 0.5f * (float)g.na - 0.5f, ib_off = 0.5f * (float)g.nb - 0.5f;
-    const int a_lo = (int)ceilf((umin - eps) * g.fp_rdy + ia_off);
-    const int a_hi = (int)floorf((umax + eps) * g.fp_rdy + ia_off);
-    const float hz = 0.5f * g.dz;
-    const float z0 = f.zc - hz, z1 = f.zc + hz;
-    const float vmin = z0 * (z0 < 0.0f ? m_near : m_far);
-    const float vmax = z1 * (z1 > 0.0f ? m_near : m_far);
-    const int b_lo = (int)ceilf((vmin - eps) * g.fp_rdz + ib_off);
-    const int b_hi = (int)floorf((vmax + eps) * g.fp_rdz + ib_off);
-    f.a_lo = a_lo > f.a_lo ? a_lo : f.a_lo;
-    f.a_hi = a_hi < f.a_hi ? a_hi : f.a_hi;
-    f.b_lo = b_lo > f.b_lo ? b_lo : f.b_lo;
-    f.b_hi = b_hi < f.b_hi ? b_hi : f.b_hi;
+    const float ia_off = 0.5f * (float)g.na - 0.5f;
+    const int cut_lo = (int)ceilf((umin - c.eps) * g.fp_rdy + ia_off);
+    const int cut_hi = (int)floorf((umax + c.eps) * g.fp_rdy + ia_off);
+    c.a_lo = cut_lo > c.a_lo ? cut_lo : c.a_lo;
+    c.a_hi = cut_hi < c.a_hi ? cut_hi : c.a_hi;
+    return c;
+}
+
+// The footprint of voxel iz of a column: the axial bin range (reference
+// bounds, clamped, and cut to the shadow of the voxel's z extent over the
+// square's depth) with the column's in-plane range.
+__host__ __device__ __forceinline__ Footprint footprint_z(const ViewGeom &g, const FootCol &c,
+                                                          int iz)
+{
+#pragma clang fp contract(off)
+    Footprint f;
+    const int nb2 = g.nb / 2, nz2 = g.nz / 2;
+    const float d = g.fp_d;
+    f.zc = ((float)(iz - nz2) + 0.5f) * g.dz;
+    f.yc = c.yc;
+    f.xc = c.xc;
+    const int b_hi = (int)floorf((f.zc + d) * c.tz + nb2);
+    const int b_lo = (int)floorf((f.zc - d) * c.tz + nb2);
+    f.a_lo = c.a_lo;
+    f.a_hi = c.a_hi;
+    f.b_lo = b_lo < 0 ? 0 : b_lo;
+    f.b_hi = b_hi > g.nb - 1 ? g.nb - 1 : b_hi;
+    if (c.cut) {
+        const float ib_off = 0.5f * (float)g.nb - 0.5f, hz = 0.5f * g.dz;
+        const float z0 = f.zc - hz, z1 = f.zc + hz;
+        const float vmin = z0 * (z0 < 0.0f ? c.m_near : c.m_far);
+        const float vmax = z1 * (z1 > 0.0f ? c.m_near : c.m_far);
+        const int cut_lo = (int)ceilf((vmin - c.eps) * g.fp_rdz + ib_off);
+        const int cut_hi = (int)floorf((vmax + c.eps) * g.fp_rdz + ib_off);
+        f.b_lo = cut_lo > f.b_lo ? cut_lo : f.b_lo;
+        f.b_hi = cut_hi < f.b_hi ? cut_hi : f.b_hi;
+    }
+    return f;
 }
 
 __host__ __device__ __forceinline__ Footprint footprint(const ViewGeom &g, int ix, int iy, int iz)
 {
-#pragma clang fp contract(off)
-    Footprint f;
-    float SD = g.SO + g.OD;
-    int na2 = g.na / 2, nb2 = g.nb / 2;
-    int nx2 = g.nx / 2, ny2 = g.ny / 2, nz2 = g.nz / 2;
-    const float d = g.fp_d;
-    // i + 0.5 - n/2 (the reference forms it in FP64 and rounds) is exact in
-    // FP32 for |i - n/2| < 2^22
-    f.zc = ((float)(iz - nz2) + 0.5f) * g.dz;
-    f.yc = (float)(iy - ny2) + 0.5f;
-    f.xc = (float)(ix - nx2) + 0.5f;
-    float xr = g.cos_phi * f.xc + g.sin_phi * f.yc;
-    float yr = -g.sin_phi * f.xc + g.cos_phi * f.yc;
-    float t = SD / ((xr + g.SO) * g.dy_det);
-    int a_hi = (int)floorf((yr + 1) * t + na2);
-    int a_lo = (int)floorf((yr - 1) * t + na2);
-    if (g.dz_det != g.dy_det) t = SD / ((xr + g.SO) * g.dz_det);
-    int b_hi = (int)floorf((f.zc + d) * t + nb2);
-    int b_lo = (int)floorf((f.zc - d) * t + nb2);
-    f.a_lo = a_lo < 0 ? 0 : a_lo;
-    f.b_lo = b_lo < 0 ? 0 : b_lo;
-    f.a_hi = a_hi > g.na - 1 ? g.na - 1 : a_hi;
-    f.b_hi = b_hi > g.nb - 1 ? g.nb - 1 : b_hi;
-    if (g.tight) shadow_cut(g, xr, yr, f);
-    return f;
+    return footprint_z(g, footprint_col(g, ix, iy), iz);
 }
 
 }  // namespace asd

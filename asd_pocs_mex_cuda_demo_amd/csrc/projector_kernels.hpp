@@ -299,7 +299,9 @@ __global__ __launch_bounds__(256) void atx_pairs_kernel(ViewGeom g, const float 
 // the voxel in sorted slot t, so each wave's voxels have similar footprints
 // while the tile's rays stay few (the tile is compact on the detector).
 // Each voxel's walk and sum are those of atx_kernel (bit-identical); only
-// the lane a voxel runs on changes.
+// the lane a voxel runs on changes.  With TZ > 1 the in-plane part of the
+// footprints (footprint_col, the costlier part) is computed once per column
+// of the tile by its first TX x TY threads and shared through LDS.
 constexpr int SORT_CLASSES = 32;  // pair counts 0..30, and 31 for "31 or more"
 
 template <int TX, int TY, int TZ, class Epi, bool KAHAN = false, int CHUNK = 2>
@@ -314,6 +316,7 @@ __global__ __launch_bounds__(TX *TY *TZ) void atx_sorted_kernel(ViewGeom g,
     __shared__ unsigned s_v[T];
     __shared__ int s_f[4][T];
     __shared__ float s_p[3][T];
+    __shared__ FootCol s_col[TZ > 1 ? TX * TY : 1];
     const bool stop = epi.skip();
     const TileDims td = tile_dims(g, TX, TY, TZ);
     const unsigned t = xcd_block(blockIdx.x, gridDim.x);
@@ -323,12 +326,20 @@ __global__ __launch_bounds__(TX *TY *TZ) void atx_sorted_kernel(ViewGeom g,
               iy = tile_y * TY + (int)((threadIdx.x / TX) % TY),
               iz = tile_z * TZ + (int)(threadIdx.x / (TX * TY));
     const bool valid = ix < g.nx && iy < g.ny && iz < g.nz;
+    if constexpr (TZ > 1) {
+        if ((int)threadIdx.x < TX * TY && ix < g.nx && iy < g.ny)
+            s_col[threadIdx.x] = footprint_col(g, ix, iy);
+        __syncthreads();
+    }
     Footprint f;
     f.a_lo = f.b_lo = 0;
     f.a_hi = f.b_hi = -1;
     float px = 0.0f, py = 0.0f, pz = 0.0f;
     if (valid) {
-        f = footprint(g, ix, iy, iz);
+        if constexpr (TZ > 1)
+            f = footprint_z(g, s_col[threadIdx.x % (TX * TY)], iz);
+        else
+            f = footprint(g, ix, iy, iz);
         px = g.cos_phi * (-g.SO) - f.xc;
         py = g.sin_phi * (-g.SO) - f.yc;
         pz = 0.0f - f.zc;
