@@ -108,7 +108,7 @@ def lib():
                        ("asd_plan_get_projections", [vp, vp]),
                        ("asd_plan_get_reference", [vp, vp]),
                        ("asd_plan_get_timing", [vp, i, dp, C.POINTER(C.c_long)]),
-                       ("asd_plan_info", [vp, C.POINTER(C.c_int), C.POINTER(vp)])]:
+                       ("asd_plan_info", [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(vp)])]:
         fn = getattr(L, name)
         fn.restype = i
         fn.argtypes = args

@@ -141,6 +141,16 @@ int launch_atx(const ViewGeom &g, float *x, const float *y, float *sc, const flo
 // Lanes cooperating on one ray of the forward projector.
 int ax_lanes_per_ray(int na, int nb, int nu);
 
+// Launch shape of the forward projector for one view: lanes per ray (per
+// bundle when rows > 1) and detector rows per bundle (1: ax_kernel, one ray
+// per lane group; 2: ax_rows_kernel).  The shape changes only the order in
+// which a ray's slab products are summed (and so the last bits of the sum),
+// never which products there are.
+struct AxShape {
+    int lpr = 1, rows = 1;
+};
+AxShape ax_shape(int na, int nb, int nu);
+
 // XCD-aware block order: hardware hands consecutive workgroups to the 8 XCDs
 // round-robin; remap so that each XCD walks a contiguous range of tiles and
 // neighbouring tiles share its L2.

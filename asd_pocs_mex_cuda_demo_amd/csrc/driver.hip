@@ -333,7 +333,7 @@ int enqueue_sweep_t(asd_plan *p, const double *relax)
         {
             StageTimer t(p, ST_AX, true);
             // null detector tables: the gateway's grid is computed in the kernel
-            if ((rc = dispatch_ax<KAHAN>(p->lpr, g, p->m.as<float>(), nullptr, nullptr, ea,
+            if ((rc = dispatch_ax<KAHAN>(p->ax, g, p->m.as<float>(), nullptr, nullptr, ea,
                                          p->stream)))
                 return rc;
         }
@@ -617,7 +617,7 @@ int asd_plan_create(const asd_para *para, const double *phi_vec, int nv, asd_pla
         zall.insert(zall.end(), zd.begin(), zd.end());
     }
     const size_t N = p->N, M = p->M;
-    p->lpr = ax_lanes_per_ray(para->na, para->nb, para->nx > para->ny ? para->nx : para->ny);
+    p->ax = ax_shape(para->na, para->nb, para->nx > para->ny ? para->nx : para->ny);
     auto fail = [&](int code) { asd_plan_destroy(p); return code; };
     if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess) {
         p->stream = nullptr;
@@ -880,11 +880,12 @@ int asd_plan_tuning(asd_plan *p, int *atx_variant, double *atx_us)
     return ASD_OK;
 }
 
-int asd_plan_info(asd_plan *p, int *lanes_per_ray, void **stream)
+int asd_plan_info(asd_plan *p, int *lanes_per_ray, int *rows_per_bundle, void **stream)
 {
     clear_error();
     ASD_ARG(p, "NULL plan");
-    if (lanes_per_ray) *lanes_per_ray = p->lpr;
+    if (lanes_per_ray) *lanes_per_ray = p->ax.lpr;
+    if (rows_per_bundle) *rows_per_bundle = p->ax.rows;
     if (stream) *stream = (void *)p->stream;
     return ASD_OK;
 }

@@ -107,29 +107,42 @@ __host__ __device__ __forceinline__ float sqrt_ge_tiny(float x)
 #endif
 }
 
-__host__ __device__ __forceinline__ AxRay ax_ray_setup(const ViewGeom &g, float ydet, float zdet)
+// The axial part of a ray's setup: everything that depends on the detector
+// row's coordinate zdet.  du is the in-plane run along the driving axis
+// (ax_ray_setup); rays of one detector column differ only in these fields.
+__host__ __device__ __forceinline__ void ax_ray_axial(AxRay &r, float du, float zdet)
 {
 #pragma clang fp contract(off)
-    AxRay r;
-    float x1 = g.cos_phi * (-g.SO), y1 = g.sin_phi * (-g.SO), z1 = 0.0f;
-    float x2 = g.cos_phi * g.OD - g.sin_phi * ydet;
-    float y2 = g.sin_phi * g.OD + g.cos_phi * ydet;
-    float z2 = zdet;
-    r.tmp = (x1 - x2) * (x1 - x2) + (y1 - y2) * (y1 - y2);
+    const float z1 = 0.0f, z2 = zdet;
     r.q = r.tmp + (z1 - z2) * (z1 - z2);
-    r.xdriven = absf(x1 - x2) > absf(y1 - y2);
-    r.u1 = r.xdriven ? x1 : y1;
-    r.t1s = r.xdriven ? y1 : x1;
-    float du = r.xdriven ? (x2 - x1) : (y2 - y1);
-    float dt = r.xdriven ? (y2 - y1) : (x2 - x1);
-    r.s1 = dt / du;
     // a ray in the source's plane (every ray of a 2-D view) has s2 = +-0
     // without a division; the sign of that zero reaches no result (s2 >= 0
     // holds for both, and s2 * (u - u1) + nz/2 rounds both to the same value)
     r.s2 = z2 == z1 ? 0.0f : (z2 - z1) / du;
+    r.mk = r.tmp >= 0x1p-60f && r.q <= 0x1p60f;
+    r.flat = r.dz1 && r.nz == 1 && r.s2 == 0.0f;
+}
+
+// Setup of the ray from the source to detector point (ydet, zdet) (the
+// reference's make_ray, cu:362-375); du receives the in-plane run that
+// ax_ray_axial needs for further rows of the same column.
+__host__ __device__ __forceinline__ AxRay ax_ray_setup(const ViewGeom &g, float ydet, float zdet,
+                                                       float &du)
+{
+#pragma clang fp contract(off)
+    AxRay r;
+    float x1 = g.cos_phi * (-g.SO), y1 = g.sin_phi * (-g.SO);
+    float x2 = g.cos_phi * g.OD - g.sin_phi * ydet;
+    float y2 = g.sin_phi * g.OD + g.cos_phi * ydet;
+    r.tmp = (x1 - x2) * (x1 - x2) + (y1 - y2) * (y1 - y2);
+    r.xdriven = absf(x1 - x2) > absf(y1 - y2);
+    r.u1 = r.xdriven ? x1 : y1;
+    r.t1s = r.xdriven ? y1 : x1;
+    du = r.xdriven ? (x2 - x1) : (y2 - y1);
+    float dt = r.xdriven ? (y2 - y1) : (x2 - x1);
+    r.s1 = dt / du;
     r.dz = g.dz;
     r.rtmp = 1.0f / r.tmp;
-    r.mk = r.tmp >= 0x1p-60f && r.q <= 0x1p60f;
     r.nu2 = r.xdriven ? g.nx / 2 : g.ny / 2;
     r.nt2 = r.xdriven ? g.ny / 2 : g.nx / 2;
     r.nt = r.xdriven ? g.ny : g.nx;
@@ -140,8 +153,14 @@ __host__ __device__ __forceinline__ AxRay ax_ray_setup(const ViewGeom &g, float 
     const bool small = g.SO + g.OD + (float)g.nx + (float)g.ny + (float)g.nz < 0x1p21f;
     r.fast_t = small && g.nx >= 2 && g.ny >= 2;
     r.fast_z = small && g.nz >= 2;
-    r.flat = r.dz1 && g.nz == 1 && r.s2 == 0.0f;
+    ax_ray_axial(r, du, zdet);
     return r;
+}
+
+__host__ __device__ __forceinline__ AxRay ax_ray_setup(const ViewGeom &g, float ydet, float zdet)
+{
+    float du;
+    return ax_ray_setup(g, ydet, zdet, du);
 }
 
 // Detector coordinate of bin i of n at pitch det (voxels), as the MEX
@@ -229,23 +248,55 @@ struct SlabW {
     bool ok[3];
 };
 
-__host__ __device__ __forceinline__ void ax_slab_geom(const AxRay &r, int iu, int nx, int ny,
-                                                      SlabW &s)
+// The transverse half of slab iu of a ray: its sorted transverse extent
+// [ta, tb], cells and crossing fraction.  It depends only on the ray's
+// in-plane line, which every ray of one detector column shares (rays that
+// differ only in their detector row), so ax_rows_kernel computes it once for
+// a bundle of rows.
+struct SlabT {
+    float ua, ub;  // the slab's bounds along the driving axis
+    float ta, tb;
+    float d2p1;    // (tb - ta)^2 + 1, the in-plane factor of the slab length
+    float rt;      // transverse crossing fraction (1 without a crossing)
+    int ct1, ct2;
+    bool in_t1, in_t2;
+};
+
+__host__ __device__ __forceinline__ SlabT ax_slab_t(const AxRay &r, int iu)
+{
+#pragma clang fp contract(off)
+    SlabT t;
+    t.ua = (float)(iu - r.nu2);
+    t.ub = t.ua + 1;
+    if (r.s1 >= 0) {
+        t.ta = r.t1s + r.s1 * (t.ua - r.u1) + r.nt2;
+        t.tb = r.t1s + r.s1 * (t.ub - r.u1) + r.nt2;
+    } else {
+        t.ta = r.t1s + r.s1 * (t.ub - r.u1) + r.nt2;
+        t.tb = r.t1s + r.s1 * (t.ua - r.u1) + r.nt2;
+    }
+    t.ct1 = (int)floorf(t.ta);
+    t.ct2 = (int)floorf(t.tb);
+    t.in_t1 = t.ct1 >= 0 && t.ct1 < r.nt;
+    t.in_t2 = t.ct2 >= 0 && t.ct2 < r.nt;
+    const float d = t.tb - t.ta;
+    t.d2p1 = d * d + 1;
+    t.rt = 1.0f;
+    if (t.ct1 != t.ct2 && (t.in_t1 || t.in_t2))
+        t.rt = r.fast_t ? div_cell((float)t.ct2 - t.ta, t.tb - t.ta)
+                        : ((float)t.ct2 - t.ta) / (t.tb - t.ta);
+    return t;
+}
+
+// The rest of slab iu of ray r given its transverse half: the axial extent
+// and cells, the slab length and the three pieces.
+__host__ __device__ __forceinline__ void ax_slab_z(const AxRay &r, const SlabT &t, int iu,
+                                                   int nx, int ny, SlabW &s)
 {
 #pragma clang fp contract(off)
     s.ok[0] = s.ok[1] = s.ok[2] = false;
     s.idx[0] = s.idx[1] = s.idx[2] = 0;
     s.wl[0] = s.wl[1] = s.wl[2] = 0.0f;
-    float ua = (float)(iu - r.nu2), ub = ua + 1;
-    float ta, tb;
-    if (r.s1 >= 0) {
-        ta = r.t1s + r.s1 * (ua - r.u1) + r.nt2;
-        tb = r.t1s + r.s1 * (ub - r.u1) + r.nt2;
-    } else {
-        ta = r.t1s + r.s1 * (ub - r.u1) + r.nt2;
-        tb = r.t1s + r.s1 * (ua - r.u1) + r.nt2;
-    }
-    int ct1 = (int)floorf(ta), ct2 = (int)floorf(tb);
     int cz1 = 0, cz2 = 0;
     float za = 0.0f, zb = 0.0f;
     if (!r.flat) {
@@ -253,11 +304,11 @@ __host__ __device__ __forceinline__ void ax_slab_geom(const AxRay &r, int iu, in
         // so dz == 1 skips the division
         float pa, pb;
         if (r.s2 >= 0) {
-            pa = r.s2 * (ua - r.u1);
-            pb = r.s2 * (ub - r.u1);
+            pa = r.s2 * (t.ua - r.u1);
+            pb = r.s2 * (t.ub - r.u1);
         } else {
-            pa = r.s2 * (ub - r.u1);
-            pb = r.s2 * (ua - r.u1);
+            pa = r.s2 * (t.ub - r.u1);
+            pb = r.s2 * (t.ua - r.u1);
         }
         if (!r.dz1) {
             pa = pa / r.dz;
@@ -268,18 +319,16 @@ __host__ __device__ __forceinline__ void ax_slab_geom(const AxRay &r, int iu, in
         cz1 = (int)floorf(za);
         cz2 = (int)floorf(zb);
     }
-    bool in_t1 = ct1 >= 0 && ct1 < r.nt, in_t2 = ct2 >= 0 && ct2 < r.nt;
-    bool in_z1 = cz1 >= 0 && cz1 < r.nz, in_z2 = cz2 >= 0 && cz2 < r.nz;
-    if (!((in_t1 || in_t2) && (in_z1 || in_z2))) return;
-    float rt = 1.0f, rz = 1.0f;
-    if (ct1 != ct2) rt = r.fast_t ? div_cell((float)ct2 - ta, tb - ta) : ((float)ct2 - ta) / (tb - ta);
+    const bool in_z1 = cz1 >= 0 && cz1 < r.nz, in_z2 = cz2 >= 0 && cz2 < r.nz;
+    if (!((t.in_t1 || t.in_t2) && (in_z1 || in_z2))) return;
+    float rz = 1.0f;
     if (cz1 != cz2) rz = r.fast_z ? div_cell((float)cz2 - za, zb - za) : ((float)cz2 - za) / (zb - za);
-    float d = tb - ta;
     // (d*d + 1) * q >= q >= tmp, so the root's argument is >= 1
-    float l = sqrt_ge_tiny(div_by_tmp(r, (d * d + 1) * r.q));
-    bool tfirst = !(rt > rz);  // the t crossing comes first (ties: t)
-    float lo = tfirst ? rt : rz, hi = tfirst ? rz : rt;
-    int ctm = tfirst ? ct2 : ct1, czm = tfirst ? cz1 : cz2;
+    const float l = sqrt_ge_tiny(div_by_tmp(r, t.d2p1 * r.q));
+    const float rt = t.rt;
+    const bool tfirst = !(rt > rz);  // the t crossing comes first (ties: t)
+    const float lo = tfirst ? rt : rz, hi = tfirst ? rz : rt;
+    const int ctm = tfirst ? t.ct2 : t.ct1, czm = tfirst ? cz1 : cz2;
     // (z * ny + y) * nx + x in 24-bit multiply-adds (index_fits); pieces out
     // of the volume get index 0
     auto index = [&](int ct, int cz) {
@@ -287,16 +336,22 @@ __host__ __device__ __forceinline__ void ax_slab_geom(const AxRay &r, int iu, in
         const unsigned x = r.xdriven ? (unsigned)iu : (unsigned)ct;
         return mad24(mad24((unsigned)cz, (unsigned)ny, y), (unsigned)nx, x);
     };
-    s.ok[0] = in_t1 && in_z1;
+    s.ok[0] = t.in_t1 && in_z1;
     // (written as mask logic, which stays in scalar lane masks)
-    s.ok[1] = (tfirst && in_t2 && in_z1) || (!tfirst && in_t1 && in_z2);
-    s.ok[2] = in_t2 && in_z2;
-    s.idx[0] = s.ok[0] ? index(ct1, cz1) : 0u;
+    s.ok[1] = (tfirst && t.in_t2 && in_z1) || (!tfirst && t.in_t1 && in_z2);
+    s.ok[2] = t.in_t2 && in_z2;
+    s.idx[0] = s.ok[0] ? index(t.ct1, cz1) : 0u;
     s.idx[1] = s.ok[1] ? index(ctm, czm) : 0u;
-    s.idx[2] = s.ok[2] ? index(ct2, cz2) : 0u;
+    s.idx[2] = s.ok[2] ? index(t.ct2, cz2) : 0u;
     s.wl[0] = lo * l;
     s.wl[1] = (hi - lo) * l;
     s.wl[2] = (1.0f - hi) * l;
+}
+
+__host__ __device__ __forceinline__ void ax_slab_geom(const AxRay &r, int iu, int nx, int ny,
+                                                      SlabW &s)
+{
+    ax_slab_z(r, ax_slab_t(r, iu), iu, nx, ny, s);
 }
 
 // U consecutive slabs of one lane (iu0, iu0 + stride, ...): all geometry

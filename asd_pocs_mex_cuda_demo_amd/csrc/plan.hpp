@@ -49,7 +49,7 @@ struct asd_plan {
     TvScratch red;
     bool have_P = false, have_I0 = false;
     bool I0_sim = false;            // the RMSE reference is a simulated phantom
-    int lpr = 1;
+    asd::AxShape ax;                // forward projector launch shape
     int *h_stop = nullptr;          // pinned copies of the stop flag (4 checkpoints)
     hipEvent_t stop_ev[4] = {};
     int atx_variant = -1;           // backprojector variant picked by the autotuner

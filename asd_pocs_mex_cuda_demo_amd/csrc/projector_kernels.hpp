@@ -87,10 +87,150 @@ __global__ __launch_bounds__(256) void ax_kernel(ViewGeom g, const float *__rest
     if (active && lane == 0 && !stop) epi(id, (float)a, (float)b, pre);
 }
 
+// Forward projector over bundles of K rays that share a detector column (ia)
+// and take consecutive rows ib0 .. ib0 + K - 1.  Their in-plane lines are the
+// same, so each slab's transverse half (ax_slab_t: extent, cells, crossing
+// fraction) is computed once for the K rays and only the axial half
+// (ax_slab_z) per ray; every ray's pieces, lengths and sums are the ones
+// ax_kernel forms for it.  LPR lanes march the bundle together as above; the
+// K ray sums are reduced the same way and lane k stores ray k.  Rows past nb
+// repeat the last row and are not stored.  A lane takes U slabs per load
+// batch (iu, iu + LPR, ...), so 3 * U * K loads are in flight together.
+template <int LPR, int K, int U, bool KAHAN, class Epi>
+__global__ __launch_bounds__(256) void ax_rows_kernel(ViewGeom g, const float *__restrict__ X,
+                                                      const float *__restrict__ y_det,
+                                                      const float *__restrict__ z_det, Epi epi)
+{
+#pragma clang fp contract(off)
+    static_assert(LPR >= K, "lane k stores row k");
+    constexpr int RPB = LPR >= 256 ? 1 : 256 / LPR;  // bundles per block
+    constexpr int WL = LPR < 64 ? LPR : 64;
+    constexpr int WPR = LPR > 64 ? LPR / 64 : 1;
+    const bool stop = epi.skip();
+    const int nbb = (g.nb + K - 1) / K;  // bundles per detector column
+    const long nbund = (long)g.na * nbb;
+    const int lane = threadIdx.x % LPR;
+    const long id = (long)xcd_block(blockIdx.x, gridDim.x) * RPB + threadIdx.x / LPR;
+    const bool active = id < nbund;
+    // id < na * nb < 2^31 (index_fits)
+    const unsigned ub = active ? fastdiv((unsigned)id, g.div_na) : 0u;
+    const int ia = (int)((unsigned)id - mad24(ub, (unsigned)g.na, 0u)), ib0 = (int)ub * K;
+    const bool mine = active && lane < K && ib0 + lane < g.nb;
+    const long ray = (long)(ib0 + lane) * g.na + ia;
+    typename Epi::Pre pre{};
+    if (mine) pre = epi.load(ray);
+    Acc<KAHAN> sa[K], sl[K];
+    if (active) {
+        const float yd = y_det ? y_det[ia] : det_coord(ia, g.na, g.dy_det);
+        float du;
+        AxRay r[K];
+        r[0] = ax_ray_setup(g, yd, z_det ? z_det[ib0] : det_coord(ib0, g.nb, g.dz_det), du);
+#pragma unroll
+        for (int k = 1; k < K; k++) {
+            const int ib = min(ib0 + k, g.nb - 1);
+            r[k] = r[0];
+            ax_ray_axial(r[k], du, z_det ? z_det[ib] : det_coord(ib, g.nb, g.dz_det));
+        }
+        const int nu = r[0].xdriven ? g.nx : g.ny;
+        for (int iu0 = lane; iu0 < nu; iu0 += U * LPR) {
+            SlabW s[U][K];
+            float xv[U][K][3];
+#pragma unroll
+            for (int j = 0; j < U; j++) {
+                const int iu = iu0 + j * LPR;
+                if (iu < nu) {
+                    const SlabT t = ax_slab_t(r[0], iu);
+#pragma unroll
+                    for (int k = 0; k < K; k++) ax_slab_z(r[k], t, iu, g.nx, g.ny, s[j][k]);
+                } else {
+#pragma unroll
+                    for (int k = 0; k < K; k++)
+#pragma unroll
+                        for (int p = 0; p < 3; p++) {
+                            s[j][k].ok[p] = false;
+                            s[j][k].idx[p] = 0;
+                            s[j][k].wl[p] = 0.0f;
+                        }
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < U; j++)
+#pragma unroll
+                for (int k = 0; k < K; k++)
+#pragma unroll
+                    for (int p = 0; p < 3; p++) xv[j][k][p] = X[s[j][k].idx[p]];
+            // each row's pieces in slab order
+#pragma unroll
+            for (int j = 0; j < U; j++)
+#pragma unroll
+                for (int k = 0; k < K; k++)
+#pragma unroll
+                    for (int p = 0; p < 3; p++)
+                        if (s[j][k].ok[p]) {
+                            sa[k].add(s[j][k].wl[p] * xv[j][k][p]);
+                            sl[k].add(s[j][k].wl[p]);
+                        }
+        }
+    }
+    using R = typename std::conditional<KAHAN, double, float>::type;
+    R a[K], b[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        a[k] = KAHAN ? (R)sa[k].s - (R)sa[k].c : (R)sa[k].s;
+        b[k] = KAHAN ? (R)sl[k].s - (R)sl[k].c : (R)sl[k].s;
+        if constexpr (WL > 1) {
+            a[k] = group_sum<WL>(a[k]);
+            b[k] = group_sum<WL>(b[k]);
+        }
+    }
+    // the butterfly leaves the group's sum in every lane; lane k of each
+    // wave keeps row k
+    const int wl = threadIdx.x % WL;
+    R am = a[0], bm = b[0];
+#pragma unroll
+    for (int k = 1; k < K; k++) {
+        am = wl == k ? a[k] : am;
+        bm = wl == k ? b[k] : bm;
+    }
+    if constexpr (WPR > 1) {
+        __shared__ R sp[RPB * WPR][2 * K];
+        const int w = threadIdx.x / 64;
+        if (wl < K) {
+            sp[w][2 * wl] = am;
+            sp[w][2 * wl + 1] = bm;
+        }
+        __syncthreads();
+        if (lane < K) {
+            const int w0 = (threadIdx.x / LPR) * WPR;
+            am = sp[w0][2 * lane];
+            bm = sp[w0][2 * lane + 1];
+#pragma unroll
+            for (int j = 1; j < WPR; j++) {
+                am += sp[w0 + j][2 * lane];
+                bm += sp[w0 + j][2 * lane + 1];
+            }
+        }
+    }
+    if (mine && !stop) epi(ray, (float)am, (float)bm, pre);
+}
+
 template <bool KAHAN = false, class Epi>
-int dispatch_ax(int lpr, const ViewGeom &g, const float *x, const float *y_det,
+int dispatch_ax(AxShape sh, const ViewGeom &g, const float *x, const float *y_det,
                 const float *z_det, const Epi &epi, hipStream_t s)
 {
+    if (sh.rows == 2) {
+        const long nbund = (long)g.na * ((g.nb + 1) / 2);
+        const unsigned nblk = (unsigned)((nbund * sh.lpr + 255) / 256);
+        if (sh.lpr == 8)
+            klaunch(ax_rows_kernel<8, 2, 2, KAHAN, Epi>, nblk, 256, 0, s, g, x, y_det, z_det, epi);
+        else {
+            set_error(ASD_ERR_ARG, "no two-row forward projector for this shape");
+            return ASD_ERR_ARG;
+        }
+        ASD_LAUNCH_CHECK("ax_rows_kernel");
+        return ASD_OK;
+    }
+    const int lpr = sh.lpr;
     long nrays = (long)g.na * g.nb;
     auto blocks = [&](int l) { return (unsigned)((nrays * l + 255) / 256); };
     switch (lpr) {

@@ -52,15 +52,42 @@ __global__ __launch_bounds__(256) void ray_table_kernel(ViewGeom g,
 
 int ax_lanes_per_ray(int na, int nb, int nu)
 {
-    // Aim for about 2^20 working lanes (4096 waves over 256 CUs) while keeping
-    // at least 2 slabs per lane; a 2-D view (a few hundred rays) gets up to
-    // four waves per ray.  (A 512-slab ray at 128 lanes puts 1.5 blocks of a
-    // 768-ray view on each CU, which leaves half the SIMDs with one wave
-    // more than the rest; at 256 lanes every SIMD gets three.)
+    // At least 8 lanes per ray, and more until about 2^20 lanes work (4096
+    // waves over 256 CUs), while keeping at least 2 slabs per lane; a 2-D
+    // view (a few hundred rays) gets up to four waves per ray.  Eight lanes
+    // on consecutive slabs of one ray read neighbouring voxels along the
+    // driving axis and share the ray's setup over a quarter of its slabs;
+    // one lane per ray, which a 1024 x 768 detector would otherwise get,
+    // takes 1.9 times as long for a 512^3 view on MI355X.  (A 512-slab ray
+    // at 128 lanes puts 1.5 blocks of a 768-ray view on each CU, which
+    // leaves half the SIMDs with one wave more than the rest; at 256 lanes
+    // every SIMD gets three.)
     long rays = (long)na * nb;
     int lpr = 1;
-    while (lpr < 256 && rays * lpr < (1L << 20) && nu / (2 * lpr) >= 2) lpr *= 2;
+    while (lpr < 256 && (rays * lpr < (1L << 20) || lpr < 8) && nu / (2 * lpr) >= 2) lpr *= 2;
     return lpr;
+}
+
+AxShape ax_shape(int na, int nb, int nu)
+{
+    AxShape s;
+    s.lpr = ax_lanes_per_ray(na, nb, nu);
+    // Two-row bundles save the second ray's transverse slab arithmetic
+    // (about 15% of the instructions) but halve the number of waves; at 8
+    // lanes they pay when the view still has at least 16 waves per SIMD (a
+    // 512^3 view with a 1024 x 768 detector: 10% faster).  With fewer rays
+    // the lane count would rise to 16, which loses more than the bundle
+    // saves (a 256^3 view with a 384 x 384 detector: 131 us against 108 us
+    // for single rays at 8 lanes).
+    if (nb >= 2) {
+        const int nbb = (nb + 1) / 2;
+        const int l = ax_lanes_per_ray(na, nbb, nu);
+        if (l == 8 && (long)na * nbb * l / 64 >= 16L * 1024) {
+            s.lpr = l;
+            s.rows = 2;
+        }
+    }
+    return s;
 }
 
 int launch_ray_table(const ViewGeom &g, const float *y_det, const float *z_det, RayRec *rt,
@@ -76,9 +103,9 @@ int launch_ray_table(const ViewGeom &g, const float *y_det, const float *z_det, 
 int launch_ax(const ViewGeom &g, const float *x, float *y, float *sr, const float *y_det,
               const float *z_det, hipStream_t s)
 {
-    int nu_max = g.nx > g.ny ? g.nx : g.ny;
-    int lpr = ax_lanes_per_ray(g.na, g.nb, nu_max);
-    return dispatch_ax(lpr, g, x, y_det, z_det, EpiPlain{y, sr, g.scale}, s);
+    const int nu_max = g.nx > g.ny ? g.nx : g.ny;
+    return dispatch_ax(ax_shape(g.na, g.nb, nu_max), g, x, y_det, z_det,
+                       EpiPlain{y, sr, g.scale}, s);
 }
 
 int launch_atx(const ViewGeom &g, float *x, const float *y, float *sc, const float *y_det,

@@ -319,10 +319,13 @@ class Plan:
                 "atx_us": {n: float(t) for n, t in zip(names, us) if t >= 0}}
 
     def info(self):
+        """(lanes per ray or bundle, detector rows per bundle, hipStream_t)
+        of the forward projector."""
         lpr = C.c_int()
+        rows = C.c_int()
         st = C.c_void_p()
-        nat.check(self._L.asd_plan_info(self.h, C.byref(lpr), C.byref(st)))
-        return lpr.value, st.value
+        nat.check(self._L.asd_plan_info(self.h, C.byref(lpr), C.byref(rows), C.byref(st)))
+        return lpr.value, rows.value, st.value
 
     def close(self):
         if getattr(self, "h", None):

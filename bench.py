@@ -407,10 +407,11 @@ def main():
     ms_d, cnt_d = stage[dom]
     achieved = nbytes[dom] / (ms_d / cnt_d * 1e-3) / 1e9 if cnt_d else 0.0
     tuning = plan.tuning()
-    lpr = plan.info()[0]
+    lpr, rows, _ = plan.info()
     kernel = None
     if args.precision == 0:
-        kernel = {0: f"ax_kernel<{lpr}, false, EpiSartAx<SinoF32> >",
+        kernel = {0: (f"ax_rows_kernel<{lpr}, {rows}, 2, false, EpiSartAx<SinoF32> >" if rows > 1
+                      else f"ax_kernel<{lpr}, false, EpiSartAx<SinoF32> >"),
                   1: ATX_KERNELS.get(tuning["atx_variant"])}.get(dom)
     traffic = profiled_traffic(args.workload, kernel) if kernel else None
     roof = {"bound": "hbm", "kernel": kernel or names[dom], "achieved": achieved,
@@ -446,6 +447,7 @@ def main():
         "ms_per_step_eager_with_events": dt_ev / prof_steps * 1e3,
         "rmse_final": float(rmse[-1]) if len(rmse) else None,
         "ax_lanes_per_ray": lpr,
+        "ax_rows_per_bundle": rows,
         "atx_tuning": tuning,
     }
     if args.precision and not sim:

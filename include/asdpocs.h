@@ -190,8 +190,10 @@ int asd_plan_tuning(asd_plan *plan, int *atx_variant, double *atx_us);
  * slices need nz > 1) or, with -1, asks for a new measurement on the next
  * sweep. */
 int asd_plan_set_atx_variant(asd_plan *plan, int atx_variant);
-/* Lanes per ray of the forward projector and the plan's hipStream_t. */
-int asd_plan_info(asd_plan *plan, int *lanes_per_ray, void **stream);
+/* Launch shape of the forward projector (lanes per ray, or per bundle of
+ * rows_per_bundle detector rows that share a column) and the plan's
+ * hipStream_t.  Any pointer may be NULL. */
+int asd_plan_info(asd_plan *plan, int *lanes_per_ray, int *rows_per_bundle, void **stream);
 void asd_plan_destroy(asd_plan *plan);
 
 /* ---------------------------------------------------------------------- */

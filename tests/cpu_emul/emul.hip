@@ -4,6 +4,7 @@
 // __host__ __device__: this file runs the exact functions the GPU kernels
 // use, in the kernels' loop order, on the CPU, so tests can compare them with
 // the oracle without a GPU.
+#include <algorithm>
 #include <cstddef>
 
 #include "geom_device.hpp"
@@ -76,60 +77,108 @@ long emul_footprint_pairs(float c, float s, const float *ydet, float SO, float O
 // det_coord, the detector grid the forward projector computes in place.
 float emul_det_coord(int i, int n, float det) { return det_coord(i, n, det); }
 
-// ax_kernel<LPR=lpr>: lane j of a ray takes slabs j, j+lpr, ...; lanes are
-// combined in the order of group_sum (quad swaps, half-row and row mirrors,
-// then the 16- and 32-lane swaps, within groups of at most 64 lanes), then
-// the wave partials of a ray in wave order.
-// Compensated lanes are combined in FP64 as on the device.
+}  // extern "C"
+
+// Lane partials of one ray combined in the order of group_sum (quad swaps,
+// half-row and row mirrors, then the 16- and 32-lane swaps, within groups of
+// at most 64 lanes), then the wave partials in wave order; compensated
+// partials in FP64 as on the device.
+static void combine_lanes(double *acc, double *accl, int lpr, int kahan, double &a, double &l)
+{
+    const int wl = lpr < 64 ? lpr : 64;
+    for (int w = 2; w <= wl; w *= 2) {
+        double na_[256], nl_[256];
+        for (int lane = 0; lane < lpr; lane++) {
+            const int o = w == 8    ? (lane & ~7) | (7 - (lane & 7))
+                          : w == 16 ? (lane & ~15) | (15 - (lane & 15))
+                                    : lane ^ (w / 2);
+            na_[lane] = kahan ? acc[lane] + acc[o] : (double)((float)acc[lane] + (float)acc[o]);
+            nl_[lane] = kahan ? accl[lane] + accl[o] : (double)((float)accl[lane] + (float)accl[o]);
+        }
+        for (int lane = 0; lane < lpr; lane++) {
+            acc[lane] = na_[lane];
+            accl[lane] = nl_[lane];
+        }
+    }
+    a = acc[0];
+    l = accl[0];
+    for (int w = 1; w * 64 < lpr; w++) {
+        a = kahan ? a + acc[w * 64] : (double)((float)a + (float)acc[w * 64]);
+        l = kahan ? l + accl[w * 64] : (double)((float)l + (float)accl[w * 64]);
+    }
+}
+
+template <bool KAHAN>
+static void lane_partial(const Acc<KAHAN> &a, const Acc<KAHAN> &l, double &pa, double &pl)
+{
+    pa = KAHAN ? (double)a.s - (double)a.c : (double)a.s;
+    pl = KAHAN ? (double)l.s - (double)l.c : (double)l.s;
+}
+
+// rows == 1: ax_kernel<LPR=lpr>, lane j of a ray takes slabs j, j+lpr, ...
+// rows > 1: ax_rows_kernel<LPR=lpr, K=rows>, the same lanes over bundles of
+// rows detector rows of one column, each slab's transverse half shared.
+template <bool KAHAN>
+static void emul_ax_t(const float *X, float *y, float *sr, const ViewGeom &g, const float *ydet,
+                      const float *zdet, float scale, int lpr, int rows)
+{
+    const int nx = g.nx, ny = g.ny, na = g.na, nb = g.nb;
+    static double acc[8][256], accl[8][256];
+    for (int ib0 = 0; ib0 < nb; ib0 += rows)
+        for (int ia = 0; ia < na; ia++) {
+            float du;
+            AxRay r[8];
+            r[0] = ax_ray_setup(g, ydet[ia], zdet[ib0], du);
+            for (int k = 1; k < rows; k++) {
+                r[k] = r[0];
+                ax_ray_axial(r[k], du, zdet[std::min(ib0 + k, nb - 1)]);
+            }
+            if (g_general)
+                for (int k = 0; k < rows; k++) r[k].flat = r[k].dz1 = 0;
+            const int nu = r[0].xdriven ? nx : ny;
+            for (int lane = 0; lane < lpr; lane++) {
+                Acc<KAHAN> a[8], l[8];
+                if (rows == 1) {
+                    for (int iu0 = lane; iu0 < nu; iu0 += 4 * lpr)
+                        ax_slabs<4, KAHAN>(r[0], X, iu0, lpr, nu, nx, ny, a[0], l[0]);
+                } else {
+                    for (int iu = lane; iu < nu; iu += lpr) {
+                        const SlabT t = ax_slab_t(r[0], iu);
+                        for (int k = 0; k < rows; k++) {
+                            SlabW s;
+                            ax_slab_z(r[k], t, iu, nx, ny, s);
+                            for (int p = 0; p < 3; p++)
+                                if (s.ok[p]) {
+                                    a[k].add(s.wl[p] * X[s.idx[p]]);
+                                    l[k].add(s.wl[p]);
+                                }
+                        }
+                    }
+                }
+                for (int k = 0; k < rows; k++) lane_partial(a[k], l[k], acc[k][lane], accl[k][lane]);
+            }
+            for (int k = 0; k < rows && ib0 + k < nb; k++) {
+                double sa, sl;
+                combine_lanes(acc[k], accl[k], lpr, KAHAN, sa, sl);
+                const long id = (long)(ib0 + k) * na + ia;
+                y[id] = (float)sa * scale;
+                sr[id] = (float)sl * scale;
+            }
+        }
+}
+
+extern "C" {
+
 void emul_ax(const float *X, float *y, float *sr, float c, float s, const float *ydet,
              const float *zdet, float SO, float OD, float scale, float dz, int nx, int ny, int nz,
-             int na, int nb, int lpr, int kahan)
+             int na, int nb, int lpr, int kahan, int rows)
 {
     ViewGeom g = mk(c, s, SO, OD, scale, 1.0f, 1.0f, dz, nx, ny, nz, na, nb);
-    double acc[256], accl[256];
-    const int wl = lpr < 64 ? lpr : 64;
-    for (long id = 0; id < (long)na * nb; id++) {
-        int ia = (int)(id % na), ib = (int)(id / na);
-        AxRay r = ray_setup(g, ydet[ia], zdet[ib]);
-        const int nu = r.xdriven ? nx : ny;
-        for (int lane = 0; lane < lpr; lane++) {
-            if (kahan) {
-                Acc<true> a, l;
-                for (int iu0 = lane; iu0 < nu; iu0 += 4 * lpr)
-                    ax_slabs<4, true>(r, X, iu0, lpr, nu, nx, ny, a, l);
-                acc[lane] = (double)a.s - (double)a.c;
-                accl[lane] = (double)l.s - (double)l.c;
-            } else {
-                Acc<false> a, l;
-                for (int iu0 = lane; iu0 < nu; iu0 += 4 * lpr)
-                    ax_slabs<4, false>(r, X, iu0, lpr, nu, nx, ny, a, l);
-                acc[lane] = a.s;
-                accl[lane] = l.s;
-            }
-        }
-        for (int w = 2; w <= wl; w *= 2) {
-            double na_[256], nl_[256];
-            for (int lane = 0; lane < lpr; lane++) {
-                const int o = w == 8    ? (lane & ~7) | (7 - (lane & 7))
-                              : w == 16 ? (lane & ~15) | (15 - (lane & 15))
-                                        : lane ^ (w / 2);
-                na_[lane] = kahan ? acc[lane] + acc[o] : (double)((float)acc[lane] + (float)acc[o]);
-                nl_[lane] = kahan ? accl[lane] + accl[o]
-                                  : (double)((float)accl[lane] + (float)accl[o]);
-            }
-            for (int lane = 0; lane < lpr; lane++) {
-                acc[lane] = na_[lane];
-                accl[lane] = nl_[lane];
-            }
-        }
-        double a = acc[0], l = accl[0];
-        for (int w = 1; w * 64 < lpr; w++) {
-            a = kahan ? a + acc[w * 64] : (double)((float)a + (float)acc[w * 64]);
-            l = kahan ? l + accl[w * 64] : (double)((float)l + (float)accl[w * 64]);
-        }
-        y[id] = (float)a * scale;
-        sr[id] = (float)l * scale;
-    }
+    if (rows < 1 || rows > 8 || lpr < 1 || lpr > 256) return;
+    if (kahan)
+        emul_ax_t<true>(X, y, sr, g, ydet, zdet, scale, lpr, rows);
+    else
+        emul_ax_t<false>(X, y, sr, g, ydet, zdet, scale, lpr, rows);
 }
 
 // atx_kernel: one voxel at a time, footprint ib outer / ia inner.

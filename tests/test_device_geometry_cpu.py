@@ -21,7 +21,7 @@ def emul():
             build.build()
         _lib = C.CDLL(path)
         f, i, vp = C.c_float, C.c_int, C.c_void_p
-        _lib.emul_ax.argtypes = [vp, vp, vp, f, f, vp, vp, f, f, f, f, i, i, i, i, i, i, i]
+        _lib.emul_ax.argtypes = [vp, vp, vp, f, f, vp, vp, f, f, f, f, i, i, i, i, i, i, i, i]
         _lib.emul_atx.argtypes = [vp, vp, vp, f, f, vp, vp, f, f, f, f, f, f, i, i, i, i, i]
     return _lib
 
@@ -52,12 +52,20 @@ def test_device_projectors_match_oracle(orc, case, phi):
     for lpr, kahan in [(1, 0), (8, 0), (64, 0), (128, 0), (256, 0), (64, 1), (256, 1)]:
         y = np.zeros(na * nb, np.float32)
         sr = np.zeros(na * nb, np.float32)
-        emul().emul_ax(P(X), P(y), P(sr), *args, np.float32(1), nx, ny, nz, na, nb, lpr, kahan)
+        emul().emul_ax(P(X), P(y), P(sr), *args, np.float32(1), nx, ny, nz, na, nb, lpr, kahan, 1)
         if lpr == 1 and not kahan:
             assert np.array_equal(y, yo) and np.array_equal(sr, sro)
         else:
             assert np.abs(y - yo).max() <= 2e-6 * np.abs(yo).max()
             assert np.abs(sr - sro).max() <= 2e-6 * np.abs(sro).max()
+        # bundles of detector rows sharing each slab's transverse half sum
+        # the same products in the same order as single rays
+        for rows in (2, 3):
+            y2 = np.zeros_like(y)
+            sr2 = np.zeros_like(sr)
+            emul().emul_ax(P(X), P(y2), P(sr2), *args, np.float32(1), nx, ny, nz, na, nb, lpr,
+                           kahan, rows)
+            assert np.array_equal(y2, y) and np.array_equal(sr2, sr), (lpr, kahan, rows)
     x = np.zeros(nx * ny * nz, np.float32)
     sc = np.zeros_like(x)
     emul().emul_atx(P(x), P(Y), P(sc), *args, g["det"], g["det"], np.float32(1), nx, ny, nz, na, nb)
@@ -182,7 +190,7 @@ def test_flat_and_unit_pitch_shortcuts_change_no_bit():
             sr = np.zeros_like(y)
             lib.emul_ax(P(X), P(y), P(sr), g["cos_phi"], g["sin_phi"], P(g["y_det"]),
                         P(g["z_det"]), g["SO"], g["OD"], g["scale"], np.float32(1), nx, ny, nz,
-                        na, nb, 1, 0)
+                        na, nb, 1, 0, 1 + general)
             x = np.zeros(nx * ny * nz, np.float32)
             sc = np.zeros_like(x)
             lib.emul_atx(P(x), P(Y), P(sc), g["cos_phi"], g["sin_phi"], P(g["y_det"]),

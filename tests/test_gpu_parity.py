@@ -70,6 +70,44 @@ def test_forward_projector_slab_arithmetic_is_bit_identical(orc, nz, nb, phi):
     assert np.array_equal(y, yo), np.abs(y - yo).max()
 
 
+def test_two_row_bundles_match_single_rays_and_oracle(orc):
+    """A 512 x 512 detector gets the two-row forward projector (8 lanes on a
+    bundle of two detector rows of one column, each slab's transverse half
+    shared).  Each ray's slab products and their lane order are the
+    single-ray kernel's, so the result equals the device functions' host run
+    at 8 lanes per ray bit for bit, and the oracle to summation rounding; a
+    SART sweep through a plan of that shape matches the oracle's sweep."""
+    from tests.test_device_geometry_cpu import emul, P as ptr
+    nx, ny, nz, na, nb = 32, 32, 24, 512, 512
+    base = geom(nx, ny, nz, na, nb, 500.0, 500.0, 1.0, 0.15, 0.0)
+    pl = ops.Plan(base, [0.0, 90.0])
+    assert pl.info()[:2] == (8, 2)
+    pl.close()
+    rng = np.random.default_rng(5)
+    X = rng.random(nx * ny * nz).astype(np.float32)
+    for phi in (-0.4, -2.0):
+        para = dict(base, phi=phi)
+        y, sr = ops.Ax_cone_gpu(X, para)
+        g = orc.gateway_geometry(phi, 500.0, 500.0, 1.0, 0.15, na, nb)
+        ye = np.zeros(na * nb, np.float32)
+        sre = np.zeros_like(ye)
+        emul().emul_ax(ptr(X), ptr(ye), ptr(sre), g["cos_phi"], g["sin_phi"], ptr(g["y_det"]),
+                       ptr(g["z_det"]), g["SO"], g["OD"], g["scale"], np.float32(1), nx, ny, nz,
+                       na, nb, 8, 0, 1)
+        assert np.array_equal(y, ye) and np.array_equal(sr, sre), phi
+        yo, sro = orc.ax_para(X, para)
+        assert rel_l2(y, yo) <= 1e-5 and rel_l2(sr, sro) <= 1e-5, phi
+    phi_vec = np.array([10.0, 100.0, 190.0, 280.0])
+    img = phantoms.shepp_logan_3d(nx, nz=nz).astype(np.float32)
+    P = np.stack([ops.Ax_cone_gpu(img, dict(base, phi=-t / 180 * np.pi))[0] for t in phi_vec],
+                 axis=1).astype(np.float64)
+    para = dict(base, phi_vec=phi_vec)
+    m, out = ops.SART_cuda_inline(P, para)
+    mo, outo = orc.sart(P, para)
+    assert rel_l2(m, mo) <= 1e-5, rel_l2(m, mo)
+    assert abs(out["dataTolerance"] - outo["dataTolerance"]) <= 1e-5 * outo["dataTolerance"]
+
+
 @pytest.mark.parametrize("name,para", CASES, ids=[c[0] for c in CASES])
 def test_atx_matches_oracle(orc, name, para):
     rng = np.random.default_rng(2)
