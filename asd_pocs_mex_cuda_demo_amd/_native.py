@@ -23,6 +23,7 @@ EXPORTS = [
     "asd_plan_reset", "asd_plan_iterate", "asd_plan_get_image", "asd_plan_get_out",
     "asd_plan_sweep", "asd_plan_get_current", "asd_plan_sync", "asd_plan_set_timing",
     "asd_plan_get_timing", "asd_plan_set_graph", "asd_plan_tuning", "asd_plan_set_atx_variant",
+    "asd_plan_set_ax_shape",
     "asd_plan_info", "asd_plan_destroy", "asd_plan_simulate", "asd_plan_get_projections",
     "asd_plan_get_reference",
     "asd_batch_create", "asd_batch_set_case", "asd_batch_iterate", "asd_batch_get_image",
@@ -104,6 +105,7 @@ def lib():
                        ("asd_plan_set_graph", [vp, i]),
                        ("asd_plan_tuning", [vp, C.POINTER(C.c_int), vp]),
                        ("asd_plan_set_atx_variant", [vp, i]),
+                       ("asd_plan_set_ax_shape", [vp, i, i]),
                        ("asd_plan_simulate", [vp, vp, i, d, d, C.c_ulonglong]),
                        ("asd_plan_get_projections", [vp, vp]),
                        ("asd_plan_get_reference", [vp, vp]),

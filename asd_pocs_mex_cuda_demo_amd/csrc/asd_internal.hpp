@@ -143,7 +143,7 @@ int ax_lanes_per_ray(int na, int nb, int nu);
 
 // Launch shape of the forward projector for one view: lanes per ray (per
 // bundle when rows > 1) and detector rows per bundle (1: ax_kernel, one ray
-// per lane group; 2: ax_rows_kernel).  The shape changes only the order in
+// per lane group; 2 or 3: ax_rows_kernel at 8 lanes).  The shape changes only the order in
 // which a ray's slab products are summed (and so the last bits of the sum),
 // never which products there are.
 struct AxShape {

@@ -117,9 +117,18 @@ __global__ __launch_bounds__(256) void ax_batch_kernel(const BItem *__restrict__
         const unsigned ub = fastdiv((unsigned)id, g.div_na);
         const int ia = (int)((unsigned)id - mad24(ub, (unsigned)g.na, 0u)), ib = (int)ub;
         AxRay r = ax_ray_setup(g, det_coord(ia, g.na, g.dy_det), det_coord(ib, g.nb, g.dz_det));
+        // every case passes vol_buffer_ok (asd_batch_create)
+        const VolRead<false> V(it.m, (size_t)g.nx * g.ny * g.nz);
+        const SlabAddr A = slab_addr(r, g.nx, VolRead<false>::unit);
         int nu = r.xdriven ? g.nx : g.ny;
-        for (int iu0 = lane; iu0 < nu; iu0 += 4 * LPR)
-            ax_slabs<4, KAHAN>(r, it.m, iu0, LPR, nu, g.nx, g.ny, sa, sl);
+        auto march = [&](auto ft) {
+            for (int iu0 = lane; iu0 < nu; iu0 += 4 * LPR)
+                ax_slabs<4, KAHAN, decltype(ft)::value>(r, A, V, iu0, LPR, nu, g.ny, sa, sl);
+        };
+        if (r.fast_t)  // a per-case constant: one branch for the march
+            march(std::true_type{});
+        else
+            march(std::false_type{});
     }
     float acc, accl;
     if constexpr (LPR > 1 && KAHAN) {
@@ -521,6 +530,8 @@ int asd_batch_create(int ncases, const asd_para *paras, const double *const *phi
         ASD_ARG(p.nx > 0 && p.ny > 0 && p.nz > 0 && p.na > 0 && p.nb > 0,
                 "every case needs positive nx, ny, nz, na, nb");
         ASD_ARG(index_fits(p.nx, p.ny, p.nz, p.na, p.nb), "case " ASD_INDEX_LIMITS);
+        ASD_ARG(vol_buffer_ok((size_t)p.nx * p.ny * p.nz, p.nx),
+                "batched cases need fewer than 2^30 voxels and nx below 2^22");
         ASD_ARG(nvs[c] > 0 && phi_vecs[c], "every case needs views");
         ASD_ARG(p.SO > 0 && p.OD > 0 && p.vxlsize > 0 && p.detsize > 0, "bad case geometry");
         ASD_ARG(p.precision == paras[0].precision, "all cases must share para.precision");

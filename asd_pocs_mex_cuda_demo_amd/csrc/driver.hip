@@ -870,6 +870,25 @@ int asd_plan_set_atx_variant(asd_plan *p, int atx_variant)
     return ASD_OK;
 }
 
+int asd_plan_set_ax_shape(asd_plan *p, int lanes_per_ray, int rows_per_bundle)
+{
+    clear_error();
+    ASD_ARG(p, "NULL plan");
+    if (lanes_per_ray == 0 && rows_per_bundle == 0) {
+        p->ax = ax_shape(p->para.na, p->para.nb, std::max(p->para.nx, p->para.ny));
+    } else {
+        const int l = lanes_per_ray, r = rows_per_bundle;
+        ASD_ARG((r == 1 && l >= 1 && l <= 256 && (l & (l - 1)) == 0) ||
+                    ((r == 2 || r == 3) && l == 8),
+                "ax shape must be (1..256 lanes, a power of two; 1 row) or (8 lanes; 2 or 3 "
+                "rows), or (0, 0) for the default");
+        p->ax.lpr = l;
+        p->ax.rows = r;
+    }
+    drop_graph(p);
+    return ASD_OK;
+}
+
 int asd_plan_tuning(asd_plan *p, int *atx_variant, double *atx_us)
 {
     clear_error();

@@ -239,20 +239,103 @@ struct Acc {
     __host__ __device__ __forceinline__ float value() const { return KAHAN ? s - c : s; }
 };
 
-// The (up to) three pieces of slab iu of ray r: voxel index, weight w*l and
-// whether the piece lies in the volume.  Out-of-volume pieces get index 0 (a
-// valid address) so the caller can issue every load before it accumulates.
+// Volume reads of the forward projector.  A piece is addressed in units of
+// VolRead::unit bytes: (cz * ny + y) * nx + x scaled by the unit, split as
+// z * (ny * nx * unit) + t * st + iu * su (x or y is the slab index iu along
+// the driving axis and the other the transverse cell t).  Pieces outside the
+// volume (ok false) carry an arbitrary address:
+//  - buffer form (volume below 2^30 voxels, nx below 2^22): byte offsets
+//    through a buffer resource whose range is the volume; the hardware
+//    returns 0 for an offset past the range and a value inside it is read
+//    but not used, so no address is selected or cleared;
+//  - wide form: element indices through a 64-bit pointer, an unused piece
+//    reading element 0.
+// On the host both read the element only for a piece in the volume.
+template <bool WIDE>
+struct VolRead;
+
+template <>
+struct VolRead<false> {
+    static constexpr unsigned unit = 4;
+    __amdgpu_buffer_rsrc_t rsrc;
+    const float *p;
+    __host__ __device__ VolRead(const float *x, size_t n) : p(x)
+    {
+#if defined(__HIP_DEVICE_COMPILE__)
+        // dword3 0x00020000: 32-bit data format, the form untyped loads use
+        rsrc = __builtin_amdgcn_make_buffer_rsrc((void *)x, (short)0, (int)(unsigned)(n * 4),
+                                                 0x00020000);
+#endif
+    }
+    __host__ __device__ __forceinline__ float operator()(unsigned off, bool ok) const
+    {
+#if defined(__HIP_DEVICE_COMPILE__)
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rsrc, off, 0, 0));
+#else
+        return ok ? p[off / 4] : 0.0f;
+#endif
+    }
+};
+
+template <>
+struct VolRead<true> {
+    static constexpr unsigned unit = 1;
+    const float *p;
+    __host__ __device__ VolRead(const float *x, size_t) : p(x) {}
+    __host__ __device__ __forceinline__ float operator()(unsigned off, bool ok) const
+    {
+        return p[ok ? off : 0u];
+    }
+};
+
+// An address past any volume's buffer range: the hardware returns 0 for it
+// without reading memory.  Slabs wholly outside the volume (a large share of
+// a fan's slabs) use it so their loads fetch nothing.
+constexpr unsigned VOL_NONE = 0xffffffffu;
+
+// Whether a volume of n voxels with rows of nx can use the buffer form.
+__host__ __device__ __forceinline__ bool vol_buffer_ok(size_t n, int nx)
+{
+    return n < (size_t(1) << 30) && nx < (1 << 22);
+}
+
+// Address strides of a ray's pieces in units of unit bytes: st per
+// transverse cell, su per slab, zs per slice (zs = nxu * ny with
+// nxu = nx * unit, formed as two 24-bit products per piece).
+struct SlabAddr {
+    unsigned st, su, nxu;
+};
+
+__host__ __device__ __forceinline__ SlabAddr slab_addr(const AxRay &r, int nx, unsigned unit)
+{
+    const unsigned nxu = (unsigned)nx * unit;
+    return SlabAddr{r.xdriven ? nxu : unit, r.xdriven ? unit : nxu, nxu};
+}
+
+// The (up to) three pieces of slab iu of ray r: address (see VolRead), weight
+// w*l and whether the piece lies in the volume.  The address and weight of a
+// piece outside the volume are unspecified.
 struct SlabW {
-    unsigned idx[3];
+    unsigned off[3];
     float wl[3];
     bool ok[3];
 };
 
+// An unspecified value (no instruction): what an unused piece carries.
+template <class T>
+__host__ __device__ __forceinline__ T unspecified(T x)
+{
+    return __builtin_nondeterministic_value(x);
+}
+
 // The transverse half of slab iu of a ray: its sorted transverse extent
-// [ta, tb], cells and crossing fraction.  It depends only on the ray's
-// in-plane line, which every ray of one detector column shares (rays that
-// differ only in their detector row), so ax_rows_kernel computes it once for
-// a bundle of rows.
+// [ta, tb], cells, crossing fraction and the transverse part of its pieces'
+// addresses.  It depends only on the ray's in-plane line, which every ray of
+// one detector column shares (rays that differ only in their detector row),
+// so ax_rows_kernel computes it once for a bundle of rows.  FT: fast_t is
+// known to hold (the caller has branched on it once for the whole march), so
+// the crossing fraction is formed only by div_cell; otherwise both forms are
+// evaluated and one is selected.
 struct SlabT {
     float ua, ub;  // the slab's bounds along the driving axis
     float ta, tb;
@@ -260,9 +343,11 @@ struct SlabT {
     float rt;      // transverse crossing fraction (1 without a crossing)
     int ct1, ct2;
     bool in_t1, in_t2;
+    unsigned tp1, tp2;  // ct * st + iu * su (meaningful for cells in the volume)
 };
 
-__host__ __device__ __forceinline__ SlabT ax_slab_t(const AxRay &r, int iu)
+template <bool FT = false>
+__host__ __device__ __forceinline__ SlabT ax_slab_t(const AxRay &r, const SlabAddr &A, int iu)
 {
 #pragma clang fp contract(off)
     SlabT t;
@@ -283,20 +368,22 @@ __host__ __device__ __forceinline__ SlabT ax_slab_t(const AxRay &r, int iu)
     t.d2p1 = d * d + 1;
     t.rt = 1.0f;
     if (t.ct1 != t.ct2 && (t.in_t1 || t.in_t2))
-        t.rt = r.fast_t ? div_cell((float)t.ct2 - t.ta, t.tb - t.ta)
-                        : ((float)t.ct2 - t.ta) / (t.tb - t.ta);
+        t.rt = (FT || r.fast_t) ? div_cell((float)t.ct2 - t.ta, t.tb - t.ta)
+                                : ((float)t.ct2 - t.ta) / (t.tb - t.ta);
+    // 24-bit products (index_fits, vol_buffer_ok); exact for cells in the
+    // volume, whose addresses lie below 2^32
+    const unsigned ub = mad24((unsigned)iu, A.su, 0u);
+    t.tp1 = mad24((unsigned)t.ct1, A.st, ub);
+    t.tp2 = mad24((unsigned)t.ct2, A.st, ub);
     return t;
 }
 
 // The rest of slab iu of ray r given its transverse half: the axial extent
 // and cells, the slab length and the three pieces.
-__host__ __device__ __forceinline__ void ax_slab_z(const AxRay &r, const SlabT &t, int iu,
-                                                   int nx, int ny, SlabW &s)
+__host__ __device__ __forceinline__ void ax_slab_z(const AxRay &r, const SlabT &t,
+                                                   const SlabAddr &A, int ny, SlabW &s)
 {
 #pragma clang fp contract(off)
-    s.ok[0] = s.ok[1] = s.ok[2] = false;
-    s.idx[0] = s.idx[1] = s.idx[2] = 0;
-    s.wl[0] = s.wl[1] = s.wl[2] = 0.0f;
     int cz1 = 0, cz2 = 0;
     float za = 0.0f, zb = 0.0f;
     if (!r.flat) {
@@ -320,45 +407,48 @@ __host__ __device__ __forceinline__ void ax_slab_z(const AxRay &r, const SlabT &
         cz2 = (int)floorf(zb);
     }
     const bool in_z1 = cz1 >= 0 && cz1 < r.nz, in_z2 = cz2 >= 0 && cz2 < r.nz;
-    if (!((t.in_t1 || t.in_t2) && (in_z1 || in_z2))) return;
+    s.ok[0] = t.in_t1 && in_z1;
+    s.ok[2] = t.in_t2 && in_z2;
+    // a slab wholly outside the volume has no piece in it
+    if (!((t.in_t1 || t.in_t2) && (in_z1 || in_z2))) {
+        s.ok[1] = false;
+        for (int p = 0; p < 3; p++) {
+            s.off[p] = VOL_NONE;
+            s.wl[p] = unspecified(0.0f);
+        }
+        return;
+    }
     float rz = 1.0f;
     if (cz1 != cz2) rz = r.fast_z ? div_cell((float)cz2 - za, zb - za) : ((float)cz2 - za) / (zb - za);
+    const bool tfirst = !(t.rt > rz);  // the t crossing comes first (ties: t)
+    s.ok[1] = (tfirst && t.in_t2 && in_z1) || (!tfirst && t.in_t1 && in_z2);
     // (d*d + 1) * q >= q >= tmp, so the root's argument is >= 1
     const float l = sqrt_ge_tiny(div_by_tmp(r, t.d2p1 * r.q));
-    const float rt = t.rt;
-    const bool tfirst = !(rt > rz);  // the t crossing comes first (ties: t)
-    const float lo = tfirst ? rt : rz, hi = tfirst ? rz : rt;
-    const int ctm = tfirst ? t.ct2 : t.ct1, czm = tfirst ? cz1 : cz2;
-    // (z * ny + y) * nx + x in 24-bit multiply-adds (index_fits); pieces out
-    // of the volume get index 0
-    auto index = [&](int ct, int cz) {
-        const unsigned y = r.xdriven ? (unsigned)ct : (unsigned)iu;
-        const unsigned x = r.xdriven ? (unsigned)iu : (unsigned)ct;
-        return mad24(mad24((unsigned)cz, (unsigned)ny, y), (unsigned)nx, x);
-    };
-    s.ok[0] = t.in_t1 && in_z1;
-    // (written as mask logic, which stays in scalar lane masks)
-    s.ok[1] = (tfirst && t.in_t2 && in_z1) || (!tfirst && t.in_t1 && in_z2);
-    s.ok[2] = t.in_t2 && in_z2;
-    s.idx[0] = s.ok[0] ? index(t.ct1, cz1) : 0u;
-    s.idx[1] = s.ok[1] ? index(ctm, czm) : 0u;
-    s.idx[2] = s.ok[2] ? index(t.ct2, cz2) : 0u;
+    const float lo = tfirst ? t.rt : rz, hi = tfirst ? rz : t.rt;
+    // slice parts cz * ny * nx * unit in two 24-bit products (index_fits:
+    // cz * ny < 2^24; vol_buffer_ok or index_fits: nx * unit < 2^24)
+    const unsigned zp1 = mad24(mad24((unsigned)cz1, (unsigned)ny, 0u), A.nxu, 0u);
+    const unsigned zp2 = mad24(mad24((unsigned)cz2, (unsigned)ny, 0u), A.nxu, 0u);
+    s.off[0] = t.tp1 + zp1;
+    s.off[1] = tfirst ? t.tp2 + zp1 : t.tp1 + zp2;
+    s.off[2] = t.tp2 + zp2;
     s.wl[0] = lo * l;
     s.wl[1] = (hi - lo) * l;
     s.wl[2] = (1.0f - hi) * l;
 }
 
-__host__ __device__ __forceinline__ void ax_slab_geom(const AxRay &r, int iu, int nx, int ny,
-                                                      SlabW &s)
+template <bool FT = false>
+__host__ __device__ __forceinline__ void ax_slab_geom(const AxRay &r, const SlabAddr &A, int iu,
+                                                      int ny, SlabW &s)
 {
-    ax_slab_z(r, ax_slab_t(r, iu), iu, nx, ny, s);
+    ax_slab_z(r, ax_slab_t<FT>(r, A, iu), A, ny, s);
 }
 
 // U consecutive slabs of one lane (iu0, iu0 + stride, ...): all geometry
 // first, then every load, then the sums in slab order.
-template <int U, bool KAHAN>
-__host__ __device__ __forceinline__ void ax_slabs(const AxRay &r, const float *__restrict__ X,
-                                                  int iu0, int stride, int nu, int nx, int ny,
+template <int U, bool KAHAN, bool FT, class Vol>
+__host__ __device__ __forceinline__ void ax_slabs(const AxRay &r, const SlabAddr &A, const Vol &V,
+                                                  int iu0, int stride, int nu, int ny,
                                                   Acc<KAHAN> &acc, Acc<KAHAN> &accl)
 {
 #pragma clang fp contract(off)
@@ -368,17 +458,19 @@ __host__ __device__ __forceinline__ void ax_slabs(const AxRay &r, const float *_
     for (int k = 0; k < U; k++) {
         const int iu = iu0 + k * stride;
         if (iu < nu) {
-            ax_slab_geom(r, iu, nx, ny, s[k]);
+            ax_slab_geom<FT>(r, A, iu, ny, s[k]);
         } else {
-            s[k].ok[0] = s[k].ok[1] = s[k].ok[2] = false;
-            s[k].idx[0] = s[k].idx[1] = s[k].idx[2] = 0;
-            s[k].wl[0] = s[k].wl[1] = s[k].wl[2] = 0.0f;
+            for (int p = 0; p < 3; p++) {
+                s[k].ok[p] = false;
+                s[k].off[p] = VOL_NONE;
+                s[k].wl[p] = unspecified(0.0f);
+            }
         }
     }
 #pragma unroll
     for (int k = 0; k < U; k++)
 #pragma unroll
-        for (int p = 0; p < 3; p++) xv[k][p] = X[s[k].idx[p]];
+        for (int p = 0; p < 3; p++) xv[k][p] = V(s[k].off[p], s[k].ok[p]);
 #pragma unroll
     for (int k = 0; k < U; k++)
 #pragma unroll

@@ -22,7 +22,7 @@ namespace asd {
 // like the reference kernel (cu:375, :576); a 2-D view (a few hundred rays)
 // gets up to four waves per ray so it still fills the chip, and lanes
 // walking along x read consecutive voxels.
-template <int LPR, bool KAHAN, class Epi>
+template <int LPR, bool KAHAN, bool WIDE, class Epi>
 __global__ __launch_bounds__(256) void ax_kernel(ViewGeom g, const float *__restrict__ X,
                                                  const float *__restrict__ y_det,
                                                  const float *__restrict__ z_det, Epi epi)
@@ -37,6 +37,7 @@ __global__ __launch_bounds__(256) void ax_kernel(ViewGeom g, const float *__rest
     // so each XCD's L2 holds one wedge of the volume instead of all of it
     const long id = (long)xcd_block(blockIdx.x, gridDim.x) * RPB + threadIdx.x / LPR;
     const bool active = id < nrays;
+    const VolRead<WIDE> V(X, (size_t)g.nx * g.ny * g.nz);
     typename Epi::Pre pre{};
     if (active && lane == 0) pre = epi.load(id);
     Acc<KAHAN> sa, sl;
@@ -48,12 +49,20 @@ __global__ __launch_bounds__(256) void ax_kernel(ViewGeom g, const float *__rest
         const float yd = y_det ? y_det[ia] : det_coord(ia, g.na, g.dy_det);
         const float zd = z_det ? z_det[ib] : det_coord(ib, g.nb, g.dz_det);
         AxRay r = ax_ray_setup(g, yd, zd);
+        const SlabAddr A = slab_addr(r, g.nx, VolRead<WIDE>::unit);
         int nu = r.xdriven ? g.nx : g.ny;
         // slabs per load batch: two when a ray has 256 lanes, else four (a
         // lane then walks its slabs in one or two batches)
         constexpr int U = LPR >= 256 ? 2 : 4;
-        for (int iu0 = lane; iu0 < nu; iu0 += U * LPR)
-            ax_slabs<U, KAHAN>(r, X, iu0, LPR, nu, g.nx, g.ny, sa, sl);
+        auto march = [&](auto ft) {
+            for (int iu0 = lane; iu0 < nu; iu0 += U * LPR)
+                ax_slabs<U, KAHAN, decltype(ft)::value>(r, A, V, iu0, LPR, nu, g.ny, sa, sl);
+        };
+        // fast_t depends on the view only: one branch for the whole march
+        if (r.fast_t)
+            march(std::true_type{});
+        else
+            march(std::false_type{});
     }
     // compensated partials are combined in FP64, plain ones in FP32
     using R = typename std::conditional<KAHAN, double, float>::type;
@@ -96,7 +105,7 @@ __global__ __launch_bounds__(256) void ax_kernel(ViewGeom g, const float *__rest
 // K ray sums are reduced the same way and lane k stores ray k.  Rows past nb
 // repeat the last row and are not stored.  A lane takes U slabs per load
 // batch (iu, iu + LPR, ...), so 3 * U * K loads are in flight together.
-template <int LPR, int K, int U, bool KAHAN, class Epi>
+template <int LPR, int K, int U, bool KAHAN, bool WIDE, class Epi>
 __global__ __launch_bounds__(256) void ax_rows_kernel(ViewGeom g, const float *__restrict__ X,
                                                       const float *__restrict__ y_det,
                                                       const float *__restrict__ z_det, Epi epi)
@@ -116,6 +125,7 @@ __global__ __launch_bounds__(256) void ax_rows_kernel(ViewGeom g, const float *_
     const unsigned ub = active ? fastdiv((unsigned)id, g.div_na) : 0u;
     const int ia = (int)((unsigned)id - mad24(ub, (unsigned)g.na, 0u)), ib0 = (int)ub * K;
     const bool mine = active && lane < K && ib0 + lane < g.nb;
+    const VolRead<WIDE> V(X, (size_t)g.nx * g.ny * g.nz);
     const long ray = (long)(ib0 + lane) * g.na + ia;
     typename Epi::Pre pre{};
     if (mine) pre = epi.load(ray);
@@ -132,45 +142,52 @@ __global__ __launch_bounds__(256) void ax_rows_kernel(ViewGeom g, const float *_
             ax_ray_axial(r[k], du, z_det ? z_det[ib] : det_coord(ib, g.nb, g.dz_det));
         }
         const int nu = r[0].xdriven ? g.nx : g.ny;
-        for (int iu0 = lane; iu0 < nu; iu0 += U * LPR) {
-            SlabW s[U][K];
-            float xv[U][K][3];
+        const SlabAddr A = slab_addr(r[0], g.nx, VolRead<WIDE>::unit);
+        auto march = [&](auto ft) {
+            for (int iu0 = lane; iu0 < nu; iu0 += U * LPR) {
+                SlabW s[U][K];
+                float xv[U][K][3];
 #pragma unroll
-            for (int j = 0; j < U; j++) {
-                const int iu = iu0 + j * LPR;
-                if (iu < nu) {
-                    const SlabT t = ax_slab_t(r[0], iu);
+                for (int j = 0; j < U; j++) {
+                    const int iu = iu0 + j * LPR;
+                    if (iu < nu) {
+                        const SlabT t = ax_slab_t<decltype(ft)::value>(r[0], A, iu);
 #pragma unroll
-                    for (int k = 0; k < K; k++) ax_slab_z(r[k], t, iu, g.nx, g.ny, s[j][k]);
-                } else {
+                        for (int k = 0; k < K; k++) ax_slab_z(r[k], t, A, g.ny, s[j][k]);
+                    } else {
+#pragma unroll
+                        for (int k = 0; k < K; k++)
+#pragma unroll
+                            for (int p = 0; p < 3; p++) {
+                                s[j][k].ok[p] = false;
+                                s[j][k].off[p] = VOL_NONE;
+                                s[j][k].wl[p] = unspecified(0.0f);
+                            }
+                    }
+                }
+#pragma unroll
+                for (int j = 0; j < U; j++)
 #pragma unroll
                     for (int k = 0; k < K; k++)
 #pragma unroll
-                        for (int p = 0; p < 3; p++) {
-                            s[j][k].ok[p] = false;
-                            s[j][k].idx[p] = 0;
-                            s[j][k].wl[p] = 0.0f;
-                        }
-                }
+                        for (int p = 0; p < 3; p++) xv[j][k][p] = V(s[j][k].off[p], s[j][k].ok[p]);
+                // each row's pieces in slab order
+#pragma unroll
+                for (int j = 0; j < U; j++)
+#pragma unroll
+                    for (int k = 0; k < K; k++)
+#pragma unroll
+                        for (int p = 0; p < 3; p++)
+                            if (s[j][k].ok[p]) {
+                                sa[k].add(s[j][k].wl[p] * xv[j][k][p]);
+                                sl[k].add(s[j][k].wl[p]);
+                            }
             }
-#pragma unroll
-            for (int j = 0; j < U; j++)
-#pragma unroll
-                for (int k = 0; k < K; k++)
-#pragma unroll
-                    for (int p = 0; p < 3; p++) xv[j][k][p] = X[s[j][k].idx[p]];
-            // each row's pieces in slab order
-#pragma unroll
-            for (int j = 0; j < U; j++)
-#pragma unroll
-                for (int k = 0; k < K; k++)
-#pragma unroll
-                    for (int p = 0; p < 3; p++)
-                        if (s[j][k].ok[p]) {
-                            sa[k].add(s[j][k].wl[p] * xv[j][k][p]);
-                            sl[k].add(s[j][k].wl[p]);
-                        }
-        }
+        };
+        if (r[0].fast_t)
+            march(std::true_type{});
+        else
+            march(std::false_type{});
     }
     using R = typename std::conditional<KAHAN, double, float>::type;
     R a[K], b[K];
@@ -214,17 +231,19 @@ __global__ __launch_bounds__(256) void ax_rows_kernel(ViewGeom g, const float *_
     if (mine && !stop) epi(ray, (float)am, (float)bm, pre);
 }
 
-template <bool KAHAN = false, class Epi>
-int dispatch_ax(AxShape sh, const ViewGeom &g, const float *x, const float *y_det,
+template <bool KAHAN, bool WIDE, class Epi>
+int dispatch_ax_t(AxShape sh, const ViewGeom &g, const float *x, const float *y_det,
                 const float *z_det, const Epi &epi, hipStream_t s)
 {
-    if (sh.rows == 2) {
-        const long nbund = (long)g.na * ((g.nb + 1) / 2);
+    if (sh.rows > 1) {
+        const long nbund = (long)g.na * ((g.nb + sh.rows - 1) / sh.rows);
         const unsigned nblk = (unsigned)((nbund * sh.lpr + 255) / 256);
-        if (sh.lpr == 8)
-            klaunch(ax_rows_kernel<8, 2, 2, KAHAN, Epi>, nblk, 256, 0, s, g, x, y_det, z_det, epi);
+        if (sh.lpr == 8 && sh.rows == 3)
+            klaunch(ax_rows_kernel<8, 3, 1, KAHAN, WIDE, Epi>, nblk, 256, 0, s, g, x, y_det, z_det, epi);
+        else if (sh.lpr == 8 && sh.rows == 2)
+            klaunch(ax_rows_kernel<8, 2, 2, KAHAN, WIDE, Epi>, nblk, 256, 0, s, g, x, y_det, z_det, epi);
         else {
-            set_error(ASD_ERR_ARG, "no two-row forward projector for this shape");
+            set_error(ASD_ERR_ARG, "no row-bundle forward projector for this shape");
             return ASD_ERR_ARG;
         }
         ASD_LAUNCH_CHECK("ax_rows_kernel");
@@ -234,18 +253,29 @@ int dispatch_ax(AxShape sh, const ViewGeom &g, const float *x, const float *y_de
     long nrays = (long)g.na * g.nb;
     auto blocks = [&](int l) { return (unsigned)((nrays * l + 255) / 256); };
     switch (lpr) {
-    case 1: klaunch(ax_kernel<1, KAHAN, Epi>, blocks(1), 256, 0, s, g, x, y_det, z_det, epi); break;
-    case 2: klaunch(ax_kernel<2, KAHAN, Epi>, blocks(2), 256, 0, s, g, x, y_det, z_det, epi); break;
-    case 4: klaunch(ax_kernel<4, KAHAN, Epi>, blocks(4), 256, 0, s, g, x, y_det, z_det, epi); break;
-    case 8: klaunch(ax_kernel<8, KAHAN, Epi>, blocks(8), 256, 0, s, g, x, y_det, z_det, epi); break;
-    case 16: klaunch(ax_kernel<16, KAHAN, Epi>, blocks(16), 256, 0, s, g, x, y_det, z_det, epi); break;
-    case 32: klaunch(ax_kernel<32, KAHAN, Epi>, blocks(32), 256, 0, s, g, x, y_det, z_det, epi); break;
-    case 64: klaunch(ax_kernel<64, KAHAN, Epi>, blocks(64), 256, 0, s, g, x, y_det, z_det, epi); break;
-    case 128: klaunch(ax_kernel<128, KAHAN, Epi>, blocks(128), 256, 0, s, g, x, y_det, z_det, epi); break;
-    default: klaunch(ax_kernel<256, KAHAN, Epi>, blocks(256), 256, 0, s, g, x, y_det, z_det, epi); break;
+    case 1: klaunch(ax_kernel<1, KAHAN, WIDE, Epi>, blocks(1), 256, 0, s, g, x, y_det, z_det, epi); break;
+    case 2: klaunch(ax_kernel<2, KAHAN, WIDE, Epi>, blocks(2), 256, 0, s, g, x, y_det, z_det, epi); break;
+    case 4: klaunch(ax_kernel<4, KAHAN, WIDE, Epi>, blocks(4), 256, 0, s, g, x, y_det, z_det, epi); break;
+    case 8: klaunch(ax_kernel<8, KAHAN, WIDE, Epi>, blocks(8), 256, 0, s, g, x, y_det, z_det, epi); break;
+    case 16: klaunch(ax_kernel<16, KAHAN, WIDE, Epi>, blocks(16), 256, 0, s, g, x, y_det, z_det, epi); break;
+    case 32: klaunch(ax_kernel<32, KAHAN, WIDE, Epi>, blocks(32), 256, 0, s, g, x, y_det, z_det, epi); break;
+    case 64: klaunch(ax_kernel<64, KAHAN, WIDE, Epi>, blocks(64), 256, 0, s, g, x, y_det, z_det, epi); break;
+    case 128: klaunch(ax_kernel<128, KAHAN, WIDE, Epi>, blocks(128), 256, 0, s, g, x, y_det, z_det, epi); break;
+    default: klaunch(ax_kernel<256, KAHAN, WIDE, Epi>, blocks(256), 256, 0, s, g, x, y_det, z_det, epi); break;
     }
     ASD_LAUNCH_CHECK("ax_kernel");
     return ASD_OK;
+}
+
+// Forward projector launch for one view (shape from ax_shape): buffer reads
+// where the volume allows them (vol_buffer_ok), 64-bit addressing otherwise.
+template <bool KAHAN = false, class Epi>
+int dispatch_ax(AxShape sh, const ViewGeom &g, const float *x, const float *y_det,
+                const float *z_det, const Epi &epi, hipStream_t s)
+{
+    if (vol_buffer_ok((size_t)g.nx * g.ny * g.nz, g.nx))
+        return dispatch_ax_t<KAHAN, false>(sh, g, x, y_det, z_det, epi, s);
+    return dispatch_ax_t<KAHAN, true>(sh, g, x, y_det, z_det, epi, s);
 }
 
 // Backprojector: one thread per voxel (x fastest) walks its detector

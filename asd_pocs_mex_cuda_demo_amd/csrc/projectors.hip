@@ -72,21 +72,14 @@ AxShape ax_shape(int na, int nb, int nu)
 {
     AxShape s;
     s.lpr = ax_lanes_per_ray(na, nb, nu);
-    // Two-row bundles save the second ray's transverse slab arithmetic
-    // (about 15% of the instructions) but halve the number of waves; at 8
-    // lanes they pay when the view still has at least 16 waves per SIMD (a
-    // 512^3 view with a 1024 x 768 detector: 10% faster).  With fewer rays
-    // the lane count would rise to 16, which loses more than the bundle
-    // saves (a 256^3 view with a 384 x 384 detector: 131 us against 108 us
-    // for single rays at 8 lanes).
-    if (nb >= 2) {
-        const int nbb = (nb + 1) / 2;
-        const int l = ax_lanes_per_ray(na, nbb, nu);
-        if (l == 8 && (long)na * nbb * l / 64 >= 16L * 1024) {
-            s.lpr = l;
-            s.rows = 2;
-        }
-    }
+    // Bundles of three rows at the same 8 lanes share each slab's transverse
+    // arithmetic among three rays (a third fewer waves, each three times as
+    // long).  Measured per view on MI355X (kernel alone, four angles): 256^3
+    // with 384 x 384 bins 105 -> 92 us, 512^3 with 1024 x 768 bins 850 ->
+    // 706 us; two rows gave 99 and 709 us.  Views with fewer rays (more than
+    // 8 lanes per ray) keep single rays, and so do detectors of fewer than
+    // 16 rows, where the last bundle's repeated rows would cost more.
+    if (s.lpr == 8 && nb >= 16) s.rows = 3;
     return s;
 }
 

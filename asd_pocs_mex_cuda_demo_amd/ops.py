@@ -309,6 +309,10 @@ class Plan:
     def set_atx_variant(self, v):
         nat.check(self._L.asd_plan_set_atx_variant(self.h, int(v)))
 
+    def set_ax_shape(self, lanes_per_ray, rows_per_bundle):
+        """Force the forward projector's shape ((0, 0): the default)."""
+        nat.check(self._L.asd_plan_set_ax_shape(self.h, int(lanes_per_ray), int(rows_per_bundle)))
+
     def tuning(self):
         v = C.c_int()
         us = np.zeros(10)

@@ -309,6 +309,8 @@ def main():
     ap.add_argument("--cpu-views", type=int, default=None)
     ap.add_argument("--prof-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--ax-shape", default=None,
+                    help="force the forward projector's shape as LANES,ROWS (for example 8,3)")
     ap.add_argument("--atx-variant", default=None,
                     choices=["direct", "tile16x16x1", "tile8x8x4", "tile16x4x4", "pairs",
                              "direct2", "direct1", "sorted16", "sorted8x8x4",
@@ -362,6 +364,8 @@ def main():
         plan.set_projections(P)
     if args.atx_variant:
         plan.set_atx_variant(list(ATX_KERNELS).index(args.atx_variant))
+    if args.ax_shape:
+        plan.set_ax_shape(*[int(v) for v in args.ax_shape.split(",")])
     N = para["nx"] * para["ny"] * para["nz"]
     M = para["na"] * para["nb"]
 
@@ -410,8 +414,9 @@ def main():
     lpr, rows, _ = plan.info()
     kernel = None
     if args.precision == 0:
-        kernel = {0: (f"ax_rows_kernel<{lpr}, {rows}, 2, false, EpiSartAx<SinoF32> >" if rows > 1
-                      else f"ax_kernel<{lpr}, false, EpiSartAx<SinoF32> >"),
+        kernel = {0: (f"ax_rows_kernel<{lpr}, {rows}, {1 if rows == 3 else 2}, false, false, "
+                      f"EpiSartAx<SinoF32> >"
+                      if rows > 1 else f"ax_kernel<{lpr}, false, false, EpiSartAx<SinoF32> >"),
                   1: ATX_KERNELS.get(tuning["atx_variant"])}.get(dom)
     traffic = profiled_traffic(args.workload, kernel) if kernel else None
     roof = {"bound": "hbm", "kernel": kernel or names[dom], "achieved": achieved,

@@ -190,6 +190,12 @@ int asd_plan_tuning(asd_plan *plan, int *atx_variant, double *atx_us);
  * slices need nz > 1) or, with -1, asks for a new measurement on the next
  * sweep. */
 int asd_plan_set_atx_variant(asd_plan *plan, int atx_variant);
+/* Forces the forward projector's launch shape: lanes per ray (1..256, a
+ * power of two) with 1 row per bundle, or 8 lanes with bundles of 2 or 3
+ * detector rows; (0, 0) restores the plan's default.  The shape changes the
+ * order in which each ray's slab products are summed (the last bits of the
+ * sums), not which products there are. */
+int asd_plan_set_ax_shape(asd_plan *plan, int lanes_per_ray, int rows_per_bundle);
 /* Launch shape of the forward projector (lanes per ray, or per bundle of
  * rows_per_bundle detector rows that share a column) and the plan's
  * hipStream_t.  Any pointer may be NULL. */

@@ -30,6 +30,10 @@ static ViewGeom mk(float c, float s, float SO, float OD, float scale, float dyd,
 // evaluated), to check that the shortcuts change no bit.
 static int g_general = 0;
 
+// When set, the forward projector's pieces are addressed in the wide form
+// (element indices) whatever the volume size.
+static int g_wide = 0;
+
 static AxRay ray_setup(const ViewGeom &g, float ydet, float zdet)
 {
     AxRay r = ax_ray_setup(g, ydet, zdet);
@@ -47,6 +51,7 @@ static RayRec ray_rec(const ViewGeom &g, float ydet, float zdet)
 extern "C" {
 
 void emul_set_general(int on) { g_general = on; }
+void emul_set_wide(int on) { g_wide = on; }
 
 // n / d by the kernels' multiply-high division (FastDiv)
 void emul_fastdiv(unsigned d, const unsigned *n, unsigned *q, long count)
@@ -136,20 +141,30 @@ static void emul_ax_t(const float *X, float *y, float *sr, const ViewGeom &g, co
             if (g_general)
                 for (int k = 0; k < rows; k++) r[k].flat = r[k].dz1 = 0;
             const int nu = r[0].xdriven ? nx : ny;
+            // the buffer form's byte offsets where the kernels use it, else
+            // element indices (the same voxels)
+            const bool buf = !g_wide && vol_buffer_ok((size_t)nx * ny * g.nz, nx);
+            const VolRead<false> Vb(X, (size_t)nx * ny * g.nz);
+            const VolRead<true> Vw(X, (size_t)nx * ny * g.nz);
+            const SlabAddr A = slab_addr(r[0], nx, buf ? VolRead<false>::unit : VolRead<true>::unit);
             for (int lane = 0; lane < lpr; lane++) {
                 Acc<KAHAN> a[8], l[8];
                 if (rows == 1) {
                     for (int iu0 = lane; iu0 < nu; iu0 += 4 * lpr)
-                        ax_slabs<4, KAHAN>(r[0], X, iu0, lpr, nu, nx, ny, a[0], l[0]);
+                        if (buf)
+                            ax_slabs<4, KAHAN, false>(r[0], A, Vb, iu0, lpr, nu, ny, a[0], l[0]);
+                        else
+                            ax_slabs<4, KAHAN, false>(r[0], A, Vw, iu0, lpr, nu, ny, a[0], l[0]);
                 } else {
                     for (int iu = lane; iu < nu; iu += lpr) {
-                        const SlabT t = ax_slab_t(r[0], iu);
+                        const SlabT t = ax_slab_t(r[0], A, iu);
                         for (int k = 0; k < rows; k++) {
                             SlabW s;
-                            ax_slab_z(r[k], t, iu, nx, ny, s);
+                            ax_slab_z(r[k], t, A, ny, s);
                             for (int p = 0; p < 3; p++)
                                 if (s.ok[p]) {
-                                    a[k].add(s.wl[p] * X[s.idx[p]]);
+                                    const float xv = buf ? Vb(s.off[p], true) : Vw(s.off[p], true);
+                                    a[k].add(s.wl[p] * xv);
                                     l[k].add(s.wl[p]);
                                 }
                         }

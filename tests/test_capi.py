@@ -77,6 +77,7 @@ def test_bad_arguments_are_rejected_before_the_device():
     with pytest.raises(nat.AsdError):
         nat.check_last()
     assert L.asd_plan_iterate(None, 1) == nat.ASD_ERR_ARG
+    assert L.asd_plan_set_ax_shape(None, 8, 3) == nat.ASD_ERR_ARG
 
 
 def test_gateway_geometry_is_bit_identical_to_oracle(orc):

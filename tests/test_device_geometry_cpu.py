@@ -59,13 +59,16 @@ def test_device_projectors_match_oracle(orc, case, phi):
             assert np.abs(y - yo).max() <= 2e-6 * np.abs(yo).max()
             assert np.abs(sr - sro).max() <= 2e-6 * np.abs(sro).max()
         # bundles of detector rows sharing each slab's transverse half sum
-        # the same products in the same order as single rays
-        for rows in (2, 3):
+        # the same products in the same order as single rays, and the wide
+        # addressing (element indices) reads the same voxels as byte offsets
+        for rows, wide in ((2, 0), (3, 0), (1, 1), (2, 1)):
             y2 = np.zeros_like(y)
             sr2 = np.zeros_like(sr)
+            emul().emul_set_wide(wide)
             emul().emul_ax(P(X), P(y2), P(sr2), *args, np.float32(1), nx, ny, nz, na, nb, lpr,
                            kahan, rows)
-            assert np.array_equal(y2, y) and np.array_equal(sr2, sr), (lpr, kahan, rows)
+            emul().emul_set_wide(0)
+            assert np.array_equal(y2, y) and np.array_equal(sr2, sr), (lpr, kahan, rows, wide)
     x = np.zeros(nx * ny * nz, np.float32)
     sc = np.zeros_like(x)
     emul().emul_atx(P(x), P(Y), P(sc), *args, g["det"], g["det"], np.float32(1), nx, ny, nz, na, nb)

@@ -70,10 +70,10 @@ def test_forward_projector_slab_arithmetic_is_bit_identical(orc, nz, nb, phi):
     assert np.array_equal(y, yo), np.abs(y - yo).max()
 
 
-def test_two_row_bundles_match_single_rays_and_oracle(orc):
-    """A 512 x 512 detector gets the two-row forward projector (8 lanes on a
-    bundle of two detector rows of one column, each slab's transverse half
-    shared).  Each ray's slab products and their lane order are the
+def test_row_bundles_match_single_rays_and_oracle(orc):
+    """A 512 x 512 detector gets the row-bundle forward projector (8 lanes on
+    a bundle of three detector rows of one column, each slab's transverse
+    half shared).  Each ray's slab products and their lane order are the
     single-ray kernel's, so the result equals the device functions' host run
     at 8 lanes per ray bit for bit, and the oracle to summation rounding; a
     SART sweep through a plan of that shape matches the oracle's sweep."""
@@ -81,7 +81,7 @@ def test_two_row_bundles_match_single_rays_and_oracle(orc):
     nx, ny, nz, na, nb = 32, 32, 24, 512, 512
     base = geom(nx, ny, nz, na, nb, 500.0, 500.0, 1.0, 0.15, 0.0)
     pl = ops.Plan(base, [0.0, 90.0])
-    assert pl.info()[:2] == (8, 2)
+    assert pl.info()[:2] == (8, 3)
     pl.close()
     rng = np.random.default_rng(5)
     X = rng.random(nx * ny * nz).astype(np.float32)
@@ -106,6 +106,53 @@ def test_two_row_bundles_match_single_rays_and_oracle(orc):
     mo, outo = orc.sart(P, para)
     assert rel_l2(m, mo) <= 1e-5, rel_l2(m, mo)
     assert abs(out["dataTolerance"] - outo["dataTolerance"]) <= 1e-5 * outo["dataTolerance"]
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_forward_shapes_give_identical_iterations(precision):
+    """At 8 lanes, single rays and bundles of 2 or 3 detector rows sum every
+    ray's slab products in the same order, so two ASD-POCS iterations are
+    bit-identical whichever shape the plan is forced to (plain sums and the
+    compensated sums of precision 1).  Other lane counts change only the
+    last bits; shapes outside the compiled set are rejected."""
+    para, img, P = _small_problem(n=32, nv=6, na=48, nz=16, nb=40)
+    para.update(maxnumIter=2, alpha=0.2, numOfgradients=5, precision=precision)
+    res = []
+    for shape in [(8, 1), (8, 2), (8, 3), (16, 1)]:
+        pl = ops.Plan(para, para["phi_vec"])
+        pl.set_projections(P)
+        pl.reset(None, img)
+        pl.set_atx_variant(0)
+        pl.set_ax_shape(*shape)
+        assert pl.info()[:2] == shape
+        pl.iterate(2)
+        pl.sync()
+        res.append(pl.image())
+        if shape == (16, 1):
+            for bad in [(8, 4), (16, 3), (12, 1), (512, 1), (0, 1)]:
+                with pytest.raises(Exception):
+                    pl.set_ax_shape(*bad)
+            pl.set_ax_shape(0, 0)
+            assert pl.info()[:2] == (16, 1)
+        pl.close()
+    assert np.array_equal(res[0], res[1]) and np.array_equal(res[0], res[2])
+    assert rel_l2(res[3], res[0]) <= 1e-4, rel_l2(res[3], res[0])
+
+
+def test_wide_addressing_matches_oracle(orc):
+    """Volumes of 2^30 voxels or more, or rows of 2^22 voxels or more, are read
+    through 64-bit element addresses instead of buffer byte offsets.  Rows of
+    4194306 voxels force that form; rays along y walk 16 short slabs near
+    the middle of the rows, so every sum has few terms and a piece read from
+    a wrong voxel shows."""
+    nx, ny = 4194306, 16
+    para = geom(nx, ny, 1, 64, 1, 1000.0, 500.0, 1.0, 1.0, -np.pi / 2)
+    X = np.random.default_rng(6).random(nx * ny).astype(np.float32)
+    y, sr = ops.Ax_cone_gpu(X, para)
+    yo, sro = orc.ax_para(X, para)
+    assert np.all(sro > 0)
+    assert np.abs(y - yo).max() <= 1e-5 * np.abs(yo).max(), np.abs(y - yo).max()
+    assert np.abs(sr - sro).max() <= 1e-5 * np.abs(sro).max(), np.abs(sr - sro).max()
 
 
 @pytest.mark.parametrize("name,para", CASES, ids=[c[0] for c in CASES])
