@@ -119,7 +119,17 @@ enum : int {
     RAY_B0_C0 = 6,        // b == 0, c == 0: |pz| <= dz/2 -> dx
     RAY_B0_XZ_U = 7,      // b == 0: (x,z) chord, |a| >  |c|
     RAY_B0_XZ_W = 8,      // b == 0: (x,z) chord, |a| <= |c|
+    // RAY_X_FIRST / RAY_Y_FIRST for a ray where some slope s has s * h
+    // inexact (h = 0.5 and dz / 2; a subnormal product, or dz != 1): the
+    // face end points c -+ s*h are formed from the rounded products
+    RAY_X_FIRST_P = 9,
+    RAY_Y_FIRST_P = 10,
 };
+
+__host__ __device__ __forceinline__ bool ray_general(int mode)
+{
+    return mode <= RAY_Y_FIRST || mode >= RAY_X_FIRST_P;
+}
 
 // Host helpers shared by the translation units.
 struct DeviceBuf {

@@ -589,22 +589,86 @@ __host__ __device__ __forceinline__ bool chord6(const bool (&f)[6], const float 
     return chord_points<6>(c, ex, ey, ez);
 }
 
-__host__ __device__ __forceinline__ bool chord3_points(
-    bool x_first, float xy_c, float xy_o, float xz_c, float xz_o, float yx_c, float yx_o,
-    float yz_c, float yz_o, float zx_c, float zx_o, float zy_c, float zy_o, float hx, float hy,
-    float hz, float &ex, float &ey, float &ez)
+// The six face candidates from the end coordinates c -+ o of each face pair
+// (xy: the y coordinate on the x faces, ...), m = c - o and p = c + o.
+struct FaceEnds {
+    float xy_m, xy_p, xz_m, xz_p, yx_m, yx_p, yz_m, yz_p, zx_m, zx_p, zy_m, zy_p;
+};
+
+__host__ __device__ __forceinline__ bool chord3_points(bool x_first, const FaceEnds &E, float hx,
+                                                       float hy, float hz, float &ex, float &ey,
+                                                       float &ez)
 {
 #pragma clang fp contract(off)
-    const bool f[6] = {absf(xy_c - xy_o) <= hy && absf(xz_c - xz_o) <= hz,
-                       absf(xy_c + xy_o) <= hy && absf(xz_c + xz_o) <= hz,
-                       absf(yx_c - yx_o) <= hx && absf(yz_c - yz_o) <= hz,
-                       absf(yx_c + yx_o) <= hx && absf(yz_c + yz_o) <= hz,
-                       absf(zx_c - zx_o) <= hx && absf(zy_c - zy_o) <= hy,
-                       absf(zx_c + zx_o) <= hx && absf(zy_c + zy_o) <= hy};
-    const float x[6] = {-hx, hx, yx_c - yx_o, yx_c + yx_o, zx_c - zx_o, zx_c + zx_o};
-    const float y[6] = {xy_c - xy_o, xy_c + xy_o, -hy, hy, zy_c - zy_o, zy_c + zy_o};
-    const float z[6] = {xz_c - xz_o, xz_c + xz_o, yz_c - yz_o, yz_c + yz_o, -hz, hz};
+    const bool f[6] = {absf(E.xy_m) <= hy && absf(E.xz_m) <= hz,
+                       absf(E.xy_p) <= hy && absf(E.xz_p) <= hz,
+                       absf(E.yx_m) <= hx && absf(E.yz_m) <= hz,
+                       absf(E.yx_p) <= hx && absf(E.yz_p) <= hz,
+                       absf(E.zx_m) <= hx && absf(E.zy_m) <= hy,
+                       absf(E.zx_p) <= hx && absf(E.zy_p) <= hy};
+    const float x[6] = {-hx, hx, E.yx_m, E.yx_p, E.zx_m, E.zx_p};
+    const float y[6] = {E.xy_m, E.xy_p, -hy, hy, E.zy_m, E.zy_p};
+    const float z[6] = {E.xz_m, E.xz_p, E.yz_m, E.yz_p, -hz, hz};
     return x_first ? chord6<true>(f, x, y, z, ex, ey, ez) : chord6<false>(f, x, y, z, ex, ey, ez);
+}
+
+// c -+ s * h as the reference forms it: the product o = s * h rounded, then
+// c - o and c + o.
+__host__ __device__ __forceinline__ void face_ends(float c, float s, float h, float &m, float &p)
+{
+#pragma clang fp contract(off)
+    const float o = s * h;
+    m = c - o;
+    p = c + o;
+}
+
+// The same values when s * h is exact (h = 0.5 and RAY_HALF_EXACT): a fused
+// multiply-add rounds c -+ s * h once, which is the rounding of c -+ o.
+__host__ __device__ __forceinline__ void face_ends_fma(float c, float s, float h, float &m,
+                                                       float &p)
+{
+    m = __builtin_fmaf(-h, s, c);
+    p = __builtin_fmaf(h, s, c);
+}
+
+// find_l_3d's chord for a general ray (a, b != 0).  FMA: the face offsets
+// s * h are exact (h = 0.5; RAY_X_FIRST / RAY_Y_FIRST), so each end point
+// c -+ s*h is one fused multiply-add with the rounding of c -+ o.
+template <bool FMA>
+__host__ __device__ __forceinline__ float chord3_general(const RayRec &R, bool x_first, float px,
+                                                         float py, float pz, float hz)
+{
+#pragma clang fp contract(off)
+    const float hx = 0.5f, hy = 0.5f;
+    // A flat ray (c == 0 in a one-slice volume, pz = -hz): s_ca = s_cb = +-0
+    // put both hits' z at pz, so every x / y face test's z condition holds
+    // (|pz| == hz) and ez = 0; s_ac, s_bc = +-inf make the z-face tests
+    // NaN / inf comparisons that fail.  What is left is the in-plane
+    // chord, evaluated by the same expressions in the same order.
+    if (R.flat) return chord2(px, py, R.s_ba, R.s_ab, R.s_ba * hx, R.s_ab * hy, hx, hy, x_first);
+    const float xy_c = R.s_ba * (-px) + py;
+    const float xz_c = R.s_ca * (-px) + pz;
+    const float yx_c = R.s_ab * (-py) + px;
+    const float yz_c = R.s_cb * (-py) + pz;
+    const float zx_c = R.s_ac * (-pz) + px;
+    const float zy_c = R.s_bc * (-pz) + py;
+    FaceEnds E;
+    auto ends = [](float c, float s, float h, float &m, float &p) {
+        if constexpr (FMA)
+            face_ends_fma(c, s, h, m, p);
+        else
+            face_ends(c, s, h, m, p);
+    };
+    ends(xy_c, R.s_ba, hx, E.xy_m, E.xy_p);
+    ends(xz_c, R.s_ca, hx, E.xz_m, E.xz_p);
+    ends(yx_c, R.s_ab, hy, E.yx_m, E.yx_p);
+    ends(yz_c, R.s_cb, hy, E.yz_m, E.yz_p);
+    ends(zx_c, R.s_ac, hz, E.zx_m, E.zx_p);
+    ends(zy_c, R.s_bc, hz, E.zy_m, E.zy_p);
+    float ex, ey, ez;
+    const bool two = chord3_points(x_first, E, hx, hy, hz, ex, ey, ez);
+    if (!two) return 0.0f;
+    return sqrtf(ex * ex + ey * ey + ez * ez);
 }
 
 __host__ __device__ __forceinline__ float find_l_3d_rec(const RayRec &R, float px, float py, float pz,
@@ -613,30 +677,12 @@ __host__ __device__ __forceinline__ float find_l_3d_rec(const RayRec &R, float p
 #pragma clang fp contract(off)
     const float hx = 0.5f, hy = 0.5f;
     // the general rays first, behind one test (the other modes are the
-    // a == 0 / b == 0 fallbacks, which only views without a footprint cut have)
-    if (R.mode <= RAY_Y_FIRST) {
-        // A flat ray (c == 0 in a one-slice volume, pz = -hz): s_ca = s_cb = +-0
-        // put both hits' z at pz, so every x / y face test's z condition holds
-        // (|pz| == hz) and ez = 0; s_ac, s_bc = +-inf make the z-face tests
-        // NaN / inf comparisons that fail.  What is left is the in-plane
-        // chord, evaluated by the same expressions in the same order.
-        if (R.flat)
-            return chord2(px, py, R.s_ba, R.s_ab, R.s_ba * hx, R.s_ab * hy, hx, hy,
-                          R.mode == RAY_X_FIRST);
-        const float xy_c = R.s_ba * (-px) + py, xy_o = R.s_ba * hx;
-        const float xz_c = R.s_ca * (-px) + pz, xz_o = R.s_ca * hx;
-        const float yx_c = R.s_ab * (-py) + px, yx_o = R.s_ab * hy;
-        const float yz_c = R.s_cb * (-py) + pz, yz_o = R.s_cb * hy;
-        const float zx_c = R.s_ac * (-pz) + px, zx_o = R.s_ac * hz;
-        const float zy_c = R.s_bc * (-pz) + py, zy_o = R.s_bc * hz;
-        float ex, ey, ez;
-        const bool two = chord3_points(R.mode == RAY_X_FIRST, xy_c, xy_o, xz_c, xz_o, yx_c, yx_o,
-                                       yz_c, yz_o, zx_c, zx_o, zy_c, zy_o, hx, hy, hz, ex, ey,
-                                       ez);
-        if (!two) return 0.0f;
-        return sqrtf(ex * ex + ey * ey + ez * ez);
-    }
+    // a == 0 / b == 0 fallbacks, which only views without a footprint cut
+    // have, and general rays whose face offsets are inexact products)
+    if (R.mode <= RAY_Y_FIRST) return chord3_general<true>(R, R.mode == RAY_X_FIRST, px, py, pz, hz);
     switch (R.mode) {
+    case RAY_X_FIRST_P: return chord3_general<false>(R, true, px, py, pz, hz);
+    case RAY_Y_FIRST_P: return chord3_general<false>(R, false, px, py, pz, hz);
     case RAY_A0_B0: return absf(py) <= hy ? dz : 0.0f;
     case RAY_A0_C0: return absf(pz) <= hz ? 1.0f : 0.0f;
     case RAY_A0_YZ_U: return chord2(py, pz, R.s_cb, R.s_bc, R.s_cb * hy, R.s_bc * hz, hy, hz, true);
@@ -658,7 +704,7 @@ __host__ __device__ __forceinline__ float find_l_3d_rec(const RayRec &R, float p
 __host__ __device__ __forceinline__ bool inplane_may_hit(const RayRec &R, float px, float py)
 {
 #pragma clang fp contract(off)
-    if (R.mode != RAY_X_FIRST && R.mode != RAY_Y_FIRST) return true;
+    if (!ray_general(R.mode)) return true;
     const float lim = 0.5f + 1e-2f;
     const float xy_c = R.s_ba * (-px) + py, xy_o = R.s_ba * 0.5f;
     const float yx_c = R.s_ab * (-py) + px, yx_o = R.s_ab * 0.5f;
@@ -690,6 +736,12 @@ __host__ __device__ __forceinline__ RayRec make_ray_rec(const ViewGeom &g, float
     } else {
         R.mode = absf(a) > absf(b) ? RAY_X_FIRST : RAY_Y_FIRST;
         R.flat = c == 0 && g.nz == 1 && g.dz == 1.0f;
+        // the face offsets s * 0.5 (and s * dz/2) are exact unless a product
+        // is subnormal (|s| < 2^-125, s != 0) or dz != 1
+        bool exact = g.dz == 1.0f;
+        for (float s : {R.s_ba, R.s_ab, R.s_ca, R.s_cb, R.s_ac, R.s_bc})
+            exact = exact && !(s != 0.0f && absf(s) < 0x1p-125f);
+        if (!exact) R.mode += RAY_X_FIRST_P;
     }
     return R;
 }

@@ -44,7 +44,11 @@ static AxRay ray_setup(const ViewGeom &g, float ydet, float zdet)
 static RayRec ray_rec(const ViewGeom &g, float ydet, float zdet)
 {
     RayRec R = make_ray_rec(g, ydet, zdet);
-    if (g_general) R.flat = 0;
+    if (g_general) {
+        // the flat shortcut off, and the product form of the face offsets
+        R.flat = 0;
+        if (R.mode <= RAY_Y_FIRST) R.mode += RAY_X_FIRST_P;
+    }
     return R;
 }
 
