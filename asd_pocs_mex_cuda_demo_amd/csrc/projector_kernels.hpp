@@ -290,16 +290,47 @@ int dispatch_ax(AxShape sh, const ViewGeom &g, const float *x, const float *y_de
 // share stay in its L2.
 constexpr int ATX_CHUNK = 4;
 
-// The footprint walk of one voxel (shared by atx_kernel and the fallback of
-// atx_pairs_kernel), CHUNK pairs per load batch.
-template <bool KAHAN, int CHUNK = ATX_CHUNK>
-__device__ __forceinline__ void atx_voxel_walk(const ViewGeom &g, const float *__restrict__ Y,
-                                               const RayRec *__restrict__ rt, const Footprint &f,
-                                               float px, float py, float pz, float hz,
-                                               Acc<KAHAN> &acc, Acc<KAHAN> &accl)
+// Reads of a view's ray records and corrections by bin index.  Buffer form
+// (fewer than 2^27 bins, so every record's byte offset fits 32 bits): one
+// 32-bit shift per read instead of 64-bit address arithmetic.  Pointer form
+// otherwise.
+struct BinsBuf {
+    __amdgpu_buffer_rsrc_t rr, ry;
+    __device__ __forceinline__ BinsBuf(const RayRec *rt, const float *Y, unsigned M)
+    {
+        // dword3 0x00020000: 32-bit data format, the form untyped loads use
+        rr = __builtin_amdgcn_make_buffer_rsrc((void *)rt, (short)0, (int)(M * 32u), 0x00020000);
+        ry = __builtin_amdgcn_make_buffer_rsrc((void *)Y, (short)0, (int)(M * 4u), 0x00020000);
+    }
+    __device__ __forceinline__ RayRec rec(unsigned i) const
+    {
+        struct Q {
+            __attribute__((ext_vector_type(4))) unsigned a, b;
+        } q;
+        q.a = __builtin_amdgcn_raw_buffer_load_b128(rr, i << 5, 0, 0);
+        q.b = __builtin_amdgcn_raw_buffer_load_b128(rr, i << 5, 16, 0);
+        return __builtin_bit_cast(RayRec, q);
+    }
+    __device__ __forceinline__ float y(unsigned i) const
+    {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(ry, i << 2, 0, 0));
+    }
+};
+
+struct BinsPtr {
+    const RayRec *__restrict__ rt;
+    const float *__restrict__ Y;
+    __device__ __forceinline__ RayRec rec(unsigned i) const { return rt[i]; }
+    __device__ __forceinline__ float y(unsigned i) const { return Y[i]; }
+};
+
+template <bool KAHAN, int CHUNK, class Bins>
+__device__ __forceinline__ void atx_voxel_walk_t(const ViewGeom &g, const Bins &B,
+                                                 const Footprint &f, float px, float py,
+                                                 float pz, float hz, Acc<KAHAN> &acc,
+                                                 Acc<KAHAN> &accl)
 {
 #pragma clang fp contract(off)
-    if (!(f.a_lo <= f.a_hi && f.b_lo <= f.b_hi)) return;
     const int npairs = (f.a_hi - f.a_lo + 1) * (f.b_hi - f.b_lo + 1);
     // cursor: the next pair in reference order, as its column ia and its bin
     // index ib * na + ia (a row wrap jumps to column a_lo of the next row)
@@ -311,8 +342,8 @@ __device__ __forceinline__ void atx_voxel_walk(const ViewGeom &g, const float *_
         float yv[CHUNK];
 #pragma unroll
         for (int k = 0; k < CHUNK; k++) {
-            R[k] = rt[idx];
-            yv[k] = Y[idx];
+            R[k] = B.rec(idx);
+            yv[k] = B.y(idx);
             // advance unless this is the last pair (the rest of the chunk
             // re-reads it and is not used)
             if (k0 + k + 1 < npairs) {
@@ -329,6 +360,22 @@ __device__ __forceinline__ void atx_voxel_walk(const ViewGeom &g, const float *_
             accl.add(l);
         }
     }
+}
+
+// The footprint walk of one voxel (shared by atx_kernel, atx_sorted_kernel
+// and the fallback of atx_pairs_kernel), CHUNK pairs per load batch.
+template <bool KAHAN, int CHUNK = ATX_CHUNK>
+__device__ __forceinline__ void atx_voxel_walk(const ViewGeom &g, const float *__restrict__ Y,
+                                               const RayRec *__restrict__ rt, const Footprint &f,
+                                               float px, float py, float pz, float hz,
+                                               Acc<KAHAN> &acc, Acc<KAHAN> &accl)
+{
+    if (!(f.a_lo <= f.a_hi && f.b_lo <= f.b_hi)) return;
+    const unsigned M = (unsigned)g.na * (unsigned)g.nb;  // < 2^31 (index_fits)
+    if (M < (1u << 27))
+        atx_voxel_walk_t<KAHAN, CHUNK>(g, BinsBuf(rt, Y, M), f, px, py, pz, hz, acc, accl);
+    else
+        atx_voxel_walk_t<KAHAN, CHUNK>(g, BinsPtr{rt, Y}, f, px, py, pz, hz, acc, accl);
 }
 
 template <class Epi, bool KAHAN = false, int CHUNK = ATX_CHUNK>
