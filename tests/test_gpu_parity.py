@@ -155,6 +155,28 @@ def test_wide_addressing_matches_oracle(orc):
     assert np.abs(sr - sro).max() <= 1e-5 * np.abs(sro).max(), np.abs(sr - sro).max()
 
 
+def test_wide_addressing_row_bundles(orc):
+    """The row-bundle forward projector in the wide form: rows of 4194306
+    voxels and a 512 x 256 detector (8 lanes, bundles of three rows).  Its
+    sums equal the device functions' host run at 8 lanes bit for bit, and
+    the oracle to summation rounding."""
+    from tests.test_device_geometry_cpu import emul, P as ptr
+    nx, ny, nz, na, nb = 4194306, 16, 2, 512, 256
+    para = geom(nx, ny, nz, na, nb, 1000.0, 500.0, 1.0, 0.01, -np.pi / 2)
+    X = np.random.default_rng(8).random(nx * ny * nz).astype(np.float32)
+    y, sr = ops.Ax_cone_gpu(X, para)
+    g = orc.gateway_geometry(-np.pi / 2, 1000.0, 500.0, 1.0, 0.01, na, nb)
+    ye = np.zeros(na * nb, np.float32)
+    sre = np.zeros_like(ye)
+    emul().emul_ax(ptr(X), ptr(ye), ptr(sre), g["cos_phi"], g["sin_phi"], ptr(g["y_det"]),
+                   ptr(g["z_det"]), g["SO"], g["OD"], g["scale"], np.float32(1), nx, ny, nz, na,
+                   nb, 8, 0, 1)
+    assert np.array_equal(y, ye) and np.array_equal(sr, sre)
+    yo, sro = orc.ax_para(X, para)
+    assert np.count_nonzero(sro) > na * nb // 2
+    assert rel_l2(y, yo) <= 1e-5 and rel_l2(sr, sro) <= 1e-5
+
+
 @pytest.mark.parametrize("name,para", CASES, ids=[c[0] for c in CASES])
 def test_atx_matches_oracle(orc, name, para):
     rng = np.random.default_rng(2)
