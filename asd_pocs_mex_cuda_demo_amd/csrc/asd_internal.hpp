@@ -148,6 +148,28 @@ int launch_ax(const ViewGeom &g, const float *x, float *y, float *sr, const floa
 int launch_atx(const ViewGeom &g, float *x, const float *y, float *sc, const float *y_det,
                const float *z_det, const RayRec *rt, hipStream_t s);
 
+// Correctly rounded square root of x >= 2^-96 (or +-0, +inf, NaN): the
+// compiler's IEEE expansion of sqrtf without its rescaling of tiny
+// arguments, which is inactive in that range, so the result is the same
+// bits as sqrtf.  The approximation s is corrected by one ulp up or down
+// from the signs of the FMA residuals x - (s -+ ulp) * s.
+__host__ __device__ __forceinline__ float sqrt_ge_tiny(float x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float sd = __int_as_float(__float_as_int(s) - 1);
+    const float su = __int_as_float(__float_as_int(s) + 1);
+    const float rd = __builtin_fmaf(-sd, s, x);
+    const float ru = __builtin_fmaf(-su, s, x);
+    float t = rd <= 0.0f ? sd : s;
+    t = ru > 0.0f ? su : t;
+    // +-0 and +inf are their own roots (class mask: -0, +0, +inf)
+    return __builtin_amdgcn_classf(x, 0x020 | 0x040 | 0x200) ? x : t;
+#else
+    return sqrtf(x);
+#endif
+}
+
 // Lanes cooperating on one ray of the forward projector.
 int ax_lanes_per_ray(int na, int nb, int nu);
 

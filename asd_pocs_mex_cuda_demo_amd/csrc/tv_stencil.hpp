@@ -131,6 +131,13 @@ struct VoxelCursor {
     }
 };
 
+// Square root of a stencil denominator eps + (sum of squares) >= eps = 1e-8.
+// In FP32 that argument is far above 2^-96, so the correctly rounded root
+// needs no rescaling of tiny arguments (sqrt_ge_tiny: the same bits as
+// sqrtf); FP64 roots are taken as they are.
+__device__ __forceinline__ float tv_sqrt(float x) { return sqrt_ge_tiny(x); }
+__device__ __forceinline__ double tv_sqrt(double x) { return sqrt(x); }
+
 // 2-D stencil at row r (fastest) and column c (imgTVGradMin.cpp:84-97).
 template <class A, class Src>
 __device__ __forceinline__ A tv2_grad(const Src &U, int r, int c)
@@ -141,9 +148,9 @@ __device__ __forceinline__ A tv2_grad(const Src &U, int r, int c)
     A rm = U(r - 1, c, 0), cm = U(r, c - 1, 0);
     A rp = U(r + 1, c, 0), rp_cm = U(r + 1, c - 1, 0);
     A cp = U(r, c + 1, 0), rm_cp = U(r - 1, c + 1, 0);
-    A g1 = ((p - rm) + (p - cm)) / sqrt(eps + (p - rm) * (p - rm) + (p - cm) * (p - cm));
-    A g2 = -(rp - p) / sqrt(eps + (rp - p) * (rp - p) + (rp - rp_cm) * (rp - rp_cm));
-    A g3 = -(cp - p) / sqrt(eps + (cp - p) * (cp - p) + (cp - rm_cp) * (cp - rm_cp));
+    A g1 = ((p - rm) + (p - cm)) / tv_sqrt(eps + (p - rm) * (p - rm) + (p - cm) * (p - cm));
+    A g2 = -(rp - p) / tv_sqrt(eps + (rp - p) * (rp - p) + (rp - rp_cm) * (rp - rp_cm));
+    A g3 = -(cp - p) / tv_sqrt(eps + (cp - p) * (cp - p) + (cp - rm_cp) * (cp - rm_cp));
     return g1 + g2 + g3;
 }
 
@@ -159,13 +166,13 @@ __device__ __forceinline__ A tv3_grad(const Src &U, int x, int y, int z)
     A yp = U(x, y + 1, z), xm_yp = U(x - 1, y + 1, z), yp_zm = U(x, y + 1, z - 1);
     A xp = U(x + 1, y, z), xp_ym = U(x + 1, y - 1, z), xp_zm = U(x + 1, y, z - 1);
     A g1 = (A(6) * p - A(2) * (xm + ym + zm)) /
-           sqrt(eps + (p - xm) * (p - xm) + (p - ym) * (p - ym) + (p - zm) * (p - zm));
+           tv_sqrt(eps + (p - xm) * (p - xm) + (p - ym) * (p - ym) + (p - zm) * (p - zm));
     A g2 = (A(2) * p - A(2) * zp) /
-           sqrt(eps + (zp - xm_zp) * (zp - xm_zp) + (zp - ym_zp) * (zp - ym_zp) + (zp - p) * (zp - p));
+           tv_sqrt(eps + (zp - xm_zp) * (zp - xm_zp) + (zp - ym_zp) * (zp - ym_zp) + (zp - p) * (zp - p));
     A g3 = (A(2) * p - A(2) * yp) /
-           sqrt(eps + (yp - xm_yp) * (yp - xm_yp) + (yp - p) * (yp - p) + (yp - yp_zm) * (yp - yp_zm));
+           tv_sqrt(eps + (yp - xm_yp) * (yp - xm_yp) + (yp - p) * (yp - p) + (yp - yp_zm) * (yp - yp_zm));
     A g4 = (A(2) * p - A(2) * xp) /
-           sqrt(eps + (xp - p) * (xp - p) + (xp - xp_ym) * (xp - xp_ym) + (xp - xp_zm) * (xp - xp_zm));
+           tv_sqrt(eps + (xp - p) * (xp - p) + (xp - xp_ym) * (xp - xp_ym) + (xp - xp_zm) * (xp - xp_zm));
     return g1 + g2 + g3 + g4;
 }
 
