@@ -291,8 +291,9 @@ __host__ __device__ __forceinline__ SlabAddr slab_addr(const AxRay &r, int nx, u
 }
 
 // The (up to) three pieces of slab iu of ray r: address (see VolRead), weight
-// w*l and whether the piece lies in the volume.  The address and weight of a
-// piece outside the volume are unspecified.
+// w*l and whether the piece lies in the volume.  A piece outside the volume
+// has an unspecified weight; its address is VOL_NONE when the whole slab lies
+// outside, else arbitrary.
 struct SlabW {
     unsigned off[3];
     float wl[3];
