@@ -108,12 +108,12 @@ def test_row_bundles_match_single_rays_and_oracle(orc):
     assert abs(out["dataTolerance"] - outo["dataTolerance"]) <= 1e-5 * outo["dataTolerance"]
 
 
-@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("precision", [0, 1, 2])
 def test_forward_shapes_give_identical_iterations(precision):
     """At 8 lanes, single rays and bundles of 2 or 3 detector rows sum every
     ray's slab products in the same order, so two ASD-POCS iterations are
-    bit-identical whichever shape the plan is forced to (plain sums and the
-    compensated sums of precision 1).  Other lane counts change only the
+    bit-identical whichever shape the plan is forced to (plain sums, and the
+    compensated sums of precision 1 and 2 with their 16-bit sinograms).  Other lane counts change only the
     last bits; shapes outside the compiled set are rejected."""
     para, img, P = _small_problem(n=32, nv=6, na=48, nz=16, nb=40)
     para.update(maxnumIter=2, alpha=0.2, numOfgradients=5, precision=precision)
