@@ -121,14 +121,19 @@ __global__ __launch_bounds__(256) void ax_batch_kernel(const BItem *__restrict__
         const VolRead<false> V(it.m, (size_t)g.nx * g.ny * g.nz);
         const SlabAddr A = slab_addr(r, g.nx, VolRead<false>::unit);
         int nu = r.xdriven ? g.nx : g.ny;
-        auto march = [&](auto ft) {
+        auto march = [&](auto ft, auto flat) {
             for (int iu0 = lane; iu0 < nu; iu0 += 4 * LPR)
-                ax_slabs<4, KAHAN, decltype(ft)::value>(r, A, V, iu0, LPR, nu, g.ny, sa, sl);
+                ax_slabs<4, KAHAN, decltype(ft)::value, VolRead<false>, decltype(flat)::value>(
+                    r, A, V, iu0, LPR, nu, g.ny, sa, sl);
         };
-        if (r.fast_t)  // a per-case constant: one branch for the march
-            march(std::true_type{});
+        // per-case constants (every ray of a 2-D case is flat): one branch
+        // for the march
+        if (r.fast_t && r.flat)
+            march(std::true_type{}, std::true_type{});
+        else if (r.fast_t)
+            march(std::true_type{}, std::false_type{});
         else
-            march(std::false_type{});
+            march(std::false_type{}, std::false_type{});
     }
     float acc, accl;
     if constexpr (LPR > 1 && KAHAN) {

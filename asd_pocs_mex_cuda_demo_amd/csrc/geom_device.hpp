@@ -358,14 +358,17 @@ __host__ __device__ __forceinline__ SlabT ax_slab_t(const AxRay &r, const SlabAd
 }
 
 // The rest of slab iu of ray r given its transverse half: the axial extent
-// and cells, the slab length and the three pieces.
+// and cells, the slab length and the three pieces.  FLAT: the ray is known
+// to be flat (the caller has branched on it once for the whole march), so
+// the axial cells are the constants of a flat ray.
+template <bool FLAT = false>
 __host__ __device__ __forceinline__ void ax_slab_z(const AxRay &r, const SlabT &t,
                                                    const SlabAddr &A, int ny, SlabW &s)
 {
 #pragma clang fp contract(off)
     int cz1 = 0, cz2 = 0;
     float za = 0.0f, zb = 0.0f;
-    if (!r.flat) {
+    if (!FLAT && !r.flat) {
         // (z1 + s2 * (u - u1)) / dz + nz/2 with z1 = 0 (see above); x / 1 == x,
         // so dz == 1 skips the division
         float pa, pb;
@@ -416,16 +419,16 @@ __host__ __device__ __forceinline__ void ax_slab_z(const AxRay &r, const SlabT &
     s.wl[2] = (1.0f - hi) * l;
 }
 
-template <bool FT = false>
+template <bool FT = false, bool FLAT = false>
 __host__ __device__ __forceinline__ void ax_slab_geom(const AxRay &r, const SlabAddr &A, int iu,
                                                       int ny, SlabW &s)
 {
-    ax_slab_z(r, ax_slab_t<FT>(r, A, iu), A, ny, s);
+    ax_slab_z<FLAT>(r, ax_slab_t<FT>(r, A, iu), A, ny, s);
 }
 
 // U consecutive slabs of one lane (iu0, iu0 + stride, ...): all geometry
 // first, then every load, then the sums in slab order.
-template <int U, bool KAHAN, bool FT, class Vol>
+template <int U, bool KAHAN, bool FT, class Vol, bool FLAT = false>
 __host__ __device__ __forceinline__ void ax_slabs(const AxRay &r, const SlabAddr &A, const Vol &V,
                                                   int iu0, int stride, int nu, int ny,
                                                   Acc<KAHAN> &acc, Acc<KAHAN> &accl)
@@ -437,7 +440,7 @@ __host__ __device__ __forceinline__ void ax_slabs(const AxRay &r, const SlabAddr
     for (int k = 0; k < U; k++) {
         const int iu = iu0 + k * stride;
         if (iu < nu) {
-            ax_slab_geom<FT>(r, A, iu, ny, s[k]);
+            ax_slab_geom<FT, FLAT>(r, A, iu, ny, s[k]);
         } else {
             for (int p = 0; p < 3; p++) {
                 s[k].ok[p] = false;

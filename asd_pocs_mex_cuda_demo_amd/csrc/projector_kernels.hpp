@@ -54,15 +54,19 @@ __global__ __launch_bounds__(256) void ax_kernel(ViewGeom g, const float *__rest
         // slabs per load batch: two when a ray has 256 lanes, else four (a
         // lane then walks its slabs in one or two batches)
         constexpr int U = LPR >= 256 ? 2 : 4;
-        auto march = [&](auto ft) {
+        auto march = [&](auto ft, auto flat) {
             for (int iu0 = lane; iu0 < nu; iu0 += U * LPR)
-                ax_slabs<U, KAHAN, decltype(ft)::value>(r, A, V, iu0, LPR, nu, g.ny, sa, sl);
+                ax_slabs<U, KAHAN, decltype(ft)::value, VolRead<WIDE>, decltype(flat)::value>(
+                    r, A, V, iu0, LPR, nu, g.ny, sa, sl);
         };
-        // fast_t depends on the view only: one branch for the whole march
-        if (r.fast_t)
-            march(std::true_type{});
+        // fast_t depends on the view only, and every ray of a 2-D view is
+        // flat: one branch for the whole march
+        if (r.fast_t && r.flat)
+            march(std::true_type{}, std::true_type{});
+        else if (r.fast_t)
+            march(std::true_type{}, std::false_type{});
         else
-            march(std::false_type{});
+            march(std::false_type{}, std::false_type{});
     }
     // compensated partials are combined in FP64, plain ones in FP32
     using R = typename std::conditional<KAHAN, double, float>::type;

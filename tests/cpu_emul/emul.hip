@@ -154,8 +154,15 @@ static void emul_ax_t(const float *X, float *y, float *sr, const ViewGeom &g, co
             for (int lane = 0; lane < lpr; lane++) {
                 Acc<KAHAN> a[8], l[8];
                 if (rows == 1) {
+                    // flat rays take the kernels' flat-specialised slab code
                     for (int iu0 = lane; iu0 < nu; iu0 += 4 * lpr)
-                        if (buf)
+                        if (r[0].flat && buf)
+                            ax_slabs<4, KAHAN, false, VolRead<false>, true>(r[0], A, Vb, iu0, lpr,
+                                                                            nu, ny, a[0], l[0]);
+                        else if (r[0].flat)
+                            ax_slabs<4, KAHAN, false, VolRead<true>, true>(r[0], A, Vw, iu0, lpr,
+                                                                           nu, ny, a[0], l[0]);
+                        else if (buf)
                             ax_slabs<4, KAHAN, false>(r[0], A, Vb, iu0, lpr, nu, ny, a[0], l[0]);
                         else
                             ax_slabs<4, KAHAN, false>(r[0], A, Vw, iu0, lpr, nu, ny, a[0], l[0]);
