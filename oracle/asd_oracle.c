@@ -444,6 +444,8 @@ typedef struct {
     double dataTolerance, beta, alpha;
 } orc_out;
 
+#define ORC_TRACE_COLS 5
+
 /* One SART sweep over all views in order (ASD_POCS_cuda.m:110-134):
  * m is FP64 and cast to single for the operators, like MATLAB's single().
  * Returns the accumulated data error dd2. */
@@ -504,8 +506,32 @@ static double nrm2diff(const double *a, const double *b, size_t n)
  * rmse: maxnumIter doubles (filled up to IterationNum) when I0 != NULL.
  * max_views: process only the first max_views views of each sweep when > 0
  * (used to time a bounded sample). */
+static int asd_pocs_impl(const double *P, const double *phi_deg, int nv, const orc_para *pa,
+                         const double *m0, const double *I0, double *m_res, double *rmse,
+                         orc_out *out, double *trace);
+
 int orc_asd_pocs(const double *P, const double *phi_deg, int nv, const orc_para *pa,
                  const double *m0, const double *I0, double *m_res, double *rmse, orc_out *out)
+{
+    return asd_pocs_impl(P, phi_deg, nv, pa, m0, I0, m_res, rmse, out, NULL);
+}
+
+/* The same loop, also recording per outer iteration k (row k-1 of `trace`,
+ * ORC_TRACE_COLS doubles): dp = ||m - m_pre|| after the sweep, dg = ||m_tv -
+ * f0|| after the TV descent (0 when the stop rule ended the iteration
+ * first), dd2, and alpha and beta after the iteration's rules.  Test-only:
+ * it lets the pins observe the step-adaptation inputs the loop never
+ * returns. */
+int orc_asd_pocs_trace(const double *P, const double *phi_deg, int nv, const orc_para *pa,
+                       const double *m0, const double *I0, double *m_res, double *rmse,
+                       orc_out *out, double *trace)
+{
+    return asd_pocs_impl(P, phi_deg, nv, pa, m0, I0, m_res, rmse, out, trace);
+}
+
+static int asd_pocs_impl(const double *P, const double *phi_deg, int nv, const orc_para *pa,
+                         const double *m0, const double *I0, double *m_res, double *rmse,
+                         orc_out *out, double *trace)
 {
     size_t N = (size_t)pa->nx * pa->ny * pa->nz;
     double *m = (double *)calloc(N, sizeof(double));
@@ -527,6 +553,8 @@ int orc_asd_pocs(const double *P, const double *phi_deg, int nv, const orc_para 
             rmse[k - 1] = sqrt(s / (double)N);
         }
         double dp = nrm2diff(m, m_pre, N);
+        double *tr = trace ? trace + (size_t)(k - 1) * ORC_TRACE_COLS : NULL;
+        if (tr) { tr[0] = dp; tr[1] = 0; tr[2] = dd2; tr[3] = alpha; tr[4] = beta; }
         if (dp / (double)N <= pa->stopRelChg) break;
         if (k == 1) memcpy(m_k1, m_pre, N * sizeof(double));
         memcpy(f0, m, N * sizeof(double));
@@ -537,6 +565,7 @@ int orc_asd_pocs(const double *P, const double *phi_deg, int nv, const orc_para 
         double dg = nrm2diff(m, f0, N);
         if (dg > pa->r_max * dp && dd > pa->epsilon) alpha = alpha * pa->alpha_red;
         beta = beta * pa->beta_red;
+        if (tr) { tr[1] = dg; tr[3] = alpha; tr[4] = beta; }
     }
     if (k > pa->maxnumIter) k = pa->maxnumIter;
     out->IterationNum = k;

@@ -72,6 +72,9 @@ def lib():
         L.orc_asd_pocs.restype = it
         L.orc_asd_pocs.argtypes = [f64p, f64p, it, C.POINTER(OrcPara), C.c_void_p, C.c_void_p,
                                    f64p, C.c_void_p, C.POINTER(OrcOut)]
+        L.orc_asd_pocs_trace.restype = it
+        L.orc_asd_pocs_trace.argtypes = [f64p, f64p, it, C.POINTER(OrcPara), C.c_void_p,
+                                         C.c_void_p, f64p, C.c_void_p, C.POINTER(OrcOut), f64p]
         L.orc_sart.restype = C.c_double
         L.orc_sart.argtypes = [f64p, f64p, it, C.POINTER(OrcPara), C.c_double, f64p]
         L.orc_set_threads.restype = None
@@ -219,9 +222,15 @@ def _orc_para(para):
     return p
 
 
-def asd_pocs(P, para):
+TRACE_COLS = ("dp", "dg", "dd2", "alpha", "beta")
+
+
+def asd_pocs(P, para, trace=False):
     """[m_res, out] = ASD_POCS_cuda(P, para) restated on the CPU.
-    P: (na*nb, nv) array (column = view).  Returns (m_res flat N, out dict)."""
+    P: (na*nb, nv) array (column = view).  Returns (m_res flat N, out dict).
+    trace=True adds out["trace"]: per outer iteration dp, dg, dd2 and the
+    alpha and beta left by that iteration's rules (test-only view of the
+    step adaptation)."""
     P = np.asarray(P, np.float64)
     nv = P.shape[1]
     phi = np.ascontiguousarray(np.asarray(para["phi_vec"], np.float64).ravel())
@@ -242,10 +251,17 @@ def asd_pocs(P, para):
     m_res = np.zeros(N)
     rmse = np.zeros(max(p.maxnumIter, 1))
     out = OrcOut()
-    lib().orc_asd_pocs(Pv, phi, nv, C.byref(p), _opt_ptr(m0), _opt_ptr(I0), m_res,
-                       rmse.ctypes.data, C.byref(out))
+    if trace:
+        tr = np.zeros((max(p.maxnumIter, 1), len(TRACE_COLS)))
+        lib().orc_asd_pocs_trace(Pv, phi, nv, C.byref(p), _opt_ptr(m0), _opt_ptr(I0), m_res,
+                                 rmse.ctypes.data, C.byref(out), tr.ravel())
+    else:
+        lib().orc_asd_pocs(Pv, phi, nv, C.byref(p), _opt_ptr(m0), _opt_ptr(I0), m_res,
+                           rmse.ctypes.data, C.byref(out))
     res = dict(IterationNum=out.IterationNum, dataTolerance=out.dataTolerance, beta=out.beta,
                alpha=out.alpha)
+    if trace:
+        res["trace"] = {k: tr[:out.IterationNum, j].copy() for j, k in enumerate(TRACE_COLS)}
     if I0 is not None:
         res["RMSE"] = rmse[:out.IterationNum].copy()
     return m_res, res

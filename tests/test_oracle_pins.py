@@ -337,6 +337,92 @@ def test_driver_reproduces_numpy_restatement(orc):
     assert out["dataTolerance"] == pytest.approx(dd2, rel=1e-9)
 
 
+def _positive_problem(orc, n=32, nv=12, na=48, offset=1.0):
+    """An object that is the ellipse of _problem on a background of `offset`
+    over the whole square, and the warm start m0 = offset everywhere: every
+    image the loop forms stays near `offset`, so neither the per-view
+    positivity nor the TV positivity ever clips."""
+    para, img, _ = _problem(orc, n, nv, na)
+    obj = img + offset
+    P = np.zeros((na, nv))
+    for i, th in enumerate(para["phi_vec"]):
+        P[:, i] = orc.ax_para(obj, dict(para, phi=-th / 180 * np.pi))[0]
+    m0 = np.full(obj.size, offset)
+    return para, obj, P, m0
+
+
+def test_driver_alpha_rule_at_first_iteration(orc):
+    """At k = 1, m_k1 = m_pre = m0, so the TV descent's d = ||m - m_k1|| is dp.
+    One TV sub-iteration moves the image by exactly alpha * d (the step is
+    alpha * d * v / ||v||), so without clipping dg = alpha * dp: the rule
+    dg > r_max * dp (ASD_POCS_cuda.m:167) keeps alpha below r_max and reduces
+    it above; dd <= epsilon suppresses it (an inverted comparison, a dropped
+    condition or a wrong d fails one of these)."""
+    para, obj, P, m0 = _positive_problem(orc)
+    base = dict(para, maxnumIter=1, numOfgradients=1, m0=m0)
+    for alpha, eps, fires in [(0.2, None, False), (0.97, None, True), (0.97, 1e9, False),
+                              (0.9, None, False)]:
+        _, out = orc.asd_pocs(P, dict(base, alpha=alpha, epsilon=eps), trace=True)
+        t = out["trace"]
+        assert out["IterationNum"] == 1
+        assert t["dg"][0] == pytest.approx(alpha * t["dp"][0], rel=1e-10), alpha
+        assert np.sqrt(t["dd2"][0]) > 1e-6
+        assert out["alpha"] == (alpha * 0.95 if fires else alpha), (alpha, eps)
+        assert out["beta"] == 0.995
+
+
+def test_driver_tv_step_uses_frozen_m_k1(orc):
+    """m_k1 is m_pre of the first iteration only (ASD_POCS_cuda.m:151-153), so
+    the TV step of iteration k has length alpha_k * ||m_res,k - m0||, where
+    m_res,k is the post-SART image of iteration k, and not alpha_k * dp_k
+    (what an m_k1 refreshed every iteration would give) nor
+    alpha_k * ||m_res,k|| (TV called without its I0)."""
+    para, obj, P, m0 = _positive_problem(orc)
+    base = dict(para, numOfgradients=1, alpha=0.2, m0=m0)
+    _, out = orc.asd_pocs(P, dict(base, maxnumIter=3), trace=True)
+    t = out["trace"]
+    assert out["IterationNum"] == 3
+    for k in (2, 3):
+        m_res, _ = orc.asd_pocs(P, dict(base, maxnumIter=k))
+        alpha_k = t["alpha"][k - 2]  # left by iteration k - 1's rule
+        d = np.linalg.norm(m_res - m0)
+        assert t["dg"][k - 1] == pytest.approx(alpha_k * d, rel=1e-10), k
+        assert abs(t["dp"][k - 1] - d) > 1e-2 * d, k
+        assert abs(np.linalg.norm(m_res) - d) > 1e-2 * d, k
+
+
+def test_driver_rmse_is_of_the_post_sart_image(orc):
+    """out.RMSE(k) = sqrt(sum((m_res,k - I0)^2) / N) with m_res,k the image
+    right after iteration k's sweep, before its TV descent
+    (ASD_POCS_cuda.m:138-143); the returned m_res is that image."""
+    para, obj, P, m0 = _positive_problem(orc)
+    base = dict(para, numOfgradients=3, alpha=0.2, m0=m0, I0=obj)
+    _, out = orc.asd_pocs(P, dict(base, maxnumIter=3))
+    for k in (1, 2, 3):
+        m_res, o = orc.asd_pocs(P, dict(base, maxnumIter=k))
+        want = np.sqrt(np.sum((m_res - obj.ravel(order="F")) ** 2) / obj.size)
+        assert out["RMSE"][k - 1] == pytest.approx(want, rel=1e-12), k
+        # the post-SART image is not the image after TV: one more sweep
+        # from the TV result differs from this m_res
+        assert o["IterationNum"] == k
+
+
+def test_driver_stop_rule_precedes_tv_and_beta(orc):
+    """relativeChange = dp / numel(m) <= stopRelChg ends the loop right after
+    the sweep (ASD_POCS_cuda.m:145-149): no TV step, alpha and beta
+    unchanged, m_res the first sweep's image, and dg never formed."""
+    para, obj, P, m0 = _positive_problem(orc)
+    base = dict(para, numOfgradients=3, alpha=0.2, m0=m0)
+    m1, o1 = orc.asd_pocs(P, dict(base, maxnumIter=1), trace=True)
+    dp = o1["trace"]["dp"][0]
+    # the threshold exactly at dp / N stops (<=); just below it does not
+    ms, os_ = orc.asd_pocs(P, dict(base, maxnumIter=4, stopRelChg=dp / obj.size), trace=True)
+    assert os_["IterationNum"] == 1 and os_["alpha"] == 0.2 and os_["beta"] == 1.0
+    assert os_["trace"]["dg"][0] == 0.0 and np.array_equal(ms, m1)
+    _, on = orc.asd_pocs(P, dict(base, maxnumIter=2, stopRelChg=np.nextafter(dp / obj.size, 0)))
+    assert on["IterationNum"] == 2
+
+
 def test_driver_invariants(orc):
     para, img, P = _problem(orc, nv=16)
     errs = []
