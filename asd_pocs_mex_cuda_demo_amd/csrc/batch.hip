@@ -264,11 +264,11 @@ __global__ __launch_bounds__(256) void tv_sub_batch(const BCase *__restrict__ cs
         const int x = cur.x, y = cur.y, z = cur.z;
         float g;
         if (it == 0) {
-            const TvSrc<float, float, false> U{src_u, nullptr, 0.0, C.nx, C.ny, C.nz};
+            const TvSrc<float, float, false> U{src_u, nullptr, TvStepVal{0.0, 0.0, 1.0}, C.nx, C.ny, C.nz};
             const TvAt<float, float, false> at{U, i, x, y, z};
             g = C.rules.three_d ? tv3_grad<float>(at, x, y, z) : tv2_grad<float>(at, x, y);
         } else {
-            const TvSrc<float, float, true> U{u, vp, st, C.nx, C.ny, C.nz};
+            const TvSrc<float, float, true> U{u, vp, TvStepVal{st, 0.0, 1.0}, C.nx, C.ny, C.nz};
             const TvAt<float, float, true> at{U, i, x, y, z};
             g = C.rules.three_d ? tv3_grad<float>(at, x, y, z) : tv2_grad<float>(at, x, y);
             un[i] = U.stored(i);

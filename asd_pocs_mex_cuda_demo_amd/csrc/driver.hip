@@ -76,18 +76,47 @@ struct EpiSartAtx {
     }
 };
 
+// The update of the reference-arithmetic plan: m in FP64 as MATLAB keeps it,
+// m = m + beta * (bProj ./ sumC), then m(m < 0) = 0 (ASD_POCS_cuda.m:126-131),
+// and its FP32 shadow ms = single(m), which the next view's forward
+// projector reads (m:117).
+struct EpiSartAtx64 {
+    double *m;
+    float *ms;
+    const double *relax;
+    const int *stopped;
+    float scale;
+    struct Pre {
+        double m;
+    };
+    __device__ __forceinline__ bool skip() const { return *stopped != 0; }
+    __device__ __forceinline__ Pre load(long v) const { return Pre{m[v]}; }
+    __device__ __forceinline__ void operator()(long v, float acc, float accl, const Pre &q) const
+    {
+#pragma clang fp contract(off)
+        float bp = acc * scale, sc = accl * scale;
+        double c = (sc == 0.0f) ? 1.0 : (double)sc;
+        double nm = q.m + (*relax) * ((double)bp / c);
+        nm = nm < 0 ? 0.0 : nm;
+        m[v] = nm;
+        ms[v] = (float)nm;
+    }
+};
+
 // After a sweep: dp^2, d^2, rmse^2 over the image (and f0 = m) and dd2 over the
-// rays (and the residual buffer re-zeroed), as block partials.
+// rays (and the residual buffer re-zeroed), as block partials.  T: the image
+// type (FP32, or FP64 in the reference-arithmetic plan).
+template <class T>
 __global__ __launch_bounds__(256) void post_sart_kernel(
-    const float *__restrict__ m, const float *__restrict__ mpre, const float *__restrict__ mk1,
-    const float *__restrict__ I0, float *__restrict__ f0, double *__restrict__ res2, size_t N,
+    const T *__restrict__ m, const T *__restrict__ mpre, const T *__restrict__ mk1,
+    const T *__restrict__ I0, T *__restrict__ f0, double *__restrict__ res2, size_t N,
     size_t M, double *partials, const DevCtl *ctl)
 {
     if (ctl->stopped) return;
     double v[4] = {0.0, 0.0, 0.0, 0.0};
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += stride) {
-        float mi = m[i];
+        T mi = m[i];
         double a = (double)mi - (double)mpre[i];
         double b = (double)mi - (double)mk1[i];
         v[0] += a * a;
@@ -106,16 +135,23 @@ __global__ __launch_bounds__(256) void post_sart_kernel(
 }
 
 // Folds post_sart's partials; applies the stop rule and records the RMSE
-// (ASD_POCS_cuda.m:136-149).  One block.
+// (ASD_POCS_cuda.m:136-149).  One block.  serial: dp^2, d^2 and rmse^2 are
+// already in scal (summed in index order) and replace the folded ones.
 __global__ __launch_bounds__(RED_THREADS) void ctl_after_sart(const double *partials, int nblocks,
                                                               size_t N, int has_I0, double *scal,
                                                               DevCtl *ctl, double *hist,
-                                                              CtlRules rules, int sweep_only)
+                                                              CtlRules rules, int sweep_only,
+                                                              int serial)
 {
     if (ctl->stopped) return;
     double tot[4];
     fold_rows<4>(partials, nblocks, tot);
     if (threadIdx.x != 0) return;
+    if (serial) {
+        tot[0] = scal[TV_SLOT_DP2];
+        tot[1] = scal[TV_SLOT_D2];
+        tot[2] = scal[TV_SLOT_RMSE2];
+    }
     scal[TV_SLOT_DP2] = tot[0];
     scal[TV_SLOT_D2] = tot[1];
     scal[TV_SLOT_RMSE2] = tot[2];
@@ -130,33 +166,39 @@ __global__ __launch_bounds__(RED_THREADS) void ctl_after_sart(const double *part
     if (dp / (double)N <= rules.stopRelChg) ctl->stopped = 1;
 }
 
-// After TV: dg^2 = ||m - f0||^2 partials and m_pre = m for the next iteration.
-__global__ __launch_bounds__(256) void post_tv_kernel(const float *__restrict__ m,
-                                                      const float *__restrict__ f0,
-                                                      float *__restrict__ mpre, size_t N,
-                                                      double *partials, const DevCtl *ctl)
+// After TV: dg^2 = ||m - f0||^2 partials and m_pre = m for the next
+// iteration; ms (when not NULL) receives single(m) for the next sweep.
+template <class T>
+__global__ __launch_bounds__(256) void post_tv_kernel(const T *__restrict__ m,
+                                                      const T *__restrict__ f0,
+                                                      T *__restrict__ mpre, float *__restrict__ ms,
+                                                      size_t N, double *partials,
+                                                      const DevCtl *ctl)
 {
     if (ctl->stopped) return;
     double v[1] = {0.0};
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += stride) {
-        float mi = m[i];
+        T mi = m[i];
         double e = (double)mi - (double)f0[i];
         v[0] += e * e;
         mpre[i] = mi;
+        if (ms) ms[i] = (float)mi;
     }
     store_partials<1>(v, partials);
 }
 
 // Folds dg^2 and applies the alpha and beta rules (ASD_POCS_cuda.m:166-170).
+// serial: dg^2 is already in scal (summed in index order).
 __global__ __launch_bounds__(RED_THREADS) void ctl_after_tv(const double *partials, int nblocks,
                                                             double *scal, DevCtl *ctl,
-                                                            CtlRules rules)
+                                                            CtlRules rules, int serial)
 {
     if (ctl->stopped) return;
     double tot[1];
     fold_rows<1>(partials, nblocks, tot);
     if (threadIdx.x != 0) return;
+    if (serial) tot[0] = scal[TV_SLOT_DG2];
     scal[TV_SLOT_DG2] = tot[0];
     const double dg = sqrt(tot[0]);
     ctl->dg = dg;
@@ -174,12 +216,13 @@ __global__ __launch_bounds__(256) void f64_to_f32(const double *__restrict__ a, 
 }
 
 
-__global__ __launch_bounds__(256) void copy3_f32(const float *__restrict__ a, float *b, float *c,
-                                                 float *d, size_t n)
+template <class T>
+__global__ __launch_bounds__(256) void copy3_kernel(const T *__restrict__ a, T *b, T *c, T *d,
+                                                    size_t n)
 {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (size_t)gridDim.x * blockDim.x) {
-        float x = a[i];
+        T x = a[i];
         b[i] = x;
         c[i] = x;
         d[i] = x;
@@ -239,7 +282,7 @@ int check_para(const asd_para *p, int nv)
     ASD_ARG(p->vxlsize > 0 && p->detsize > 0, "para.vxlsize and para.detsize must be positive");
     ASD_ARG(p->SO > 0 && p->OD > 0, "para.SO and para.OD must be positive");
     ASD_ARG(p->numOfgradients >= 0, "para.numOfgradients must be >= 0");
-    ASD_ARG(p->precision >= 0 && p->precision <= 3, "para.precision must be 0, 1, 2 or 3");
+    ASD_ARG(p->precision >= 0 && p->precision <= 4, "para.precision must be 0, 1, 2, 3 or 4");
     return ASD_OK;
 }
 
@@ -314,6 +357,11 @@ SinoF16 view_codec<SinoF16>(asd_plan *p, int iv)
 {
     return SinoF16{p->P.as<__half>() + (size_t)iv * p->M};
 }
+template <>
+SinoF64 view_codec<SinoF64>(asd_plan *p, int iv)
+{
+    return SinoF64{p->P.as<double>() + (size_t)iv * p->M};
+}
 
 // One SART sweep over all views with relaxation *relax.  The precision
 // variants read a 16-bit sinogram and compensate every FP32 sum; EXACT
@@ -351,6 +399,40 @@ int enqueue_sweep_t(asd_plan *p, const double *relax)
     return ASD_OK;
 }
 
+// The sweep of the reference-arithmetic plan: the FP64 sinogram and image,
+// the forward projector reading single(m) and summing every ray in slab
+// order (as the reference kernel does), the backprojector's update in FP64.
+// Every per-view quantity is the reference's arithmetic, so the sweep
+// follows the reference bit for bit.  A forward shape forced with
+// asd_plan_set_ax_shape replaces the slab-order projector (for measurement).
+int enqueue_sweep_ref(asd_plan *p, const double *relax)
+{
+    const int *stopped = &p->ctl()->stopped;
+    for (int iv = 0; iv < p->nv; iv++) {
+        const ViewGeom &g = p->views[iv];
+        const RayRec *rt = p->rt.as<RayRec>() + (size_t)iv * p->M;
+        EpiSartAx<SinoF64> ea{view_codec<SinoF64>(p, iv), p->cor.as<float>(),
+                              p->res2.as<double>(), stopped, g.scale};
+        int rc;
+        {
+            StageTimer t(p, ST_AX, true);
+            rc = p->ax_forced
+                     ? dispatch_ax<false>(p->ax, g, p->ms.as<float>(), nullptr, nullptr, ea,
+                                          p->stream)
+                     : dispatch_ax_ordered(g, p->ms.as<float>(), nullptr, nullptr, ea, p->stream);
+            if (rc) return rc;
+        }
+        {
+            StageTimer t(p, ST_ATX, true);
+            const EpiSartAtx64 eb{p->m.as<double>(), p->ms.as<float>(), relax, stopped, g.scale};
+            if ((rc = dispatch_atx_variant<false>(p->atx_variant, g, p->cor.as<float>(), rt, eb,
+                                                  p->stream)))
+                return rc;
+        }
+    }
+    return ASD_OK;
+}
+
 // Backprojector epilogue for the autotuner (plain x and sc).
 struct EpiTune {
     float *x, *sc;
@@ -376,7 +458,8 @@ int autotune_atx(asd_plan *p)
     if (p->atx_variant >= 0) return ASD_OK;
     for (int i = 0; i < ATX_NVARIANT; i++) p->atx_us[i] = -1.0;
     switch (p->para.precision) {
-    case 0: return autotune_atx_t<false>(p);
+    case 0:
+    case 4: return autotune_atx_t<false>(p);
     case 3: p->atx_variant = ATX_DIRECT; return ASD_OK;  // own tiled FP64-geometry kernel
     default: return autotune_atx_t<true>(p);
     }
@@ -437,22 +520,73 @@ int enqueue_sweep(asd_plan *p, const double *relax)
     case 1: return enqueue_sweep_t<SinoU16, true, false>(p, relax);
     case 2: return enqueue_sweep_t<SinoF16, true, false>(p, relax);
     case 3: return enqueue_sweep_t<SinoU16, true, true>(p, relax);
+    case 4: return enqueue_sweep_ref(p, relax);
     default: return enqueue_sweep_t<SinoF32, false, false>(p, relax);
     }
 }
 
-int enqueue_post_sart(asd_plan *p, int sweep_only)
+template <class T>
+int enqueue_post_sart_t(asd_plan *p, int sweep_only)
 {
     const unsigned nb = red_blocks(p->N > p->M ? p->N : p->M);
-    post_sart_kernel<<<nb, RED_THREADS, 0, p->stream>>>(
-        p->m.as<float>(), p->mpre.as<float>(), p->mk1.as<float>(),
-        p->have_I0 ? p->I0.as<float>() : nullptr, p->f0.as<float>(), p->res2.as<double>(), p->N,
-        p->M, p->red.partials, p->ctl());
+    post_sart_kernel<T><<<nb, RED_THREADS, 0, p->stream>>>(
+        p->m.as<T>(), p->mpre.as<T>(), p->mk1.as<T>(), p->have_I0 ? p->I0.as<T>() : nullptr,
+        p->f0.as<T>(), p->res2.as<double>(), p->N, p->M, p->red.partials, p->ctl());
     ASD_LAUNCH_CHECK("post_sart_kernel");
+    // the reference-arithmetic plan sums dp^2, d^2 and rmse^2 in index order,
+    // as the reference's loops (and the oracle) do
+    const bool serial = p->ref64();
+    if (serial) {
+        const int *stopped = &p->ctl()->stopped;
+        double *sc = p->red.scalars;
+        int rc;
+        if ((rc = launch_serial_sumsq<T>(p->m.as<T>(), p->mpre.as<T>(), p->N, sc + TV_SLOT_DP2,
+                                         p->stream, stopped)) ||
+            (rc = launch_serial_sumsq<T>(p->m.as<T>(), p->mk1.as<T>(), p->N, sc + TV_SLOT_D2,
+                                         p->stream, stopped)) ||
+            (p->have_I0 && (rc = launch_serial_sumsq<T>(p->m.as<T>(), p->I0.as<T>(), p->N,
+                                                        sc + TV_SLOT_RMSE2, p->stream, stopped))))
+            return rc;
+    }
     ctl_after_sart<<<1, RED_THREADS, 0, p->stream>>>(p->red.partials, (int)nb, p->N,
                                                      p->have_I0 ? 1 : 0, p->red.scalars, p->ctl(),
-                                                     p->hist.as<double>(), p->rules(), sweep_only);
+                                                     p->hist.as<double>(), p->rules(), sweep_only,
+                                                     serial ? 1 : 0);
     ASD_LAUNCH_CHECK("ctl_after_sart");
+    return ASD_OK;
+}
+
+int enqueue_post_sart(asd_plan *p, int sweep_only)
+{
+    return p->ref64() ? enqueue_post_sart_t<double>(p, sweep_only)
+                      : enqueue_post_sart_t<float>(p, sweep_only);
+}
+
+// TV descent of the loop (ASD_POCS_cuda.m:155-164) on the image type T, and
+// post_tv (dg, m_pre = m, and the shadow of the reference-arithmetic plan).
+template <class T>
+int enqueue_tv_post(asd_plan *p)
+{
+    int rc;
+    DevCtl *ctl = p->ctl();
+    const asd_para &pa = p->para;
+    const bool three_d = pa.nz >= 2;
+    {
+        StageTimer t(p, ST_TVGRAD, false);
+        if ((rc = tv_descent<T, T>(p->m.as<T>(), p->mk1.as<T>(), p->v.as<T>(), p->u2.as<T>(),
+                                   p->v2.as<T>(), pa.nx, pa.ny, three_d ? pa.nz : 1, three_d, 0.0,
+                                   &ctl->alpha, pa.numOfgradients, 1, p->red, p->stream, true,
+                                   &ctl->stopped, p->ref64())))
+            return rc;
+    }
+    const unsigned nb = red_blocks(p->N);
+    post_tv_kernel<T><<<nb, RED_THREADS, 0, p->stream>>>(
+        p->m.as<T>(), p->f0.as<T>(), p->mpre.as<T>(), p->ref64() ? p->ms.as<float>() : nullptr,
+        p->N, p->red.partials, ctl);
+    ASD_LAUNCH_CHECK("post_tv_kernel");
+    if (p->ref64())
+        return launch_serial_sumsq<T>(p->m.as<T>(), p->f0.as<T>(), p->N,
+                                      p->red.scalars + TV_SLOT_DG2, p->stream, &ctl->stopped);
     return ASD_OK;
 }
 
@@ -463,24 +597,10 @@ int enqueue_iteration(asd_plan *p)
     DevCtl *ctl = p->ctl();
     if ((rc = enqueue_sweep(p, &ctl->beta))) return rc;
     if ((rc = enqueue_post_sart(p, 0))) return rc;
-    const asd_para &pa = p->para;
-    const bool three_d = pa.nz >= 2;
-    {
-        StageTimer t(p, ST_TVGRAD, false);
-        if ((rc = tv_descent<float, float>(p->m.as<float>(), p->mk1.as<float>(),
-                                           p->v.as<float>(), p->u2.as<float>(), p->v2.as<float>(),
-                                           pa.nx, pa.ny, three_d ? pa.nz : 1,
-                                           three_d, 0.0, &ctl->alpha, pa.numOfgradients, 1,
-                                           p->red, p->stream, true, &ctl->stopped)))
-            return rc;
-    }
+    if ((rc = p->ref64() ? enqueue_tv_post<double>(p) : enqueue_tv_post<float>(p))) return rc;
     const unsigned nb = red_blocks(p->N);
-    post_tv_kernel<<<nb, RED_THREADS, 0, p->stream>>>(p->m.as<float>(), p->f0.as<float>(),
-                                                      p->mpre.as<float>(), p->N, p->red.partials,
-                                                      ctl);
-    ASD_LAUNCH_CHECK("post_tv_kernel");
     ctl_after_tv<<<1, RED_THREADS, 0, p->stream>>>(p->red.partials, (int)nb, p->red.scalars, ctl,
-                                                   p->rules());
+                                                   p->rules(), p->ref64() ? 1 : 0);
     ASD_LAUNCH_CHECK("ctl_after_tv");
     return ASD_OK;
 }
@@ -533,9 +653,13 @@ int write_ctl(asd_plan *p)
 // (m_k1 is m_pre of the first iteration, ASD_POCS_cuda.m:151-153).
 int rebase(asd_plan *p)
 {
-    copy3_f32<<<grid_for(p->N), 256, 0, p->stream>>>(p->m.as<float>(), p->mpre.as<float>(),
-                                                      p->mk1.as<float>(), p->f0.as<float>(), p->N);
-    ASD_LAUNCH_CHECK("copy3_f32");
+    if (p->ref64())
+        copy3_kernel<double><<<grid_for(p->N), 256, 0, p->stream>>>(
+            p->m.as<double>(), p->mpre.as<double>(), p->mk1.as<double>(), p->f0.as<double>(), p->N);
+    else
+        copy3_kernel<float><<<grid_for(p->N), 256, 0, p->stream>>>(
+            p->m.as<float>(), p->mpre.as<float>(), p->mk1.as<float>(), p->f0.as<float>(), p->N);
+    ASD_LAUNCH_CHECK("copy3_kernel");
     ASD_HIP(hipStreamSynchronize(p->stream));
     return ASD_OK;
 }
@@ -552,12 +676,37 @@ int upload_f64_as_f32(asd_plan *p, DeviceBuf &dst, const double *src, size_t n)
     return ASD_OK;
 }
 
+// An image of the plan from the host's FP64: FP64 as it is in the
+// reference-arithmetic plan (with `shadow`, when not NULL, receiving
+// single()), FP32 otherwise.
+int upload_image(asd_plan *p, DeviceBuf &dst, const double *src, size_t n, DeviceBuf *shadow)
+{
+    if (!p->ref64()) return upload_f64_as_f32(p, dst, src, n);
+    ASD_HIP(hipMemcpyAsync(dst.p, src, n * sizeof(double), hipMemcpyHostToDevice, p->stream));
+    if (shadow) {
+        f64_to_f32<<<grid_for(n), 256, 0, p->stream>>>(dst.as<double>(), shadow->as<float>(), n);
+        ASD_LAUNCH_CHECK("f64_to_f32");
+    }
+    ASD_HIP(hipStreamSynchronize(p->stream));
+    return ASD_OK;
+}
+
+int download_image(asd_plan *p, const DeviceBuf &src, double *dst, size_t n);
+
 int download_f32_as_f64(asd_plan *p, const DeviceBuf &src, double *dst, size_t n)
 {
     std::vector<float> h(n);
     ASD_HIP(hipMemcpyAsync(h.data(), src.p, n * sizeof(float), hipMemcpyDeviceToHost, p->stream));
     ASD_HIP(hipStreamSynchronize(p->stream));
     for (size_t i = 0; i < n; i++) dst[i] = (double)h[i];
+    return ASD_OK;
+}
+
+int download_image(asd_plan *p, const DeviceBuf &src, double *dst, size_t n)
+{
+    if (!p->ref64()) return download_f32_as_f64(p, src, dst, n);
+    ASD_HIP(hipMemcpyAsync(dst, src.p, n * sizeof(double), hipMemcpyDeviceToHost, p->stream));
+    ASD_HIP(hipStreamSynchronize(p->stream));
     return ASD_OK;
 }
 
@@ -635,10 +784,12 @@ int asd_plan_create(const asd_para *para, const double *phi_vec, int nv, asd_pla
             set_error(ASD_ERR_HIP, "hipEventCreate failed");
             return fail(ASD_ERR_HIP);
         }
+    const size_t ib = p->img_bytes();
     if ((rc = p->ydet.ensure(yall.size() * 4)) || (rc = p->zdet.ensure(zall.size() * 4)) ||
-        (rc = p->m.ensure(N * 4)) || (rc = p->mpre.ensure(N * 4)) ||
-        (rc = p->mk1.ensure(N * 4)) || (rc = p->f0.ensure(N * 4)) || (rc = p->v.ensure(N * 4)) ||
-        (rc = p->u2.ensure(N * 4)) || (rc = p->v2.ensure(N * 4)) ||
+        (rc = p->m.ensure(N * ib)) || (rc = p->mpre.ensure(N * ib)) ||
+        (rc = p->mk1.ensure(N * ib)) || (rc = p->f0.ensure(N * ib)) || (rc = p->v.ensure(N * ib)) ||
+        (rc = p->u2.ensure(N * ib)) || (rc = p->v2.ensure(N * ib)) ||
+        (p->ref64() && (rc = p->ms.ensure(N * 4))) ||
         (rc = p->cor.ensure(M * 4)) || (rc = p->res2.ensure(M * 8)) ||
         (rc = p->rt.ensure((size_t)nv * M * sizeof(RayRec))) ||
         (rc = p->P.ensure((size_t)nv * M * sino_bytes(para->precision))) ||
@@ -647,7 +798,7 @@ int asd_plan_create(const asd_para *para, const double *phi_vec, int nv, asd_pla
         return fail(rc);
     if (hipMemcpy(p->ydet.p, yall.data(), yall.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(p->zdet.p, zall.data(), zall.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemset(p->m.p, 0, N * 4) != hipSuccess || hipMemset(p->res2.p, 0, M * 8) != hipSuccess) {
+        hipMemset(p->m.p, 0, N * ib) != hipSuccess || hipMemset(p->res2.p, 0, M * 8) != hipSuccess) {
         set_error(ASD_ERR_HIP, "plan initialisation failed");
         return fail(ASD_ERR_HIP);
     }
@@ -699,6 +850,8 @@ int asd_plan_set_projections(asd_plan *p, const double *P)
         std::vector<__half> h(n);
         for (size_t i = 0; i < n; i++) h[i] = __float2half_rn((float)P[i]);
         ASD_HIP(hipMemcpy(p->P.p, h.data(), n * 2, hipMemcpyHostToDevice));
+    } else if (p->para.precision == 4) {
+        ASD_HIP(hipMemcpy(p->P.p, P, n * 8, hipMemcpyHostToDevice));  // FP64, as MATLAB holds it
     } else if ((rc = upload_f64_as_f32(p, p->P, P, n))) {
         return rc;
     }
@@ -712,14 +865,15 @@ int asd_plan_reset(asd_plan *p, const double *m0, const double *I0)
     ASD_ARG(p, "NULL plan");
     int rc;
     if (m0) {
-        if ((rc = upload_f64_as_f32(p, p->m, m0, p->N))) return rc;
+        if ((rc = upload_image(p, p->m, m0, p->N, &p->ms))) return rc;
     } else {
-        ASD_HIP(hipMemsetAsync(p->m.p, 0, p->N * 4, p->stream));
+        ASD_HIP(hipMemsetAsync(p->m.p, 0, p->N * p->img_bytes(), p->stream));
+        if (p->ref64()) ASD_HIP(hipMemsetAsync(p->ms.p, 0, p->N * 4, p->stream));
     }
     bool had_I0 = p->have_I0;
     if (I0) {
-        if ((rc = p->I0.ensure(p->N * 4))) return rc;
-        if ((rc = upload_f64_as_f32(p, p->I0, I0, p->N))) return rc;
+        if ((rc = p->I0.ensure(p->N * p->img_bytes()))) return rc;
+        if ((rc = upload_image(p, p->I0, I0, p->N, nullptr))) return rc;
         p->have_I0 = true;
         p->I0_sim = false;
     } else {
@@ -792,14 +946,14 @@ int asd_plan_get_image(asd_plan *p, double *m_res)
 {
     clear_error();
     ASD_ARG(p && m_res, "NULL argument");
-    return download_f32_as_f64(p, p->f0, m_res, p->N);
+    return download_image(p, p->f0, m_res, p->N);
 }
 
 int asd_plan_get_current(asd_plan *p, double *m)
 {
     clear_error();
     ASD_ARG(p && m, "NULL argument");
-    return download_f32_as_f64(p, p->m, m, p->N);
+    return download_image(p, p->m, m, p->N);
 }
 
 int asd_plan_get_out(asd_plan *p, asd_out *o, double *rmse)
@@ -876,6 +1030,7 @@ int asd_plan_set_ax_shape(asd_plan *p, int lanes_per_ray, int rows_per_bundle)
     ASD_ARG(p, "NULL plan");
     if (lanes_per_ray == 0 && rows_per_bundle == 0) {
         p->ax = ax_shape(p->para.na, p->para.nb, std::max(p->para.nx, p->para.ny));
+        p->ax_forced = false;
     } else {
         const int l = lanes_per_ray, r = rows_per_bundle;
         ASD_ARG((r == 1 && l >= 1 && l <= 256 && (l & (l - 1)) == 0) ||
@@ -884,6 +1039,7 @@ int asd_plan_set_ax_shape(asd_plan *p, int lanes_per_ray, int rows_per_bundle)
                 "rows), or (0, 0) for the default");
         p->ax.lpr = l;
         p->ax.rows = r;
+        p->ax_forced = true;
     }
     drop_graph(p);
     return ASD_OK;
@@ -903,8 +1059,10 @@ int asd_plan_info(asd_plan *p, int *lanes_per_ray, int *rows_per_bundle, void **
 {
     clear_error();
     ASD_ARG(p, "NULL plan");
-    if (lanes_per_ray) *lanes_per_ray = p->ax.lpr;
-    if (rows_per_bundle) *rows_per_bundle = p->ax.rows;
+    // the reference-arithmetic plan's slab-order projector reports 0 lanes
+    const bool ordered = p->ref64() && !p->ax_forced;
+    if (lanes_per_ray) *lanes_per_ray = ordered ? 0 : p->ax.lpr;
+    if (rows_per_bundle) *rows_per_bundle = ordered ? 1 : p->ax.rows;
     if (stream) *stream = (void *)p->stream;
     return ASD_OK;
 }

@@ -36,7 +36,8 @@ extern "C" void mexFunction(int nlhs, mxArray *plhs[], int nrhs, const mxArray *
     p.r_max = asdmex::opt_field(pa, "r_max", p.r_max);
     p.epsilon = asdmex::opt_field(pa, "epsilon", p.epsilon);
     p.stopRelChg = asdmex::opt_field(pa, "stopRelChg", p.stopRelChg);
-    p.initBySART = asdmex::opt_field(pa, "initBySART", 0.0) != 0.0;
+    // ASD_POCS_cuda.m:24 tests para.initBySART == true, i.e. == 1
+    p.initBySART = asdmex::opt_field(pa, "initBySART", 0.0) == 1.0;
     p.lambda = asdmex::opt_field(pa, "lambda", 1.0);
     p.precision = (int)asdmex::opt_field(pa, "precision", 0.0);
     if (p.beta >= 2.0 || p.beta <= 0.0)

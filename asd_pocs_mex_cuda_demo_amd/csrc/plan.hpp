@@ -44,12 +44,16 @@ struct asd_plan {
     size_t N = 0, M = 0;
     std::vector<ViewGeom> views;
     hipStream_t stream = nullptr;
-    DeviceBuf ydet, zdet, P, m, mpre, mk1, f0, I0, cor, res2, rt, v, u2, v2, ctlbuf, hist;
+    // Images m, mpre, mk1, f0, I0 and the TV buffers v, u2, v2 hold FP32
+    // values, or FP64 ones in the reference-arithmetic plan (precision 4),
+    // which also keeps ms = single(m), the image its forward projector reads.
+    DeviceBuf ydet, zdet, P, m, mpre, mk1, f0, I0, cor, res2, rt, v, u2, v2, ctlbuf, hist, ms;
     std::vector<double> u16_lo, u16_step;  // per-view scale of the 16-bit fixed-point sinogram
     TvScratch red;
     bool have_P = false, have_I0 = false;
     bool I0_sim = false;            // the RMSE reference is a simulated phantom
     asd::AxShape ax;                // forward projector launch shape
+    bool ax_forced = false;         // ax set by asd_plan_set_ax_shape
     int *h_stop = nullptr;          // pinned copies of the stop flag (4 checkpoints)
     hipEvent_t stop_ev[4] = {};
     int atx_variant = -1;           // backprojector variant picked by the autotuner
@@ -68,6 +72,9 @@ struct asd_plan {
     long stage_cnt[ST_N] = {0, 0, 0};
 
     DevCtl *ctl() { return ctlbuf.as<DevCtl>(); }
+    // the reference-arithmetic plan: FP64 sinogram, image state and TV
+    bool ref64() const { return para.precision == 4; }
+    size_t img_bytes() const { return ref64() ? 8 : 4; }  // per image element
     CtlRules rules() const
     {
         return CtlRules{para.alpha_red, para.beta_red, para.r_max, para.epsilon, para.stopRelChg};

@@ -235,6 +235,97 @@ __global__ __launch_bounds__(256) void ax_rows_kernel(ViewGeom g, const float *_
     if (mine && !stop) epi(ray, (float)am, (float)bm, pre);
 }
 
+// Forward projector that sums every ray in slab order, the order of the
+// reference kernel, where one thread per ray adds slab after slab into
+// y[id] and sr[id] (cu:375-772): one wave per ray.  Lane j forms the pieces
+// of slab c + j of each run of 64 slabs (the products w*l*x and weights w*l
+// of ax_slabs); then the wave adds the run's terms in slab order and, per
+// slab, in piece order, reading each lane's terms as wave-uniform values.  A
+// piece outside the volume adds +0, which leaves every sum unchanged (a sum
+// that starts at +0 never becomes -0).  The sums are therefore the
+// reference's bit for bit, while a run's geometry and loads are spread over
+// the wave.  Used where results must follow the reference's order exactly
+// (the reference-arithmetic plan and the slab-ordered drop-in form).
+template <bool WIDE, class Epi>
+__global__ __launch_bounds__(256) void ax_ordered_kernel(ViewGeom g, const float *__restrict__ X,
+                                                         const float *__restrict__ y_det,
+                                                         const float *__restrict__ z_det, Epi epi)
+{
+#pragma clang fp contract(off)
+    const bool stop = epi.skip();
+    const long nrays = (long)g.na * g.nb;
+    const int lane = threadIdx.x & 63;
+    const long id = (long)xcd_block(blockIdx.x, gridDim.x) * 4 + threadIdx.x / 64;
+    if (id >= nrays) return;  // whole waves
+    const VolRead<WIDE> V(X, (size_t)g.nx * g.ny * g.nz);
+    typename Epi::Pre pre{};
+    if (lane == 0) pre = epi.load(id);
+    // id < na * nb < 2^31 and ib < nb < 2^24 (index_fits)
+    const unsigned ub = fastdiv((unsigned)id, g.div_na);
+    const int ia = (int)((unsigned)id - mad24(ub, (unsigned)g.na, 0u)), ib = (int)ub;
+    const float yd = y_det ? y_det[ia] : det_coord(ia, g.na, g.dy_det);
+    const float zd = z_det ? z_det[ib] : det_coord(ib, g.nb, g.dz_det);
+    const AxRay r = ax_ray_setup(g, yd, zd);
+    const SlabAddr A = slab_addr(r, g.nx, VolRead<WIDE>::unit);
+    const int nu = r.xdriven ? g.nx : g.ny;
+    float acc = 0.0f, accl = 0.0f;  // wave-uniform running sums
+    auto march = [&](auto ft, auto flat) {
+        for (int c0 = 0; c0 < nu; c0 += 64) {
+            const int iu = c0 + lane;
+            SlabW sw;
+            if (iu < nu) {
+                ax_slab_geom<decltype(ft)::value, decltype(flat)::value>(r, A, iu, g.ny, sw);
+            } else {
+                for (int p = 0; p < 3; p++) {
+                    sw.ok[p] = false;
+                    sw.off[p] = VOL_NONE;
+                    sw.wl[p] = unspecified(0.0f);
+                }
+            }
+            float t[3], tl[3];
+#pragma unroll
+            for (int p = 0; p < 3; p++) {
+                const float xv = V(sw.off[p], sw.ok[p]);
+                t[p] = sw.ok[p] ? sw.wl[p] * xv : 0.0f;
+                tl[p] = sw.ok[p] ? sw.wl[p] : 0.0f;
+            }
+            // only slabs with a piece in the volume add anything
+            unsigned long long live = __ballot(sw.ok[0] || sw.ok[1] || sw.ok[2]);
+            while (live) {
+                const int l = __builtin_ctzll(live);
+                live &= live - 1;
+#pragma unroll
+                for (int p = 0; p < 3; p++) {
+                    acc += __int_as_float(__builtin_amdgcn_readlane(__float_as_int(t[p]), l));
+                    accl += __int_as_float(__builtin_amdgcn_readlane(__float_as_int(tl[p]), l));
+                }
+            }
+        }
+    };
+    if (r.fast_t && r.flat)
+        march(std::true_type{}, std::true_type{});
+    else if (r.fast_t)
+        march(std::true_type{}, std::false_type{});
+    else
+        march(std::false_type{}, std::false_type{});
+    if (lane == 0 && !stop) epi(id, acc, accl, pre);
+}
+
+// Slab-order forward projector launch for one view (4 rays per block).
+template <class Epi>
+int dispatch_ax_ordered(const ViewGeom &g, const float *x, const float *y_det,
+                        const float *z_det, const Epi &epi, hipStream_t s)
+{
+    const long nrays = (long)g.na * g.nb;
+    const unsigned nblk = (unsigned)((nrays + 3) / 4);
+    if (vol_buffer_ok((size_t)g.nx * g.ny * g.nz, g.nx))
+        klaunch(ax_ordered_kernel<false, Epi>, nblk, 256, 0, s, g, x, y_det, z_det, epi);
+    else
+        klaunch(ax_ordered_kernel<true, Epi>, nblk, 256, 0, s, g, x, y_det, z_det, epi);
+    ASD_LAUNCH_CHECK("ax_ordered_kernel");
+    return ASD_OK;
+}
+
 template <bool KAHAN, bool WIDE, class Epi>
 int dispatch_ax_t(AxShape sh, const ViewGeom &g, const float *x, const float *y_det,
                 const float *z_det, const Epi &epi, hipStream_t s)

@@ -93,6 +93,50 @@ __device__ __forceinline__ void fold_rows(const double *partials, int nblocks, d
     for (int k = 0; k < K; k++) tot[k] = res[k];
 }
 
+// sum over i of (a[i] - b[i])^2 (b NULL: zeros) in index order, rounded
+// exactly as a serial loop forms it (the FP64 norms of the reference's TV
+// loops, imgTVGradMin.cpp:58-59, :96, imgTVGradMin3D.cpp:150-153): one wave
+// loads SER_U * 64 consecutive values at a time and adds their squares lane
+// by lane from wave-uniform reads.  Lanes past n add +0, which leaves the
+// non-negative sum unchanged.  Latency-bound (one dependent FP64 add per
+// element); the reference-arithmetic plan uses it where results must round
+// like the reference's.  Launch with one block of 64 threads.
+constexpr int SER_U = 8;
+
+__device__ __forceinline__ double readlane_f64(double v, int l)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double((long long)(unsigned)lo | ((long long)hi << 32));
+}
+
+template <class T>
+__global__ __launch_bounds__(64) void serial_sumsq_kernel(const T *__restrict__ a,
+                                                          const T *__restrict__ b, size_t n,
+                                                          double *out, const int *skip)
+{
+#pragma clang fp contract(off)
+    if (skip && *skip) return;
+    const int lane = threadIdx.x;
+    double acc = 0.0;
+    for (size_t base = 0; base < n; base += 64 * SER_U) {
+        double sq[SER_U];
+#pragma unroll
+        for (int u = 0; u < SER_U; u++) {
+            const size_t i = base + (size_t)u * 64 + lane;
+            double e = 0.0;
+            if (i < n) e = (double)a[i] - (b ? (double)b[i] : 0.0);
+            sq[u] = e * e;
+        }
+#pragma unroll
+        for (int u = 0; u < SER_U; u++)
+#pragma unroll 8
+            for (int l = 0; l < 64; l++) acc += readlane_f64(sq[u], l);
+    }
+    if (lane == 0) *out = acc;
+}
+
 // out[k] = sum of row k (one block); skipped when *skip != 0.
 template <int K>
 __global__ __launch_bounds__(RED_THREADS) void fold_partials(const double *partials, int nblocks,

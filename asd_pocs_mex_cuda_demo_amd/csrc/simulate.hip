@@ -250,16 +250,25 @@ int asd_plan_simulate(asd_plan *p, const double *ellipsoids, int n_ell, double s
         const double t = r[7] * (M_PI / 180.0);
         E[k] = Ellipsoid{r[0], r[1], r[2], r[3], r[4], r[5], r[6], std::cos(t), std::sin(t)};
     }
-    DeviceBuf dE, stage, mm, srb;
+    DeviceBuf dE, stage, mm, srb, img32;
+    // the phantom in FP32 (the reference-arithmetic plan keeps its RMSE
+    // reference in FP64 and rasterises into a separate FP32 image first)
+    DeviceBuf &img = p->ref64() ? img32 : p->I0;
     if ((rc = dE.ensure(sizeof(Ellipsoid) * (n_ell > 0 ? n_ell : 1))) ||
-        (rc = p->I0.ensure(p->N * 4)) || (rc = srb.ensure(p->M * 4)))
+        (rc = img.ensure(p->N * 4)) || (rc = p->I0.ensure(p->N * p->img_bytes())) ||
+        (rc = srb.ensure(p->M * 4)))
         return rc;
     if (n_ell) ASD_HIP(hipMemcpyAsync(dE.p, E.data(), sizeof(Ellipsoid) * n_ell,
                                       hipMemcpyHostToDevice, p->stream));
     phantom_kernel<<<blocks_for(p->N), 256, 0, p->stream>>>(dE.as<Ellipsoid>(), n_ell, pa.nx,
                                                             pa.ny, pa.nz, scale,
-                                                            p->I0.as<float>());
+                                                            img.as<float>());
     ASD_LAUNCH_CHECK("phantom_kernel");
+    if (p->ref64()) {
+        f32_to_f64<<<blocks_for(p->N), 256, 0, p->stream>>>(img.as<float>(), p->I0.as<double>(),
+                                                            p->N);
+        ASD_LAUNCH_CHECK("f32_to_f64");
+    }
     // every view forward projected (the drop-in projector's kernel and lane
     // count) into an FP64 staging sinogram, then noise
     const size_t M = p->M, n = (size_t)p->nv * M;
@@ -268,7 +277,7 @@ int asd_plan_simulate(asd_plan *p, const double *ellipsoids, int n_ell, double s
         const ViewGeom &g = p->views[iv];
         const float *yd = p->ydet.as<float>() + (size_t)iv * pa.na;
         const float *zd = p->zdet.as<float>() + (size_t)iv * pa.nb;
-        if ((rc = launch_ax(g, p->I0.as<float>(), p->cor.as<float>(), srb.as<float>(), yd, zd,
+        if ((rc = launch_ax(g, img.as<float>(), p->cor.as<float>(), srb.as<float>(), yd, zd,
                             p->stream)))
             return rc;
         f32_to_f64<<<blocks_for(M), 256, 0, p->stream>>>(p->cor.as<float>(),
@@ -305,6 +314,8 @@ int asd_plan_simulate(asd_plan *p, const double *ellipsoids, int n_ell, double s
         encode_f16<<<blocks_for(n), 256, 0, p->stream>>>(stage.as<double>(), p->P.as<__half>(),
                                                           n);
         ASD_LAUNCH_CHECK("encode_f16");
+    } else if (pa.precision == 4) {
+        ASD_HIP(hipMemcpyAsync(p->P.p, stage.p, n * 8, hipMemcpyDeviceToDevice, p->stream));
     } else {
         encode_f32<<<blocks_for(n), 256, 0, p->stream>>>(stage.as<double>(), p->P.as<float>(),
                                                           n);
@@ -332,6 +343,8 @@ int asd_plan_get_projections(asd_plan *p, double *P)
         std::vector<__half> h(n);
         ASD_HIP(hipMemcpy(h.data(), p->P.p, n * 2, hipMemcpyDeviceToHost));
         for (size_t i = 0; i < n; i++) P[i] = (double)__half2float(h[i]);
+    } else if (p->para.precision == 4) {
+        ASD_HIP(hipMemcpy(P, p->P.p, n * 8, hipMemcpyDeviceToHost));
     } else {
         std::vector<uint16_t> h(n);
         ASD_HIP(hipMemcpy(h.data(), p->P.p, n * 2, hipMemcpyDeviceToHost));
@@ -348,6 +361,10 @@ int asd_plan_get_reference(asd_plan *p, double *I0)
     clear_error();
     ASD_ARG(p && I0, "NULL argument");
     ASD_ARG(p->have_I0, "the plan has no RMSE reference image");
+    if (p->ref64()) {
+        ASD_HIP(hipMemcpy(I0, p->I0.p, p->N * 8, hipMemcpyDeviceToHost));
+        return ASD_OK;
+    }
     std::vector<float> h(p->N);
     ASD_HIP(hipMemcpy(h.data(), p->I0.p, p->N * 4, hipMemcpyDeviceToHost));
     for (size_t i = 0; i < p->N; i++) I0[i] = h[i];

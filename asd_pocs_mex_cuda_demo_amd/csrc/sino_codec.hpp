@@ -5,6 +5,7 @@
 //                lo / step from the view's range (error <= step / 2)
 //   precision 2  FP16 (11-bit significand; kept for comparison)
 //   precision 3  as 1 (the backprojector also sets up its rays in FP64)
+//   precision 4  FP64, as MATLAB holds P (the reference-arithmetic plan)
 //
 // Each codec reads one value of one view as FP64, the type the SART
 // correction (P - vxl*y) is formed in.
@@ -32,6 +33,11 @@ struct SinoF16 {
     __device__ __forceinline__ double at(long i) const { return (double)__half2float(p[i]); }
 };
 
-inline size_t sino_bytes(int precision) { return precision == 0 ? 4 : 2; }
+struct SinoF64 {
+    const double *p;
+    __device__ __forceinline__ double at(long i) const { return p[i]; }
+};
+
+inline size_t sino_bytes(int precision) { return precision == 0 ? 4 : precision == 4 ? 8 : 2; }
 
 }  // namespace asd

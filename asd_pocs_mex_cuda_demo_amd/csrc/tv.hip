@@ -17,14 +17,20 @@ namespace {
 // the previous kernel's block partials inside every block, in the fixed
 // order of fold_partials, so each block gets the same value and no separate
 // fold kernel runs between sub-iterations.
-__device__ __forceinline__ double tv_step_size(double alpha, const double *alpha_dev,
-                                               const double *d2, const double *partials_prev,
-                                               int nb_prev)
+// nrm2_dev, when not NULL, holds ||v||^2 already (summed in index order by
+// serial_sumsq_kernel) and the partials are not folded.
+__device__ __forceinline__ TvStepVal tv_step_size(double alpha, const double *alpha_dev,
+                                                  const double *d2, const double *partials_prev,
+                                                  int nb_prev, const double *nrm2_dev)
 {
+#pragma clang fp contract(off)
     double nrm2[1];
-    fold_rows<1>(partials_prev, nb_prev, nrm2);
+    if (nrm2_dev)
+        nrm2[0] = *nrm2_dev;
+    else
+        fold_rows<1>(partials_prev, nb_prev, nrm2);
     const double a = alpha_dev ? *alpha_dev : alpha;
-    return a * sqrt(*d2) / sqrt(nrm2[0]);
+    return tv_step_val(a * sqrt(*d2), sqrt(nrm2[0]));
 }
 
 // One TV gradient: v = grad TV(u) with FP64 block partials of ||v||^2.  With
@@ -38,10 +44,11 @@ __global__ __launch_bounds__(256) void tv_grad_kernel(const T *__restrict__ u,
                                                       int nx, int ny, int nz, double alpha,
                                                       const double *alpha_dev, const double *d2,
                                                       const double *partials_prev, int nb_prev,
-                                                      double *partials, const int *skip)
+                                                      double *partials, const int *skip,
+                                                      const double *nrm2_dev)
 {
     if (skip && *skip) return;
-    TvSrc<T, A, STEPPED> U{u, v_prev, 0.0, nx, ny, nz};
+    TvSrc<T, A, STEPPED> U{u, v_prev, TvStepVal{0.0, 0.0, 1.0}, nx, ny, nz};
     const size_t n = (size_t)nx * ny * nz;
     double s = 0.0;
     // Each block walks one contiguous chunk of the image, and each XCD owns a
@@ -64,14 +71,15 @@ __global__ __launch_bounds__(256) void tv_grad_kernel(const T *__restrict__ u,
             (void)(THREE_D ? tv3_grad<A>(gat, cur.x, cur.y, cur.z)
                            : tv2_grad<A>(gat, cur.x, cur.y));
         }
-        U.s = tv_step_size(alpha, alpha_dev, d2, partials_prev, nb_prev);
+        U.st = tv_step_size(alpha, alpha_dev, d2, partials_prev, nb_prev, nrm2_dev);
         // the gathered values are used only after the fold: the empty asm ties
         // them to the step size, which keeps the compiler from converting them
         // (and so waiting for their loads) before the fold starts
 #pragma unroll
-        for (int k = 0; k < K; k++) asm volatile("" : "+v"(raw.u[k]), "+v"(raw.v[k]) : "v"(U.s));
+        for (int k = 0; k < K; k++)
+            asm volatile("" : "+v"(raw.u[k]), "+v"(raw.v[k]) : "v"(U.st.s), "v"(U.st.nrm));
         if (i < i1) {
-            const TvReplay<T, A, K> rep{raw, U.s, 0};
+            const TvReplay<T, A, K> rep{raw, U.st, 0};
             A g = THREE_D ? tv3_grad<A>(rep, cur.x, cur.y, cur.z) : tv2_grad<A>(rep, cur.x, cur.y);
             T gt = T(g);
             v[i] = gt;
@@ -102,16 +110,24 @@ __global__ __launch_bounds__(256) void tv_step_kernel(const T *__restrict__ u,
                                                       const double *__restrict__ alpha_dev,
                                                       const double *__restrict__ d2,
                                                       const double *__restrict__ partials_prev,
-                                                      int nb_prev, int clamp, const int *skip)
+                                                      int nb_prev, int clamp, const int *skip,
+                                                      const double *nrm2_dev)
 {
 #pragma clang fp contract(off)
     if (skip && *skip) return;
-    const double st = tv_step_size(alpha, alpha_dev, d2, partials_prev, nb_prev);
+    const TvStepVal st = tv_step_size(alpha, alpha_dev, d2, partials_prev, nb_prev, nrm2_dev);
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (size_t)gridDim.x * blockDim.x) {
-        double w = (double)u[i] - st * (double)v[i];
-        if (clamp && w < 0) w = 0;
-        u_out[i] = T(w);
+        if constexpr (sizeof(T) == sizeof(double)) {
+            T w = st.apply<T>(u[i], v[i]);
+            if (clamp && w < 0) w = 0;
+            u_out[i] = w;
+        } else {
+            // clamped in FP64 before the rounding to T
+            double w = (double)u[i] - st.s * (double)v[i];
+            if (clamp && w < 0) w = 0;
+            u_out[i] = T(w);
+        }
     }
 }
 
@@ -150,17 +166,33 @@ int launch_sqdiff(const T *a, const T *b, size_t n, TvScratch &s, int slot, hipS
     return ASD_OK;
 }
 
+template <class T>
+int launch_serial_sumsq(const T *a, const T *b, size_t n, double *out, hipStream_t st,
+                        const int *skip)
+{
+    serial_sumsq_kernel<T><<<1, 64, 0, st>>>(a, b, n, out, skip);
+    ASD_LAUNCH_CHECK("serial_sumsq_kernel");
+    return ASD_OK;
+}
+
 template <class T, class A>
 int tv_descent(T *u, const T *img_old, T *v, T *u2, T *v2, int nx, int ny, int nz, bool three_d,
                double alpha, const double *alpha_dev, int nTVIter, int pos, TvScratch &s,
-               hipStream_t st, bool have_d2, const int *skip)
+               hipStream_t st, bool have_d2, const int *skip, bool serial_norms)
 {
     const size_t n = (size_t)nx * ny * nz;
     int rc;
-    if (!have_d2 && (rc = launch_sqdiff<T>(u, img_old, n, s, TV_SLOT_D2, st))) return rc;
+    if (!have_d2) {
+        rc = serial_norms ? launch_serial_sumsq<T>(u, img_old, n, s.scalars + TV_SLOT_D2, st, skip)
+                          : launch_sqdiff<T>(u, img_old, n, s, TV_SLOT_D2, st);
+        if (rc) return rc;
+    }
     if (nTVIter <= 0) return ASD_OK;
     const unsigned gb = red_blocks(n), ub = stream_blocks(n);
     double *d2 = s.scalars + TV_SLOT_D2;
+    // serial norms: ||v||^2 of each gradient summed in index order into one
+    // slot, read by the next kernel instead of the block partials
+    double *nrm2 = serial_norms ? s.scalars + TV_SLOT_NRM2 : nullptr;
     // gradient partials alternate between two rows: kernel i writes one while
     // reading kernel i-1's in the other
     double *pp[2] = {s.partials, s.partials + RED_MAX_BLOCKS};
@@ -173,19 +205,21 @@ int tv_descent(T *u, const T *img_old, T *v, T *u2, T *v2, int nx, int ny, int n
             if (three_d)
                 tv_grad_kernel<T, A, true, false><<<gb, RED_THREADS, 0, st>>>(
                     cu, nullptr, nullptr, cv, nx, ny, nz, alpha, alpha_dev, d2, nullptr, 0, out,
-                    skip);
+                    skip, nullptr);
             else
                 tv_grad_kernel<T, A, false, false><<<gb, RED_THREADS, 0, st>>>(
                     cu, nullptr, nullptr, cv, nx, ny, 1, alpha, alpha_dev, d2, nullptr, 0, out,
-                    skip);
+                    skip, nullptr);
         } else {
             // step of sub-iteration it-1 fused into gradient it
             if (three_d)
                 tv_grad_kernel<T, A, true, true><<<gb, RED_THREADS, 0, st>>>(
-                    cu, cv, nu, nv, nx, ny, nz, alpha, alpha_dev, d2, prev, (int)gb, out, skip);
+                    cu, cv, nu, nv, nx, ny, nz, alpha, alpha_dev, d2, prev, (int)gb, out, skip,
+                    nrm2);
             else
                 tv_grad_kernel<T, A, false, true><<<gb, RED_THREADS, 0, st>>>(
-                    cu, cv, nu, nv, nx, ny, 1, alpha, alpha_dev, d2, prev, (int)gb, out, skip);
+                    cu, cv, nu, nv, nx, ny, 1, alpha, alpha_dev, d2, prev, (int)gb, out, skip,
+                    nrm2);
             T *t1 = cu, *t2 = cv;
             cu = nu;
             cv = nv;
@@ -193,11 +227,13 @@ int tv_descent(T *u, const T *img_old, T *v, T *u2, T *v2, int nx, int ny, int n
             nv = t2;
         }
         ASD_LAUNCH_CHECK("tv_grad_kernel");
+        if (serial_norms && (rc = launch_serial_sumsq<T>(cv, nullptr, n, nrm2, st, skip)))
+            return rc;
     }
     // last step (with positivity) lands in the caller's buffer u
     tv_step_kernel<T><<<ub, 256, 0, st>>>(cu, cv, u, n, alpha, alpha_dev, d2,
                                           pp[(nTVIter - 1) & 1], (int)gb, pos == 1 ? 1 : 0,
-                                          skip);
+                                          skip, nrm2);
     ASD_LAUNCH_CHECK("tv_step_kernel");
     return ASD_OK;
 }
@@ -206,12 +242,16 @@ template int launch_sqdiff<float>(const float *, const float *, size_t, TvScratc
                                   hipStream_t);
 template int launch_sqdiff<double>(const double *, const double *, size_t, TvScratch &, int,
                                    hipStream_t);
+template int launch_serial_sumsq<float>(const float *, const float *, size_t, double *,
+                                        hipStream_t, const int *);
+template int launch_serial_sumsq<double>(const double *, const double *, size_t, double *,
+                                         hipStream_t, const int *);
 template int tv_descent<float, float>(float *, const float *, float *, float *, float *, int, int,
                                       int, bool, double, const double *, int, int, TvScratch &,
-                                      hipStream_t, bool, const int *);
+                                      hipStream_t, bool, const int *, bool);
 template int tv_descent<double, double>(double *, const double *, double *, double *, double *,
                                         int, int, int, bool, double, const double *, int, int,
-                                        TvScratch &, hipStream_t, bool, const int *);
+                                        TvScratch &, hipStream_t, bool, const int *, bool);
 
 int TvScratch::init()
 {
@@ -270,10 +310,13 @@ int tv_host(const double *img, int nx, int ny, int nz, bool three_d, double alph
     ASD_HIP(hipMemcpyAsync(c.img.p, img, n * 8, hipMemcpyHostToDevice, c.stream));
     if (img_old)
         ASD_HIP(hipMemcpyAsync(c.old.p, img_old, n * 8, hipMemcpyHostToDevice, c.stream));
+    // the drop-in forms sum d^2 and ||v||^2 in the reference's loop order
+    // (imgTVGradMin.cpp:58-59, :96; imgTVGradMin3D.cpp:150-153), so the
+    // result rounds like the reference's
     rc = tv_descent<double, double>(c.img.as<double>(), img_old ? c.old.as<double>() : nullptr,
                                     c.v.as<double>(), c.u2.as<double>(), c.v2.as<double>(), nx, ny,
                                     nz, three_d, alpha, nullptr, nTVIter, pos, c.scratch, c.stream,
-                                    false);
+                                    false, nullptr, true);
     if (rc) return rc;
     ASD_HIP(hipMemcpyAsync(out, c.img.p, n * 8, hipMemcpyDeviceToHost, c.stream));
     ASD_HIP(hipStreamSynchronize(c.stream));

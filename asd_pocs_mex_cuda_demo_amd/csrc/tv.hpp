@@ -34,9 +34,18 @@ int launch_sqdiff(const T *a, const T *b, size_t n, TvScratch &s, int slot, hipS
 // may be NULL) makes every kernel return at entry.
 // v, u2, v2: n-element work buffers (u2/v2 hold every other sub-iteration:
 // each step is fused into the next gradient).
+// serial_norms: d^2 and every ||v||^2 are summed in index order
+// (serial_sumsq_kernel), the order of the reference's loops, instead of
+// from block partials.
 template <class T, class A>
 int tv_descent(T *u, const T *img_old, T *v, T *u2, T *v2, int nx, int ny, int nz, bool three_d,
                double alpha, const double *alpha_dev, int nTVIter, int pos, TvScratch &s,
-               hipStream_t st, bool have_d2, const int *skip = nullptr);
+               hipStream_t st, bool have_d2, const int *skip = nullptr, bool serial_norms = false);
+
+// *out = sum over i of (a[i] - b[i])^2 (b NULL: zeros) in index order (one
+// wave; see serial_sumsq_kernel); skipped when *skip != 0.
+template <class T>
+int launch_serial_sumsq(const T *a, const T *b, size_t n, double *out, hipStream_t st,
+                        const int *skip);
 
 }  // namespace asd

@@ -7,19 +7,42 @@
 
 namespace asd {
 
+// One normalised TV step, u - alpha * d * v / ||v|| (imgTVGradMin.cpp:104-107,
+// imgTVGradMin3D.cpp:157-160, where d = ||I - I0|| is fixed for the call).
+// FP64 images are stepped in the reference's order, u - ((alpha * d) * v) /
+// ||v||, so the drop-in TV and the reference-arithmetic plan round like the
+// reference; FP32 images take the product with the step size
+// s = (alpha * d) / ||v|| in FP64 and round once.
+struct TvStepVal {
+    double s;         // (alpha * d) / ||v||
+    double ad, nrm;   // alpha * d and ||v||
+    template <class T>
+    __device__ __forceinline__ T apply(T u, T v) const
+    {
+#pragma clang fp contract(off)
+        if constexpr (sizeof(T) == sizeof(double)) return u - (ad * v) / nrm;
+        else return T((double)u - s * (double)v);
+    }
+};
+
+__device__ __forceinline__ TvStepVal tv_step_val(double alpha_d, double nrm)
+{
+#pragma clang fp contract(off)
+    return TvStepVal{alpha_d / nrm, alpha_d, nrm};
+}
+
 // Image values as the stencil sees them: zero outside the image, and with
 // STEPPED the previous sub-iteration's step applied on the fly,
-// u_i = T(u_{i-1} - s * v_{i-1}), s = alpha * d / ||v_{i-1}|| (the value the
-// unfused step kernel would have stored).
+// u_i = step(u_{i-1}, v_{i-1}) (the value the unfused step kernel would have
+// stored).
 template <class T, class A, bool STEPPED>
 struct TvSrc {
     const T *u, *v;
-    double s;
+    TvStepVal st;
     int nx, ny, nz;
     __device__ __forceinline__ T stored(size_t i) const
     {
-#pragma clang fp contract(off)
-        if constexpr (STEPPED) return T((double)u[i] - s * (double)v[i]);
+        if constexpr (STEPPED) return st.apply<T>(u[i], v[i]);
         else return u[i];
     }
     __device__ __forceinline__ A operator()(int x, int y, int z) const
@@ -88,14 +111,13 @@ struct TvGather {
 template <class T, class A, int K>
 struct TvReplay {
     const TvRaw<T, K> &raw;
-    double s;
+    TvStepVal st;
     mutable int k;
     __device__ __forceinline__ A operator()(int, int, int) const
     {
-#pragma clang fp contract(off)
         const int j = k++;
         // the value TvSrc<T, A, true>::stored gives for the same point
-        const A val = A(T((double)raw.u[j] - s * (double)raw.v[j]));
+        const A val = A(st.apply<T>(raw.u[j], raw.v[j]));
         return raw.ok[j] ? val : A(0);
     }
 };

@@ -154,7 +154,8 @@ def make_para(para):
     p.numOfgradients = int(_get(para, "numOfgradients", p.numOfgradients))
     for k in ("alpha", "alpha_red", "beta", "beta_red", "r_max", "epsilon", "stopRelChg"):
         setattr(p, k, float(_get(para, k, getattr(p, k))))
-    p.initBySART = int(bool(_get(para, "initBySART", False)))
+    # ASD_POCS_cuda.m:24 tests para.initBySART == true, i.e. == 1
+    p.initBySART = int(float(_get(para, "initBySART", 0)) == 1.0)
     p.lambda_ = float(_get(para, "lambda", 1.0))
     p.precision = int(_get(para, "precision", 0))
     return p

@@ -112,7 +112,11 @@ typedef struct asd_para {
                              * 1: 16-bit fixed-point sinogram per view, compensated sums;
                              * 2: FP16 sinogram, compensated sums;
                              * 3: as 1, and the backprojector sets up its rays in FP64
-                             *    per voxel tile (lengths within ~1e-6 voxel of exact) */
+                             *    per voxel tile (lengths within ~1e-6 voxel of exact);
+                             * 4: the reference's arithmetic: sinogram, image, m_pre,
+                             *    m_k1, f0 and TV in FP64 as ASD_POCS_cuda.m holds
+                             *    them, single(m) fed to a forward projector that sums
+                             *    each ray in slab order, the update in FP64 */
 } asd_para;
 
 typedef struct asd_out {
@@ -192,13 +196,15 @@ int asd_plan_tuning(asd_plan *plan, int *atx_variant, double *atx_us);
 int asd_plan_set_atx_variant(asd_plan *plan, int atx_variant);
 /* Forces the forward projector's launch shape: lanes per ray (1..256, a
  * power of two) with 1 row per bundle, or 8 lanes with bundles of 2 or 3
- * detector rows; (0, 0) restores the plan's default.  The shape changes the
- * order in which each ray's slab products are summed (the last bits of the
- * sums), not which products there are. */
+ * detector rows; (0, 0) restores the plan's default (for precision 4, the
+ * slab-order projector).  The shape changes the order in which each ray's
+ * slab products are summed (the last bits of the sums), not which products
+ * there are. */
 int asd_plan_set_ax_shape(asd_plan *plan, int lanes_per_ray, int rows_per_bundle);
 /* Launch shape of the forward projector (lanes per ray, or per bundle of
- * rows_per_bundle detector rows that share a column) and the plan's
- * hipStream_t.  Any pointer may be NULL. */
+ * rows_per_bundle detector rows that share a column; 0 lanes: the
+ * slab-order projector of precision 4) and the plan's hipStream_t.  Any
+ * pointer may be NULL. */
 int asd_plan_info(asd_plan *plan, int *lanes_per_ray, int *rows_per_bundle, void **stream);
 void asd_plan_destroy(asd_plan *plan);
 

@@ -341,6 +341,41 @@ void orc_atx(float *X, const float *y, float *sc, float cos_phi, float sin_phi,
 
 static inline double sq(double a) { return a * a; }
 
+/* Summation order of the FP64 sums of squares behind every norm of the TV
+ * descent and the driver (d, ||v||, dp, dg, dd2, RMSE).  The reference forms
+ * them with serial loops (imgTVGradMin.cpp:58-59, :96; imgTVGradMin3D.cpp:150-153)
+ * and MATLAB's norm / sum (ASD_POCS_cuda.m:122, 141, 145, 166), whose order is
+ * not specified.  0 (default): serial, index order.  1: pairwise (index-order
+ * blocks of 8, halved recursively).  Test-only: the two orders differ only
+ * in rounding, so comparing them measures how far the algorithm itself
+ * amplifies an FP64 rounding difference. */
+static int g_sum_order = 0;
+
+void orc_set_sum_order(int order) { g_sum_order = order; }
+
+static double pairwise_sumsq(const double *a, const double *b, size_t n)
+{
+    if (n <= 8) {
+        double s = 0;
+        for (size_t i = 0; i < n; i++) s += sq(a[i] - (b ? b[i] : 0.0));
+        return s;
+    }
+    size_t h = n / 2;
+    return pairwise_sumsq(a, b, h) + pairwise_sumsq(a + h, b ? b + h : NULL, n - h);
+}
+
+/* sum over i of (a[i] - b[i])^2 (b NULL: zeros) in the selected order */
+static double sumsq_diff(const double *a, const double *b, size_t n)
+{
+    if (g_sum_order == 1) return pairwise_sumsq(a, b, n);
+    double s = 0;
+    for (size_t i = 0; i < n; i++) {
+        double e = a[i] - (b ? b[i] : 0.0);
+        s += e * e;
+    }
+    return s;
+}
+
 /* TV gradient of a 2-D image, rows r (fastest) x cols c, zero outside. */
 void orc_tv2d_grad(const double *u, int R, int C, double *v)
 {
@@ -395,18 +430,11 @@ static int tv_common(const double *img, int Nx, int Ny, int Nz, int three_d, dou
     size_t n = (size_t)Nx * Ny * Nz;
     double *v = (double *)malloc(n * sizeof(double));
     if (out != img) memcpy(out, img, n * sizeof(double));
-    double d = 0;
-    for (size_t i = 0; i < n; i++) {
-        double e = out[i] - (img_old ? img_old[i] : 0.0);
-        d += e * e;
-    }
-    d = sqrt(d);
+    double d = sqrt(sumsq_diff(out, img_old, n));
     for (int it = 0; it < nTVIter; it++) {
         if (three_d) orc_tv3d_grad(out, Nx, Ny, Nz, v);
         else orc_tv2d_grad(out, Nx, Ny, v);
-        double nrm = 0;
-        for (size_t i = 0; i < n; i++) nrm += v[i] * v[i];
-        nrm = sqrt(nrm);
+        double nrm = sqrt(sumsq_diff(v, NULL, n));
         for (size_t i = 0; i < n; i++) out[i] -= alpha * d * v[i] / nrm;
     }
     if (pos == 1 && nTVIter > 0)
@@ -457,6 +485,7 @@ static double sart_sweep(double *m, const double *P, const double *phi_deg, int 
     float *mf = (float *)malloc(N * sizeof(float));
     float *proj = (float *)malloc(M * sizeof(float)), *sumR = (float *)malloc(M * sizeof(float));
     float *corf = (float *)malloc(M * sizeof(float));
+    double *res = (double *)malloc(M * sizeof(double));
     float *bp = (float *)malloc(N * sizeof(float)), *sumC = (float *)malloc(N * sizeof(float));
     float *ydet = (float *)malloc(na * sizeof(float)), *zdet = (float *)malloc(nb * sizeof(float));
     double dd2 = 0;
@@ -468,15 +497,14 @@ static double sart_sweep(double *m, const double *P, const double *phi_deg, int 
                              &SO, &OD, &det, &scale, ydet, zdet);
         for (size_t i = 0; i < N; i++) mf[i] = (float)m[i];
         orc_ax(mf, proj, sumR, cp, sp, ydet, zdet, SO, OD, scale, 1.0f, nx, ny, nz, na, nb);
-        double e2 = 0;
         for (size_t i = 0; i < M; i++) {
             double r = (double)sumR[i];
             if (r == 0) r = 1;
             double c0 = P[(size_t)v * M + i] - (double)proj[i];
-            e2 += c0 * c0;
+            res[i] = c0;
             corf[i] = (float)(c0 / r);
         }
-        dd2 += e2;
+        dd2 += sumsq_diff(res, NULL, M);
         orc_atx(bp, corf, sumC, cp, sp, ydet, zdet, SO, OD, scale, det, det, 1.0f, nx, ny, nz,
                 na, nb);
         for (size_t i = 0; i < N; i++) {
@@ -486,18 +514,14 @@ static double sart_sweep(double *m, const double *P, const double *phi_deg, int 
             m[i] = nm < 0 ? 0 : nm;
         }
     }
-    free(mf); free(proj); free(sumR); free(corf); free(bp); free(sumC); free(ydet); free(zdet);
+    free(mf); free(proj); free(sumR); free(corf); free(res); free(bp); free(sumC); free(ydet);
+    free(zdet);
     return dd2;
 }
 
 static double nrm2diff(const double *a, const double *b, size_t n)
 {
-    double s = 0;
-    for (size_t i = 0; i < n; i++) {
-        double e = a[i] - (b ? b[i] : 0.0);
-        s += e * e;
-    }
-    return sqrt(s);
+    return sqrt(sumsq_diff(a, b, n));
 }
 
 /* ASD_POCS_cuda(P, para) with m0 given (zeros when m0 is NULL).
@@ -548,9 +572,7 @@ static int asd_pocs_impl(const double *P, const double *phi_deg, int nv, const o
         double dd = sqrt(dd2);
         memcpy(m_res, m, N * sizeof(double));
         if (I0 && rmse) {
-            double s = 0;
-            for (size_t i = 0; i < N; i++) s += (m[i] - I0[i]) * (m[i] - I0[i]);
-            rmse[k - 1] = sqrt(s / (double)N);
+            rmse[k - 1] = sqrt(sumsq_diff(m, I0, N) / (double)N);
         }
         double dp = nrm2diff(m, m_pre, N);
         double *tr = trace ? trace + (size_t)(k - 1) * ORC_TRACE_COLS : NULL;

@@ -77,12 +77,19 @@ def lib():
                                          C.c_void_p, f64p, C.c_void_p, C.POINTER(OrcOut), f64p]
         L.orc_sart.restype = C.c_double
         L.orc_sart.argtypes = [f64p, f64p, it, C.POINTER(OrcPara), C.c_double, f64p]
+        L.orc_set_sum_order.restype = None
+        L.orc_set_sum_order.argtypes = [it]
         L.orc_set_threads.restype = None
         L.orc_set_threads.argtypes = [it]
         L.orc_get_threads.restype = it
         L.orc_get_threads.argtypes = []
         _lib = L
     return _lib
+
+
+def set_sum_order(order):
+    """0: serial FP64 sums of squares (default); 1: pairwise (test-only)."""
+    lib().orc_set_sum_order(int(order))
 
 
 def set_threads(n):
@@ -242,7 +249,8 @@ def asd_pocs(P, para, trace=False):
     m0 = para.get("m0", None)
     if m0 is not None:
         m0 = np.ascontiguousarray(np.asarray(m0, np.float64).reshape(-1, order="F"))
-    elif para.get("initBySART", False):
+    elif para.get("initBySART", None) is not None and float(para["initBySART"]) == 1.0:
+        # ASD_POCS_cuda.m:23-25: a non-empty para.initBySART == true (== 1)
         m0 = np.zeros(N)
         lib().orc_sart(Pv, phi, nv, C.byref(p), float(para.get("lambda", 1.0) or 1.0), m0)
     I0 = para.get("I0", None)

@@ -1,27 +1,30 @@
-"""GPU parity on the reference's own data and geometries (fixtures written by
-tests/golden/make_fixtures.py from its testdata/):
+"""GPU parity on the geometries of the reference's own scripts, with
+synthetic data (the reference's measured and clinical arrays are not copied):
 
-  swine   the measured fan-beam sinogram of testdata/swine_2d.mat with the
-          geometry of utilities/test_asdpocs.m (512^2, 1537 bins - an odd
-          count, which exercises the backprojector footprint's integer na/2 -
-          0.148 mm bins, sod 405.941, sdd 906.251), every 4th of its 360 views
-          (the script indexes columns (1:360)*4 of a 360-column array, so the
-          views are taken at their own 1-degree angles)
-  head    the clinical slice of testdata/P10240000_head_slice190.mat with the
-          geometry of demo_head190.m (90 views over 180 degrees, 1024 x 2 bins
-          for a one-slice volume: rays off the slice's faces)
+  swine   utilities/test_asdpocs.m: a 512^2 fan-beam image, 1537 bins - an
+          odd count, which exercises the backprojector footprint's integer
+          na/2 - of 0.148 mm, sod 405.941, sdd 906.251 (the geometry of
+          testdata/swine_2d.mat; tests/test_matio.py checks these constants
+          against the file), views at every 4th of the script's 360
+          1-degree angles; the data is a Poisson-noisy (I0 = 1e5)
+          modified Shepp-Logan sinogram in place of the measured one
+  head    demo_head190.m: 90 views over 180 degrees, 1024 x 2 bins for a
+          one-slice 512^2 volume (rays off the slice's faces), the same
+          sod / sdd / bin, with the phantom in place of the clinical slice
+  demo    demo_test.m at reduced size (the central 16 slices of a 64^3
+          phantom, Uy = 2 Nx by Vz = 2 Nz bins)
 
 Bars as in test_gpu_parity: projectors relative L2 <= 1e-5 (backprojector
 bit-identical), reconstructions against the oracle's FP64 driver."""
-import os
-
 import numpy as np
 import pytest
 
 from asd_pocs_mex_cuda_demo_amd import ops, phantoms
 
 pytestmark = pytest.mark.gpu
-GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+# utilities/test_asdpocs.m:11-20 (and testdata/swine_2d.mat)
+SWINE = dict(n=512, dt=0.148, sod=405.9410, sdd=906.2510, na=1537)
 
 
 def rel_l2(a, b):
@@ -31,23 +34,24 @@ def rel_l2(a, b):
 
 
 def swine():
-    fx = np.load(os.path.join(GOLD, "swine_2d_90.npz"))
-    n, dt, sod, sdd, na = 512, float(fx["det"]), float(fx["sod"]), float(fx["sdd"]), int(fx["na"])
+    n, dt, sod, sdd, na = (SWINE[k] for k in ("n", "dt", "sod", "sdd", "na"))
     voxel = sod / sdd * dt * na / n  # test_asdpocs.m: d = sod/sdd*dt*Uy; voxel = d/Nx
     para = dict(nx=n, ny=n, nz=1, SO=sod, OD=sdd - sod, vxlsize=voxel, detsize=dt, na=na, nb=1,
-                phi_vec=fx["phi_deg"])
-    return para, fx["P"].astype(np.float64)
+                phi_vec=np.arange(4, 361, 4).astype(np.float64))
+    img = phantoms.shepp_logan_2d(n)
+    P = np.zeros((na, len(para["phi_vec"])))
+    for i, th in enumerate(para["phi_vec"]):
+        P[:, i] = ops.Ax_cone_gpu(img.astype(np.float32), dict(para, phi=-th / 180 * np.pi))[0]
+    return para, phantoms.poisson_log(P, 1e5, seed=190)
 
 
 def head():
-    img = np.load(os.path.join(GOLD, "head190.npz"))["img"].astype(np.float64)
-    img = img / img.max()  # I3d = img_slice190/max(img_slice190(:))
     n, dt, sod, sdd, nv = 512, 0.148, 405.9410, 906.2510, 90
     Uy, Vz = 2 * n, 2
-    voxel = sod / sdd * dt * Uy / n
+    voxel = sod / sdd * dt * Uy / n  # demo_head190.m
     para = dict(nx=n, ny=n, nz=1, SO=sod, OD=sdd - sod, vxlsize=voxel, detsize=dt, na=Uy, nb=Vz,
                 phi_vec=np.arange(1, nv + 1) * 180.0 / nv)
-    return para, img
+    return para, phantoms.shepp_logan_2d(n)
 
 
 @pytest.mark.parametrize("case", ["swine", "head"])
@@ -66,9 +70,10 @@ def test_projectors_on_reference_geometries(orc, case):
         assert np.array_equal(x, xo) and np.array_equal(sc, sco), (case, phi_deg)
 
 
-def test_swine_reconstruction_matches_oracle(orc):
-    """ASD-POCS on the measured swine sinogram (no ground truth: images and
-    data residuals are compared)."""
+def test_swine_geometry_reconstruction_matches_oracle(orc):
+    """ASD-POCS on a noisy sinogram in the swine geometry (odd bin count):
+    images and data residuals against the oracle, and the reference
+    arithmetic (precision 4) sweep for sweep."""
     para, P = swine()
     para.update(maxnumIter=2, alpha=0.2, numOfgradients=5)
     m, out = ops.ASD_POCS_cuda(P, para)
@@ -77,11 +82,14 @@ def test_swine_reconstruction_matches_oracle(orc):
     assert rel_l2(m.ravel(order="F"), mo) <= 1e-3, rel_l2(m.ravel(order="F"), mo)
     assert abs(out["dataTolerance"] - outo["dataTolerance"]) <= 1e-4 * outo["dataTolerance"]
     assert abs(out["alpha"] - outo["alpha"]) <= 1e-12 * outo["alpha"]
+    m4, out4 = ops.ASD_POCS_cuda(P, dict(para, precision=4))
+    assert rel_l2(m4.ravel(order="F"), mo) <= 1e-12, rel_l2(m4.ravel(order="F"), mo)
+    assert out4["dataTolerance"] == pytest.approx(outo["dataTolerance"], rel=1e-13)
 
 
-def test_head_reconstruction_matches_oracle(orc):
-    """demo_head190.m: projections of the clinical slice, reconstructed with
-    the slice as I0; RMSE per iteration against the oracle."""
+def test_head_geometry_reconstruction_matches_oracle(orc):
+    """demo_head190.m's geometry: projections of the phantom reconstructed
+    with the phantom as I0; RMSE per iteration against the oracle."""
     para, img = head()
     P = np.zeros((para["na"] * para["nb"], len(para["phi_vec"])))
     for i, th in enumerate(para["phi_vec"]):
@@ -91,6 +99,9 @@ def test_head_reconstruction_matches_oracle(orc):
     mo, outo = orc.asd_pocs(P, para)
     assert np.allclose(out["RMSE"], outo["RMSE"], rtol=1e-4, atol=0), (out["RMSE"], outo["RMSE"])
     assert rel_l2(m.ravel(order="F"), mo) <= 1e-3
+    m4, out4 = ops.ASD_POCS_cuda(P, dict(para, precision=4))
+    assert np.allclose(out4["RMSE"], outo["RMSE"], rtol=1e-12, atol=0)
+    assert rel_l2(m4.ravel(order="F"), mo) <= 1e-12
 
 
 def demo3d(n=64):
@@ -134,3 +145,6 @@ def test_demo_test_geometry_projectors_and_reconstruction(orc):
     assert out["IterationNum"] == outo["IterationNum"] == 2
     assert np.allclose(out["RMSE"], outo["RMSE"], rtol=1e-4, atol=0), (out["RMSE"], outo["RMSE"])
     assert rel_l2(m.ravel(order="F"), mo) <= 1e-3, rel_l2(m.ravel(order="F"), mo)
+    m4, out4 = ops.ASD_POCS_cuda(P, dict(para, precision=4))
+    assert np.allclose(out4["RMSE"], outo["RMSE"], rtol=1e-12, atol=0)
+    assert rel_l2(m4.ravel(order="F"), mo) <= 1e-12

@@ -12,7 +12,6 @@ import pytest
 from asd_pocs_mex_cuda_demo_amd.matio import MatError, read_mat
 
 REF = "/root/reference/testdata"
-GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
 def _savemat(d, compress):
@@ -63,7 +62,11 @@ def test_malformed_input_raises():
 
 
 @pytest.mark.skipif(not os.path.isdir(REF), reason="reference test data not present")
-def test_reference_testdata_and_fixtures():
+def test_reference_testdata():
+    """The reader against scipy on the reference's own MAT files, and the
+    swine file's geometry scalars against the constants of
+    utilities/test_asdpocs.m that the real-geometry GPU tests use (the
+    clinical arrays themselves are read here only, never copied)."""
     sio = pytest.importorskip("scipy.io")
     for f in ("swine_2d.mat", "P10240000_head_slice190.mat", "P11570000_abdomen_slice148.mat"):
         a = read_mat(os.path.join(REF, f))
@@ -71,14 +74,9 @@ def test_reference_testdata_and_fixtures():
         for k, v in a.items():
             assert v.dtype == b[k].dtype and np.array_equal(v, b[k]), (f, k)
     s = read_mat(os.path.join(REF, "swine_2d.mat"))
-    fx = np.load(os.path.join(GOLD, "swine_2d_90.npz"))
-    assert np.array_equal(fx["P"], s["prjSwine"][:, 3::4].astype(np.float32))
+    from tests.test_gpu_realdata import SWINE
+    assert s["prjSwine"].shape == (SWINE["na"], 360)
+    assert float(s["sod"][0, 0]) == SWINE["sod"] and float(s["sdd"][0, 0]) == SWINE["sdd"]
+    assert float(s["detPixelSize"][0, 0]) == SWINE["dt"] and int(s["na"][0, 0]) == SWINE["na"]
     h = read_mat(os.path.join(REF, "P10240000_head_slice190.mat"))
-    assert np.array_equal(np.load(os.path.join(GOLD, "head190.npz"))["img"], h["img_slice190"])
-
-
-def test_fixtures_shape():
-    fx = np.load(os.path.join(GOLD, "swine_2d_90.npz"))
-    assert fx["P"].shape == (1537, 90) and int(fx["na"]) == 1537
-    assert np.array_equal(fx["phi_deg"], np.arange(4, 361, 4))
-    assert np.load(os.path.join(GOLD, "head190.npz"))["img"].shape == (512, 512)
+    assert h["img_slice190"].shape == (512, 512)
