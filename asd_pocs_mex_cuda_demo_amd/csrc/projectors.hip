@@ -210,6 +210,41 @@ int atx_host(float *X, const float *y, float *sc, const ViewGeom &g, const float
 // Device-pointer forms keep one ray table per thread of the caller.
 thread_local DeviceBuf tl_rt;
 
+// ---------------------------------------------------------------------------
+// The backprojector's ray table for asd_atx_device_noalloc: a fixed device
+// array instead of a library allocation, so that entry point allocates
+// nothing.  Each device holds its own copy (32 MiB) until the process ends;
+// every call on a device shares it and overwrites the first na * nb records
+// (nothing in it needs to start at zero), so calls that overlap on one
+// device (different streams) race.  In host code `scratch` names no device
+// address: its address on the current device comes from
+// hipGetSymbolAddress, once per device (scratch_table).
+// ---------------------------------------------------------------------------
+constexpr size_t NOALLOC_MAX_BINS = size_t(1) << 20;  // e.g. a 1024 x 1024 detector
+static __device__ __align__(256) RayRec scratch[NOALLOC_MAX_BINS];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex scratch_mu;
+RayRec *scratch_dev[NOALLOC_MAX_DEVICES] = {};
+
+// The address of `scratch` on the current device (resolved on the first call
+// for that device, with it current).
+int scratch_table(RayRec **out)
+{
+    int dev = 0;
+    ASD_HIP(hipGetDevice(&dev));
+    ASD_ARG(dev >= 0 && dev < NOALLOC_MAX_DEVICES, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lk(scratch_mu);
+    if (!scratch_dev[dev]) {
+        void *ptr = nullptr;
+        ASD_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        ASD_ARG(ptr, "hipGetSymbolAddress returned no address for the scratch table");
+        scratch_dev[dev] = static_cast<RayRec *>(ptr);
+    }
+    *out = scratch_dev[dev];
+    return ASD_OK;
+}
+
 }  // namespace
 }  // namespace asd
 
@@ -264,6 +299,27 @@ int asd_atx_device(float *X, const float *y, float *sc, float cos_phi, float sin
     hipStream_t s = (hipStream_t)stream;
     if ((rc = launch_ray_table(g, y_det, z_det, tl_rt.as<RayRec>(), s))) return rc;
     return launch_atx(g, X, y, sc, y_det, z_det, tl_rt.as<RayRec>(), s);
+}
+
+int asd_atx_device_noalloc(float *X, const float *y, float *sc, float cos_phi, float sin_phi,
+                           const float *y_det, const float *z_det, float SO, float OD,
+                           float scale, float dy_det, float dz_det, float dz, int nx, int ny,
+                           int nz, int na, int nb, void *stream)
+{
+    clear_error();
+    ViewGeom g = make_geom(cos_phi, sin_phi, SO, OD, scale, dy_det, dz_det, dz, nx, ny, nz, na,
+                           nb);
+    int rc = check_geom(g);
+    if (rc) return rc;
+    ASD_ARG(X && y && sc && y_det && z_det, "NULL pointer argument");
+    ASD_ARG((size_t)na * (size_t)nb <= NOALLOC_MAX_BINS,
+            "asd_atx_device_noalloc: na * nb exceeds its fixed ray table of 1048576 bins "
+            "(use asd_atx_device)");
+    RayRec *rt = nullptr;
+    if ((rc = scratch_table(&rt))) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    if ((rc = launch_ray_table(g, y_det, z_det, rt, s))) return rc;
+    return launch_atx(g, X, y, sc, y_det, z_det, rt, s);
 }
 
 }  // extern "C"

@@ -66,6 +66,20 @@ int asd_atx_device(float *X, const float *y, float *sc, float cos_phi, float sin
                    float dy_det, float dz_det, float dz, int nx, int ny, int nz, int na, int nb,
                    void *stream);
 
+/* asd_atx_device without any allocation: no counterpart in the reference.
+ * For callers that cannot allocate device memory (work under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  Same arguments
+ * and results as asd_atx_device; its ray-slope table lives in a fixed
+ * device array of 2^20 records (32 MiB, held by each device the library
+ * runs on until the process ends), so na * nb must not exceed 1048576
+ * (ASD_ERR_ARG otherwise).  Calls on one device share that array: calls
+ * that overlap on one device (on different streams) race; calls on one
+ * stream are ordered and safe. */
+int asd_atx_device_noalloc(float *X, const float *y, float *sc, float cos_phi, float sin_phi,
+                           const float *y_det, const float *z_det, float SO, float OD,
+                           float scale, float dy_det, float dz_det, float dz, int nx, int ny,
+                           int nz, int na, int nb, void *stream);
+
 /* ---------------------------------------------------------------------- */
 /* TV steepest descent                                                     */
 /* ---------------------------------------------------------------------- */
