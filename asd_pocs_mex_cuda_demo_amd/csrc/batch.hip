@@ -502,8 +502,11 @@ int b_sync(asd_batch *B)
     return ASD_OK;
 }
 
+// As the single plan's drop_graph: the stream is drained before the
+// executable graph is destroyed.
 void b_drop_graph(asd_batch *B)
 {
+    if (B->exec && B->stream) (void)hipStreamSynchronize(B->stream);
     if (B->exec) (void)hipGraphExecDestroy(B->exec);
     if (B->graph) (void)hipGraphDestroy(B->graph);
     B->exec = nullptr;
@@ -835,8 +838,15 @@ int asd_batch_iterate(asd_batch *B, int n_iter)
                 set_error(ASD_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
                 return ASD_ERR_HIP;
             }
+            hipError_t ei = hipGraphInstantiate(&B->exec, g, nullptr, nullptr, 0);
+            if (ei != hipSuccess) {
+                (void)hipGraphDestroy(g);
+                B->exec = nullptr;
+                set_error(ASD_ERR_HIP, std::string("hipGraphInstantiate: ") +
+                                           hipGetErrorString(ei));
+                return ASD_ERR_HIP;
+            }
             B->graph = g;
-            ASD_HIP(hipGraphInstantiate(&B->exec, g, nullptr, nullptr, 0));
         }
         for (int i = 0; i < n_iter; i++) ASD_HIP(hipGraphLaunch(B->exec, B->stream));
     } else {

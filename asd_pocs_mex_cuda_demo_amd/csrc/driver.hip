@@ -605,8 +605,12 @@ int enqueue_iteration(asd_plan *p)
     return ASD_OK;
 }
 
+// An executable graph must not be destroyed while a replay of it may still
+// run: the stream is drained first (every caller has drained it already; the
+// drain keeps that true for any caller).
 void drop_graph(asd_plan *p)
 {
+    if (p->exec && p->stream) (void)hipStreamSynchronize(p->stream);
     if (p->exec) (void)hipGraphExecDestroy(p->exec);
     if (p->graph) (void)hipGraphDestroy(p->graph);
     p->exec = nullptr;
@@ -629,8 +633,14 @@ int build_graph(asd_plan *p)
         set_error(ASD_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
         return ASD_ERR_HIP;
     }
+    hipError_t ei = hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0);
+    if (ei != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        p->exec = nullptr;
+        set_error(ASD_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+        return ASD_ERR_HIP;
+    }
     p->graph = g;
-    ASD_HIP(hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0));
     return ASD_OK;
 }
 

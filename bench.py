@@ -1,22 +1,30 @@
 """ASD-POCS iteration benchmark on one MI355X.
 
-Workload (default): 2-D fan-beam ASD-POCS at 512x512 (pixel 0.5 mm), 120
-views over 360 deg, 768 bins at 0.8 mm, SID 541 mm, SDD 949 mm, modified
-Shepp-Logan phantom (0.02 mm^-1), Poisson noise at I0 = 1e5 photons per bin
-(seeded) and log transform, 50 ASD-POCS iterations (ng = 20, alpha = 0.2).
-A step is one ASD-POCS outer iteration (a SART sweep over the 120 views,
-positivity, 20 TV sub-iterations, step adaptation) with the sinogram and
-image resident in HBM.
+Headline workload (default, the largest configuration): cone3d_512, 3-D
+cone-beam ASD-POCS on a 512^3 volume (0.5 mm voxels, Shepp-Logan plus 200
+seeded spheres), 720 views over 360 deg, 1024 x 768 bins at 0.6 mm, SID
+600 mm / SDD 1100 mm, FP16-stored projections with FP32 compensated sums,
+simulated on the device; 20 iterations of ASD-POCS with 20 TV sub-iterations
+(alpha = 0.2).  A step is one outer iteration (a SART sweep over the 720
+views with positivity, the TV descent and the step adaptation) with the
+sinogram and image resident in HBM.  The other workloads of
+asd_pocs_mex_cuda_demo_amd/workloads.py are selected with --workload.
 
-    python bench.py [--gpus 1] [--steps 50] [--warmup 2] [--precision 0|1]
+    python bench.py [--gpus 1] [--steps 20] [--warmup 5] [--workload NAME]
+                    [--precision P] [--no-cpu] [--no-extras]
 
-Prints one JSON line (value = iterations/s) with the roofline of the
-dominant kernel (HIP events on the plan's stream over the timed region) and
-the CPU oracle timed on a bounded sample of the same workload.
+Prints one JSON line: value = iterations/s of the headline workload, the
+roofline of its dominant kernel (HIP events on the plan's stream, per
+launch), the CPU oracle timed on a bounded sample of the same workload, and
+(unless --no-extras) the rates of the other single-plan workloads, the
+64-case batch against 64 sequential plans, and the accuracy of the 512^2
+fan-beam reconstruction against the CPU oracle over its 50 iterations.
 """
 import argparse
 import json
 import os
+import platform
+import subprocess
 import sys
 import time
 
@@ -27,221 +35,52 @@ sys.path.insert(0, ROOT)
 
 METRIC = "ASD-POCS iters/sec and projector GB/s (% HBM roofline), batched-mixed vs plain at RMSE"
 HBM_PEAK_GBS = 8000.0
+WORKLOAD_CHOICES = ["fan2d_256", "fan2d_512", "cone3d_256", "cone3d_512", "batch64"]
 
-WORKLOADS = {
-    "fan2d_512": dict(n=512, nz=1, vxl=0.5, nv=120, na=768, nb=1, det=0.8, SO=541.0,
-                      SDD=949.0, I0=1e5, iters=50),
-    "cone3d_256": dict(n=256, nz=256, vxl=1.0, nv=180, na=384, nb=384, det=1.2, SO=500.0,
-                       SDD=1000.0, I0=5e4, iters=30),
-    # 512^3 at 0.5 mm, Shepp-Logan plus 200 seeded spheres (radius 1-4 mm,
-    # attenuation 0.01-0.03 /mm), 720 views, 1024 x 768 bins at 0.6 mm, SID
-    # 600 mm, SDD 1100 mm, FP16-stored projections (precision 2), simulated on
-    # the device; a bench run times 2 of its 20 iterations by default
-    "cone3d_512": dict(n=512, nz=512, vxl=0.5, nv=720, na=1024, nb=768, det=0.6, SO=600.0,
-                       SDD=1100.0, I0=0.0, iters=20, steps=2, precision=2, sim="gpu",
-                       spheres=200),
-}
+CODEC = {0: "SinoF32", 1: "SinoU16", 2: "SinoF16", 3: "SinoU16", 4: "SinoF64"}
+ATX_VARIANTS = ["direct", "tile16x16x1", "tile8x8x4", "tile16x4x4", "pairs", "direct2",
+                "direct1", "sorted16", "sorted8x8x4", "sorted8x4x8"]
 
 
-def sphere_table(k, seed, width_mm):
-    """k seeded spheres (radius 1-4 mm, attenuation 0.01-0.03 /mm, centres in
-    the inner 60% of the phantom) as phantom rows, values relative to the
-    phantom's scale of 0.02 /mm, coordinates in units of half the width."""
-    rng = np.random.default_rng(seed)
-    r = rng.uniform(1.0, 4.0, k) / (width_mm / 2)
-    mu = rng.uniform(0.01, 0.03, k) / 0.02
-    c = rng.uniform(-0.6, 0.6, (k, 3)) * np.array([0.69, 0.92, 0.81])
-    return np.column_stack([mu, r, r, r, c, np.zeros(k)])
+def atx_kernel_name(variant, kahan):
+    k = "true" if kahan else "false"
+    return {"direct": f"atx_kernel<EpiSartAtx, {k}>",
+            "tile16x16x1": f"atx_tile_plain_kernel<16, 16, 1, {k}, EpiSartAtx>",
+            "tile8x8x4": f"atx_tile_plain_kernel<8, 8, 4, {k}, EpiSartAtx>",
+            "tile16x4x4": f"atx_tile_plain_kernel<16, 4, 4, {k}, EpiSartAtx>",
+            "pairs": f"atx_pairs_kernel<EpiSartAtx, {k}>",
+            "direct2": f"atx_kernel<EpiSartAtx, {k}, 2>",
+            "direct1": f"atx_kernel<EpiSartAtx, {k}, 1>",
+            "sorted16": f"atx_sorted_kernel<16, 16, 1, EpiSartAtx, {k}, 2>",
+            "sorted8x8x4": f"atx_sorted_kernel<8, 8, 4, EpiSartAtx, {k}, 2>",
+            "sorted8x4x8": f"atx_sorted_kernel<8, 4, 8, EpiSartAtx, {k}, 2>"}.get(variant)
 
 
-def make_sim_problem(w, precision, seed=20241203):
-    """The workload simulated on the device: phantom, projections, noise."""
-    from asd_pocs_mex_cuda_demo_amd import ops, phantoms
-    n, nz = w["n"], w["nz"]
-    para = dict(nx=n, ny=n, nz=nz, SO=w["SO"], OD=w["SDD"] - w["SO"], vxlsize=w["vxl"],
-                detsize=w["det"], na=w["na"], nb=w["nb"], alpha=0.2, numOfgradients=20)
-    para["phi_vec"] = np.arange(1, w["nv"] + 1) * 360.0 / w["nv"]
-    table = np.vstack([phantoms.ellipses(),
-                       sphere_table(w.get("spheres", 0), seed, n * w["vxl"])])
-    plan = ops.Plan(dict(para, maxnumIter=w["iters"], precision=precision), para["phi_vec"])
-    plan.simulate(table, scale=0.02, I0=w["I0"], seed=seed)
-    return para, plan
+def ax_kernel_name(lpr, rows, precision):
+    k = "true" if precision in (1, 2, 3) else "false"
+    epi = f"EpiSartAx<{CODEC[precision]}>"
+    if rows > 1:
+        return f"ax_rows_kernel<{lpr}, {rows}, {1 if rows == 3 else 2}, {k}, false, {epi} >"
+    return f"ax_kernel<{lpr}, {k}, false, {epi} >"
 
 
-def make_problem(w, seed=20241203):
-    from asd_pocs_mex_cuda_demo_amd import ops, phantoms
-    n, nz = w["n"], w["nz"]
-    para = dict(nx=n, ny=n, nz=nz, SO=w["SO"], OD=w["SDD"] - w["SO"], vxlsize=w["vxl"],
-                detsize=w["det"], na=w["na"], nb=w["nb"], alpha=0.2, numOfgradients=20)
-    para["phi_vec"] = np.arange(1, w["nv"] + 1) * 360.0 / w["nv"]
-    img = phantoms.shepp_logan_2d(n) if nz == 1 else phantoms.shepp_logan_3d(n, nz=nz)
-    plan = ops.Plan(dict(para, maxnumIter=w["iters"]), para["phi_vec"])
-    # noiseless line integrals from our forward projector, then Poisson + log
-    Pclean = np.zeros((w["na"] * w["nb"], w["nv"]))
-    for i, th in enumerate(para["phi_vec"]):
-        Pclean[:, i] = ops.Ax_cone_gpu(img.astype(np.float32), dict(para, phi=-th / 180 * np.pi))[0]
-    P = phantoms.poisson_log(Pclean, w["I0"], seed)
-    return para, img, P, plan
-
-
-def make_batch_cases(ncases=64, seed=20241204):
-    """64 independent 2-D fan-beam cases: N from {128,192,256,384,512},
-    views uniform in [60, 360] over 360 deg, 1.5N bins, perturbed modified
-    Shepp-Logan phantoms (intensity +-10%, axes +-5%), Poisson I0 = 1e5.
-    Geometry (not fixed by the workload): SID 541 mm, SDD 949 mm, pixel
-    256/N mm (a 256 mm field), bin 1.6 x pixel."""
-    from asd_pocs_mex_cuda_demo_amd import ops, phantoms
-    rng = np.random.default_rng(seed)
-    cases = []
-    for c in range(ncases):
-        n = int(rng.choice([128, 192, 256, 384, 512]))
-        nv = int(rng.integers(60, 361))
-        vxl = 256.0 / n
-        para = dict(nx=n, ny=n, nz=1, SO=541.0, OD=408.0, vxlsize=vxl, detsize=1.6 * vxl,
-                    na=int(1.5 * n), nb=1, alpha=0.2, numOfgradients=20)
-        para["phi_vec"] = np.arange(1, nv + 1) * 360.0 / nv
-        img = phantoms.shepp_logan_2d(n, table=phantoms.ellipses(rng))
-        Pc = np.zeros((para["na"], nv))
-        for i, th in enumerate(para["phi_vec"]):
-            Pc[:, i] = ops.Ax_cone_gpu(img.astype(np.float32), dict(para, phi=-th / 180 * np.pi))[0]
-        cases.append((para, img, phantoms.poisson_log(Pc, 1e5, seed + c)))
-    return cases
-
-
-def bench_batch(args):
-    """Batched-mixed (one launch per stage for all 64 cases, 16-bit sinogram,
-    compensated sums) vs the same cases run one after another with the plain
-    single-case plan; value = case-iterations per second of the batch."""
-    from asd_pocs_mex_cuda_demo_amd import ops
-    steps = args.steps if args.steps is not None else 20
-    cases = make_batch_cases()
-    K = len(cases)
-    # plain: one plan per case, sequentially
-    t_plain = 0.0
-    rmse_plain, alpha_plain = [], []
-    for para, img, P in ([] if args.batch_only else cases):
-        pl = ops.Plan(dict(para, maxnumIter=steps), para["phi_vec"])
-        pl.set_projections(P)
-        pl.reset(None, img)
-        pl.iterate(1)  # graph capture + warm-up
-        pl.reset(None, img)
-        pl.sync()
-        t0 = time.perf_counter()
-        pl.iterate(steps)
-        pl.sync()
-        t_plain += time.perf_counter() - t0
-        o, r = pl.out()
-        rmse_plain.append(r[-1])
-        alpha_plain.append(o["alpha"])
-        pl.close()
-    out = {}
-    for prec in (0, 1):
-        B = ops.Batch([dict(p, maxnumIter=steps, precision=prec) for p, _, _ in cases])
-        if args.eager:
-            B.set_graph(False)
-        for c, (para, img, P) in enumerate(cases):
-            B.set_case(c, P, None, img)
-        B.iterate(1)  # graph capture + warm-up
-        for c, (para, img, P) in enumerate(cases):
-            B.set_case(c, P, None, img)
-        B.sync()
-        t0 = time.perf_counter()
-        B.iterate(steps)
-        B.sync()
-        dt = time.perf_counter() - t0
-        rm = np.array([B.out(c)[1][-1] for c in range(K)])
-        al = np.array([B.out(c)[0]["alpha"] for c in range(K)])
-        out[prec] = (dt, rm, al)
-        B.close()
-    dt_mixed, rm_mixed, al_mixed = out[1]
-    dt_bplain, rm_bplain, al_bplain = out[0]
-    if args.batch_only:
-        print(json.dumps({"workload": "batch64_fan2d", "cases": K, "iterations": steps,
-                          "batched_mixed_case_iters_per_sec": K * steps / dt_mixed,
-                          "batched_plain_case_iters_per_sec": K * steps / dt_bplain}),
-              flush=True)
-        return
-    rp = np.array(rmse_plain)
-    ap = np.array(alpha_plain)
-    # the spread of the plain result under a 1e-7 relative data perturbation
-    rq = np.array([plain_rmse_and_floor(para, img, P, steps, steps)[1][-1]
-                   for para, img, P in cases])
-
-    def rel(a, al=None):
-        d = np.abs(np.asarray(a) - rp) / rp
-        r = {"max": float(d.max()), "median": float(np.median(d))}
-        if al is not None:
-            # the alpha rule is a discrete decision; cases where it fired at a
-            # different iteration differ by far more than the arithmetic
-            same = al == ap
-            r["cases_with_other_alpha"] = int((~same).sum())
-            r["max_same_alpha"] = float(d[same].max()) if same.any() else None
-        return r
-    result = {
-        "metric": METRIC,
-        "value": K * steps / dt_mixed,
-        "unit": "case-iters/sec",
-        "n_gpus": 1, "steps": steps, "warmup": 1,
-        "ms_per_step": dt_mixed / steps * 1e3,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 (16-bit fixed-point sinogram, compensated sums)",
-        "data": "synthetic: 64 perturbed modified Shepp-Logan phantoms, Poisson I0=1e5 (seeded)",
-        "config": {"workload": "batch64_fan2d", "cases": K, "iterations": steps,
-                   "sizes": sorted(set(int(p["nx"]) for p, _, _ in cases)),
-                   "views_min": int(min(len(p["phi_vec"]) for p, _, _ in cases)),
-                   "views_max": int(max(len(p["phi_vec"]) for p, _, _ in cases))},
-        "plain_sequential_case_iters_per_sec": K * steps / t_plain,
-        "batched_plain_case_iters_per_sec": K * steps / dt_bplain,
-        "speedup_batched_mixed_vs_plain": t_plain / dt_mixed,
-        "rmse_rel_diff_mixed_vs_plain": rel(rm_mixed, al_mixed),
-        "rmse_rel_diff_batched_plain_vs_plain": rel(rm_bplain, al_bplain),
-        "rmse_rel_diff_plain_under_1e-7_data_perturbation": rel(rq),
-    }
-    print(json.dumps(result), flush=True)
-
-
-ATX_KERNELS = {"direct": "atx_kernel<EpiSartAtx, false>",
-               "tile16x16x1": "atx_tile_plain_kernel<16, 16, 1, false, EpiSartAtx>",
-               "tile8x8x4": "atx_tile_plain_kernel<8, 8, 4, false, EpiSartAtx>",
-               "tile16x4x4": "atx_tile_plain_kernel<16, 4, 4, false, EpiSartAtx>",
-               "pairs": "atx_pairs_kernel<EpiSartAtx, false>",
-               "direct2": "atx_kernel<EpiSartAtx, false, 2>",
-               "direct1": "atx_kernel<EpiSartAtx, false, 1>",
-               "sorted16": "atx_sorted_kernel<16, 16, 1, EpiSartAtx, false, 2>",
-               "sorted8x8x4": "atx_sorted_kernel<8, 8, 4, EpiSartAtx, false, 2>",
-               "sorted8x4x8": "atx_sorted_kernel<8, 4, 8, EpiSartAtx, false, 2>"}
-
-
-def profiled_traffic(workload, kernel):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
-    summary of this workload (FETCH_SIZE + WRITE_SIZE, each scaled by a
-    256 MiB copy), or None when that summary has no row for it."""
-    path = os.path.join(ROOT, "profiles", f"{workload}_hbm_traffic.md")
-    if not os.path.exists(path):
+def profiled(workload, kernel, what):
+    """A per-launch figure of `kernel` from a committed rocprofv3 summary of
+    this workload: what = "traffic" (HBM bytes per launch, FETCH_SIZE +
+    WRITE_SIZE scaled by a plain copy) from profiles/<workload>_hbm_traffic.md,
+    or "valu" (the VALU issue fraction and L2 hit rate) from
+    profiles/<workload>_valu_l2.md; None when there is no row for it."""
+    name = {"traffic": "hbm_traffic", "valu": "valu_l2"}[what]
+    path = os.path.join(ROOT, "profiles", f"{workload}_{name}.md")
+    if not kernel or not os.path.exists(path):
         return None
     for line in open(path):
         if line.startswith("| `") and f"`{kernel}`" in line:
-            return int(line.strip().strip("|").split("|")[-1])
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            if what == "traffic":
+                return int(cells[-1])
+            return {"valu_issue_frac": float(cells[-2]), "l2_hit_rate": float(cells[-1])}
     return None
-
-
-def perturbed(P, seed=99, eps=1e-7):
-    return P * (1 + eps * np.random.default_rng(seed).standard_normal(P.shape))
-
-
-def plain_rmse_and_floor(para, img, P, steps, max_iter):
-    """RMSE history of the plain FP32 plan, and of the same plan on the data
-    perturbed by 1e-7 relative."""
-    from asd_pocs_mex_cuda_demo_amd import ops
-    res = []
-    for data in (P, perturbed(P)):
-        pl = ops.Plan(dict(para, maxnumIter=max_iter), para["phi_vec"])
-        pl.set_projections(data)
-        pl.reset(None, img)
-        pl.iterate(steps)
-        res.append(pl.out()[1])
-        pl.close()
-    return res
 
 
 def stage_bytes(N, M, ng, sino=4):
@@ -253,77 +92,360 @@ def stage_bytes(N, M, ng, sino=4):
     return {0: 4 * N + (sino + 4) * M, 1: 8 * N + 4 * M, 2: ng * 20 * N}
 
 
-def cpu_baseline(para, P, sample_iters, sample_views=0, I0=None, tv_iters=20):
-    """The CPU oracle on the GPU box's host cores.  2-D: `sample_iters` full
-    outer iterations (their result is returned for the accuracy check).
-    3-D: `sample_views` SART view updates plus one full TV descent,
-    extrapolated to one outer iteration (nv view updates + TV).
-    Returns (baseline record, (image, out) of the 2-D run or None)."""
-    from oracle import oracle
-    threads = min(16, os.cpu_count() or 1)
-    oracle.set_threads(threads)
-    if not sample_views:
-        t0 = time.perf_counter()
-        run = oracle.asd_pocs(P, dict(para, maxnumIter=sample_iters, I0=I0))
-        dt = time.perf_counter() - t0
-        return ({"value": sample_iters / dt, "unit": "iters/sec", "cores": threads,
-                 "kind": "port",
-                 "sample": f"{sample_iters} full ASD-POCS iterations of the same workload "
-                           f"(FP32 projectors, FP64 TV/driver; OpenMP over rays/voxels), "
-                           f"{dt:.2f} s wall"}, run)
-    nv = P.shape[1]
-    N = para["nx"] * para["ny"] * para["nz"]
-    m = np.zeros(N)
+def build_plan(name, precision):
+    """(para, plan, img, P) of a workload; img and P are None when the plan
+    simulated them on the device (its reference image and sinogram stay
+    there)."""
+    from asd_pocs_mex_cuda_demo_amd import ops
+    from asd_pocs_mex_cuda_demo_amd import workloads as W
+    if W.CONFIGS[name].get("sim") == "gpu":
+        p, plan = W.sim_plan(name, precision)
+        return p, plan, None, None
+    p, img, P = W.problem(name, precision=precision)
+    plan = ops.Plan(p, p["phi_vec"])
+    plan.set_projections(P)
+    return p, plan, img, P
+
+
+def time_iterations(plan, img, steps, warmup):
+    """`warmup` untimed iterations, then `steps` timed ones (the captured
+    iteration graph replayed) from a fresh start; returns (seconds,
+    iterations done, rmse history)."""
+    plan.reset(None, img)
+    plan.iterate(warmup)
+    plan.reset(None, img)
+    plan.sync()
     t0 = time.perf_counter()
-    for i in range(sample_views):
-        q = dict(para, phi=-para["phi_vec"][i] / 180 * np.pi)
-        proj, sumR = (a.astype(np.float64) for a in oracle.ax_para(m.astype(np.float32), q))
+    plan.iterate(steps)
+    plan.sync()
+    dt = time.perf_counter() - t0
+    out, rmse = plan.out()
+    return dt, out["IterationNum"], rmse
+
+
+def cpu_threads():
+    """Threads of the CPU baseline: the host share of this GPU where the
+    environment states it (OMP_NUM_THREADS), else every CPU this process may
+    run on."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    aff = len(os.sched_getaffinity(0))
+    return min(int(env), aff) if env.isdigit() and int(env) > 0 else aff
+
+
+def host_info():
+    info = {"cpu_model": platform.processor() or None, "online_cpus": os.cpu_count(),
+            "affinity_cpus": len(os.sched_getaffinity(0)),
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k == "Model name":
+                info["cpu_model"] = v
+            elif k in ("Socket(s)", "Core(s) per socket", "Thread(s) per core"):
+                info[k.lower().replace("(s)", "s").replace(" ", "_")] = int(v)
+    except (OSError, subprocess.SubprocessError, ValueError):
+        pass
+    return info
+
+
+def cpu_sample_2d(oracle, p, P, iters, threads):
+    oracle.set_threads(threads)
+    t0 = time.perf_counter()
+    oracle.asd_pocs(P, dict(p, maxnumIter=iters))
+    return time.perf_counter() - t0
+
+
+def cpu_sample_3d(oracle, p, P, nviews, tv_steps, stride, threads):
+    """Seconds of one outer iteration of the oracle on a 3-D workload,
+    extrapolated from `nviews` SART view updates and `tv_steps` TV3D steps
+    (the reference's FP32 projector arithmetic and FP64 TV/driver).  With
+    stride > 1 every stride-th ray and voxel of the views and every
+    stride-th slice of the TV step are evaluated and the time scaled by
+    stride (a uniform subset of the same work)."""
+    oracle.set_threads(threads)
+    nv = len(p["phi_vec"])
+    N = p["nx"] * p["ny"] * p["nz"]
+    M = p["na"] * p["nb"]
+    m = np.zeros(N, np.float32)
+    rays = np.arange(0, M, stride, dtype=np.int64)
+    vox = np.arange(0, N, stride, dtype=np.int64)
+    t0 = time.perf_counter()
+    for i in range(nviews):
+        q = dict(p, phi=-p["phi_vec"][i] / 180 * np.pi)
+        if stride == 1:
+            proj, sumR = oracle.ax_para(m, q)
+            sel = slice(None)
+        else:
+            proj, sumR = oracle.ax_rays_para(m, q, rays)
+            sel = rays
+        sumR = sumR.astype(np.float64)
         sumR[sumR == 0] = 1
-        cor = ((P[:, i] - proj) / sumR).astype(np.float32)
-        bp, sumC = (a.astype(np.float64) for a in oracle.atx_para(cor, q))
+        cor = np.zeros(M, np.float32)  # the sampled rays' corrections (all of them at stride 1)
+        cor[sel] = (P[sel, i] - proj) / sumR
+        if stride == 1:
+            bp, sumC = oracle.atx_para(cor, q)
+        else:
+            bp, sumC = oracle.atx_voxels_para(cor, q, vox)
+        sumC = sumC.astype(np.float64)
         sumC[sumC == 0] = 1
-        m = np.maximum(m + bp / sumC, 0)
-    t_views = time.perf_counter() - t0
-    vol = m.reshape(para["nx"], para["ny"], para["nz"], order="F")
+        if stride == 1:
+            m = np.maximum(m + bp / sumC, 0).astype(np.float32)
+    t_views = (time.perf_counter() - t0) * stride
+    nz = max(p["nz"] // stride, 2)
+    vol = np.full((p["nx"], p["ny"], nz), 1e-3) + np.random.default_rng(1).random(
+        (p["nx"], p["ny"], nz)) * 1e-4
     t1 = time.perf_counter()
-    oracle.tv3d(vol, 0.2, tv_iters, 1, np.zeros_like(vol) + 1e-3)
-    t_tv = (time.perf_counter() - t1) * 20 / tv_iters
-    t_iter = t_views / sample_views * nv + t_tv
-    return ({"value": 1.0 / t_iter, "unit": "iters/sec", "cores": threads, "kind": "port",
-             "sample": f"{sample_views} SART view updates ({t_views:.2f} s) + {tv_iters} TV3D "
-                       f"steps (scaled to 20: {t_tv:.2f} s), extrapolated to one iteration of "
-                       f"{nv} views ({t_iter:.1f} s)"}, None)
+    oracle.tv3d(vol, 0.2, tv_steps, 1, np.zeros_like(vol))
+    t_tv = (time.perf_counter() - t1) * (p["nz"] / nz) * 20 / tv_steps
+    return t_views / nviews * nv + t_tv, t_views, t_tv
+
+
+def cpu_baseline(name, p, P, threads):
+    """The CPU oracle on this host: a bounded sample of the same workload on
+    `threads` threads, and the same sample's single-thread time (on a
+    uniform 1/16 subset where a full one would take minutes)."""
+    from oracle import oracle
+    nz = p["nz"]
+    rec = {"unit": "iters/sec", "cores": threads, "kind": "port"}
+    if nz == 1:
+        iters = 5
+        dt = cpu_sample_2d(oracle, p, P, iters, threads)
+        dt1 = cpu_sample_2d(oracle, p, P, 1, 1)
+        rec.update(value=iters / dt,
+                   sample=f"{iters} full ASD-POCS iterations of the same workload (FP32 "
+                          f"projectors in the reference's order, FP64 TV and driver; OpenMP over "
+                          f"rays and voxels), {dt:.2f} s wall",
+                   one_thread={"value": 1 / dt1, "sample": f"1 full iteration, {dt1:.2f} s"})
+    else:
+        big = p["nx"] * p["ny"] * nz > 64 ** 3 * 64
+        stride = 16 if big else 1
+        t_it, tv_, tt_ = cpu_sample_3d(oracle, p, P, 1, 1, stride, threads)
+        rec.update(value=1 / t_it,
+                   sample=f"1 SART view update ({'every 16th ray and voxel, x16: ' if big else ''}"
+                          f"{tv_:.1f} s) and 1 TV3D step ({tt_:.1f} s scaled to 20), extrapolated "
+                          f"to one iteration of {len(p['phi_vec'])} views ({t_it:.0f} s)")
+        t1, tv1, tt1 = cpu_sample_3d(oracle, p, P, 1, 1, 16, 1)
+        rec["one_thread"] = {"value": 1 / t1,
+                             "sample": f"every 16th ray and voxel of 1 view update (x16: "
+                                       f"{tv1:.1f} s) and 1/16 of a TV3D step (x16, x20: "
+                                       f"{tt1:.1f} s), extrapolated ({t1:.0f} s per iteration)"}
+    oracle.set_threads(threads)
+    return rec
+
+
+def headline(args):
+    from asd_pocs_mex_cuda_demo_amd import workloads as W
+    name = args.workload
+    w = W.CONFIGS[name]
+    precision = args.precision if args.precision is not None else w.get("precision", 0)
+    p, plan, img, P = build_plan(name, precision)
+    if args.atx_variant:
+        plan.set_atx_variant(ATX_VARIANTS.index(args.atx_variant))
+    if args.ax_shape:
+        plan.set_ax_shape(*[int(v) for v in args.ax_shape.split(",")])
+    N = p["nx"] * p["ny"] * p["nz"]
+    M = p["na"] * p["nb"]
+    dt, done, rmse = time_iterations(plan, img, args.steps, args.warmup)
+    # the same iterations enqueued kernel by kernel, each projector launch
+    # stamped with HIP events by its dispatch on the plan's stream
+    prof = max(1, min(args.steps, args.prof_steps))
+    plan.reset(None, img)
+    plan.set_timing(True)
+    plan.sync()
+    t1 = time.perf_counter()
+    plan.iterate(prof)
+    plan.sync()
+    dt_ev = time.perf_counter() - t1
+    stage = {s: plan.timing(s) for s in (0, 1, 2)}
+    plan.set_timing(False)
+    nbytes = stage_bytes(N, M, p["numOfgradients"], 8 if precision == 4 else
+                         4 if precision == 0 else 2)
+    tuning = plan.tuning()
+    lpr, rows, _ = plan.info()
+    names = {0: ax_kernel_name(lpr, rows, precision),
+             1: atx_kernel_name(tuning["atx_variant"], precision in (1, 2, 3)),
+             2: "TV descent (tv_grad_kernel x ng + tv_step_kernel)"}
+    per_stage = {}
+    for s, (ms, cnt) in stage.items():
+        if cnt:
+            avg = ms / cnt
+            per_stage[names[s] or f"stage {s}"] = {
+                "avg_us": avg * 1e3, "launches": cnt, "total_ms": ms,
+                "algorithmic_bytes": nbytes[s], "GBps": nbytes[s] / (avg * 1e-3) / 1e9}
+    dom = max(stage, key=lambda s: stage[s][0])
+    ms_d, cnt_d = stage[dom]
+    achieved = nbytes[dom] / (ms_d / cnt_d * 1e-3) / 1e9 if cnt_d else 0.0
+    kernel = names[dom]
+    roof = {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+            "traffic": profiled(name, kernel, "traffic"),
+            "algorithmic_bytes_per_launch": nbytes[dom], "avg_launch_us": ms_d / cnt_d * 1e3}
+    valu = profiled(name, kernel, "valu")
+    if valu:
+        roof.update(valu)
+    result = {
+        "metric": METRIC, "value": done / dt, "unit": "iters/sec", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt / done * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": {0: "f32", 1: "f32 (16-bit fixed-point sinogram, compensated sums)",
+                  2: "f32 (fp16 sinogram, compensated sums)",
+                  3: "f32 (16-bit fixed-point sinogram, compensated sums, f64 tile ray setup)",
+                  4: "f64 state, f32 projectors (the reference's arithmetic)"}[precision],
+        "data": ("synthetic, simulated on the device: modified Shepp-Logan phantom"
+                 + (f" + {w['spheres']} seeded spheres" if w.get("spheres") else "")
+                 + (f", Poisson noise I0={w['I0']:g} (seeded), log transform" if w["I0"]
+                    else ", noiseless projections")) if P is None else
+                ("synthetic: modified Shepp-Logan phantom"
+                 + (f", Poisson noise I0={w['I0']:g} (seeded), log transform" if w["I0"]
+                    else ", noiseless projections")),
+        "config": {"workload": name, "nx": p["nx"], "ny": p["ny"], "nz": p["nz"],
+                   "views": len(p["phi_vec"]), "na": p["na"], "nb": p["nb"],
+                   "iterations": args.steps, "ng": p["numOfgradients"], "precision": precision},
+        "roofline": roof,
+        "stages": per_stage,
+        "ms_per_step_eager_with_events": dt_ev / prof * 1e3,
+        "rmse_final": float(rmse[-1]) if len(rmse) else None,
+        "ax_lanes_per_ray": lpr, "ax_rows_per_bundle": rows, "atx_tuning": tuning,
+    }
+    if not args.no_cpu:
+        if P is None:
+            P = plan.projections()
+        cb = cpu_baseline(name, p, P, cpu_threads())
+        result["cpu_baseline"] = cb
+        result["speedup_vs_cpu_baseline"] = result["value"] / cb["value"]
+        result["host"] = host_info()
+    plan.close()
+    return result
+
+
+def workload_rate(name, cpu, threads):
+    """Iterations/s of a single-plan workload over its own iteration count
+    (one warm-up iteration), with the CPU oracle's rate beside it."""
+    from asd_pocs_mex_cuda_demo_amd import workloads as W
+    p, plan, img, P = build_plan(name, W.CONFIGS[name].get("precision", 0))
+    iters = W.CONFIGS[name]["iters"]
+    dt, done, rmse = time_iterations(plan, img, iters, 1)
+    r = {"iters_per_sec": done / dt, "ms_per_step": dt / done * 1e3, "iterations": done,
+         "rmse_final": float(rmse[-1]) if len(rmse) else None}
+    plan.close()
+    if cpu:
+        from oracle import oracle
+        if p["nz"] == 1:
+            t = cpu_sample_2d(oracle, p, P, 2, threads) / 2
+            r["cpu_sample"] = "2 full iterations"
+        else:
+            t, _, _ = cpu_sample_3d(oracle, p, P, 2, 2, 1, threads)
+            r["cpu_sample"] = "2 SART view updates + 2 TV3D steps, extrapolated"
+        r["cpu_iters_per_sec"] = 1.0 / t
+        r["speedup_vs_cpu"] = r["iters_per_sec"] * t
+    return r
+
+
+def batch_rates(steps=20):
+    """The 64-case batch: batched-mixed (one launch per stage for all cases,
+    16-bit sinogram, compensated sums) and batched-plain against the same
+    cases as 64 sequential plain plans; case-iterations per second."""
+    from asd_pocs_mex_cuda_demo_amd import ops
+    from asd_pocs_mex_cuda_demo_amd import workloads as W
+    cases = W.batch_cases()
+    K = len(cases)
+    t_plain = 0.0
+    for p, img, P in cases:
+        pl = ops.Plan(dict(p, maxnumIter=steps), p["phi_vec"])
+        pl.set_projections(P)
+        dt, _, _ = time_iterations(pl, img, steps, 1)
+        t_plain += dt
+        pl.close()
+    res = {"cases": K, "iterations": steps,
+           "sizes": sorted(set(int(p["nx"]) for p, _, _ in cases)),
+           "views_min": int(min(len(p["phi_vec"]) for p, _, _ in cases)),
+           "views_max": int(max(len(p["phi_vec"]) for p, _, _ in cases)),
+           "plain_sequential_case_iters_per_sec": K * steps / t_plain}
+    for prec, key in ((0, "batched_plain"), (1, "batched_mixed")):
+        B = ops.Batch([dict(p, maxnumIter=steps, precision=prec) for p, _, _ in cases])
+        for c, (p, img, P) in enumerate(cases):
+            B.set_case(c, P, None, img)
+        B.iterate(1)
+        for c, (p, img, P) in enumerate(cases):
+            B.set_case(c, P, None, img)
+        B.sync()
+        t0 = time.perf_counter()
+        B.iterate(steps)
+        B.sync()
+        res[f"{key}_case_iters_per_sec"] = K * steps / (time.perf_counter() - t0)
+        B.close()
+    res["speedup_batched_mixed_vs_sequential_plain"] = (
+        res["batched_mixed_case_iters_per_sec"] / res["plain_sequential_case_iters_per_sec"])
+    res["speedup_batched_mixed_vs_batched_plain"] = (
+        res["batched_mixed_case_iters_per_sec"] / res["batched_plain_case_iters_per_sec"])
+    return res
+
+
+def accuracy_fan2d_512(threads):
+    """The 512^2 fan-beam reconstruction (50 iterations) against the CPU
+    oracle: the reference-arithmetic plan (precision 4) and the plain FP32
+    plan, beside how far the oracle moves itself when only the order of its
+    FP64 sums changes, or its data by 1e-9 relative."""
+    from asd_pocs_mex_cuda_demo_amd import ops
+    from asd_pocs_mex_cuda_demo_amd import workloads as W
+    from oracle import oracle
+    oracle.set_threads(threads)
+    p, img, P = W.problem("fan2d_512")
+    q = dict(p, I0=img, precision=0)
+    mo, oo = oracle.asd_pocs(P, q)
+
+    def gap(o):
+        return np.abs(np.asarray(o["RMSE"]) - oo["RMSE"]) / oo["RMSE"]
+
+    def rel(m):
+        return float(np.linalg.norm(np.ravel(m, order="F") - mo) / np.linalg.norm(mo))
+    oracle.set_sum_order(1)
+    try:
+        _, o_pw = oracle.asd_pocs(P, q)
+    finally:
+        oracle.set_sum_order(0)
+    Pp = P * (1 + 1e-9 * np.random.default_rng(1).standard_normal(P.shape))
+    m9, o9 = oracle.asd_pocs(Pp, q)
+    m4, o4 = ops.ASD_POCS_cuda(P, dict(q, precision=4))
+    m0, o0 = ops.ASD_POCS_cuda(P, q)
+    g4, g0, gpw, g9 = gap(o4), gap(o0), gap(o_pw), gap(o9)
+    return {
+        "workload": "fan2d_512", "iterations": len(oo["RMSE"]),
+        "oracle": "FP32 projectors in the reference's expression and summation order, FP64 "
+                  "TV and driver with serial sums",
+        "reference_arithmetic_plan": {
+            "image_bit_identical": bool(np.array_equal(np.ravel(m4, order="F"), mo)),
+            "rmse_rel_diff_max": float(g4.max()), "alpha_equal": o4["alpha"] == oo["alpha"]},
+        "plain_fp32_plan": {"rmse_rel_diff_first_iteration": float(g0[0]),
+                            "rmse_rel_diff_max": float(g0.max()),
+                            "rmse_rel_diff_final": float(g0[-1]), "image_rel_l2": rel(m0)},
+        "oracle_vs_itself": {
+            "pairwise_fp64_sums": {"rmse_rel_diff_max": float(gpw.max()),
+                                   "rmse_rel_diff_first_three": [float(x) for x in gpw[:3]]},
+            "data_perturbed_1e-9": {"rmse_rel_diff_max": float(g9.max()),
+                                    "image_rel_l2": rel(m9)}},
+    }
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=None)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="fan2d_512", choices=sorted(WORKLOADS) + ["batch64"])
-    ap.add_argument("--cpu-tv-iters", type=int, default=None,
-                    help="TV sub-iterations the CPU sample times (scaled to ng)")
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="cone3d_512", choices=WORKLOAD_CHOICES)
     ap.add_argument("--precision", type=int, default=None,
                     help="sinogram storage / arithmetic variant (default: the workload's)")
-    ap.add_argument("--cpu-iters", type=int, default=5)
-    ap.add_argument("--cpu-views", type=int, default=None)
-    ap.add_argument("--prof-steps", type=int, default=10)
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--prof-steps", type=int, default=2,
+                    help="iterations of the per-kernel event timing pass")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="only the headline workload (for profiling runs)")
     ap.add_argument("--ax-shape", default=None,
                     help="force the forward projector's shape as LANES,ROWS (for example 8,3)")
-    ap.add_argument("--atx-variant", default=None,
-                    choices=["direct", "tile16x16x1", "tile8x8x4", "tile16x4x4", "pairs",
-                             "direct2", "direct1", "sorted16", "sorted8x8x4",
-                             "sorted8x4x8"],
-                    help="force the backprojector variant instead of the autotuner's pick "
-                         "(profiling runs use this to profile one variant)")
-    ap.add_argument("--batch-only", action="store_true",
-                    help="batch64: time only the batched forms (skips the sequential plain "
-                         "plans and the perturbation floor; for kernel traces of the batch)")
-    ap.add_argument("--eager", action="store_true",
-                    help="batch64: launch the batched iterations eagerly instead of replaying "
-                         "their hipGraph (same kernels; for tracers that cannot follow graph "
-                         "launches)")
+    ap.add_argument("--atx-variant", default=None, choices=ATX_VARIANTS,
+                    help="force the backprojector variant instead of the autotuner's pick")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 256 MiB device to device three times (a plain copy "
                          "kernel of known size, to scale rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -342,148 +464,32 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         del a, b
-
     if args.workload == "batch64":
-        bench_batch(args)
+        r = batch_rates(args.steps)
+        result = {"metric": METRIC, "value": r["batched_mixed_case_iters_per_sec"],
+                  "unit": "case-iters/sec", "n_gpus": 1, "steps": args.steps, "warmup": 1,
+                  "ms_per_step": r["cases"] / r["batched_mixed_case_iters_per_sec"] * 1e3,
+                  "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                  "dtype": "f32 (16-bit fixed-point sinogram, compensated sums)",
+                  "data": "synthetic: 64 perturbed modified Shepp-Logan phantoms, Poisson "
+                          "I0=1e5 (seeded)",
+                  "config": {"workload": "batch64", **{k: r[k] for k in
+                                                       ("cases", "iterations", "sizes",
+                                                        "views_min", "views_max")}},
+                  "batch": r}
+        print(json.dumps(result), flush=True)
         return
-    w = WORKLOADS[args.workload]
-    steps = args.steps if args.steps is not None else w.get("steps", w["iters"])
-    sim = w.get("sim") == "gpu"
-    if args.precision is None:
-        args.precision = w.get("precision", 0)
-    if sim:
-        para, plan = make_sim_problem(w, args.precision)
-        img = P = None  # the plan holds the phantom (RMSE reference) and sinogram
-    else:
-        para, img, P, plan = make_problem(w)
-        if args.precision:
-            plan.close()
-            from asd_pocs_mex_cuda_demo_amd import ops
-            plan = ops.Plan(dict(para, maxnumIter=w["iters"], precision=args.precision),
-                            para["phi_vec"])
-        plan.set_projections(P)
-    if args.atx_variant:
-        plan.set_atx_variant(list(ATX_KERNELS).index(args.atx_variant))
-    if args.ax_shape:
-        plan.set_ax_shape(*[int(v) for v in args.ax_shape.split(",")])
-    N = para["nx"] * para["ny"] * para["nz"]
-    M = para["na"] * para["nb"]
-
-    # warm-up iterations (untimed), then restart the reconstruction from zero
-    plan.reset(None, img)
-    plan.iterate(args.warmup)
-    plan.reset(None, img)
-    plan.sync()
-
-    # timed region: `steps` outer iterations replayed as a hipGraph
-    plan.sync()
-    t0 = time.perf_counter()
-    plan.iterate(steps)
-    plan.sync()
-    dt = time.perf_counter() - t0
-    out, rmse = plan.out()
-    done = out["IterationNum"]
-
-    # the same workload enqueued kernel by kernel with HIP events around every
-    # projector and TV-gradient launch on the plan stream (per-kernel durations)
-    prof_steps = min(steps, args.prof_steps)
-    plan.reset(None, img)
-    plan.set_timing(True)
-    plan.sync()
-    t1 = time.perf_counter()
-    plan.iterate(prof_steps)
-    plan.sync()
-    dt_ev = time.perf_counter() - t1
-    stage = {s: plan.timing(s) for s in (0, 1, 2)}
-    plan.set_timing(False)
-
-    nbytes = stage_bytes(N, M, 20, 4 if args.precision == 0 else 2)
-    names = {0: "ax_kernel (fused SART forward step)", 1: "atx_kernel (fused backprojection + update)",
-             2: "TV descent (20 x tv_grad_kernel + tv_step_kernel)"}
-    per_stage = {}
-    for s, (ms, cnt) in stage.items():
-        if cnt:
-            avg = ms / cnt
-            per_stage[names[s]] = {"avg_us": avg * 1e3, "launches": cnt,
-                                   "total_ms": ms,
-                                   "GBps": nbytes[s] / (avg * 1e-3) / 1e9}
-    dom = max(stage, key=lambda s: stage[s][0])
-    ms_d, cnt_d = stage[dom]
-    achieved = nbytes[dom] / (ms_d / cnt_d * 1e-3) / 1e9 if cnt_d else 0.0
-    tuning = plan.tuning()
-    lpr, rows, _ = plan.info()
-    kernel = None
-    if args.precision == 0:
-        kernel = {0: (f"ax_rows_kernel<{lpr}, {rows}, {1 if rows == 3 else 2}, false, false, "
-                      f"EpiSartAx<SinoF32> >"
-                      if rows > 1 else f"ax_kernel<{lpr}, false, false, EpiSartAx<SinoF32> >"),
-                  1: ATX_KERNELS.get(tuning["atx_variant"])}.get(dom)
-    traffic = profiled_traffic(args.workload, kernel) if kernel else None
-    roof = {"bound": "hbm", "kernel": kernel or names[dom], "achieved": achieved,
-            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic, "algorithmic_bytes_per_launch": nbytes[dom]}
-
-    result = {
-        "metric": METRIC,
-        "value": done / dt,
-        "unit": "iters/sec",
-        "n_gpus": 1,
-        "steps": steps,
-        "warmup": args.warmup,
-        "ms_per_step": dt / done * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": {0: "f32", 1: "f32 (16-bit fixed-point sinogram, compensated sums)",
-                  2: "f32 (fp16 sinogram, compensated sums)",
-                  3: "f32 (16-bit fixed-point sinogram, compensated sums, "
-                     "f64 tile ray setup)"}[args.precision],
-        "data": ("synthetic, simulated on the device: modified Shepp-Logan phantom"
-                 + (f" + {w['spheres']} seeded spheres" if w.get("spheres") else "")
-                 + (f", Poisson noise I0={w['I0']:g} (seeded), log transform" if w["I0"]
-                    else ", noiseless projections")) if sim else
-                f"synthetic: modified Shepp-Logan phantom, Poisson noise I0={w['I0']:g} (seeded), "
-                f"log transform",
-        "config": {"workload": args.workload, "nx": para["nx"], "ny": para["ny"],
-                   "nz": para["nz"], "views": w["nv"], "na": para["na"], "nb": para["nb"],
-                   "iterations": steps, "ng": 20, "precision": args.precision},
-        "roofline": roof,
-        "stages": per_stage,
-        "ms_per_step_eager_with_events": dt_ev / prof_steps * 1e3,
-        "rmse_final": float(rmse[-1]) if len(rmse) else None,
-        "ax_lanes_per_ray": lpr,
-        "ax_rows_per_bundle": rows,
-        "atx_tuning": tuning,
-    }
-    if args.precision and not sim:
-        # matched accuracy: the plain FP32 plan over the same iterations, and
-        # the same plan on data perturbed by 1e-7 relative (the spread the TV
-        # descent gives to any change of the data at FP32 level)
-        rp, rq = plain_rmse_and_floor(para, img, P, steps, w["iters"])
-        result["rmse_rel_diff_vs_plain_fp32"] = float(np.max(np.abs(rmse - rp) / rp))
-        result["rmse_rel_diff_plain_under_1e-7_data_perturbation"] = float(
-            np.max(np.abs(rq - rp) / rp))
-    if not args.no_cpu:
-        views = args.cpu_views if args.cpu_views is not None else (
-            0 if para["nz"] == 1 else 1 if sim else 2)
-        if P is None:
-            P = plan.projections()
-        tv_iters = args.cpu_tv_iters if args.cpu_tv_iters is not None else (1 if sim else 20)
-        cb, orc_run = cpu_baseline(para, P, args.cpu_iters, views, I0=img, tv_iters=tv_iters)
-        result["cpu_baseline"] = cb
-        if orc_run is not None:
-            # the GPU plan over the oracle's iteration count, against the oracle
-            mo, outo = orc_run
-            plan.reset(None, img)
-            plan.iterate(args.cpu_iters)
-            _, rg = plan.out()
-            mg = plan.image().ravel(order="F").astype(np.float64)
-            result["accuracy_vs_oracle"] = {
-                "iterations": args.cpu_iters,
-                "rmse_rel_diff_max": float(np.max(np.abs(rg - outo["RMSE"]) / outo["RMSE"])),
-                "image_rel_l2": float(np.linalg.norm(mg - mo) / np.linalg.norm(mo)),
-                "oracle": "FP32 projectors in the reference's expression order, FP64 TV and driver"}
-    plan.close()
+    result = headline(args)
+    if not args.no_extras:
+        threads = cpu_threads()
+        extras = {}
+        for name in ("fan2d_256", "fan2d_512", "cone3d_256"):
+            if name != args.workload:
+                extras[name] = workload_rate(name, not args.no_cpu, threads)
+        result["workloads"] = extras
+        result["batch64"] = batch_rates()
+        if not args.no_cpu:
+            result["accuracy_vs_oracle"] = accuracy_fan2d_512(threads)
     print(json.dumps(result), flush=True)
 
 

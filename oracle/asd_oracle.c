@@ -281,7 +281,8 @@ void orc_ax(const float *X, float *y, float *sr, float cos_phi, float sin_phi,
 /* Backprojector (cu:214-331)                                                */
 /* ------------------------------------------------------------------------ */
 
-static void atx_one_voxel(float *X, const float *y, float *sc, long v, float cos_phi,
+/* Sums of voxel v, written to *xo and *sco. */
+static void atx_one_voxel(float *xo, const float *y, float *sco, long v, float cos_phi,
                           float sin_phi, const float *y_det, const float *z_det, float SO,
                           float OD, float scale, float dy_det, float dz_det, float dz, int nx,
                           int ny, int nz, int na, int nb)
@@ -318,8 +319,8 @@ static void atx_one_voxel(float *X, const float *y, float *sc, long v, float cos
             accl += l;
         }
     }
-    X[v] = acc * scale;
-    sc[v] = accl * scale;
+    *xo = acc * scale;
+    *sco = accl * scale;
 }
 
 /* One view of the backprojector; same arguments as the reference launcher
@@ -331,8 +332,126 @@ void orc_atx(float *X, const float *y, float *sc, float cos_phi, float sin_phi,
     long N = (long)nx * ny * nz;
 #pragma omp parallel for schedule(static) num_threads(NTH)
     for (long v = 0; v < N; v++)
-        atx_one_voxel(X, y, sc, v, cos_phi, sin_phi, y_det, z_det, SO, OD, scale, dy_det,
-                      dz_det, dz, nx, ny, nz, na, nb);
+        atx_one_voxel(&X[v], y, &sc[v], v, cos_phi, sin_phi, y_det, z_det, SO, OD, scale,
+                      dy_det, dz_det, dz, nx, ny, nz, na, nb);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Subsets of one view, for full-size views whose whole evaluation would take */
+/* minutes on the CPU: the same per-ray and per-voxel arithmetic as above.    */
+/* ------------------------------------------------------------------------ */
+
+/* y[k], sr[k] = the forward projector's sums of ray ids[k] (id = ib*na + ia),
+ * exactly as orc_ax forms them. */
+void orc_ax_rays(const float *X, float *y, float *sr, const long *ids, long nids, float cos_phi,
+                 float sin_phi, const float *y_det, const float *z_det, float SO, float OD,
+                 float scale, float dz, int nx, int ny, int nz, int na, int nb)
+{
+#pragma omp parallel for schedule(dynamic, 16) num_threads(NTH)
+    for (long k = 0; k < nids; k++) {
+        long id = ids[k];
+        ax_one_ray(X, &y[k], &sr[k], (int)(id % na), (int)(id / na), cos_phi, sin_phi, y_det,
+                   z_det, SO, OD, scale, dz, nx, ny, nz, na);
+    }
+}
+
+/* x[k], sc[k] = the backprojector's sums of voxel ids[k] (idx = (iz*ny +
+ * iy)*nx + ix), exactly as orc_atx forms them. */
+void orc_atx_voxels(float *x, const float *y, float *sc, const long *ids, long nids,
+                    float cos_phi, float sin_phi, const float *y_det, const float *z_det,
+                    float SO, float OD, float scale, float dy_det, float dz_det, float dz,
+                    int nx, int ny, int nz, int na, int nb)
+{
+#pragma omp parallel for schedule(static) num_threads(NTH)
+    for (long k = 0; k < nids; k++)
+        atx_one_voxel(&x[k], y, &sc[k], ids[k], cos_phi, sin_phi, y_det, z_det, SO, OD, scale,
+                      dy_det, dz_det, dz, nx, ny, nz, na, nb);
+}
+
+/* Bins [a_lo, a_hi] x [b_lo, b_hi] of voxel v's footprint (cu:230-257), as
+ * atx_one_voxel bounds them. */
+static void voxel_footprint(long v, float cos_phi, float sin_phi, float SO, float OD,
+                            float dy_det, float dz_det, float dz, int nx, int ny, int nz, int na,
+                            int nb, int *a_lo, int *a_hi, int *b_lo, int *b_hi)
+{
+    int ix = (int)(v % nx);
+    long iyz = v / nx;
+    int iy = (int)(iyz % ny), iz = (int)(iyz / ny);
+    float SD = SO + OD;
+    int na2 = na / 2, nb2 = nb / 2, nx2 = nx / 2, ny2 = ny / 2, nz2 = nz / 2;
+    float d = (float)sqrt((1 + dz * dz) / 2);
+    float zc = (float)(iz + 0.5 - nz2) * dz;
+    float yc = (float)(iy + 0.5 - ny2);
+    float xc = (float)(ix + 0.5 - nx2);
+    float xr = cos_phi * xc + sin_phi * yc;
+    float yr = -sin_phi * xc + cos_phi * yc;
+    float t = SD / ((xr + SO) * dy_det);
+    *a_hi = (int)floorf((yr + 1) * t + na2);
+    *a_lo = (int)floorf((yr - 1) * t + na2);
+    t = SD / ((xr + SO) * dz_det);
+    *b_hi = (int)floorf((zc + d) * t + nb2);
+    *b_lo = (int)floorf((zc - d) * t + nb2);
+    if (*a_lo < 0) *a_lo = 0;
+    if (*b_lo < 0) *b_lo = 0;
+    if (*a_hi > na - 1) *a_hi = na - 1;
+    if (*b_hi > nb - 1) *b_hi = nb - 1;
+}
+
+/* One SART view update (ASD_POCS_cuda.m:116-131 with relaxation `relax`) of
+ * the FP64 image m, evaluated only at voxels ids[k]: m_new[k] = max(0,
+ * m[v] + relax * bp[v] / sumC[v]) with bp the backprojection of
+ * cor = (P_v - proj) ./ sumR (sumR == 0 -> 1), proj and sumR from
+ * single(m).  Only the corrections of bins in those voxels' footprints are
+ * formed (each from its ray's sums, as orc_ax forms them), so the result
+ * equals the full update at those voxels.  phi, SO, OD, vxlsize, detsize as
+ * the gateway receives them. */
+void orc_sart_view_voxels(const double *m, const double *Pv, double phi, double SO_mm,
+                          double OD_mm, double vxlsize, double detsize, int nx, int ny, int nz,
+                          int na, int nb, double relax, const long *ids, long nids,
+                          double *m_new)
+{
+    size_t N = (size_t)nx * ny * nz, M = (size_t)na * nb;
+    float cp, sp, SO, OD, det, scale;
+    float *ydet = (float *)malloc(na * sizeof(float)), *zdet = (float *)malloc(nb * sizeof(float));
+    orc_gateway_geometry(phi, SO_mm, OD_mm, vxlsize, detsize, na, nb, &cp, &sp, &SO, &OD, &det,
+                         &scale, ydet, zdet);
+    float *mf = (float *)malloc(N * sizeof(float));
+    float *cor = (float *)calloc(M, sizeof(float));
+    unsigned char *need = (unsigned char *)calloc(M, 1);
+#pragma omp parallel for schedule(static) num_threads(NTH)
+    for (size_t i = 0; i < N; i++) mf[i] = (float)m[i];
+    /* the bins the listed voxels read (each flag is only ever set to 1) */
+#pragma omp parallel for schedule(static) num_threads(NTH)
+    for (long k = 0; k < nids; k++) {
+        int a0, a1, b0, b1;
+        voxel_footprint(ids[k], cp, sp, SO, OD, det, det, 1.0f, nx, ny, nz, na, nb, &a0, &a1,
+                        &b0, &b1);
+        for (int ib = b0; ib <= b1; ib++)
+            for (int ia = a0; ia <= a1; ia++) need[(size_t)ib * na + ia] = 1;
+    }
+#pragma omp parallel for schedule(dynamic, 16) num_threads(NTH)
+    for (long id = 0; id < (long)M; id++) {
+        if (!need[id]) continue;
+        float proj, sumR;
+        ax_one_ray(mf, &proj, &sumR, (int)(id % na), (int)(id / na), cp, sp, ydet, zdet, SO, OD,
+                   scale, 1.0f, nx, ny, nz, na);
+        double r = (double)sumR;
+        if (r == 0) r = 1;
+        double c0 = Pv[id] - (double)proj;
+        cor[id] = (float)(c0 / r);
+    }
+#pragma omp parallel for schedule(static) num_threads(NTH)
+    for (long k = 0; k < nids; k++) {
+        float bp, sumC;
+        long v = ids[k];
+        atx_one_voxel(&bp, cor, &sumC, v, cp, sp, ydet, zdet, SO, OD, scale, det, det, 1.0f,
+                      nx, ny, nz, na, nb);
+        double c = (double)sumC;
+        if (c == 0) c = 1;
+        double nm = m[v] + relax * ((double)bp / c);
+        m_new[k] = nm < 0 ? 0 : nm;
+    }
+    free(ydet); free(zdet); free(mf); free(cor); free(need);
 }
 
 /* ------------------------------------------------------------------------ */

@@ -61,6 +61,16 @@ def lib():
         L.orc_atx.restype = None
         L.orc_atx.argtypes = [f32p, f32p, f32p, fl, fl, f32p, f32p, fl, fl, fl, fl, fl, fl,
                               it, it, it, it, it]
+        lp = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+        L.orc_ax_rays.restype = None
+        L.orc_ax_rays.argtypes = [f32p, f32p, f32p, lp, C.c_long, fl, fl, f32p, f32p, fl, fl, fl,
+                                  fl, it, it, it, it, it]
+        L.orc_atx_voxels.restype = None
+        L.orc_atx_voxels.argtypes = [f32p, f32p, f32p, lp, C.c_long, fl, fl, f32p, f32p, fl, fl,
+                                     fl, fl, fl, fl, it, it, it, it, it]
+        L.orc_sart_view_voxels.restype = None
+        L.orc_sart_view_voxels.argtypes = [f64p, f64p] + [C.c_double] * 5 + [it] * 5 + [
+            C.c_double, lp, C.c_long, f64p]
         L.orc_tv2d_grad.restype = None
         L.orc_tv2d_grad.argtypes = [f64p, it, it, f64p]
         L.orc_tv3d_grad.restype = None
@@ -164,6 +174,55 @@ def atx_para(Y, para):
     Y = np.asarray(Y, np.float32).reshape(-1, order="F")
     return atx(Y, g["cos_phi"], g["sin_phi"], g["y_det"], g["z_det"], g["SO"], g["OD"],
                g["scale"], g["det"], g["det"], np.float32(1.0), nx, ny, nz, na, nb)
+
+
+def _dims(para):
+    return tuple(int(para[k]) for k in ("nx", "ny", "nz", "na", "nb"))
+
+
+def ax_rays_para(X, para, ids):
+    """Forward-projector sums (y, sr) of the rays `ids` (id = ib*na + ia) of
+    the view para['phi'], as ax_para would give them."""
+    g = _geom_from_para(para)
+    nx, ny, nz, na, nb = _dims(para)
+    X = np.ascontiguousarray(np.asarray(X, np.float32).reshape(-1, order="F"))
+    ids = np.ascontiguousarray(ids, np.int64)
+    y = np.zeros(ids.size, np.float32)
+    sr = np.zeros(ids.size, np.float32)
+    lib().orc_ax_rays(X, y, sr, ids, ids.size, g["cos_phi"], g["sin_phi"], g["y_det"],
+                      g["z_det"], g["SO"], g["OD"], g["scale"], np.float32(1.0), nx, ny, nz, na,
+                      nb)
+    return y, sr
+
+
+def atx_voxels_para(Y, para, ids):
+    """Backprojector sums (x, sc) of the voxels `ids` (MATLAB linear index)
+    of the view para['phi'], as atx_para would give them."""
+    g = _geom_from_para(para)
+    nx, ny, nz, na, nb = _dims(para)
+    Y = np.ascontiguousarray(np.asarray(Y, np.float32).reshape(-1, order="F"))
+    ids = np.ascontiguousarray(ids, np.int64)
+    x = np.zeros(ids.size, np.float32)
+    sc = np.zeros(ids.size, np.float32)
+    lib().orc_atx_voxels(x, Y, sc, ids, ids.size, g["cos_phi"], g["sin_phi"], g["y_det"],
+                         g["z_det"], g["SO"], g["OD"], g["scale"], g["det"], g["det"],
+                         np.float32(1.0), nx, ny, nz, na, nb)
+    return x, sc
+
+
+def sart_view_voxels(m, Pv, para, phi_deg, ids, relax=1.0):
+    """One SART view update (ASD_POCS_cuda.m:116-131) of the FP64 image m
+    with the view's FP64 projection Pv at angle phi_deg (degrees), evaluated
+    at the voxels `ids` only."""
+    nx, ny, nz, na, nb = _dims(para)
+    m = np.ascontiguousarray(np.asarray(m, np.float64).reshape(-1, order="F"))
+    Pv = np.ascontiguousarray(np.asarray(Pv, np.float64).ravel())
+    ids = np.ascontiguousarray(ids, np.int64)
+    out = np.zeros(ids.size)
+    lib().orc_sart_view_voxels(m, Pv, -float(phi_deg) / 180 * np.pi, float(para["SO"]),
+                               float(para["OD"]), float(para["vxlsize"]), float(para["detsize"]),
+                               nx, ny, nz, na, nb, float(relax), ids, ids.size, out)
+    return out
 
 
 def tv2d_grad(u):
@@ -284,7 +343,8 @@ def sart(P, para):
     p = _orc_para(para)
     N = p.nx * p.ny * p.nz
     m0 = para.get("m0", None)
-    m = np.zeros(N) if m0 is None else np.ascontiguousarray(np.asarray(m0, np.float64).reshape(-1, order="F"))
+    # a copy: the sweep updates m in place and must not touch the caller's m0
+    m = np.zeros(N) if m0 is None else np.array(m0, np.float64).reshape(-1, order="F").copy()
     lam = para.get("lambda", None)
     tol = lib().orc_sart(Pv, phi, nv, C.byref(p), float(1.0 if lam is None else lam), m)
     return m, dict(IterationNum=1, dataTolerance=tol, reconImg=m)

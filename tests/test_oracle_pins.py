@@ -473,3 +473,31 @@ def test_tv_descent_amplifies_data_perturbations(orc):
     d0, d10 = rmse_diff(0), rmse_diff(10)
     assert d0.max() < 1e-7
     assert d10[0] < 1e-7 and d10[-1] > 1e-5
+
+
+# ---------------------------------------------------------------------------
+# Subset entry points (for full-size views on the GPU tests)
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("dims", [(24, 20, 1, 37, 1), (16, 12, 9, 30, 14)])
+def test_subset_entry_points_equal_full_ones(orc, dims):
+    nx, ny, nz, na, nb = dims
+    para = dict(nx=nx, ny=ny, nz=nz, na=na, nb=nb, SO=60.0, OD=50.0, vxlsize=0.8, detsize=1.1)
+    rng = np.random.default_rng(31)
+    X = rng.random(nx * ny * nz).astype(np.float32)
+    Y = rng.standard_normal(na * nb).astype(np.float32)
+    q = dict(para, phi=-1.1)
+    y, sr = orc.ax_para(X, q)
+    x, sc = orc.atx_para(Y, q)
+    rays = rng.choice(na * nb, min(40, na * nb), replace=False)
+    vox = rng.choice(nx * ny * nz, 60, replace=False)
+    ys, srs = orc.ax_rays_para(X, q, rays)
+    xs, scs = orc.atx_voxels_para(Y, q, vox)
+    assert np.array_equal(ys, y[rays]) and np.array_equal(srs, sr[rays])
+    assert np.array_equal(xs, x[vox]) and np.array_equal(scs, sc[vox])
+    # one SART view at voxels = the full one-view sweep there
+    m = rng.random(nx * ny * nz) * 0.02
+    Pv = rng.random(na * nb)
+    full, _ = orc.sart(Pv[:, None], dict(para, phi_vec=[63.0], m0=m, **{"lambda": 0.9}))
+    part = orc.sart_view_voxels(m, Pv, para, 63.0, vox, relax=0.9)
+    assert np.array_equal(part, full[vox])
