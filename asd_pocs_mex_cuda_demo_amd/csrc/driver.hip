@@ -502,7 +502,9 @@ int autotune_atx_t(asd_plan *p)
         }
         if (rc) break;
         p->atx_us[v] = ms_min * 1e3;
-        if (ms_min * 1e3 < best_us) {
+        // a later candidate must be faster by more than 3%: near-ties keep the
+        // earlier one, so the choice does not flip between runs on timing noise
+        if (ms_min * 1e3 < best_us * 0.97) {
             best_us = ms_min * 1e3;
             best = v;
         }

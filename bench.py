@@ -44,7 +44,7 @@ ATX_VARIANTS = ["direct", "tile16x16x1", "tile8x8x4", "tile16x4x4", "pairs", "di
 
 def atx_kernel_name(variant, kahan):
     k = "true" if kahan else "false"
-    return {"direct": f"atx_kernel<EpiSartAtx, {k}>",
+    return {"direct": f"atx_kernel<EpiSartAtx, {k}, 4>",
             "tile16x16x1": f"atx_tile_plain_kernel<16, 16, 1, {k}, EpiSartAtx>",
             "tile8x8x4": f"atx_tile_plain_kernel<8, 8, 4, {k}, EpiSartAtx>",
             "tile16x4x4": f"atx_tile_plain_kernel<16, 4, 4, {k}, EpiSartAtx>",
