@@ -18,7 +18,8 @@ ASD_ERR_STATE = 4
 # Every symbol declared in include/asdpocs.h.
 EXPORTS = [
     "Ax_cone_mf_gpu_new", "Atx_cone_mf_gpu_new", "asd_last_status", "asd_last_error",
-    "asd_ax_device", "asd_atx_device", "asd_atx_device_noalloc", "asd_tv_min_2d", "asd_tv_min_3d", "asd_tv_min_device",
+    "asd_ax_device", "asd_atx_device", "asd_ax_slab_device", "asd_atx_slab_device",
+    "asd_atx_device_noalloc", "asd_tv_min_2d", "asd_tv_min_3d", "asd_tv_min_device",
     "asd_para_defaults", "asd_pocs", "asd_sart", "asd_plan_create", "asd_plan_set_projections",
     "asd_plan_reset", "asd_plan_iterate", "asd_plan_get_image", "asd_plan_get_out",
     "asd_plan_sweep", "asd_plan_get_current", "asd_plan_sync", "asd_plan_set_timing",
@@ -83,6 +84,12 @@ def lib():
     L.asd_ax_device.argtypes = [vp, vp, vp, f, f, vp, vp, f, f, f, f, i, i, i, i, i, vp]
     L.asd_atx_device.restype = i
     L.asd_atx_device.argtypes = [vp, vp, vp, f, f, vp, vp, f, f, f, f, f, f, i, i, i, i, i, vp]
+    L.asd_ax_slab_device.restype = i
+    L.asd_ax_slab_device.argtypes = [vp, vp, vp, f, f, vp, vp, f, f, f, f, i, i, i, i, i, i, i,
+                                     vp]
+    L.asd_atx_slab_device.restype = i
+    L.asd_atx_slab_device.argtypes = [vp, vp, vp, f, f, vp, vp, f, f, f, f, f, f, i, i, i, i, i,
+                                      i, i, vp]
     L.asd_atx_device_noalloc.restype = i
     L.asd_atx_device_noalloc.argtypes = [vp, vp, vp, f, f, vp, vp, f, f, f, f, f, f, i, i, i, i,
                                          i, vp]

@@ -97,6 +97,12 @@ struct ViewGeom {
     int tight;               // footprints may be cut to the voxel's shadow (view_tight)
     FastDiv div_nx, div_ny;  // voxel index -> (x, row) and row -> (y, z) (geom_finish)
     FastDiv div_na;          // bin index -> (ia, ib) (geom_finish)
+    // z-window [z0, z0 + nzs) of the volume the buffers hold (geom_finish:
+    // the whole volume).  The slab entry points narrow it: the forward
+    // kernels then sum only the pieces in those slices (a slab's partial ray
+    // sums, read from a slab-resident volume) and atx_kernel evaluates only
+    // those slices' voxels; geometry stays the whole volume's (nz).
+    int z0, nzs;
 };
 
 // Per-ray record for the backprojector: the six slope quotients the

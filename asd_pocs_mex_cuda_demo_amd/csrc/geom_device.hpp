@@ -29,6 +29,7 @@ struct AxRay {
     // fast_t / fast_z: the slab's crossing fractions may use div_cell (below)
     int fast_t, fast_z;
     int nu2, nt2, nz2, nt, nz;
+    int zlo, zhi;    // the slices summed: [zlo, zhi) (the view's z-window)
     int xdriven;
     // flat: a one-slice volume and a ray in its z = 0 face (s2 = +-0, dz = 1),
     // where every piece lies in slice 0 with rz = 1; dz1: dz == 1, where
@@ -126,6 +127,8 @@ __host__ __device__ __forceinline__ AxRay ax_ray_setup(const ViewGeom &g, float 
     r.nt = r.xdriven ? g.ny : g.nx;
     r.nz = g.nz;
     r.nz2 = g.nz / 2;
+    r.zlo = g.z0;
+    r.zhi = g.z0 + g.nzs;
     r.dz1 = g.dz == 1.0f;
     // coordinates below 2^22 and at least two cells along each axis (div_cell)
     const bool small = g.SO + g.OD + (float)g.nx + (float)g.ny + (float)g.nz < 0x1p21f;
@@ -168,6 +171,8 @@ __host__ __device__ __forceinline__ void geom_finish(ViewGeom &g)
     g.div_nx = make_fastdiv((unsigned)g.nx);
     g.div_ny = make_fastdiv((unsigned)g.ny);
     g.div_na = make_fastdiv((unsigned)g.na);
+    g.z0 = 0;
+    g.nzs = g.nz;
 }
 
 // Whether the backprojector footprint may be cut to the voxel's shadow: no
@@ -388,7 +393,8 @@ __host__ __device__ __forceinline__ void ax_slab_z(const AxRay &r, const SlabT &
         cz1 = (int)floorf(za);
         cz2 = (int)floorf(zb);
     }
-    const bool in_z1 = cz1 >= 0 && cz1 < r.nz, in_z2 = cz2 >= 0 && cz2 < r.nz;
+    // in the volume's z-window (the whole volume, or a slab's slices)
+    const bool in_z1 = cz1 >= r.zlo && cz1 < r.zhi, in_z2 = cz2 >= r.zlo && cz2 < r.zhi;
     s.ok[0] = t.in_t1 && in_z1;
     s.ok[2] = t.in_t2 && in_z2;
     // a slab wholly outside the volume has no piece in it
@@ -407,10 +413,12 @@ __host__ __device__ __forceinline__ void ax_slab_z(const AxRay &r, const SlabT &
     // (d*d + 1) * q >= q >= tmp, so the root's argument is >= 1
     const float l = sqrt_ge_tiny(div_by_tmp(r, t.d2p1 * r.q));
     const float lo = tfirst ? t.rt : rz, hi = tfirst ? rz : t.rt;
-    // slice parts cz * ny * nx * unit in two 24-bit products (index_fits:
-    // cz * ny < 2^24; vol_buffer_ok or index_fits: nx * unit < 2^24)
-    const unsigned zp1 = mad24(mad24((unsigned)cz1, (unsigned)ny, 0u), A.nxu, 0u);
-    const unsigned zp2 = mad24(mad24((unsigned)cz2, (unsigned)ny, 0u), A.nxu, 0u);
+    // slice parts (cz - zlo) * ny * nx * unit, relative to the window's first
+    // slice, in two 24-bit products (index_fits: cz * ny < 2^24;
+    // vol_buffer_ok or index_fits: nx * unit < 2^24); a slice outside the
+    // window gives an address of a piece that is not used
+    const unsigned zp1 = mad24(mad24((unsigned)(cz1 - r.zlo), (unsigned)ny, 0u), A.nxu, 0u);
+    const unsigned zp2 = mad24(mad24((unsigned)(cz2 - r.zlo), (unsigned)ny, 0u), A.nxu, 0u);
     s.off[0] = t.tp1 + zp1;
     s.off[1] = tfirst ? t.tp2 + zp1 : t.tp1 + zp2;
     s.off[2] = t.tp2 + zp2;

@@ -37,7 +37,7 @@ __global__ __launch_bounds__(256) void ax_kernel(ViewGeom g, const float *__rest
     // so each XCD's L2 holds one wedge of the volume instead of all of it
     const long id = (long)xcd_block(blockIdx.x, gridDim.x) * RPB + threadIdx.x / LPR;
     const bool active = id < nrays;
-    const VolRead<WIDE> V(X, (size_t)g.nx * g.ny * g.nz);
+    const VolRead<WIDE> V(X, (size_t)g.nx * g.ny * g.nzs);  // the window's slices
     typename Epi::Pre pre{};
     if (active && lane == 0) pre = epi.load(id);
     Acc<KAHAN> sa, sl;
@@ -129,7 +129,7 @@ __global__ __launch_bounds__(256) void ax_rows_kernel(ViewGeom g, const float *_
     const unsigned ub = active ? fastdiv((unsigned)id, g.div_na) : 0u;
     const int ia = (int)((unsigned)id - mad24(ub, (unsigned)g.na, 0u)), ib0 = (int)ub * K;
     const bool mine = active && lane < K && ib0 + lane < g.nb;
-    const VolRead<WIDE> V(X, (size_t)g.nx * g.ny * g.nz);
+    const VolRead<WIDE> V(X, (size_t)g.nx * g.ny * g.nzs);  // the window's slices
     const long ray = (long)(ib0 + lane) * g.na + ia;
     typename Epi::Pre pre{};
     if (mine) pre = epi.load(ray);
@@ -257,7 +257,7 @@ __global__ __launch_bounds__(256) void ax_ordered_kernel(ViewGeom g, const float
     const int lane = threadIdx.x & 63;
     const long id = (long)xcd_block(blockIdx.x, gridDim.x) * 4 + threadIdx.x / 64;
     if (id >= nrays) return;  // whole waves
-    const VolRead<WIDE> V(X, (size_t)g.nx * g.ny * g.nz);
+    const VolRead<WIDE> V(X, (size_t)g.nx * g.ny * g.nzs);  // the window's slices
     typename Epi::Pre pre{};
     if (lane == 0) pre = epi.load(id);
     // id < na * nb < 2^31 and ib < nb < 2^24 (index_fits)
@@ -318,7 +318,7 @@ int dispatch_ax_ordered(const ViewGeom &g, const float *x, const float *y_det,
 {
     const long nrays = (long)g.na * g.nb;
     const unsigned nblk = (unsigned)((nrays + 3) / 4);
-    if (vol_buffer_ok((size_t)g.nx * g.ny * g.nz, g.nx))
+    if (vol_buffer_ok((size_t)g.nx * g.ny * g.nzs, g.nx))
         klaunch(ax_ordered_kernel<false, Epi>, nblk, 256, 0, s, g, x, y_det, z_det, epi);
     else
         klaunch(ax_ordered_kernel<true, Epi>, nblk, 256, 0, s, g, x, y_det, z_det, epi);
@@ -368,7 +368,7 @@ template <bool KAHAN = false, class Epi>
 int dispatch_ax(AxShape sh, const ViewGeom &g, const float *x, const float *y_det,
                 const float *z_det, const Epi &epi, hipStream_t s)
 {
-    if (vol_buffer_ok((size_t)g.nx * g.ny * g.nz, g.nx))
+    if (vol_buffer_ok((size_t)g.nx * g.ny * g.nzs, g.nx))
         return dispatch_ax_t<KAHAN, false>(sh, g, x, y_det, z_det, epi, s);
     return dispatch_ax_t<KAHAN, true>(sh, g, x, y_det, z_det, epi, s);
 }
@@ -481,14 +481,15 @@ __global__ __launch_bounds__(256) void atx_kernel(ViewGeom g, const float *__res
     const bool stop = epi.skip();
     const unsigned b = xcd_block(blockIdx.x, gridDim.x);
     const unsigned v = b * 256u + threadIdx.x;
-    const unsigned N = (unsigned)g.nx * (unsigned)g.ny * (unsigned)g.nz;
+    // the voxels of the z-window (the whole volume, or a slab's slices)
+    const unsigned N = (unsigned)g.nx * (unsigned)g.ny * (unsigned)g.nzs;
     if (v >= N) return;
     const typename Epi::Pre pre = epi.load(v);
     // r < ny*nz < 2^24 and iz < nz (index_fits): 24-bit products
     const unsigned r = fastdiv(v, g.div_nx), iz = fastdiv(r, g.div_ny);
     const int ix = (int)(v - mad24(r, (unsigned)g.nx, 0u));
     const int iy = (int)(r - mad24(iz, (unsigned)g.ny, 0u));
-    Footprint f = footprint(g, ix, iy, iz);
+    Footprint f = footprint(g, ix, iy, (int)iz + g.z0);
     const float sx = g.cos_phi * (-g.SO), sy = g.sin_phi * (-g.SO);
     const float px = sx - f.xc, py = sy - f.yc, pz = 0.0f - f.zc;
     Acc<KAHAN> acc, accl;
@@ -822,7 +823,7 @@ int dispatch_atx_variant(int variant, const ViewGeom &g, const float *Y, const R
     case ATX_T8x8x4: return launch_atx_tile_t<8, 8, 4, KAHAN>(g, Y, rt, epi, s);
     case ATX_T16x4x4: return launch_atx_tile_t<16, 4, 4, KAHAN>(g, Y, rt, epi, s);
     case ATX_DIRECT2: {
-        const long N = (long)g.nx * g.ny * g.nz;
+        const long N = (long)g.nx * g.ny * g.nzs;
         klaunch(atx_kernel<Epi, KAHAN, 2>, (unsigned)((N + 255) / 256), 256, 0, s, g, Y, rt, epi);
         ASD_LAUNCH_CHECK("atx_kernel");
         return ASD_OK;
@@ -847,7 +848,7 @@ int dispatch_atx_variant(int variant, const ViewGeom &g, const float *Y, const R
         return ASD_OK;
     }
     case ATX_DIRECT1: {
-        const long N = (long)g.nx * g.ny * g.nz;
+        const long N = (long)g.nx * g.ny * g.nzs;
         klaunch(atx_kernel<Epi, KAHAN, 1>, (unsigned)((N + 255) / 256), 256, 0, s, g, Y, rt, epi);
         ASD_LAUNCH_CHECK("atx_kernel");
         return ASD_OK;
@@ -859,7 +860,7 @@ int dispatch_atx_variant(int variant, const ViewGeom &g, const float *Y, const R
         return ASD_OK;
     }
     default: {
-        const long N = (long)g.nx * g.ny * g.nz;
+        const long N = (long)g.nx * g.ny * g.nzs;
         klaunch(atx_kernel<Epi, KAHAN>, (unsigned)((N + 255) / 256), 256, 0, s, g, Y, rt, epi);
         ASD_LAUNCH_CHECK("atx_kernel");
         return ASD_OK;

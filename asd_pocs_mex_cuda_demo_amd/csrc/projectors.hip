@@ -301,6 +301,42 @@ int asd_atx_device(float *X, const float *y, float *sc, float cos_phi, float sin
     return launch_atx(g, X, y, sc, y_det, z_det, tl_rt.as<RayRec>(), s);
 }
 
+int asd_ax_slab_device(const float *X, float *y, float *sr, float cos_phi, float sin_phi,
+                       const float *y_det, const float *z_det, float SO, float OD, float scale,
+                       float dz, int nx, int ny, int nz, int na, int nb, int z0, int nzs,
+                       void *stream)
+{
+    clear_error();
+    ViewGeom g = make_geom(cos_phi, sin_phi, SO, OD, scale, 1.0f, 1.0f, dz, nx, ny, nz, na, nb);
+    int rc = check_geom(g);
+    if (rc) return rc;
+    ASD_ARG(X && y && sr && y_det && z_det, "NULL pointer argument");
+    ASD_ARG(z0 >= 0 && nzs > 0 && z0 <= nz - nzs, "the slab [z0, z0 + nzs) must lie in [0, nz)");
+    g.z0 = z0;
+    g.nzs = nzs;
+    return launch_ax(g, X, y, sr, y_det, z_det, (hipStream_t)stream);
+}
+
+int asd_atx_slab_device(float *X, const float *y, float *sc, float cos_phi, float sin_phi,
+                        const float *y_det, const float *z_det, float SO, float OD, float scale,
+                        float dy_det, float dz_det, float dz, int nx, int ny, int nz, int na,
+                        int nb, int z0, int nzs, void *stream)
+{
+    clear_error();
+    ViewGeom g = make_geom(cos_phi, sin_phi, SO, OD, scale, dy_det, dz_det, dz, nx, ny, nz, na,
+                           nb);
+    int rc = check_geom(g);
+    if (rc) return rc;
+    ASD_ARG(X && y && sc && y_det && z_det, "NULL pointer argument");
+    ASD_ARG(z0 >= 0 && nzs > 0 && z0 <= nz - nzs, "the slab [z0, z0 + nzs) must lie in [0, nz)");
+    g.z0 = z0;
+    g.nzs = nzs;
+    if ((rc = tl_rt.ensure((size_t)na * nb * sizeof(RayRec)))) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    if ((rc = launch_ray_table(g, y_det, z_det, tl_rt.as<RayRec>(), s))) return rc;
+    return launch_atx(g, X, y, sc, y_det, z_det, tl_rt.as<RayRec>(), s);
+}
+
 int asd_atx_device_noalloc(float *X, const float *y, float *sc, float cos_phi, float sin_phi,
                            const float *y_det, const float *z_det, float SO, float OD,
                            float scale, float dy_det, float dz_det, float dz, int nx, int ny,

@@ -66,6 +66,25 @@ int asd_atx_device(float *X, const float *y, float *sc, float cos_phi, float sin
                    float dy_det, float dz_det, float dz, int nx, int ny, int nz, int na, int nb,
                    void *stream);
 
+/* The operators on one z-slab of an nx x ny x nz volume, the unit of a
+ * volume sharded over GPUs by slices (no counterpart in the reference, whose
+ * kernels take the whole volume, src/AxAtx/Ax_cone_gpu.cu:315-316, :796).
+ * The geometry is the whole volume's; the slab is slices [z0, z0 + nzs),
+ * and X (nx * ny * nzs floats, x fastest) holds only those slices.
+ * asd_ax_slab_device: y and sr are the slab's partial ray sums (the pieces
+ * of each ray inside the slab); the partial sums of all slabs add up to
+ * asd_ax_device's y and sr (to the order of the additions).
+ * asd_atx_slab_device: X and sc receive the slab's voxels, each equal bit
+ * for bit to asd_atx_device's value for that voxel. */
+int asd_ax_slab_device(const float *X, float *y, float *sr, float cos_phi, float sin_phi,
+                       const float *y_det, const float *z_det, float SO, float OD, float scale,
+                       float dz, int nx, int ny, int nz, int na, int nb, int z0, int nzs,
+                       void *stream);
+int asd_atx_slab_device(float *X, const float *y, float *sc, float cos_phi, float sin_phi,
+                        const float *y_det, const float *z_det, float SO, float OD, float scale,
+                        float dy_det, float dz_det, float dz, int nx, int ny, int nz, int na,
+                        int nb, int z0, int nzs, void *stream);
+
 /* asd_atx_device without any allocation: no counterpart in the reference.
  * For callers that cannot allocate device memory (work under HIP graph
  * capture, serving loops, harnesses that forbid hipMalloc).  Same arguments

@@ -200,8 +200,8 @@ void orc_gateway_geometry(double phi, double SO, double OD, double vxlsize, doub
  * (ct1,cz1), (ct2,cz1) or (ct1,cz2), and (ct2,cz2); each in-range piece adds
  * w*l*x to the ray sum and w*l to the row sum.  This rule reproduces all
  * sixteen labelled boundary cases of the reference (// 11 ... // 44). */
-static inline void slab_pieces(float t1, float t2, float zz1, float zz2, int nt, int nz,
-                               float *w, int *ct, int *cz)
+static inline void slab_pieces(float t1, float t2, float zz1, float zz2, int nt, int zlo,
+                               int zhi, float *w, int *ct, int *cz)
 {
     int ct1 = (int)floorf(t1), ct2 = (int)floorf(t2);
     int cz1 = (int)floorf(zz1), cz2 = (int)floorf(zz2);
@@ -213,14 +213,17 @@ static inline void slab_pieces(float t1, float t2, float zz1, float zz2, int nt,
     if (rt > rz) { ct[1] = ct1; cz[1] = cz2; } else { ct[1] = ct2; cz[1] = cz1; }
     w[2] = 1.0f - hi; ct[2] = ct2; cz[2] = cz2;
     for (int k = 0; k < 3; k++)
-        if (ct[k] < 0 || ct[k] >= nt || cz[k] < 0 || cz[k] >= nz) ct[k] = -1;
+        if (ct[k] < 0 || ct[k] >= nt || cz[k] < zlo || cz[k] >= zhi) ct[k] = -1;
     /* The reference adds only the pieces its case analysis names; a piece of
      * zero weight (no crossing) adds +0 and leaves the sums unchanged. */
 }
 
+/* The sums of ray (ia, ib) over the pieces in slices [zlo, zhi) (the whole
+ * volume: 0, nz); X holds those slices only (slice zlo first). */
 static void ax_one_ray(const float *X, float *yout, float *srout, int ia, int ib,
                        float cos_phi, float sin_phi, const float *y_det, const float *z_det,
-                       float SO, float OD, float scale, float dz, int nx, int ny, int nz, int na)
+                       float SO, float OD, float scale, float dz, int nx, int ny, int nz, int na,
+                       int zlo, int zhi)
 {
     int nx2 = nx / 2, ny2 = ny / 2, nz2 = nz / 2;
     float x1 = cos_phi * (-SO), y1 = sin_phi * (-SO), z1 = 0.0f;
@@ -245,14 +248,14 @@ static void ax_one_ray(const float *X, float *yout, float *srout, int ia, int ib
         if (s2 >= 0) { za = (z1 + s2 * (ua - u1)) / dz + nz2; zb = (z1 + s2 * (ub - u1)) / dz + nz2; }
         else         { za = (z1 + s2 * (ub - u1)) / dz + nz2; zb = (z1 + s2 * (ua - u1)) / dz + nz2; }
         float w[3]; int ct[3], cz[3];
-        slab_pieces(ta, tb, za, zb, nt, nz, w, ct, cz);
+        slab_pieces(ta, tb, za, zb, nt, zlo, zhi, w, ct, cz);
         if (ct[0] < 0 && ct[1] < 0 && ct[2] < 0) continue;
         float d = tb - ta;
         float l = (float)sqrt((d * d + 1) * q / tmp);
         for (int k = 0; k < 3; k++) {
             if (ct[k] < 0) continue;
             int ix = xdriven ? iu : ct[k], iy = xdriven ? ct[k] : iu;
-            size_t idx = (size_t)cz[k] * ny * nx + (size_t)iy * nx + ix;
+            size_t idx = (size_t)(cz[k] - zlo) * ny * nx + (size_t)iy * nx + ix;
             float wl = w[k] * l;
             acc += wl * X[idx];
             accl += wl;
@@ -273,7 +276,7 @@ void orc_ax(const float *X, float *y, float *sr, float cos_phi, float sin_phi,
     for (long id = 0; id < nrays; id++) {
         int ia = (int)(id % na), ib = (int)(id / na);
         ax_one_ray(X, &y[id], &sr[id], ia, ib, cos_phi, sin_phi, y_det, z_det, SO, OD, scale,
-                   dz, nx, ny, nz, na);
+                   dz, nx, ny, nz, na, 0, nz);
     }
 }
 
@@ -351,7 +354,23 @@ void orc_ax_rays(const float *X, float *y, float *sr, const long *ids, long nids
     for (long k = 0; k < nids; k++) {
         long id = ids[k];
         ax_one_ray(X, &y[k], &sr[k], (int)(id % na), (int)(id / na), cos_phi, sin_phi, y_det,
-                   z_det, SO, OD, scale, dz, nx, ny, nz, na);
+                   z_det, SO, OD, scale, dz, nx, ny, nz, na, 0, nz);
+    }
+}
+
+/* The forward projector's partial sums over the z-slab [z0, z0 + nzs) of an
+ * nx x ny x nz volume (the slab's pieces of every ray, with the whole
+ * volume's geometry), of the rays ids[k]; X holds the slab's slices only. */
+void orc_ax_slab_rays(const float *X, float *y, float *sr, const long *ids, long nids,
+                      float cos_phi, float sin_phi, const float *y_det, const float *z_det,
+                      float SO, float OD, float scale, float dz, int nx, int ny, int nz, int na,
+                      int nb, int z0, int nzs)
+{
+#pragma omp parallel for schedule(dynamic, 16) num_threads(NTH)
+    for (long k = 0; k < nids; k++) {
+        long id = ids[k];
+        ax_one_ray(X, &y[k], &sr[k], (int)(id % na), (int)(id / na), cos_phi, sin_phi, y_det,
+                   z_det, SO, OD, scale, dz, nx, ny, nz, na, z0, z0 + nzs);
     }
 }
 
@@ -434,7 +453,7 @@ void orc_sart_view_voxels(const double *m, const double *Pv, double phi, double 
         if (!need[id]) continue;
         float proj, sumR;
         ax_one_ray(mf, &proj, &sumR, (int)(id % na), (int)(id / na), cp, sp, ydet, zdet, SO, OD,
-                   scale, 1.0f, nx, ny, nz, na);
+                   scale, 1.0f, nx, ny, nz, na, 0, nz);
         double r = (double)sumR;
         if (r == 0) r = 1;
         double c0 = Pv[id] - (double)proj;

@@ -65,6 +65,9 @@ def lib():
         L.orc_ax_rays.restype = None
         L.orc_ax_rays.argtypes = [f32p, f32p, f32p, lp, C.c_long, fl, fl, f32p, f32p, fl, fl, fl,
                                   fl, it, it, it, it, it]
+        L.orc_ax_slab_rays.restype = None
+        L.orc_ax_slab_rays.argtypes = [f32p, f32p, f32p, lp, C.c_long, fl, fl, f32p, f32p, fl,
+                                       fl, fl, fl, it, it, it, it, it, it, it]
         L.orc_atx_voxels.restype = None
         L.orc_atx_voxels.argtypes = [f32p, f32p, f32p, lp, C.c_long, fl, fl, f32p, f32p, fl, fl,
                                      fl, fl, fl, fl, it, it, it, it, it]
@@ -192,6 +195,23 @@ def ax_rays_para(X, para, ids):
     lib().orc_ax_rays(X, y, sr, ids, ids.size, g["cos_phi"], g["sin_phi"], g["y_det"],
                       g["z_det"], g["SO"], g["OD"], g["scale"], np.float32(1.0), nx, ny, nz, na,
                       nb)
+    return y, sr
+
+
+def ax_slab_rays_para(X_slab, para, ids, z0, nzs):
+    """Partial forward-projector sums (y, sr) of the rays `ids` over the
+    z-slab [z0, z0 + nzs) of the volume para describes; X_slab holds that
+    slab's slices only (x fastest)."""
+    g = _geom_from_para(para)
+    nx, ny, nz, na, nb = _dims(para)
+    X = np.ascontiguousarray(np.asarray(X_slab, np.float32).reshape(-1, order="F"))
+    assert X.size == nx * ny * nzs
+    ids = np.ascontiguousarray(ids, np.int64)
+    y = np.zeros(ids.size, np.float32)
+    sr = np.zeros(ids.size, np.float32)
+    lib().orc_ax_slab_rays(X, y, sr, ids, ids.size, g["cos_phi"], g["sin_phi"], g["y_det"],
+                           g["z_det"], g["SO"], g["OD"], g["scale"], np.float32(1.0), nx, ny, nz,
+                           na, nb, int(z0), int(nzs))
     return y, sr
 
 

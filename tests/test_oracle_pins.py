@@ -501,3 +501,28 @@ def test_subset_entry_points_equal_full_ones(orc, dims):
     full, _ = orc.sart(Pv[:, None], dict(para, phi_vec=[63.0], m0=m, **{"lambda": 0.9}))
     part = orc.sart_view_voxels(m, Pv, para, 63.0, vox, relax=0.9)
     assert np.array_equal(part, full[vox])
+
+
+def test_slab_partial_sums(orc):
+    """A z-slab's partial forward sums: y equals the whole-volume projection
+    of the volume zeroed outside the slab bit for bit (the pieces outside add
+    0), and the partial y and sr of the slabs of a partition add up to the
+    whole volume's to FP32 rounding."""
+    nx, ny, nz, na, nb = 14, 12, 10, 26, 16
+    para = dict(nx=nx, ny=ny, nz=nz, na=na, nb=nb, SO=40.0, OD=30.0, vxlsize=1.0, detsize=1.3,
+                phi=-0.8)
+    rng = np.random.default_rng(41)
+    X = rng.random((nx, ny, nz)).astype(np.float32)
+    y, sr = orc.ax_para(X, para)
+    ids = np.arange(na * nb)
+    ys, srs = np.zeros(na * nb), np.zeros(na * nb)
+    for z0, nzs in [(0, 3), (3, 4), (7, 3)]:
+        yp, srp = orc.ax_slab_rays_para(X[:, :, z0:z0 + nzs], para, ids, z0, nzs)
+        Xz = np.zeros_like(X)
+        Xz[:, :, z0:z0 + nzs] = X[:, :, z0:z0 + nzs]
+        assert np.array_equal(yp, orc.ax_para(Xz, para)[0]), z0
+        assert srp.max() > 0 and np.all(srp <= sr + 1e-6)
+        ys += yp
+        srs += srp
+    assert np.allclose(ys, y, rtol=2e-6, atol=0) and np.allclose(srs, sr, rtol=2e-6, atol=0)
+    assert np.array_equal(orc.ax_slab_rays_para(X, para, ids, 0, nz)[0], y)
