@@ -282,7 +282,7 @@ int check_para(const asd_para *p, int nv)
     ASD_ARG(p->vxlsize > 0 && p->detsize > 0, "para.vxlsize and para.detsize must be positive");
     ASD_ARG(p->SO > 0 && p->OD > 0, "para.SO and para.OD must be positive");
     ASD_ARG(p->numOfgradients >= 0, "para.numOfgradients must be >= 0");
-    ASD_ARG(p->precision >= 0 && p->precision <= 4, "para.precision must be 0, 1, 2, 3 or 4");
+    ASD_ARG(p->precision >= 0 && p->precision <= 5, "para.precision must be 0, 1, 2, 3, 4 or 5");
     return ASD_OK;
 }
 
@@ -459,7 +459,8 @@ int autotune_atx(asd_plan *p)
     for (int i = 0; i < ATX_NVARIANT; i++) p->atx_us[i] = -1.0;
     switch (p->para.precision) {
     case 0:
-    case 4: return autotune_atx_t<false>(p);
+    case 4:
+    case 5: return autotune_atx_t<false>(p);
     case 3: p->atx_variant = ATX_DIRECT; return ASD_OK;  // own tiled FP64-geometry kernel
     default: return autotune_atx_t<true>(p);
     }
@@ -521,6 +522,7 @@ int enqueue_sweep(asd_plan *p, const double *relax)
     switch (p->para.precision) {
     case 1: return enqueue_sweep_t<SinoU16, true, false>(p, relax);
     case 2: return enqueue_sweep_t<SinoF16, true, false>(p, relax);
+    case 5: return enqueue_sweep_t<SinoF16, false, false>(p, relax);
     case 3: return enqueue_sweep_t<SinoU16, true, true>(p, relax);
     case 4: return enqueue_sweep_ref(p, relax);
     default: return enqueue_sweep_t<SinoF32, false, false>(p, relax);
@@ -858,7 +860,7 @@ int asd_plan_set_projections(asd_plan *p, const double *P)
         }
         ASD_HIP(hipMemcpy(p->P.p, q.data(), n * 2, hipMemcpyHostToDevice));
         drop_graph(p);  // the view scales are kernel arguments
-    } else if (p->para.precision == 2) {
+    } else if (p->para.precision == 2 || p->para.precision == 5) {
         std::vector<__half> h(n);
         for (size_t i = 0; i < n; i++) h[i] = __float2half_rn((float)P[i]);
         ASD_HIP(hipMemcpy(p->P.p, h.data(), n * 2, hipMemcpyHostToDevice));

@@ -524,6 +524,14 @@ __host__ __device__ __forceinline__ bool chord_points(const Cands<K> &c, float &
     return two;
 }
 
+// sqrtf of a squared chord: the same bits as sqrtf, without the rescaling
+// of tiny arguments where the argument is at least 2^-96 (sqrt_ge_tiny); a
+// shorter chord (two hits within 2^-48 voxel) takes sqrtf.
+__host__ __device__ __forceinline__ float chord_sqrt(float x)
+{
+    return x >= 0x1p-96f ? sqrt_ge_tiny(x) : sqrtf(x);
+}
+
 // General 2-D chord (find_l with a' != 0, b' != 0) in plane (u,w):
 // pu = u1_0 - uc, pw = w1_0 - wc, s_wu = b'/a', s_uw = a'/b', and the face
 // offsets ow = s_wu * hu, ou = s_uw * hw (per ray).
@@ -556,7 +564,7 @@ __host__ __device__ __forceinline__ float chord2(float pu, float pw, float s_wu,
     const bool two = u_first ? chord2_points<true>(cw, ow, cu, ou, hu, hw, ex, ey)
                              : chord2_points<false>(cw, ow, cu, ou, hu, hw, ex, ey);
     if (!two) return 0.0f;
-    return sqrtf(ex * ex + ey * ey);
+    return chord_sqrt(ex * ex + ey * ey);
 }
 
 // The chord of find_l_3d from its six face candidates, given in face order
@@ -658,7 +666,7 @@ __host__ __device__ __forceinline__ float chord3_general(const RayRec &R, bool x
     float ex, ey, ez;
     const bool two = chord3_points(x_first, E, hx, hy, hz, ex, ey, ez);
     if (!two) return 0.0f;
-    return sqrtf(ex * ex + ey * ey + ez * ez);
+    return chord_sqrt(ex * ex + ey * ey + ez * ez);
 }
 
 __host__ __device__ __forceinline__ float find_l_3d_rec(const RayRec &R, float px, float py, float pz,

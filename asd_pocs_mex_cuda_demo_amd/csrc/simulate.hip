@@ -310,7 +310,7 @@ int asd_plan_simulate(asd_plan *p, const double *ellipsoids, int n_ell, double s
                                                           mm.as<double>());
         ASD_LAUNCH_CHECK("encode_u16");
         plan_drop_graph(p);  // the view scales are kernel arguments
-    } else if (pa.precision == 2) {
+    } else if (pa.precision == 2 || pa.precision == 5) {
         encode_f16<<<blocks_for(n), 256, 0, p->stream>>>(stage.as<double>(), p->P.as<__half>(),
                                                           n);
         ASD_LAUNCH_CHECK("encode_f16");
@@ -339,7 +339,7 @@ int asd_plan_get_projections(asd_plan *p, double *P)
         std::vector<float> h(n);
         ASD_HIP(hipMemcpy(h.data(), p->P.p, n * 4, hipMemcpyDeviceToHost));
         for (size_t i = 0; i < n; i++) P[i] = h[i];
-    } else if (p->para.precision == 2) {
+    } else if (p->para.precision == 2 || p->para.precision == 5) {
         std::vector<__half> h(n);
         ASD_HIP(hipMemcpy(h.data(), p->P.p, n * 2, hipMemcpyDeviceToHost));
         for (size_t i = 0; i < n; i++) P[i] = (double)__half2float(h[i]);

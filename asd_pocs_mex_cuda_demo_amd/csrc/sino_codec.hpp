@@ -6,6 +6,7 @@
 //   precision 2  FP16 (11-bit significand; kept for comparison)
 //   precision 3  as 1 (the backprojector also sets up its rays in FP64)
 //   precision 4  FP64, as MATLAB holds P (the reference-arithmetic plan)
+//   precision 5  FP16, with plain FP32 sums (half the sinogram traffic of 0)
 //
 // Each codec reads one value of one view as FP64, the type the SART
 // correction (P - vxl*y) is formed in.

@@ -12,8 +12,9 @@ tests run, and their synthetic inputs.
   batch64     64 independent 2-D fan-beam cases (batch_cases)
   cone3d_512  3-D cone beam, 512^3 at 0.5 mm, 720 views, 1024 x 768 bins at
               0.6 mm, SID 600 / SDD 1100 mm, Shepp-Logan plus 200 seeded
-              spheres, FP16-stored projections, 20 iterations; simulated on
-              the device (phantom, projections)
+              spheres, FP16-stored projections with plain FP32 sums
+              (para.precision 5), 20 iterations; simulated on the device
+              (phantom, projections)
 
 Every workload runs ASD-POCS with alpha = 0.2, 20 TV sub-iterations and the
 driver's other defaults (beta = 1 decaying by 0.995, alpha_red = 0.95,
@@ -31,7 +32,7 @@ CONFIGS = {
     "cone3d_256": dict(n=256, nz=256, vxl=1.0, nv=180, na=384, nb=384, det=1.2, SO=500.0,
                        SDD=1000.0, I0=5e4, iters=30),
     "cone3d_512": dict(n=512, nz=512, vxl=0.5, nv=720, na=1024, nb=768, det=0.6, SO=600.0,
-                       SDD=1100.0, I0=0.0, iters=20, precision=2, sim="gpu", spheres=200),
+                       SDD=1100.0, I0=0.0, iters=20, precision=5, sim="gpu", spheres=200),
 }
 SEED = 20241203
 BATCH_SEED = 20241204

@@ -3,7 +3,7 @@
 Headline workload (default, the largest configuration): cone3d_512, 3-D
 cone-beam ASD-POCS on a 512^3 volume (0.5 mm voxels, Shepp-Logan plus 200
 seeded spheres), 720 views over 360 deg, 1024 x 768 bins at 0.6 mm, SID
-600 mm / SDD 1100 mm, FP16-stored projections with FP32 compensated sums,
+600 mm / SDD 1100 mm, FP16-stored projections with FP32 sums,
 simulated on the device; 20 iterations of ASD-POCS with 20 TV sub-iterations
 (alpha = 0.2).  A step is one outer iteration (a SART sweep over the 720
 views with positivity, the TV descent and the step adaptation) with the
@@ -37,7 +37,7 @@ METRIC = "ASD-POCS iters/sec and projector GB/s (% HBM roofline), batched-mixed 
 HBM_PEAK_GBS = 8000.0
 WORKLOAD_CHOICES = ["fan2d_256", "fan2d_512", "cone3d_256", "cone3d_512", "batch64"]
 
-CODEC = {0: "SinoF32", 1: "SinoU16", 2: "SinoF16", 3: "SinoU16", 4: "SinoF64"}
+CODEC = {0: "SinoF32", 1: "SinoU16", 2: "SinoF16", 3: "SinoU16", 4: "SinoF64", 5: "SinoF16"}
 ATX_VARIANTS = ["direct", "tile16x16x1", "tile8x8x4", "tile16x4x4", "pairs", "direct2",
                 "direct1", "sorted16", "sorted8x8x4", "sorted8x4x8"]
 
@@ -260,8 +260,7 @@ def headline(args):
     dt_ev = time.perf_counter() - t1
     stage = {s: plan.timing(s) for s in (0, 1, 2)}
     plan.set_timing(False)
-    nbytes = stage_bytes(N, M, p["numOfgradients"], 8 if precision == 4 else
-                         4 if precision == 0 else 2)
+    nbytes = stage_bytes(N, M, p["numOfgradients"], {0: 4, 4: 8}.get(precision, 2))
     tuning = plan.tuning()
     lpr, rows, _ = plan.info()
     names = {0: ax_kernel_name(lpr, rows, precision),
@@ -292,7 +291,8 @@ def headline(args):
         "dtype": {0: "f32", 1: "f32 (16-bit fixed-point sinogram, compensated sums)",
                   2: "f32 (fp16 sinogram, compensated sums)",
                   3: "f32 (16-bit fixed-point sinogram, compensated sums, f64 tile ray setup)",
-                  4: "f64 state, f32 projectors (the reference's arithmetic)"}[precision],
+                  4: "f64 state, f32 projectors (the reference's arithmetic)",
+                  5: "f32 (fp16 sinogram)"}[precision],
         "data": ("synthetic, simulated on the device: modified Shepp-Logan phantom"
                  + (f" + {w['spheres']} seeded spheres" if w.get("spheres") else "")
                  + (f", Poisson noise I0={w['I0']:g} (seeded), log transform" if w["I0"]

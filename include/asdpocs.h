@@ -149,7 +149,8 @@ typedef struct asd_para {
                              * 4: the reference's arithmetic: sinogram, image, m_pre,
                              *    m_k1, f0 and TV in FP64 as ASD_POCS_cuda.m holds
                              *    them, single(m) fed to a forward projector that sums
-                             *    each ray in slab order, the update in FP64 */
+                             *    each ray in slab order, the update in FP64;
+                             * 5: FP16 sinogram, plain FP32 sums */
 } asd_para;
 
 typedef struct asd_out {

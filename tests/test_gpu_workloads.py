@@ -24,8 +24,9 @@
         by FP32 rounding.
   cone3d_512  one 1024 x 768 view of the 512^3 phantom simulated on the device:
         forward projector on sampled rays, backprojector on sampled voxels, and
-        the SART update of the view for the FP16-sinogram plan (the workload's
-        own precision), the FP32 plan and precision 4, on three z-slices.
+        the SART update of the view for the FP16-sinogram plans (plain sums:
+        the workload's own precision 5; compensated: precision 2), the FP32
+        plan and precision 4, on three z-slices.
 """
 import numpy as np
 import pytest
@@ -75,7 +76,7 @@ def reconstruction_checks(orc, name, variants):
     for prec in variants:
         if prec in (1, 3):
             Pv = W.quantize_u16(P)  # the data the 16-bit codec holds
-        elif prec == 2:
+        elif prec in (2, 5):
             Pv = P.astype(np.float32).astype(np.float16).astype(np.float64)
         else:
             Pv = P
@@ -93,7 +94,7 @@ def test_fan2d_256_reconstruction(orc):
 
 
 def test_fan2d_512_reconstruction_and_precision_variants(orc):
-    reconstruction_checks(orc, "fan2d_512", [0, 1, 2, 3])
+    reconstruction_checks(orc, "fan2d_512", [0, 1, 2, 3, 5])
 
 
 def _one_view_update(plan, m0):
@@ -171,7 +172,7 @@ def test_cone3d_512_view(orc):
     ids = np.concatenate([np.arange(z * n, (z + 1) * n) for z in (100, 256, 400)])
     res = {}
     img = None
-    for prec in (2, 0, 4):
+    for prec in (5, 2, 0, 4):
         _, pl = W.sim_plan("cone3d_512", precision=prec, phi_vec=[th])
         if img is None:
             img = pl.reference()
