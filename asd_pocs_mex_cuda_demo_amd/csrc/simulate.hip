@@ -26,13 +26,20 @@ namespace asd {
 namespace {
 
 struct Ellipsoid {
-    double A, a, b, c, x0, y0, z0, cos_t, sin_t;
+    double A, a, b, c, x0, y0, z0;
+    double cos_t, sin_t;     // 2-D: rotation by phi about the ellipse's centre
+    double m[9];             // 3-D: phantom3d.m's Euler matrix of (phi, theta, psi)
+    double asq, bsq, csq;    // a^2, b^2, c^2
 };
 
 // One voxel per thread: the sum of A over the ellipsoids holding the voxel
-// centre, in table order, times scale, rounded to FP32.  Coordinates as in
-// phantoms.py: x_i = (i - (n-1)/2) / ((n-1)/2) in-plane, z scaled by the x
-// half-width; with one slice the ellipses are tested in-plane only.
+// centre, in table order, times scale, rounded to FP32, with the FP64
+// expressions of the CPU rasteriser (phantoms.py) in the same order.
+//  - 2-D (nz == 1): x = r(ix), y = r(iy) with r(i) = (i - (n-1)/2) / ((n-1)/2),
+//    each ellipse rotated by phi about its centre.
+//  - 3-D: phantom3d.m:76-108 - the point (x, y, z) = (r(iy), r(ix), r(iz +
+//    (nx - nz) floor/ 2)) (MATLAB's meshgrid puts x along the second index)
+//    rotated by the Euler matrix, then (p - offset)^2 / axis^2 summed.
 __global__ __launch_bounds__(256) void phantom_kernel(const Ellipsoid *__restrict__ E, int ne,
                                                       int nx, int ny, int nz, double scale,
                                                       float *__restrict__ out)
@@ -42,22 +49,31 @@ __global__ __launch_bounds__(256) void phantom_kernel(const Ellipsoid *__restric
     const size_t N = (size_t)nx * ny * nz;
     if (i >= N) return;
     const int ix = (int)(i % nx), iy = (int)((i / nx) % ny), iz = (int)(i / ((size_t)nx * ny));
-    const double hx = (nx - 1) / 2.0, hy = (ny - 1) / 2.0, hz = (nz - 1) / 2.0;
-    const double X = ((double)ix - hx) / hx, Y = ((double)iy - hy) / hy;
-    const double Z = ((double)iz - hz) / hx;
+    const double hx = (nx - 1) / 2.0, hy = (ny - 1) / 2.0;
     double v = 0.0;
-    for (int k = 0; k < ne; k++) {
-        const Ellipsoid e = E[k];
-        const double dx = X - e.x0, dy = Y - e.y0;
-        const double xr = dx * e.cos_t + dy * e.sin_t;
-        const double yr = (-dx) * e.sin_t + dy * e.cos_t;
-        const double tx = xr / e.a, ty = yr / e.b;
-        double q = tx * tx + ty * ty;
-        if (nz > 1) {
-            const double tz = (Z - e.z0) / e.c;
-            q = q + tz * tz;
+    if (nz == 1) {
+        const double X = ((double)ix - hx) / hx, Y = ((double)iy - hy) / hy;
+        for (int k = 0; k < ne; k++) {
+            const Ellipsoid e = E[k];
+            const double dx = X - e.x0, dy = Y - e.y0;
+            const double xr = dx * e.cos_t + dy * e.sin_t;
+            const double yr = (-dx) * e.sin_t + dy * e.cos_t;
+            const double tx = xr / e.a, ty = yr / e.b;
+            if (tx * tx + ty * ty <= 1.0) v += e.A;
         }
-        if (q <= 1.0) v += e.A;
+    } else {
+        const int off = nx >= nz ? (nx - nz) / 2 : -((nz - nx + 1) / 2);
+        const double x = ((double)iy - hy) / hy, y = ((double)ix - hx) / hx;
+        const double z = ((double)(iz + off) - hx) / hx;
+        for (int k = 0; k < ne; k++) {
+            const Ellipsoid &e = E[k];
+            const double p1 = e.m[0] * x + e.m[1] * y + e.m[2] * z;
+            const double p2 = e.m[3] * x + e.m[4] * y + e.m[5] * z;
+            const double p3 = e.m[6] * x + e.m[7] * y + e.m[8] * z;
+            const double d1 = p1 - e.x0, d2 = p2 - e.y0, d3 = p3 - e.z0;
+            const double q = (d1 * d1) / e.asq + (d2 * d2) / e.bsq + (d3 * d3) / e.csq;
+            if (q <= 1.0) v += e.A;
+        }
     }
     out[i] = (float)(v * scale);
 }
@@ -242,13 +258,29 @@ int asd_plan_simulate(asd_plan *p, const double *ellipsoids, int n_ell, double s
     const asd_para &pa = p->para;
     ASD_ARG(pa.nx > 1 && pa.ny > 1, "the phantom grid needs nx, ny >= 2");
     int rc;
-    // ellipsoid table: A, a, b, c, x0, y0, z0, phi (degrees) per row; the
-    // rotation's cos / sin are formed here (phantoms.py: np.deg2rad, np.cos)
+    // ellipsoid table: A, a, b, c, x0, y0, z0, phi, theta, psi (degrees) per
+    // row (phantom3d.m's layout); the rotations are formed here with the
+    // expressions of phantoms.py (2-D: phi * (pi / 180); 3-D: the Euler
+    // matrix of phantom3d.m:94-101, each angle as deg * pi / 180)
     std::vector<Ellipsoid> E(n_ell);
     for (int k = 0; k < n_ell; k++) {
-        const double *r = ellipsoids + 8 * k;
+        const double *r = ellipsoids + 10 * k;
+        Ellipsoid &e = E[k];
+        e.A = r[0]; e.a = r[1]; e.b = r[2]; e.c = r[3];
+        e.x0 = r[4]; e.y0 = r[5]; e.z0 = r[6];
         const double t = r[7] * (M_PI / 180.0);
-        E[k] = Ellipsoid{r[0], r[1], r[2], r[3], r[4], r[5], r[6], std::cos(t), std::sin(t)};
+        e.cos_t = std::cos(t);
+        e.sin_t = std::sin(t);
+        const double ph = r[7] * M_PI / 180, th = r[8] * M_PI / 180, ps = r[9] * M_PI / 180;
+        const double cph = std::cos(ph), sph = std::sin(ph), cth = std::cos(th),
+                     sth = std::sin(th), cps = std::cos(ps), sps = std::sin(ps);
+        const double m[9] = {cps * cph - cth * sph * sps, cps * sph + cth * cph * sps, sps * sth,
+                             -sps * cph - cth * sph * cps, -sps * sph + cth * cph * cps,
+                             cps * sth, sth * sph, -sth * cph, cth};
+        for (int j = 0; j < 9; j++) e.m[j] = m[j];
+        e.asq = r[1] * r[1];
+        e.bsq = r[2] * r[2];
+        e.csq = r[3] * r[3];
     }
     DeviceBuf dE, stage, mm, srb, img32;
     // the phantom in FP32 (the reference-arithmetic plan keeps its RMSE

@@ -250,9 +250,10 @@ class Plan:
         nat.check(self._L.asd_plan_iterate(self.h, int(n)))
 
     def simulate(self, table, scale=0.02, I0=0.0, seed=0):
-        """Phantom (rows A, a, b, c, x0, y0, z0, phi_deg), its projections
+        """Phantom (rows A, a, b, c, x0, y0, z0, phi, theta, psi in degrees),
+        its projections
         and optional Poisson noise, made on the device (asd_plan_simulate)."""
-        t = np.ascontiguousarray(np.asarray(table, np.float64).reshape(-1, 8))
+        t = np.ascontiguousarray(np.asarray(table, np.float64).reshape(-1, 10))
         nat.check(self._L.asd_plan_simulate(self.h, _ptr(t), t.shape[0], float(scale),
                                             float(I0), int(seed)))
 

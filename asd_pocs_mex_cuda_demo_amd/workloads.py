@@ -84,7 +84,7 @@ def sphere_table(k, seed, width_mm):
     r = rng.uniform(1.0, 4.0, k) / (width_mm / 2)
     mu = rng.uniform(0.01, 0.03, k) / 0.02
     c = rng.uniform(-0.6, 0.6, (k, 3)) * np.array([0.69, 0.92, 0.81])
-    return np.column_stack([mu, r, r, r, c, np.zeros(k)])
+    return np.column_stack([mu, r, r, r, c, np.zeros((k, 3))])
 
 
 def phantom_table(name, seed=SEED):

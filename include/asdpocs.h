@@ -195,10 +195,14 @@ int asd_plan_get_current(asd_plan *plan, double *m_host);
 int asd_plan_sync(asd_plan *plan);
 /* Fills the plan with simulated data on the device.  No counterpart in the
  * reference, whose demos build their inputs in MATLAB (demo_test.m:35-52).
- *  - The ellipsoid phantom (n_ell rows of A, a, b, c, x0, y0, z0, phi in
- *    degrees; coordinates in [-1, 1] across the grid, rotation about z;
- *    values summed in row order, times scale) becomes the plan's RMSE
- *    reference; a later asd_plan_reset with I0 == NULL keeps it.
+ *  - The ellipsoid phantom (n_ell rows of A, a, b, c, x0, y0, z0, phi,
+ *    theta, psi in degrees, the layout of utilities/phantom3d.m:183-200;
+ *    coordinates in [-1, 1] across the grid; values summed in row order,
+ *    times scale) becomes the plan's RMSE reference; a later asd_plan_reset
+ *    with I0 == NULL keeps it.  3-D volumes follow phantom3d.m:76-108 (x
+ *    along the second index, the Euler rotation applied before the
+ *    offsets); a one-slice image rotates each ellipse by phi about its
+ *    centre, x along the first index.
  *  - Every view is forward projected.  With I0_photons > 0 each projection
  *    value p becomes -ln(max(k, 1) / I0_photons), k ~ Poisson(I0_photons *
  *    exp(-p)), from a Philox4x32-10 stream per value keyed by seed.

@@ -341,51 +341,85 @@ def test_stop_rule_ends_the_loop():
     np.testing.assert_allclose(m.ravel(order="F"), sweep, rtol=0, atol=0)
 
 
-def test_precision_variant_matches_plain_rmse():
-    """16-bit fixed-point sinogram and compensated sums (precision 1): the
-    reconstruction RMSE stays within 1e-4 relative of the plain FP32 form on
-    noisy data.  r_max is raised so the
-    alpha rule (a discrete decision that any perturbation can flip) never
-    fires.  The TV descent with the reference's eps = 1e-8 amplifies any
-    perturbation with every sub-iteration (a 1e-9 relative change of the data
-    reaches ~4e-4 in RMSE after 5 iterations at ng = 10, pinned on the
-    oracle), so the 1e-4 bar is checked where the problem is conditioned:
-    5 iterations at ng = 5, and the first 2 iterations at ng = 10.  FP16
-    storage (precision 2, 11-bit significand) is looser and kept for
-    comparison."""
+def _decoded(P, prec):
+    """The sinogram a plan of precision `prec` holds, decoded to FP64."""
+    from asd_pocs_mex_cuda_demo_amd import workloads as W
+    if prec in (1, 3):
+        return W.quantize_u16(P)
+    if prec in (2, 5):
+        return P.astype(np.float32).astype(np.float16).astype(np.float64)
+    return P
+
+
+def test_precision_variants_against_oracle(orc):
+    """Every storage / arithmetic variant against the oracle run on the data
+    the variant holds (16-bit fixed point or FP16 decoded), 5 iterations of
+    ng = 5 on noisy 64^2 data: the first iteration (no TV step yet) within
+    1e-4, every iteration within max(1e-4, 2 * floor), where floor is how far
+    the oracle moves itself when only the summation order of its FP64 norms
+    changes (the TV descent amplifies any rounding difference; see the
+    per-workload tests).  r_max is raised so the alpha rule, a discrete
+    decision, does not fire.  Precision 3 changes the backprojector's lengths
+    on purpose and is held to 1e-3 after the first iteration."""
     para, img, P = _small_problem(n=64, nv=24, na=96)
     Pn = phantoms.poisson_log(P, 1e5, seed=7)
     para.update(maxnumIter=5, alpha=0.2, numOfgradients=5, I0=img, r_max=1e9)
-    m0, o0 = ops.ASD_POCS_cuda(Pn, para)
-    m1, o1 = ops.ASD_POCS_cuda(Pn, dict(para, precision=1))
-    assert o0["alpha"] == o1["alpha"] == 0.2
-    assert np.all(np.abs(o1["RMSE"] - o0["RMSE"]) <= 1e-4 * o0["RMSE"]), (o0["RMSE"], o1["RMSE"])
-    # Pixel by pixel the images differ by what the TV descent makes of any
-    # change of the data.  The 16-bit codec changes the data by up to half a
-    # step (~1e-5 of a view's range), so the image is compared with the plain
-    # form run on the same quantised data, against the spread a 1e-7
-    # perturbation of the data gives the plain form.
-    Pq = np.empty_like(Pn)
-    for v in range(Pn.shape[1]):
-        lo, hi = Pn[:, v].min(), Pn[:, v].max()
-        step = (hi - lo) / 65535.0
-        Pq[:, v] = lo + step * np.clip(np.floor((Pn[:, v] - lo) / step + 0.5), 0, 65535)
-    m0q, _ = ops.ASD_POCS_cuda(Pq, para)
-    mp, _ = ops.ASD_POCS_cuda(Pq * (1 + 1e-7 * np.random.default_rng(3).standard_normal(Pq.shape)),
-                              para)
-    floor = rel_l2(mp, m0q)
-    assert rel_l2(m1, m0q) <= max(3 * floor, 1e-4), (rel_l2(m1, m0q), floor)
-    m2, o2 = ops.ASD_POCS_cuda(Pn, dict(para, precision=2))
-    assert np.all(np.abs(o2["RMSE"] - o0["RMSE"]) <= 1e-2 * o0["RMSE"])
-    # precision 3 computes the backprojector's lengths to ~1e-6 voxel where
-    # the reference's FP32 arithmetic is off by ~1e-4: a deliberate,
-    # systematic difference (about 4e-4 in RMSE after 5 iterations here)
-    m3, o3 = ops.ASD_POCS_cuda(Pn, dict(para, precision=3))
-    assert np.all(np.abs(o3["RMSE"] - o0["RMSE"]) <= 2e-3 * o0["RMSE"])
-    q = dict(para, maxnumIter=2, numOfgradients=10)
-    _, a = ops.ASD_POCS_cuda(Pn, q)
-    _, b = ops.ASD_POCS_cuda(Pn, dict(q, precision=1))
-    assert np.all(np.abs(b["RMSE"] - a["RMSE"]) <= 1e-4 * a["RMSE"]), (a["RMSE"], b["RMSE"])
+    _, oo = orc.asd_pocs(Pn, para)
+    orc.set_sum_order(1)
+    try:
+        _, op = orc.asd_pocs(Pn, para)
+    finally:
+        orc.set_sum_order(0)
+    floor = float(np.max(np.abs(op["RMSE"] - oo["RMSE"]) / oo["RMSE"]))
+    bar = max(1e-4, 2 * floor)
+    for prec in (0, 1, 2, 3, 5):
+        _, ov = orc.asd_pocs(_decoded(Pn, prec), para)
+        _, o = ops.ASD_POCS_cuda(Pn, dict(para, precision=prec))
+        g = np.abs(o["RMSE"] - ov["RMSE"]) / ov["RMSE"]
+        assert o["alpha"] == 0.2 and ov["alpha"] == 0.2
+        assert g[0] <= 1e-4, (prec, g)
+        # precision 3 departs from the reference's lengths by design (FP64 ray
+        # setup: ~1e-6 voxel of exact, where the reference's FP32 lengths carry
+        # ~1e-4 of cancellation); the TV descent amplifies that systematic
+        # difference (5.3e-4 after 5 iterations here), so it is held to 1e-3
+        assert g.max() <= (1e-3 if prec == 3 else bar), (prec, g, bar)
+
+
+def test_precision_variants_against_exact_geometry(orc):
+    """One SART view update of each variant against the same update with the
+    exact FP64 system matrix (oracle/exact64: every ray-pixel length clipped
+    in FP64, the rays' end points as the operators form them in FP32) on a
+    512^2-like fan: the reference's FP32 lengths (precision 0, 4, and the
+    16-bit / FP16 storage variants) carry ~1e-4 voxel of cancellation, the
+    FP64 tile setup of precision 3 removes most of it."""
+    from oracle import exact64 as E
+    n, na = 64, 96
+    para = dict(nx=n, ny=n, nz=1, SO=541.0, OD=408.0, vxlsize=0.5, detsize=0.8, na=na, nb=1)
+    th = 37.0
+    q = dict(para, phi=-th / 180 * np.pi)
+    g = orc.gateway_geometry(q["phi"], para["SO"], para["OD"], 0.5, 0.8, na, 1)
+    A = E.system_matrix_2d(g["cos_phi"], g["sin_phi"], float(g["SO"]), float(g["OD"]),
+                           g["y_det"], n, n, f32_points=True)
+    img = phantoms.shepp_logan_2d(n).ravel(order="F")
+    m0 = (0.9 * img).astype(np.float32).astype(np.float64)  # held exactly by every plan
+    Pv = 0.5 * (A @ img)  # exact line integrals (vxl = 0.5 mm)
+    errs = {}
+    for prec in (0, 1, 2, 3, 4, 5):
+        Pd = _decoded(Pv[:, None], prec)[:, 0]
+        proj, sr = 0.5 * (A @ m0), 0.5 * A.sum(axis=1)
+        cor = (Pd - proj) / np.where(sr == 0, 1, sr)
+        bp, sc = 0.5 * (A.T @ cor), 0.5 * A.sum(axis=0)
+        upd = bp / np.where(sc == 0, 1, sc)
+        pl = ops.Plan(dict(para, precision=prec), [th])
+        pl.set_projections(Pv[:, None])
+        pl.reset(m0, None)
+        pl.sweep(1.0)
+        got = pl.current() - m0
+        pl.close()
+        errs[prec] = rel_l2(got, upd)
+    print("one-view update, rel L2 vs exact geometry:", errs)
+    assert all(e <= 1e-3 for e in errs.values()), errs
+    assert errs[3] < errs[0], errs
 
 
 def test_batch_matches_single_plans():

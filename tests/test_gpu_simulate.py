@@ -55,7 +55,7 @@ def _counts(lam, seed, nv=512, na=1024):
     """Deviates of Poisson(lam) from an empty phantom (every p = 0), clipped
     below at 1 as the log transform does."""
     _, _, pl = plan(nv=nv, na=na)
-    pl.simulate(np.zeros((0, 8)), I0=lam, seed=seed)
+    pl.simulate(np.zeros((0, 10)), I0=lam, seed=seed)
     v = pl.projections().ravel()
     pl.close()
     return np.rint(lam * np.exp(-v))
