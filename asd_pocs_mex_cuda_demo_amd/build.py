@@ -53,11 +53,21 @@ def _run(cmd):
     return r
 
 
-def build(verbose=False, jobs=None):
+def build(verbose=False, jobs=None, gateways_only=False):
+    """Compile libasdpocs.so and the MEX gateways (only the gateways, against
+    the library already built, with gateways_only)."""
     os.makedirs(BUILD, exist_ok=True)
     os.makedirs(MEXDIR, exist_ok=True)
     jobs = jobs or min(8, os.cpu_count() or 4)
+    if not gateways_only:
+        build_library(jobs)
+    build_gateways(jobs)
+    if verbose:
+        print("built", LIB, "and the MEX gateways")
+    return LIB
 
+
+def build_library(jobs):
     def obj(src):
         o = os.path.join(BUILD, src.replace(".hip", ".o"))
         r = _run([HIPCC, *HIP_FLAGS, "-Rpass-analysis=kernel-resource-usage", "-c",
@@ -80,12 +90,21 @@ def build(verbose=False, jobs=None):
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, LIB)
 
+
+def build_gateways(jobs):
+
     # MEX gateways: host C++ against our own mex_api.h, linked to libasdpocs.so
     # with an rpath to this directory; mx*/mex* stay undefined until load.
+    # ASDPOCS_SANITIZE=1: the gateways (host C++) with AddressSanitizer and
+    # UndefinedBehaviorSanitizer
+    san = (["-g", "-fno-omit-frame-pointer", "-fsanitize=address,undefined",
+            "-fno-sanitize-recover=undefined"]
+           if os.environ.get("ASDPOCS_SANITIZE") == "1" else [])
+
     def mex(src):
         name = src.replace(".cpp", ".mexa64")
         out = os.path.join(MEXDIR, name)
-        _run(["g++", "-O2", "-fPIC", "-shared", "-std=c++17", "-Wall",
+        _run(["g++", "-O2", "-fPIC", "-shared", "-std=c++17", "-Wall", *san,
               f"-I{INC}", os.path.join(CSRC, "mex", src), "-o", out + ".tmp",
               f"-L{PKG}", "-lasdpocs", "-Wl,-rpath,$ORIGIN/.."])
         os.replace(out + ".tmp", out)
@@ -94,9 +113,6 @@ def build(verbose=False, jobs=None):
     present = [s for s in MEX_SOURCES if os.path.exists(os.path.join(CSRC, "mex", s))]
     with ThreadPoolExecutor(jobs) as ex:
         list(ex.map(mex, present))
-    if verbose:
-        print("built", LIB, "and", len(present), "MEX gateways")
-    return LIB
 
 
 def clean():

@@ -38,9 +38,11 @@ class OrcOut(C.Structure):
 
 
 def build():
-    """Compile liboracle.so with the Makefile beside this file."""
+    """Compile liboracle.so with the Makefile beside this file (with the
+    sanitizers when ASDPOCS_SANITIZE=1)."""
     import subprocess
-    subprocess.run(["make", "-B", "-s", "-C", _HERE], check=True)
+    extra = ["SANITIZE=1"] if os.environ.get("ASDPOCS_SANITIZE") == "1" else []
+    subprocess.run(["make", "-B", "-s", "-C", _HERE] + extra, check=True)
 
 
 def lib():

@@ -21,9 +21,14 @@ def stale(out):
 def build():
     out = os.path.join(HERE, "libemul.so")
     pkg = os.path.join(ROOT, "asd_pocs_mex_cuda_demo_amd")
+    # ASDPOCS_SANITIZE=1: the host code (the emulated device functions) with
+    # AddressSanitizer and UndefinedBehaviorSanitizer; device code untouched
+    san = ["-g", "-fno-omit-frame-pointer", "-Xarch_host", "-fsanitize=address",
+           "-Xarch_host", "-fsanitize=undefined", "-Xarch_host",
+           "-fno-sanitize-recover=undefined"] if os.environ.get("ASDPOCS_SANITIZE") == "1" else []
     subprocess.run(["/opt/rocm/bin/hipcc", "--offload-arch=gfx950", "-O2", "-fPIC", "-shared",
                     "-std=c++17", "-ffp-contract=off", "-fhip-fp32-correctly-rounded-divide-sqrt",
-                    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+                    *san, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
                     f"-I{os.path.join(ROOT, 'include')}", f"-I{os.path.join(pkg, 'csrc')}",
                     os.path.join(HERE, "emul.hip"), "-o", out + ".tmp"],
                    check=True, capture_output=True)
