@@ -212,6 +212,22 @@ def test_tv3d_matches_oracle(orc):
     assert rel_l2(out, orc.tv3d(I, 0.3, 4, 0, I0)) <= 1e-12
 
 
+def test_tv_drop_ins_are_bit_identical(orc):
+    """The drop-in descents sum d^2 and every ||v||^2 in index order and step
+    as u - ((alpha*d)*v)/||v|| (imgTVGradMin.cpp:58-59, :96, :104-107), so
+    they reproduce the oracle exactly, odd sizes and a stopped descent
+    (nTVIter 0) included."""
+    rng = np.random.default_rng(11)
+    for shape, fn, ref in [((37, 29), ops.imgTVGradMin, orc.tv2d),
+                           ((17, 13, 11), ops.imgTVGradMin3D, orc.tv3d)]:
+        I = rng.random(shape)
+        I0 = rng.random(shape) * 0.1
+        for alpha, nit, pos in [(0.2, 20, 1), (0.05, 3, 0), (0.1, 0, 1)]:
+            out = fn(I, alpha, nit, pos, I0)
+            exp = ref(I, alpha, nit, pos, I0)
+            assert np.array_equal(out, exp), (shape, alpha, nit, np.abs(out - exp).max())
+
+
 def _small_problem(n=64, nv=24, na=96, nz=1, nb=1):
     para = dict(nx=n, ny=n, nz=nz, SO=500.0, OD=400.0, vxlsize=1.0, detsize=1.2, na=na, nb=nb)
     para["phi_vec"] = np.arange(1, nv + 1) * 360.0 / nv
