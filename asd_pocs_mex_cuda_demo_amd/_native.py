@@ -28,7 +28,9 @@ EXPORTS = [
     "asd_plan_info", "asd_plan_destroy", "asd_plan_simulate", "asd_plan_get_projections",
     "asd_plan_get_reference",
     "asd_batch_create", "asd_batch_set_case", "asd_batch_iterate", "asd_batch_get_image",
-    "asd_batch_get_out", "asd_batch_set_graph", "asd_batch_sync", "asd_batch_destroy",
+    "asd_batch_get_out", "asd_batch_set_graph", "asd_batch_tuning", "asd_batch_set_atx_chunk",
+    "asd_batch_sync",
+    "asd_batch_destroy",
 ]
 
 
@@ -131,7 +133,9 @@ def lib():
     for name, args in [("asd_batch_set_case", [vp, i, vp, vp, vp]), ("asd_batch_iterate", [vp, i]),
                        ("asd_batch_get_image", [vp, i, vp]),
                        ("asd_batch_get_out", [vp, i, C.POINTER(AsdOut), vp]),
-                       ("asd_batch_set_graph", [vp, i]), ("asd_batch_sync", [vp])]:
+                       ("asd_batch_set_graph", [vp, i]), ("asd_batch_sync", [vp]),
+                       ("asd_batch_tuning", [vp, C.POINTER(i), vp]),
+                       ("asd_batch_set_atx_chunk", [vp, i])]:
         fn = getattr(L, name)
         fn.restype = i
         fn.argtypes = args

@@ -160,9 +160,13 @@ __global__ __launch_bounds__(256) void ax_batch_kernel(const BItem *__restrict__
     }
 }
 
-template <bool KAHAN>
+// CHUNK: footprint pairs per load batch (the batch's autotuner picks it; every
+// choice gives the same bits).  dry: compute everything but store nothing
+// (the autotuner's timing runs).
+template <bool KAHAN, int CHUNK>
 __global__ __launch_bounds__(256) void atx_batch_kernel(const BItem *__restrict__ items,
-                                                        const int *__restrict__ bpref, int ncase)
+                                                        const int *__restrict__ bpref, int ncase,
+                                                        int dry)
 {
 #pragma clang fp contract(off)
     const int b = blockIdx.x;
@@ -182,11 +186,24 @@ __global__ __launch_bounds__(256) void atx_batch_kernel(const BItem *__restrict_
     const float sx = g.cos_phi * (-g.SO), sy = g.sin_phi * (-g.SO);
     const float px = sx - f.xc, py = sy - f.yc, pz = 0.0f - f.zc;
     Acc<KAHAN> acc, accl;
-    atx_voxel_walk<KAHAN, 2>(g, it.cor, it.rt, f, px, py, pz, g.dz / 2.0f, acc, accl);
+    atx_voxel_walk<KAHAN, CHUNK>(g, it.cor, it.rt, f, px, py, pz, g.dz / 2.0f, acc, accl);
     float bp = acc.value() * g.scale, sc = accl.value() * g.scale;
     double cc = (sc == 0.0f) ? 1.0 : (double)sc;
     double nm = (double)m0 + it.ctl->beta * ((double)bp / cc);
-    it.m[v] = nm < 0 ? 0.0f : (float)nm;
+    if (!dry) it.m[v] = nm < 0 ? 0.0f : (float)nm;
+}
+
+// The backprojector stage with `chunk` pairs per load batch (1, 2 or 4).
+template <bool KAHAN>
+void launch_atx_batch(int chunk, unsigned blocks, hipStream_t s, const BItem *items,
+                      const int *bp, int A, int dry)
+{
+    if (chunk == 1)
+        klaunch(atx_batch_kernel<KAHAN, 1>, blocks, 256, 0, s, items, bp, A, dry);
+    else if (chunk == 4)
+        klaunch(atx_batch_kernel<KAHAN, 4>, blocks, 256, 0, s, items, bp, A, dry);
+    else
+        klaunch(atx_batch_kernel<KAHAN, 2>, blocks, 256, 0, s, items, bp, A, dry);
 }
 
 // Per-case folds: block c of a <<<ncase>>> launch sums rows of the partials
@@ -420,6 +437,8 @@ struct asd_batch {
     std::vector<size_t> voff, roff, poff;
     size_t Ntot = 0, Mtot = 0, Ptot = 0;
     int max_nv = 0, max_ng = 0, lpr = 64;
+    int atx_chunk = 0;                    // backprojector pairs per load batch (0: not tuned)
+    double atx_us[3] = {-1.0, -1.0, -1.0};  // tuning times of chunks 1, 2, 4 (µs)
     std::vector<int> active;   // active cases per view step (prefix of `order`)
     hipStream_t stream = nullptr;
     DeviceBuf m, mpre, mk1, f0, I0, v, u2, v2, cor, res2, rt, P, ydet, zdet, ctlb, hist;
@@ -457,7 +476,7 @@ int b_enqueue_iteration(asd_batch *B, int sweep_only, int iterate)
             default: ax_batch_kernel<1, true, SinoU16><<<ab, 256, 0, B->stream>>>(items, rp, A); break;
             }
             ASD_LAUNCH_CHECK("ax_batch_kernel");
-            atx_batch_kernel<true><<<blocks, 256, 0, B->stream>>>(items, bp, A);
+            launch_atx_batch<true>(B->atx_chunk, blocks, B->stream, items, bp, A, 0);
         } else {
             switch (B->lpr) {
             case 64: ax_batch_kernel<64, false, SinoF32><<<ab, 256, 0, B->stream>>>(items, rp, A); break;
@@ -466,7 +485,7 @@ int b_enqueue_iteration(asd_batch *B, int sweep_only, int iterate)
             default: ax_batch_kernel<1, false, SinoF32><<<ab, 256, 0, B->stream>>>(items, rp, A); break;
             }
             ASD_LAUNCH_CHECK("ax_batch_kernel");
-            atx_batch_kernel<false><<<blocks, 256, 0, B->stream>>>(items, bp, A);
+            launch_atx_batch<false>(B->atx_chunk, blocks, B->stream, items, bp, A, 0);
         }
         ASD_LAUNCH_CHECK("atx_batch_kernel");
     }
@@ -491,6 +510,62 @@ int b_enqueue_iteration(asd_batch *B, int sweep_only, int iterate)
     ASD_LAUNCH_CHECK("post_tv_batch");
     ctl_tv_batch<<<K, 256, 0, B->stream>>>(cs, cbp, K, ctl, part);
     ASD_LAUNCH_CHECK("ctl_tv_batch");
+    return ASD_OK;
+}
+
+// Picks the backprojector's pairs per load batch: each choice runs the first
+// view step (every case active) dry, timed by the dispatch itself, best of
+// three after a warm-up; a choice must beat the default (2) by more than 3%,
+// so near-ties do not flip between runs.  All choices give the same bits.
+int b_autotune(asd_batch *B)
+{
+    const int A = B->active[0];
+    const BItem *items = B->items.as<BItem>();
+    const int *bp = B->bpref.as<int>();
+    const unsigned blocks = (unsigned)B->h_bpref_tot[0];
+    ASD_HIP(hipMemsetAsync(B->cor.p, 0, B->Mtot * 4, B->stream));
+    hipEvent_t e0, e1;
+    ASD_HIP(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        set_error(ASD_ERR_HIP, "hipEventCreate failed");
+        return ASD_ERR_HIP;
+    }
+    const int chunks[3] = {2, 1, 4};
+    int best = 2;
+    double best_us = 1e30;
+    int rc = ASD_OK;
+    for (int chunk : chunks) {
+        float ms_min = 1e30f;
+        for (int rep = 0; rep < 4 && rc == ASD_OK; rep++) {
+            LaunchTiming lt{e0, e1, 0};
+            t_launch_timing = &lt;
+            if (B->precision == 1)
+                launch_atx_batch<true>(chunk, blocks, B->stream, items, bp, A, 1);
+            else
+                launch_atx_batch<false>(chunk, blocks, B->stream, items, bp, A, 1);
+            t_launch_timing = nullptr;
+            if (hipGetLastError() != hipSuccess) {
+                set_error(ASD_ERR_HIP, "atx_batch_kernel launch failed");
+                rc = ASD_ERR_HIP;
+                break;
+            }
+            (void)hipEventSynchronize(e1);
+            float ms = 0;
+            if (rep > 0 && hipEventElapsedTime(&ms, e0, e1) == hipSuccess && ms < ms_min)
+                ms_min = ms;
+        }
+        if (rc) break;
+        B->atx_us[chunk == 4 ? 2 : chunk - 1] = ms_min * 1e3;
+        if (ms_min * 1e3 < best_us * 0.97) {
+            best_us = ms_min * 1e3;
+            best = chunk;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    B->atx_chunk = best;
     return ASD_OK;
 }
 
@@ -824,6 +899,7 @@ int asd_batch_iterate(asd_batch *B, int n_iter)
             return ASD_ERR_STATE;
         }
     int rc;
+    if (!B->atx_chunk && (rc = b_autotune(B))) return rc;
     if (B->use_graph) {
         if (!B->exec) {
             ASD_HIP(hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal));
@@ -862,6 +938,26 @@ int asd_batch_set_graph(asd_batch *B, int enable)
     ASD_ARG(B, "NULL batch");
     B->use_graph = enable != 0;
     if (!B->use_graph) b_drop_graph(B);
+    return ASD_OK;
+}
+
+int asd_batch_set_atx_chunk(asd_batch *B, int chunk)
+{
+    clear_error();
+    ASD_ARG(B, "NULL batch");
+    ASD_ARG(chunk == 0 || chunk == 1 || chunk == 2 || chunk == 4, "chunk must be 0, 1, 2 or 4");
+    b_drop_graph(B);
+    B->atx_chunk = chunk;
+    return ASD_OK;
+}
+
+int asd_batch_tuning(asd_batch *B, int *chunk, double *us3)
+{
+    clear_error();
+    ASD_ARG(B, "NULL batch");
+    if (chunk) *chunk = B->atx_chunk;
+    if (us3)
+        for (int k = 0; k < 3; k++) us3[k] = B->atx_us[k];
     return ASD_OK;
 }
 

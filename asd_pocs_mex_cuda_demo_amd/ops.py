@@ -382,6 +382,20 @@ class Batch:
     def set_graph(self, on):
         nat.check(self._L.asd_batch_set_graph(self.h, int(on)))
 
+    def tuning(self):
+        """{"atx_chunk": footprint pairs per load batch of the backprojector
+        stage (None before the first iterate), "atx_us": {1: µs, 2: µs, 4: µs}}
+        from the batch's autotuner."""
+        ch = C.c_int()
+        us = np.zeros(3)
+        nat.check(self._L.asd_batch_tuning(self.h, C.byref(ch), _ptr(us)))
+        return {"atx_chunk": ch.value or None,
+                "atx_us": {k: float(t) for k, t in zip((1, 2, 4), us) if t >= 0}}
+
+    def set_atx_chunk(self, chunk):
+        """Forces the backprojector's pairs per load batch (1, 2, 4; 0: tune)."""
+        nat.check(self._L.asd_batch_set_atx_chunk(self.h, int(chunk)))
+
     def sync(self):
         nat.check(self._L.asd_batch_sync(self.h))
 

@@ -375,6 +375,8 @@ def batch_rates(steps=20):
         B.iterate(steps)
         B.sync()
         res[f"{key}_case_iters_per_sec"] = K * steps / (time.perf_counter() - t0)
+        res[f"{key}_atx_tuning"] = {"pairs_per_load_batch": B.tuning()["atx_chunk"],
+                                    "us": {str(k): v for k, v in B.tuning()["atx_us"].items()}}
         B.close()
     res["speedup_batched_mixed_vs_sequential_plain"] = (
         res["batched_mixed_case_iters_per_sec"] / res["plain_sequential_case_iters_per_sec"])

@@ -266,6 +266,14 @@ int asd_batch_iterate(asd_batch *batch, int n_iter);
 int asd_batch_get_image(asd_batch *batch, int c, double *m_res);
 int asd_batch_get_out(asd_batch *batch, int c, asd_out *out, double *rmse);
 int asd_batch_set_graph(asd_batch *batch, int enable);
+/* The backprojector stage's footprint pairs per load batch, picked by the
+ * batch's autotuner on its first asd_batch_iterate (0 before), and the
+ * tuning times of 1, 2 and 4 pairs in µs (-1 before); every choice gives the
+ * same bits.  Either pointer may be NULL. */
+int asd_batch_tuning(asd_batch *batch, int *chunk, double *us3);
+/* Forces the backprojector stage's pairs per load batch (1, 2 or 4; 0: let
+ * the autotuner pick on the next asd_batch_iterate). */
+int asd_batch_set_atx_chunk(asd_batch *batch, int chunk);
 int asd_batch_sync(asd_batch *batch);
 void asd_batch_destroy(asd_batch *batch);
 

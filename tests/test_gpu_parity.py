@@ -465,7 +465,22 @@ def test_batch_matches_single_plans():
         assert rel_l2(B.image(c), m.ravel(order="F")) <= 1e-5, c
         np.testing.assert_allclose(rb, out["RMSE"], rtol=1e-5)
         assert ob["beta"] == pytest.approx(out["beta"], rel=1e-12)
+    tune = B.tuning()
+    assert tune["atx_chunk"] in (1, 2, 4) and set(tune["atx_us"]) == {1, 2, 4}, tune
+    ref = [B.image(c) for c in range(len(paras))]
     B.close()
+    # every pairs-per-load-batch choice of the backprojector stage gives the
+    # same bits
+    for chunk in (1, 2, 4):
+        Bc = ops.Batch(paras)
+        Bc.set_atx_chunk(chunk)
+        for c in range(len(paras)):
+            Bc.set_case(c, Ps[c], None, imgs[c])
+        Bc.iterate(3)
+        assert Bc.tuning()["atx_chunk"] == chunk
+        for c in range(len(paras)):
+            assert np.array_equal(Bc.image(c), ref[c]), (chunk, c)
+        Bc.close()
 
 
 def test_stage_timing_counts_every_launch_and_changes_nothing():
