@@ -136,6 +136,13 @@ int check_geom(const ViewGeom &g)
     ASD_ARG(g.nx > 0 && g.ny > 0 && g.nz > 0 && g.na > 0 && g.nb > 0,
             "sizes nx, ny, nz, na, nb must be positive");
     ASD_ARG(index_fits(g.nx, g.ny, g.nz, g.na, g.nb), ASD_INDEX_LIMITS);
+    // distances and pitches in voxel units (the gateways' SO / vxl, det / vxl
+    // and dz = 1); NaN fails every comparison
+    ASD_ARG(g.SO > 0 && g.OD > 0 && g.SO < 3.0e38f && g.OD < 3.0e38f,
+            "SO and OD must be positive and finite");
+    ASD_ARG(g.dz > 0 && g.dy_det > 0 && g.dz_det > 0 && g.dz < 3.0e38f && g.dy_det < 3.0e38f &&
+                g.dz_det < 3.0e38f,
+            "dz and the detector pitches must be positive and finite");
     return ASD_OK;
 }
 

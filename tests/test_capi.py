@@ -70,6 +70,16 @@ def test_bad_arguments_are_rejected_before_the_device():
     x = np.zeros(4, np.float32)
     assert L.asd_ax_device(x.ctypes.data_as(C.c_void_p), None, None, 1.0, 0.0, None, None, 1.0,
                            1.0, 1.0, 1.0, 0, 1, 1, 1, 1, None) == nat.ASD_ERR_ARG
+    # geometry in voxel units: distances, slice and detector pitches positive
+    for bad in [dict(SO=0.0), dict(OD=-1.0), dict(dz=0.0), dict(dz=float("nan"))]:
+        a = dict(SO=500.0, OD=500.0, dz=1.0)
+        a.update(bad)
+        assert L.asd_ax_device(x.ctypes.data_as(C.c_void_p), x.ctypes.data_as(C.c_void_p),
+                               x.ctypes.data_as(C.c_void_p), 1.0, 0.0,
+                               x.ctypes.data_as(C.c_void_p), x.ctypes.data_as(C.c_void_p),
+                               a["SO"], a["OD"], 1.0, a["dz"], 2, 2, 1, 2, 1,
+                               None) == nat.ASD_ERR_ARG, bad
+        assert b"positive" in L.asd_last_error()
     # the void drop-ins report through asd_last_status
     L.Ax_cone_mf_gpu_new(None, None, None, 1.0, 0.0, None, None, 1.0, 1.0, 1.0, 1.0,
                          -1, 1, 1, 1, 1)
