@@ -127,3 +127,16 @@ def test_size_mismatch_messages():
         ops.imgTVGradMin(np.zeros((4, 4)), 0.1, 1, 1, np.zeros((4, 5)))
     with pytest.raises(ops.MexError, match="phi_vec mismatch"):
         ops.ASD_POCS_cuda(np.zeros((4, 3)), dict(para, phi_vec=[1.0, 2.0]))
+
+
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    """No fallback: without the built library every entry point raises
+    instead of computing anything another way."""
+    monkeypatch.setattr(nat, "_lib", None)
+    monkeypatch.setattr(nat, "LIB_PATH", str(tmp_path / "libasdpocs.so"))
+    with pytest.raises(RuntimeError, match="is missing"):
+        nat.lib()
+    with pytest.raises(RuntimeError, match="is missing"):
+        ops.Ax_cone_gpu(np.zeros(16, np.float32), dict(nx=4, ny=4, nz=1, SO=1.0, OD=1.0,
+                                                        vxlsize=1.0, phi=0.0, detsize=1.0,
+                                                        na=4, nb=1))
