@@ -42,8 +42,14 @@ ATX_VARIANTS = ["direct", "tile16x16x1", "tile8x8x4", "tile16x4x4", "pairs", "di
                 "direct1", "sorted16", "sorted8x8x4", "sorted8x4x8"]
 
 
-def atx_kernel_name(variant, kahan):
+def atx_kernel_name(variant, kahan, precision=0, three_d=True):
     k = "true" if kahan else "false"
+    if precision == 3:  # the tiled backprojector with FP64 per-tile ray setup
+        t = "8, 8, 4" if three_d else "16, 16, 1"
+        return f"atx_tile_kernel<{t}, true, EpiSartAtx>"
+    if precision == 4:  # FP64 image and its FP32 shadow
+        return atx_kernel_name(variant, kahan).replace("EpiSartAtx,", "EpiSartAtx64,").replace(
+            "EpiSartAtx>", "EpiSartAtx64>")
     return {"direct": f"atx_kernel<EpiSartAtx, {k}, 4>",
             "tile16x16x1": f"atx_tile_plain_kernel<16, 16, 1, {k}, EpiSartAtx>",
             "tile8x8x4": f"atx_tile_plain_kernel<8, 8, 4, {k}, EpiSartAtx>",
@@ -59,6 +65,8 @@ def atx_kernel_name(variant, kahan):
 def ax_kernel_name(lpr, rows, precision):
     k = "true" if precision in (1, 2, 3) else "false"
     epi = f"EpiSartAx<{CODEC[precision]}>"
+    if precision == 4:  # each ray summed in slab order, one wave per ray
+        return f"ax_ordered_kernel<false, {epi} >"
     if rows > 1:
         return f"ax_rows_kernel<{lpr}, {rows}, {1 if rows == 3 else 2}, {k}, false, {epi} >"
     return f"ax_kernel<{lpr}, {k}, false, {epi} >"
@@ -264,7 +272,8 @@ def headline(args):
     tuning = plan.tuning()
     lpr, rows, _ = plan.info()
     names = {0: ax_kernel_name(lpr, rows, precision),
-             1: atx_kernel_name(tuning["atx_variant"], precision in (1, 2, 3)),
+             1: atx_kernel_name(tuning["atx_variant"], precision in (1, 2, 3), precision,
+                                p["nz"] > 1),
              2: "TV descent (tv_grad_kernel x ng + tv_step_kernel)"}
     per_stage = {}
     for s, (ms, cnt) in stage.items():
