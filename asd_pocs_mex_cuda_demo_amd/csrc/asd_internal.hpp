@@ -95,6 +95,7 @@ struct ViewGeom {
     float fp_cx[4], fp_cy[4];  // voxel corner offsets in the view frame (bounds only)
     float fp_rdy, fp_rdz;    // 1 / dy_det, 1 / dz_det (bounds only)
     int tight;               // footprints may be cut to the voxel's shadow (view_tight)
+    int general;             // every ray is a general ray with exact face offsets (view_general)
     FastDiv div_nx, div_ny;  // voxel index -> (x, row) and row -> (y, z) (geom_finish)
     FastDiv div_na;          // bin index -> (ia, ib) (geom_finish)
     // z-window [z0, z0 + nzs) of the volume the buffers hold (geom_finish:
@@ -111,7 +112,12 @@ struct ViewGeom {
 struct alignas(32) RayRec {
     float s_ba, s_ab, s_ca, s_cb, s_ac, s_bc;
     int mode;   // RAY_* below
-    int flat;   // ray in the z = 0 face of a one-slice volume (make_ray_rec)
+    // General rays with exact face offsets (RAY_X_FIRST, RAY_Y_FIRST): bytes
+    // 0, 1 and 3 hold, for x, y and z, the top byte of the FP32 coordinate
+    // -sign(d_k) / 2 of the face of that axis the ray enters through, with
+    // (a, b, c) = d the ray's direction (0xbf: -1/2, 0x3f: +1/2).  Zero for
+    // the other modes.
+    unsigned side;
 };
 
 // find_l_3d dispatch (cu:131-200 and find_l cu:26-113)
@@ -130,6 +136,10 @@ enum : int {
     // face end points c -+ s*h are formed from the rounded products
     RAY_X_FIRST_P = 9,
     RAY_Y_FIRST_P = 10,
+    // RAY_X_FIRST / RAY_Y_FIRST for a ray in the z = 0 face of a one-slice
+    // volume (c == 0, nz == 1, dz == 1): the in-plane chord (chord2)
+    RAY_FLAT_X = 11,
+    RAY_FLAT_Y = 12,
 };
 
 __host__ __device__ __forceinline__ bool ray_general(int mode)

@@ -705,6 +705,8 @@ int asd_batch_create(int ncases, const asd_para *paras, const double *const *phi
                 zall[zoff[c] + (size_t)j * p.nb + i] =
                     (float)(((double)(float)i - (double)(float)p.nb / 2.0 + 0.5) * (double)det);
             g.tight = view_tight(g, &yall[yoff[c] + (size_t)j * p.na]);
+            g.general = view_general(g, &yall[yoff[c] + (size_t)j * p.na],
+                                     &zall[zoff[c] + (size_t)j * p.nb]);
         }
     }
     std::vector<BItem> items((size_t)B->max_nv * K);

@@ -270,6 +270,7 @@ void view_geom_from_para(const asd_para &p, double phi_deg, ViewGeom &g, std::ve
     for (int j = 0; j < p.nb; j++)
         zdet[j] = (float)(((double)(float)j - (double)(float)p.nb / 2.0 + 0.5) * (double)det);
     g.tight = view_tight(g, ydet.data());
+    g.general = view_general(g, ydet.data(), zdet.data());
 }
 
 int check_para(const asd_para *p, int nv)

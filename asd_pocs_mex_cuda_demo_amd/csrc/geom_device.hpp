@@ -168,6 +168,7 @@ __host__ __device__ __forceinline__ void geom_finish(ViewGeom &g)
     g.fp_rdy = 1.0f / g.dy_det;
     g.fp_rdz = 1.0f / g.dz_det;
     g.tight = 0;
+    g.general = 0;
     g.div_nx = make_fastdiv((unsigned)g.nx);
     g.div_ny = make_fastdiv((unsigned)g.ny);
     g.div_na = make_fastdiv((unsigned)g.na);
@@ -189,6 +190,43 @@ inline int view_tight(const ViewGeom &g, const float *y_det)
         if (x2 - x1 == 0.0f || y2 - y1 == 0.0f) return 0;
     }
     return 1;
+}
+
+// Whether make_ray_rec gives every ray of the view a general mode with exact
+// face offsets (RAY_X_FIRST or RAY_Y_FIRST), so the backprojector may call
+// chord3_sides without dispatching on the mode: the view is tight (no a == 0
+// or b == 0), dz == 1, no ray is flat (c == 0 in a one-slice volume), and no
+// slope is a nonzero value below 2^-125.  The last is shown from bounds: every
+// slope is a quotient of two of the nonzero |a|, |b|, |c| (a zero c gives the
+// slopes 0 and +-inf), so it is at least min / max of those, rounded
+// monotonically.  A view the bounds cannot clear keeps the dispatching walk.
+// a, b and c are formed exactly as make_ray_rec forms them.
+inline int view_general(const ViewGeom &g, const float *y_det, const float *z_det)
+{
+#pragma clang fp contract(off)
+    if (!g.tight || g.dz != 1.0f) return 0;
+    const float x1 = g.cos_phi * (-g.SO), y1 = g.sin_phi * (-g.SO), z1 = 0.0f;
+    float lo = 3.0e38f, hi = 0.0f;
+    for (int ia = 0; ia < g.na; ia++) {
+        const float x2 = g.cos_phi * g.OD - g.sin_phi * y_det[ia];
+        const float y2 = g.sin_phi * g.OD + g.cos_phi * y_det[ia];
+        const float a = absf(x2 - x1), b = absf(y2 - y1);
+        lo = a < lo ? a : lo;
+        lo = b < lo ? b : lo;
+        hi = a > hi ? a : hi;
+        hi = b > hi ? b : hi;
+    }
+    for (int ib = 0; ib < g.nb; ib++) {
+        const float c = absf(z_det[ib] - z1);
+        if (c == 0.0f) {
+            if (g.nz == 1) return 0;
+            continue;
+        }
+        if (!(c == c)) return 0;
+        lo = c < lo ? c : lo;
+        hi = c > hi ? c : hi;
+    }
+    return lo > 0.0f && (double)lo / (double)hi >= 0x1p-120;
 }
 
 // Reciprocal for conservative bounds only (one ulp on the device).
@@ -217,6 +255,22 @@ struct Acc {
             s = t;
         } else {
             s += x;
+        }
+    }
+    // add x where ok holds, without a branch.  Uncompensated: +0 where ok
+    // fails, which leaves the sum's bits unchanged (a sum that starts at +0
+    // is never -0, and s + 0 == s for every other s).  Compensated: the sum
+    // and carry are kept where ok fails (adding 0 would apply the carry).
+    __host__ __device__ __forceinline__ void add_if(bool ok, float x)
+    {
+#pragma clang fp contract(off)
+        if constexpr (KAHAN) {
+            const float s0 = s, c0 = c;
+            add(x);
+            s = ok ? s : s0;
+            c = ok ? c : c0;
+        } else {
+            s += ok ? x : 0.0f;
         }
     }
     __host__ __device__ __forceinline__ float value() const { return KAHAN ? s - c : s; }
@@ -638,12 +692,6 @@ __host__ __device__ __forceinline__ float chord3_general(const RayRec &R, bool x
 {
 #pragma clang fp contract(off)
     const float hx = 0.5f, hy = 0.5f;
-    // A flat ray (c == 0 in a one-slice volume, pz = -hz): s_ca = s_cb = +-0
-    // put both hits' z at pz, so every x / y face test's z condition holds
-    // (|pz| == hz) and ez = 0; s_ac, s_bc = +-inf make the z-face tests
-    // NaN / inf comparisons that fail.  What is left is the in-plane
-    // chord, evaluated by the same expressions in the same order.
-    if (R.flat) return chord2(px, py, R.s_ba, R.s_ab, R.s_ba * hx, R.s_ab * hy, hx, hy, x_first);
     const float xy_c = R.s_ba * (-px) + py;
     const float xz_c = R.s_ca * (-px) + pz;
     const float yx_c = R.s_ab * (-py) + px;
@@ -669,6 +717,85 @@ __host__ __device__ __forceinline__ float chord3_general(const RayRec &R, bool x
     return chord_sqrt(ex * ex + ey * ey + ez * ez);
 }
 
+// The coordinate -sign(d_k) / 2 of the entry face of axis K (RayRec::side:
+// the top byte of that FP32 value in byte 0, 1 or 3 for x, y or z, so each
+// takes one instruction: a shift, a byte permute or a mask).
+template <int K>
+__host__ __device__ __forceinline__ float entry_half(unsigned side)
+{
+    static_assert(K >= 0 && K <= 2, "axis");
+    if constexpr (K == 0) return __builtin_bit_cast(float, side << 24);
+    if constexpr (K == 2) return __builtin_bit_cast(float, side & 0xff000000u);
+#if defined(__HIP_DEVICE_COMPILE__)
+    // result byte 3 = byte 1 of side, bytes 0..2 = 0 (selector 0x0c)
+    return __builtin_bit_cast(float, __builtin_amdgcn_perm(0u, side, 0x010c0c0cu));
+#else
+    return __builtin_bit_cast(float, (side << 16) & 0xff000000u);
+#endif
+}
+
+// find_l_3d's chord for a general ray with exact face offsets (RAY_X_FIRST,
+// RAY_Y_FIRST), from the two sides of the voxel.  The six candidates of
+// find_l_3d split into the three faces the ray enters through (E: the x, y
+// or z face at -sign(d_k) / 2) and the three it leaves through (X).  The
+// reference keeps the first two candidates that pass its tests, in its test
+// order (cu:141-205), and returns their distance.  When exactly one E and
+// one X candidate pass (a ray through the voxel away from its edges), those
+// are the two, whatever the order, and the distance of two points does not
+// depend on which comes first (p1 - p0 and p0 - p1 differ only in sign, so
+// their squares are equal): the entry point is a select among three E
+// candidates and the exit point among three X candidates.  Each candidate
+// coordinate is the reference's c -+ s/2, formed as one fused multiply-add
+// of the exact product s * (+-1/2), so every test and every coordinate has
+// the reference's bits.  With fewer than two passes the length is 0, as in
+// the reference.  The common cases run without a branch; every other
+// pattern (several passes on one side, near an edge or a corner of the
+// voxel) and chords shorter than 2^-48 (chord_sqrt) take the reference's
+// ordered rule behind one branch.
+__host__ __device__ __forceinline__ float chord3_sides(const RayRec &R, float px, float py,
+                                                       float pz, float hz)
+{
+#pragma clang fp contract(off)
+    const float hxe = entry_half<0>(R.side), hye = entry_half<1>(R.side),
+                hze = entry_half<2>(R.side);
+    const float xy_c = R.s_ba * (-px) + py;
+    const float xz_c = R.s_ca * (-px) + pz;
+    const float yx_c = R.s_ab * (-py) + px;
+    const float yz_c = R.s_cb * (-py) + pz;
+    const float zx_c = R.s_ac * (-pz) + px;
+    const float zy_c = R.s_bc * (-pz) + py;
+    // end points on the entry faces (c + s * hE) and the exit faces (c - s * hE)
+    const float xy_e = __builtin_fmaf(hxe, R.s_ba, xy_c), xy_x = __builtin_fmaf(-hxe, R.s_ba, xy_c);
+    const float xz_e = __builtin_fmaf(hxe, R.s_ca, xz_c), xz_x = __builtin_fmaf(-hxe, R.s_ca, xz_c);
+    const float yx_e = __builtin_fmaf(hye, R.s_ab, yx_c), yx_x = __builtin_fmaf(-hye, R.s_ab, yx_c);
+    const float yz_e = __builtin_fmaf(hye, R.s_cb, yz_c), yz_x = __builtin_fmaf(-hye, R.s_cb, yz_c);
+    const float zx_e = __builtin_fmaf(hze, R.s_ac, zx_c), zx_x = __builtin_fmaf(-hze, R.s_ac, zx_c);
+    const float zy_e = __builtin_fmaf(hze, R.s_bc, zy_c), zy_x = __builtin_fmaf(-hze, R.s_bc, zy_c);
+    const float hx = 0.5f, hy = 0.5f;
+    const bool fxe = absf(xy_e) <= hy && absf(xz_e) <= hz;
+    const bool fye = absf(yx_e) <= hx && absf(yz_e) <= hz;
+    const bool fze = absf(zx_e) <= hx && absf(zy_e) <= hy;
+    const bool fxx = absf(xy_x) <= hy && absf(xz_x) <= hz;
+    const bool fyx = absf(yx_x) <= hx && absf(yz_x) <= hz;
+    const bool fzx = absf(zx_x) <= hx && absf(zy_x) <= hy;
+    // exactly one pass on each side
+    const bool one_e = (fxe != fye) != fze && !(fxe && fye && fze);
+    const bool one_x = (fxx != fyx) != fzx && !(fxx && fyx && fzx);
+    const bool any_e = fxe || fye || fze, any_x = fxx || fyx || fzx;
+    const bool two = one_e && one_x;
+    const float ex = (fxx ? -hxe : fyx ? yx_x : zx_x) - (fxe ? hxe : fye ? yx_e : zx_e);
+    const float ey = (fxx ? xy_x : fyx ? -hye : zy_x) - (fxe ? xy_e : fye ? hye : zy_e);
+    const float ez = (fxx ? xz_x : fyx ? yz_x : -hze) - (fxe ? xz_e : fye ? yz_e : hze);
+    const float q = ex * ex + ey * ey + ez * ez;
+    // every lane takes the root (of 1 where there is no chord)
+    const float l = sqrt_ge_tiny(two ? q : 1.0f);
+    // at most one pass in all: no chord
+    const bool none = !(any_e && any_x) && (!any_e || one_e) && (!any_x || one_x);
+    if ((!two && !none) || (two && !(q >= 0x1p-96f)))
+        return chord3_general<true>(R, R.mode == RAY_X_FIRST, px, py, pz, hz);
+    return two ? l : 0.0f;
+}
+
 __host__ __device__ __forceinline__ float find_l_3d_rec(const RayRec &R, float px, float py, float pz,
                                                float hz, float dz)
 {
@@ -676,11 +803,19 @@ __host__ __device__ __forceinline__ float find_l_3d_rec(const RayRec &R, float p
     const float hx = 0.5f, hy = 0.5f;
     // the general rays first, behind one test (the other modes are the
     // a == 0 / b == 0 fallbacks, which only views without a footprint cut
-    // have, and general rays whose face offsets are inexact products)
-    if (R.mode <= RAY_Y_FIRST) return chord3_general<true>(R, R.mode == RAY_X_FIRST, px, py, pz, hz);
+    // have, general rays whose face offsets are inexact products, and rays in
+    // the face of a one-slice volume)
+    if (R.mode <= RAY_Y_FIRST) return chord3_sides(R, px, py, pz, hz);
     switch (R.mode) {
     case RAY_X_FIRST_P: return chord3_general<false>(R, true, px, py, pz, hz);
     case RAY_Y_FIRST_P: return chord3_general<false>(R, false, px, py, pz, hz);
+    // A flat ray (c == 0 in a one-slice volume, pz = -hz): s_ca = s_cb = +-0
+    // put both hits' z at pz, so every x / y face test's z condition holds
+    // (|pz| == hz) and ez = 0; s_ac, s_bc = +-inf make the z-face tests
+    // NaN / inf comparisons that fail.  What is left is the in-plane
+    // chord, evaluated by the same expressions in the same order.
+    case RAY_FLAT_X: return chord2(px, py, R.s_ba, R.s_ab, R.s_ba * hx, R.s_ab * hy, hx, hy, true);
+    case RAY_FLAT_Y: return chord2(px, py, R.s_ba, R.s_ab, R.s_ba * hx, R.s_ab * hy, hx, hy, false);
     case RAY_A0_B0: return absf(py) <= hy ? dz : 0.0f;
     case RAY_A0_C0: return absf(pz) <= hz ? 1.0f : 0.0f;
     case RAY_A0_YZ_U: return chord2(py, pz, R.s_cb, R.s_bc, R.s_cb * hy, R.s_bc * hz, hy, hz, true);
@@ -723,7 +858,7 @@ __host__ __device__ __forceinline__ RayRec make_ray_rec(const ViewGeom &g, float
     R.s_ba = b / a; R.s_ab = a / b;
     R.s_ca = c / a; R.s_cb = c / b;
     R.s_ac = a / c; R.s_bc = b / c;
-    R.flat = 0;
+    R.side = 0;
     if (a == 0) {
         if (b == 0) R.mode = RAY_A0_B0;
         else if (c == 0) R.mode = RAY_A0_C0;
@@ -733,13 +868,18 @@ __host__ __device__ __forceinline__ RayRec make_ray_rec(const ViewGeom &g, float
         else R.mode = absf(a) > absf(c) ? RAY_B0_XZ_U : RAY_B0_XZ_W;
     } else {
         R.mode = absf(a) > absf(b) ? RAY_X_FIRST : RAY_Y_FIRST;
-        R.flat = c == 0 && g.nz == 1 && g.dz == 1.0f;
         // the face offsets s * 0.5 (and s * dz/2) are exact unless a product
         // is subnormal (|s| < 2^-125, s != 0) or dz != 1
         bool exact = g.dz == 1.0f;
         for (float s : {R.s_ba, R.s_ab, R.s_ca, R.s_cb, R.s_ac, R.s_bc})
             exact = exact && !(s != 0.0f && absf(s) < 0x1p-125f);
-        if (!exact) R.mode += RAY_X_FIRST_P;
+        if (c == 0 && g.nz == 1 && g.dz == 1.0f)
+            R.mode += RAY_FLAT_X;
+        else if (!exact)
+            R.mode += RAY_X_FIRST_P;
+        else  // entry faces at -sign(d)/2 (c == 0: no z face passes, either sign)
+            R.side = (a > 0 ? 0xbfu : 0x3fu) | (b > 0 ? 0xbfu : 0x3fu) << 8 |
+                     (c > 0 ? 0xbfu : 0x3fu) << 24;
     }
     return R;
 }

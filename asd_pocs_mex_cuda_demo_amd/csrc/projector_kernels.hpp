@@ -415,9 +415,48 @@ struct BinsBuf {
 struct BinsPtr {
     const RayRec *__restrict__ rt;
     const float *__restrict__ Y;
-    __device__ __forceinline__ RayRec rec(unsigned i) const { return rt[i]; }
-    __device__ __forceinline__ float y(unsigned i) const { return Y[i]; }
+    __host__ __device__ __forceinline__ RayRec rec(unsigned i) const { return rt[i]; }
+    __host__ __device__ __forceinline__ float y(unsigned i) const { return Y[i]; }
 };
+
+// The footprint walk on a view whose rays are all general rays with exact
+// face offsets (ViewGeom::general): the chord of every pair is chord3_sides,
+// with no dispatch on the ray's mode, and the pairs past the footprint's end
+// in the last load batch are evaluated and added as nothing (Acc::add_if)
+// instead of branched around.  Same terms in the same order as the walk below.
+template <bool KAHAN, int CHUNK, class Bins>
+__host__ __device__ __forceinline__ void atx_voxel_walk_general(const ViewGeom &g, const Bins &B,
+                                                       const Footprint &f, float px, float py,
+                                                       float pz, float hz, Acc<KAHAN> &acc,
+                                                       Acc<KAHAN> &accl)
+{
+#pragma clang fp contract(off)
+    const int npairs = (f.a_hi - f.a_lo + 1) * (f.b_hi - f.b_lo + 1);
+    int ia = f.a_lo;
+    unsigned idx = mad24((unsigned)f.b_lo, (unsigned)g.na, (unsigned)f.a_lo);
+    const unsigned jump = (unsigned)(g.na - (f.a_hi - f.a_lo));
+    for (int k0 = 0; k0 < npairs; k0 += CHUNK) {
+        RayRec R[CHUNK];
+        float yv[CHUNK];
+#pragma unroll
+        for (int k = 0; k < CHUNK; k++) {
+            R[k] = B.rec(idx);
+            yv[k] = B.y(idx);
+            if (k0 + k + 1 < npairs) {
+                const bool wrap = ia == f.a_hi;
+                ia = wrap ? f.a_lo : ia + 1;
+                idx += wrap ? jump : 1u;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < CHUNK; k++) {
+            const bool ok = k0 + k < npairs;
+            const float l = chord3_sides(R[k], px, py, pz, hz);
+            acc.add_if(ok, l * yv[k]);
+            accl.add_if(ok, l);
+        }
+    }
+}
 
 template <bool KAHAN, int CHUNK, class Bins>
 __device__ __forceinline__ void atx_voxel_walk_t(const ViewGeom &g, const Bins &B,
@@ -426,6 +465,10 @@ __device__ __forceinline__ void atx_voxel_walk_t(const ViewGeom &g, const Bins &
                                                  Acc<KAHAN> &accl)
 {
 #pragma clang fp contract(off)
+    if (g.general) {
+        atx_voxel_walk_general<KAHAN, CHUNK>(g, B, f, px, py, pz, hz, acc, accl);
+        return;
+    }
     const int npairs = (f.a_hi - f.a_lo + 1) * (f.b_hi - f.b_lo + 1);
     // cursor: the next pair in reference order, as its column ia and its bin
     // index ib * na + ia (a row wrap jumps to column a_lo of the next row)

@@ -205,6 +205,7 @@ int atx_host(float *X, const float *y, float *sc, const ViewGeom &g, const float
         return rc;
     ViewGeom gt = g;
     gt.tight = view_tight(g, y_det);  // host table: the rays' a and b can be checked here
+    gt.general = view_general(gt, y_det, z_det);
     if ((rc = launch_atx(gt, d.vol.as<float>(), d.proj.as<float>(), d.vol2.as<float>(),
                          d.ydet.as<float>(), d.zdet.as<float>(), d.rt.as<RayRec>(), d.stream)))
         return rc;

@@ -8,6 +8,7 @@
 #include <cstddef>
 
 #include "geom_device.hpp"
+#include "projector_kernels.hpp"
 #include "projector_tile.hpp"
 
 #include <vector>
@@ -45,9 +46,13 @@ static RayRec ray_rec(const ViewGeom &g, float ydet, float zdet)
 {
     RayRec R = make_ray_rec(g, ydet, zdet);
     if (g_general) {
-        // the flat shortcut off, and the product form of the face offsets
-        R.flat = 0;
-        if (R.mode <= RAY_Y_FIRST) R.mode += RAY_X_FIRST_P;
+        // the flat shortcut and the two-sided chord off: the product form of
+        // the face offsets and the reference's ordered rule
+        if (R.mode == RAY_FLAT_X || R.mode == RAY_FLAT_Y)
+            R.mode += RAY_X_FIRST_P - RAY_FLAT_X;
+        else if (R.mode <= RAY_Y_FIRST)
+            R.mode += RAY_X_FIRST_P;
+        R.side = 0;
     }
     return R;
 }
@@ -85,6 +90,32 @@ long emul_footprint_pairs(float c, float s, const float *ydet, float SO, float O
 
 // det_coord, the detector grid the forward projector computes in place.
 float emul_det_coord(int i, int n, float det) { return det_coord(i, n, det); }
+
+// view_general for a view (after view_tight, as the plans set them).
+int emul_view_general(float c, float s, const float *ydet, const float *zdet, float SO, float OD,
+                      float dyd, float dzd, float dz, int nx, int ny, int nz, int na, int nb)
+{
+    ViewGeom g = mk(c, s, SO, OD, 1.0f, dyd, dzd, dz, nx, ny, nz, na, nb);
+    g.tight = view_tight(g, ydet);
+    return view_general(g, ydet, zdet);
+}
+
+// Lengths of n (ray, voxel) pairs by the backprojector's record path
+// (make_ray_rec, find_l_3d_rec): pair k is the ray of the view (c[k], s[k],
+// SO[k], OD[k]) to detector point (ydet[k], zdet[k]) and the unit voxel
+// centred at (xc[k], yc[k], zc[k]) in a volume of nz slices.
+void emul_find_l_3d_pairs(long n, const float *c, const float *s, const float *SO,
+                          const float *OD, const float *ydet, const float *zdet, const float *xc,
+                          const float *yc, const float *zc, int nz, float *out)
+{
+    for (long k = 0; k < n; k++) {
+        ViewGeom g = mk(c[k], s[k], SO[k], OD[k], 1.0f, 1.0f, 1.0f, 1.0f, 4, 4, nz, 4, 4);
+        const RayRec R = ray_rec(g, ydet[k], zdet[k]);
+        const float px = g.cos_phi * (-g.SO) - xc[k], py = g.sin_phi * (-g.SO) - yc[k],
+                    pz = 0.0f - zc[k];
+        out[k] = find_l_3d_rec(R, px, py, pz, g.dz / 2.0f, g.dz);
+    }
+}
 
 }  // extern "C"
 
@@ -214,6 +245,7 @@ void emul_atx(float *X, const float *y, float *sc, float c, float s, const float
 {
     ViewGeom g = mk(c, s, SO, OD, scale, dyd, dzd, dz, nx, ny, nz, na, nb);
     if (!g_general) g.tight = view_tight(g, ydet);
+    if (!g_general) g.general = view_general(g, ydet, zdet);
     long M = (long)na * nb;
     RayRec *rt = new RayRec[M];
     for (long id = 0; id < M; id++) rt[id] = ray_rec(g, ydet[id % na], zdet[id / na]);
@@ -226,6 +258,14 @@ void emul_atx(float *X, const float *y, float *sc, float c, float s, const float
         float px = sx - f.xc, py = sy - f.yc, pz = 0.0f - f.zc;
         float hz = g.dz / 2.0f;
         Acc<false> acc, accl;
+        if (g.general) {
+            // the kernels' walk for views of general rays (two pairs per load batch)
+            if (f.a_lo <= f.a_hi && f.b_lo <= f.b_hi)
+                atx_voxel_walk_general<false, 2>(g, BinsPtr{rt, y}, f, px, py, pz, hz, acc, accl);
+            X[v] = acc.value() * scale;
+            sc[v] = accl.value() * scale;
+            continue;
+        }
         for (int ib = f.b_lo; ib <= f.b_hi; ib++)
             for (int ia = f.a_lo; ia <= f.a_hi; ia++) {
                 const RayRec &R = rt[(long)ib * na + ia];

@@ -290,3 +290,88 @@ def test_fastdiv_is_exact():
         q = np.zeros_like(n)
         lib.emul_fastdiv(d, P(n), P(q), n.size)
         assert np.array_equal(q, n // np.uint32(d)), d
+
+
+def _pairs(rng, n, near_edges):
+    """n (ray, voxel) pairs: random rays of random views (angles, distances,
+    detector points) and unit voxels near a point of the ray; with
+    near_edges the ray passes within 0 to 3e-4 voxel of an edge or a corner
+    of its voxel, where FP32 face tests pass on several faces at once."""
+    phi = rng.uniform(-7, 7, n)
+    c = np.cos(phi).astype(np.float32)
+    s = np.sin(phi).astype(np.float32)
+    SO = rng.uniform(30, 1500, n).astype(np.float32)
+    OD = rng.uniform(30, 1500, n).astype(np.float32)
+    yd = rng.uniform(-400, 400, n).astype(np.float32)
+    zd = rng.uniform(-300, 300, n).astype(np.float32)
+    Sx, Sy = (c * -SO).astype(np.float64), (s * -SO).astype(np.float64)
+    Dx = (c * OD - s * yd).astype(np.float64)
+    Dy = (s * OD + c * yd).astype(np.float64)
+    t = rng.uniform(0.2, 0.8, n)
+    P = np.stack([Sx + t * (Dx - Sx), Sy + t * (Dy - Sy), t * zd], 1)
+    if near_edges:
+        off = rng.choice([-0.5, 0.5], (n, 3))
+        free = rng.integers(0, 4, n)  # the axis along the edge (3: a corner)
+        for k in range(3):
+            off[free == k, k] = rng.uniform(-0.5, 0.5, (free == k).sum())
+        off += rng.choice([0, 1e-7, -1e-7, 1e-6, -1e-6, 1e-5, -1e-5, 1e-4, -1e-4, 3e-4], (n, 3))
+    else:
+        off = rng.uniform(-0.9, 0.9, (n, 3))
+    C = (P + off).astype(np.float32)
+    return c, s, SO, OD, yd, zd, C[:, 0].copy(), C[:, 1].copy(), C[:, 2].copy()
+
+
+@pytest.mark.parametrize("near_edges", [False, True])
+def test_two_sided_chord_matches_reference_rule(orc, near_edges):
+    """find_l_3d_rec (the backprojector's chord: entry and exit candidates
+    selected when one passes on each side, the reference's ordered rule
+    otherwise) equals the oracle's find_l_3d bit for bit on 40000 pairs;
+    near voxel edges and corners several faces pass their FP32 tests and the
+    ordered rule decides (including the chords the reference drops)."""
+    lib = emul()
+    lib.emul_find_l_3d_pairs.argtypes = [C.c_long] + [C.c_void_p] * 9 + [C.c_int, C.c_void_p]
+    n = 40000
+    args = _pairs(np.random.default_rng(31 + near_edges), n, near_edges)
+    out = np.zeros(n, np.float32)
+    lib.emul_find_l_3d_pairs(n, *[a.ctypes.data for a in args], 8, out.ctypes.data)
+    c, s, SO, OD, yd, zd, xc, yc, zc = args
+    L = orc.lib()
+    ref = np.array([L.orc_find_l_3d(float(c[k] * -SO[k]), float(s[k] * -SO[k]), 0.0,
+                                    float(np.float32(c[k] * OD[k]) - np.float32(s[k] * yd[k])),
+                                    float(np.float32(s[k] * OD[k]) + np.float32(c[k] * yd[k])),
+                                    float(zd[k]), 1.0, 1.0, 1.0, float(xc[k]), float(yc[k]),
+                                    float(zc[k])) for k in range(n)], np.float32)
+    assert (out > 0).sum() > n // 3
+    assert np.array_equal(out.view(np.uint32), ref.view(np.uint32))
+
+
+def test_general_views_take_the_mode_free_walk(orc):
+    """Cone views whose rays are all general with exact face offsets
+    (view_general) take the walk without a mode dispatch: a view with an odd
+    row count (a row with c == 0) is one, a fan view (flat rays) and a view
+    with a b == 0 ray are not.  The backprojection of the general views
+    equals the oracle bit for bit."""
+    lib = emul()
+    f, i, vp = C.c_float, C.c_int, C.c_void_p
+    lib.emul_view_general.argtypes = [f, f, vp, vp, f, f, f, f, f, i, i, i, i, i]
+    for nx, ny, nz, na, nb, SO, OD, det, phi, want in [
+            (20, 18, 10, 32, 16, 300.0, 300.0, 1.5, -0.3, 1),
+            (20, 18, 9, 31, 15, 200.0, 300.0, 1.1, -2.2, 1),   # odd nb: z_det == 0 row
+            (16, 16, 6, 24, 7, 25.0, 25.0, 1.5, -1.2, 1),
+            (40, 36, 1, 64, 1, 500.0, 400.0, 1.3, -0.3, 0),   # fan: flat rays
+            (16, 16, 4, 31, 5, 200.0, 200.0, 1.0, 0.0, 0)]:   # b == 0 at the central bin
+        g = orc.gateway_geometry(phi, SO, OD, 1.0, det, na, nb)
+        got = lib.emul_view_general(g["cos_phi"], g["sin_phi"], P(g["y_det"]), P(g["z_det"]),
+                                    g["SO"], g["OD"], g["det"], g["det"], np.float32(1), nx, ny,
+                                    nz, na, nb)
+        assert got == want, (nx, ny, nz, na, nb, phi)
+        Y = np.random.default_rng(nx + na).standard_normal(na * nb).astype(np.float32)
+        x = np.zeros(nx * ny * nz, np.float32)
+        sc = np.zeros_like(x)
+        emul().emul_atx(P(x), P(Y), P(sc), g["cos_phi"], g["sin_phi"], P(g["y_det"]),
+                        P(g["z_det"]), g["SO"], g["OD"], g["scale"], g["det"], g["det"],
+                        np.float32(1), nx, ny, nz, na, nb)
+        xo, sco = orc.atx(Y, g["cos_phi"], g["sin_phi"], g["y_det"], g["z_det"], g["SO"],
+                          g["OD"], g["scale"], g["det"], g["det"], np.float32(1), nx, ny, nz,
+                          na, nb)
+        assert np.array_equal(x, xo) and np.array_equal(sc, sco)
