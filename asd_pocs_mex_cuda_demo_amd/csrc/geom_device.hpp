@@ -912,16 +912,31 @@ struct FootCol {
     int cut;         // the axial cut applies (tight view, square in front of the source)
 };
 
+// Voxel centre coordinates in the reference's voxel frame (cu:243-245):
+// i + 0.5 - n/2, which the reference forms in FP64 and rounds, is exact in
+// FP32 for |i - n/2| < 2^22.
+__host__ __device__ __forceinline__ float voxel_xc(const ViewGeom &g, int ix)
+{
+    return (float)(ix - g.nx / 2) + 0.5f;
+}
+__host__ __device__ __forceinline__ float voxel_yc(const ViewGeom &g, int iy)
+{
+    return (float)(iy - g.ny / 2) + 0.5f;
+}
+__host__ __device__ __forceinline__ float voxel_zc(const ViewGeom &g, int iz)
+{
+#pragma clang fp contract(off)
+    return ((float)(iz - g.nz / 2) + 0.5f) * g.dz;
+}
+
 __host__ __device__ __forceinline__ FootCol footprint_col(const ViewGeom &g, int ix, int iy)
 {
 #pragma clang fp contract(off)
     FootCol c;
     const float SD = g.SO + g.OD;
-    const int na2 = g.na / 2, nx2 = g.nx / 2, ny2 = g.ny / 2;
-    // i + 0.5 - n/2 (the reference forms it in FP64 and rounds) is exact in
-    // FP32 for |i - n/2| < 2^22
-    c.yc = (float)(iy - ny2) + 0.5f;
-    c.xc = (float)(ix - nx2) + 0.5f;
+    const int na2 = g.na / 2;
+    c.yc = voxel_yc(g, iy);
+    c.xc = voxel_xc(g, ix);
     // (xr, yr): the voxel centre in the view frame (x' towards the detector)
     const float xr = g.cos_phi * c.xc + g.sin_phi * c.yc;
     const float yr = -g.sin_phi * c.xc + g.cos_phi * c.yc;
@@ -972,9 +987,9 @@ __host__ __device__ __forceinline__ Footprint footprint_z(const ViewGeom &g, con
 {
 #pragma clang fp contract(off)
     Footprint f;
-    const int nb2 = g.nb / 2, nz2 = g.nz / 2;
+    const int nb2 = g.nb / 2;
     const float d = g.fp_d;
-    f.zc = ((float)(iz - nz2) + 0.5f) * g.dz;
+    f.zc = voxel_zc(g, iz);
     f.yc = c.yc;
     f.xc = c.xc;
     const int b_hi = (int)floorf((f.zc + d) * c.tz + nb2);
