@@ -186,6 +186,23 @@ __host__ __device__ __forceinline__ float sqrt_ge_tiny(float x)
 #endif
 }
 
+// Correctly rounded square root of a finite x >= 2^-96: sqrt_ge_tiny without
+// its fixup for +-0 and +inf, which such an x never is.
+__host__ __device__ __forceinline__ float sqrt_finite_ge_tiny(float x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float sd = __int_as_float(__float_as_int(s) - 1);
+    const float su = __int_as_float(__float_as_int(s) + 1);
+    const float rd = __builtin_fmaf(-sd, s, x);
+    const float ru = __builtin_fmaf(-su, s, x);
+    const float t = rd <= 0.0f ? sd : s;
+    return ru > 0.0f ? su : t;
+#else
+    return sqrtf(x);
+#endif
+}
+
 // Lanes cooperating on one ray of the forward projector.
 int ax_lanes_per_ray(int na, int nb, int nu);
 

@@ -488,6 +488,80 @@ __host__ __device__ __forceinline__ void ax_slab_geom(const AxRay &r, const Slab
     ax_slab_z<FLAT>(r, ax_slab_t<FT>(r, A, iu), A, ny, s);
 }
 
+// The slabs [lo, hi] of a bundle of K rays that share their in-plane line
+// (rays of one detector column, or K = 1) outside which no slab has a piece
+// in the volume's window: a slab's pieces lie in cells floor(t) and
+// floor(z) of its end points' transverse and axial coordinates
+// (ax_slab_t / ax_slab_z), so a slab with a piece in the window has an end
+// point with 0 <= t < nt and, for one of the rays, zlo <= z < zhi.  Those
+// coordinates are linear in the slab position u; the u where they hold is
+// solved in FP64 and widened by a margin far above the FP32 rounding of t and
+// z (1e-2 cell plus 1e-5 of the coordinates' magnitudes), so no slab with a
+// piece is left out.  Skipping the other slabs adds nothing to any sum (their
+// pieces are never added), so the ray sums keep their bits.
+// Reciprocal for the bound below (about 2^-26 relative on the device, where
+// its error moves a bound by far less than the margin).
+__host__ __device__ __forceinline__ double bound_rcp64(double x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_rcp(x);
+#else
+    return 1.0 / x;
+#endif
+}
+
+__host__ __device__ __forceinline__ void ax_bundle_slabs(const AxRay *r, int K, int nu, int &lo,
+                                                         int &hi)
+{
+    const double u1 = r[0].u1, nu2 = r[0].nu2;
+    const double mag = fabs((double)r[0].u1) + (double)nu + 1.0;
+    // u interval (in the reference's slab coordinate ua = iu - nu2) where the
+    // transverse coordinate lies in [-dt, nt + dt)
+    double uL = -1e30, uR = 1e30;
+    {
+        const double s1 = r[0].s1, t0 = (double)r[0].t1s + (double)r[0].nt2;
+        const double dt = 1e-2 + 1e-5 * (fabs((double)r[0].t1s) + fabs(s1) * mag + r[0].nt);
+        if (s1 == 0.0) {
+            if (!(t0 >= -dt && t0 < r[0].nt + dt)) uL = 1e30, uR = -1e30;
+        } else {
+            const double rs = bound_rcp64(s1);
+            const double a = u1 + (-dt - t0) * rs, b = u1 + (r[0].nt + dt - t0) * rs;
+            uL = a < b ? a : b;
+            uR = a < b ? b : a;
+        }
+    }
+    // union over the rays of the u intervals where the axial coordinate lies
+    // in [zlo - dz, zhi + dz)
+    double zL = 1e30, zR = -1e30;
+    for (int k = 0; k < K; k++) {
+        const double s2 = r[k].s2, z0 = r[k].nz2;
+        const double dz = 1e-2 + 1e-5 * (fabs(s2) * mag / r[k].dz + r[k].nz);
+        double a, b;
+        if (s2 == 0.0) {
+            if (!(z0 >= r[k].zlo - dz && z0 < r[k].zhi + dz)) continue;
+            a = -1e30;
+            b = 1e30;
+        } else {
+            const double rs = r[k].dz * bound_rcp64(s2);
+            a = u1 + (r[k].zlo - dz - z0) * rs;
+            b = u1 + (r[k].zhi + dz - z0) * rs;
+            if (a > b) {
+                const double t = a;
+                a = b;
+                b = t;
+            }
+        }
+        zL = a < zL ? a : zL;
+        zR = b > zR ? b : zR;
+    }
+    uL = zL > uL ? zL : uL;
+    uR = zR < uR ? zR : uR;
+    // slab iu spans u in [iu - nu2, iu - nu2 + 1]; one more slab each side
+    const double fl = floor(uL + nu2) - 2.0, fh = floor(uR + nu2) + 1.0;
+    lo = fl < 0.0 ? 0 : fl > (double)nu ? nu : (int)fl;
+    hi = fh < -1.0 ? -1 : fh > (double)(nu - 1) ? nu - 1 : (int)fh;
+}
+
 // U consecutive slabs of one lane (iu0, iu0 + stride, ...): all geometry
 // first, then every load, then the sums in slab order.
 template <int U, bool KAHAN, bool FT, class Vol, bool FLAT = false>
@@ -787,8 +861,9 @@ __host__ __device__ __forceinline__ float chord3_sides(const RayRec &R, float px
     const float ey = (fxx ? xy_x : fyx ? -hye : zy_x) - (fxe ? xy_e : fye ? hye : zy_e);
     const float ez = (fxx ? xz_x : fyx ? yz_x : -hze) - (fxe ? xz_e : fye ? yz_e : hze);
     const float q = ex * ex + ey * ey + ez * ez;
-    // every lane takes the root (of 1 where there is no chord)
-    const float l = sqrt_ge_tiny(two ? q : 1.0f);
+    // every lane takes the root (of 1 where there is no chord); a chord's q
+    // is finite (|e| <= 1 per axis) and, off the branch below, >= 2^-96
+    const float l = sqrt_finite_ge_tiny(two ? q : 1.0f);
     // at most one pass in all: no chord
     const bool none = !(any_e && any_x) && (!any_e || one_e) && (!any_x || one_x);
     if ((!two && !none) || (two && !(q >= 0x1p-96f)))

@@ -108,7 +108,9 @@ __global__ __launch_bounds__(256) void ax_kernel(ViewGeom g, const float *__rest
 // ax_kernel forms for it.  LPR lanes march the bundle together as above; the
 // K ray sums are reduced the same way and lane k stores ray k.  Rows past nb
 // repeat the last row and are not stored.  A lane takes U slabs per load
-// batch (iu, iu + LPR, ...), so 3 * U * K loads are in flight together.
+// batch (iu, iu + LPR, ...), so 3 * U * K loads are in flight together.  The
+// march covers only the slabs ax_bundle_slabs leaves (a 512^3 volume under a
+// 1024 x 768 detector puts about a quarter of the slabs outside it).
 template <int LPR, int K, int U, bool KAHAN, bool WIDE, class Epi>
 __global__ __launch_bounds__(256) void ax_rows_kernel(ViewGeom g, const float *__restrict__ X,
                                                       const float *__restrict__ y_det,
@@ -146,15 +148,20 @@ __global__ __launch_bounds__(256) void ax_rows_kernel(ViewGeom g, const float *_
             ax_ray_axial(r[k], du, z_det ? z_det[ib] : det_coord(ib, g.nb, g.dz_det));
         }
         const int nu = r[0].xdriven ? g.nx : g.ny;
+        // the slabs that may hold a piece in the volume; lane j still takes
+        // the slabs iu == j (mod LPR), so its partial sums keep their terms
+        int lo, hi;
+        ax_bundle_slabs(r, K, nu, lo, hi);
+        const int first = lo + ((lane - lo) % LPR + LPR) % LPR;
         const SlabAddr A = slab_addr(r[0], g.nx, VolRead<WIDE>::unit);
         auto march = [&](auto ft) {
-            for (int iu0 = lane; iu0 < nu; iu0 += U * LPR) {
+            for (int iu0 = first; iu0 <= hi; iu0 += U * LPR) {
                 SlabW s[U][K];
                 float xv[U][K][3];
 #pragma unroll
                 for (int j = 0; j < U; j++) {
                     const int iu = iu0 + j * LPR;
-                    if (iu < nu) {
+                    if (iu <= hi) {
                         const SlabT t = ax_slab_t<decltype(ft)::value>(r[0], A, iu);
 #pragma unroll
                         for (int k = 0; k < K; k++) ax_slab_z(r[k], t, A, g.ny, s[j][k]);

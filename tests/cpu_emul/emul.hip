@@ -91,6 +91,46 @@ long emul_footprint_pairs(float c, float s, const float *ydet, float SO, float O
 // det_coord, the detector grid the forward projector computes in place.
 float emul_det_coord(int i, int n, float det) { return det_coord(i, n, det); }
 
+// ax_bundle_slabs against the slabs themselves: over every bundle of K rows
+// of the view (rows past nb repeat the last), the number of slabs with a
+// piece in the volume outside the bundle's [lo, hi] (must be 0); *kept and
+// *total receive the slabs the range keeps and all slabs, summed over
+// bundles.
+long emul_ax_slab_range_misses(float c, float s, const float *ydet, const float *zdet, float SO,
+                               float OD, float dz, int nx, int ny, int nz, int na, int nb, int K,
+                               long *kept, long *total)
+{
+    ViewGeom g = mk(c, s, SO, OD, 1.0f, 1.0f, 1.0f, dz, nx, ny, nz, na, nb);
+    long miss = 0;
+    *kept = *total = 0;
+    for (int ia = 0; ia < na; ia++)
+        for (int ib0 = 0; ib0 < nb; ib0 += K) {
+            AxRay r[4];
+            float du;
+            r[0] = ax_ray_setup(g, ydet[ia], zdet[ib0], du);
+            for (int k = 1; k < K; k++) {
+                r[k] = r[0];
+                ax_ray_axial(r[k], du, zdet[std::min(ib0 + k, nb - 1)]);
+            }
+            const int nu = r[0].xdriven ? nx : ny;
+            int lo, hi;
+            ax_bundle_slabs(r, K, nu, lo, hi);
+            const SlabAddr A = slab_addr(r[0], nx, 4);
+            *total += nu;
+            *kept += hi >= lo ? hi - lo + 1 : 0;
+            for (int iu = 0; iu < nu; iu++) {
+                if (iu >= lo && iu <= hi) continue;
+                const SlabT t = ax_slab_t(r[0], A, iu);
+                for (int k = 0; k < K; k++) {
+                    SlabW w;
+                    ax_slab_z(r[k], t, A, ny, w);
+                    if (w.ok[0] || w.ok[1] || w.ok[2]) miss++;
+                }
+            }
+        }
+    return miss;
+}
+
 // view_general for a view (after view_tight, as the plans set them).
 int emul_view_general(float c, float s, const float *ydet, const float *zdet, float SO, float OD,
                       float dyd, float dzd, float dz, int nx, int ny, int nz, int na, int nb)

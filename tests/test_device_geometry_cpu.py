@@ -375,3 +375,38 @@ def test_general_views_take_the_mode_free_walk(orc):
                           g["OD"], g["scale"], g["det"], g["det"], np.float32(1), nx, ny, nz,
                           na, nb)
         assert np.array_equal(x, xo) and np.array_equal(sc, sco)
+
+
+def test_forward_slab_range_keeps_every_slab_with_a_piece(orc):
+    """ax_bundle_slabs (the slabs a row bundle of the forward projector
+    marches) leaves out no slab with a piece in the volume, on random cone
+    and fan geometries and on the 512^3 workload's geometry (a subset of its
+    detector columns), where it drops about a quarter of the slabs."""
+    lib = emul()
+    f, i, vp = C.c_float, C.c_int, C.c_void_p
+    lib.emul_ax_slab_range_misses.restype = C.c_long
+    lib.emul_ax_slab_range_misses.argtypes = [f, f, vp, vp, f, f, f, i, i, i, i, i, i, vp, vp]
+    rng = np.random.default_rng(41)
+    cases = []
+    for _ in range(30):
+        nz = 1 if rng.random() < 0.3 else int(rng.integers(2, 40))
+        cases.append((int(rng.integers(4, 60)), int(rng.integers(4, 60)), nz,
+                      int(rng.integers(3, 80)), 1 if nz == 1 else int(rng.integers(2, 40)),
+                      float(rng.uniform(20, 900)), float(rng.uniform(20, 900)),
+                      float(rng.uniform(0.3, 3.0)), float(rng.uniform(-7, 7)), int(rng.integers(1, 4))))
+    for phi in (-0.5 / 180 * np.pi, -37.3 / 180 * np.pi, -np.pi / 2):
+        cases.append((512, 512, 512, 1024, 768, 600.0 / 0.5, 500.0 / 0.5, 0.6 / 0.5, phi, 3))
+    dropped = []
+    for nx, ny, nz, na, nb, SO, OD, det, phi, K in cases:
+        g = orc.gateway_geometry(phi, SO, OD, 1.0, det, na, nb)
+        ydet = g["y_det"]
+        if na == 1024:  # every 16th detector column of the full-size view
+            ydet = np.ascontiguousarray(ydet[::16])
+        kept, total = C.c_long(), C.c_long()
+        miss = lib.emul_ax_slab_range_misses(g["cos_phi"], g["sin_phi"], P(ydet), P(g["z_det"]),
+                                             g["SO"], g["OD"], np.float32(1), nx, ny, nz,
+                                             ydet.size, nb, K, C.byref(kept), C.byref(total))
+        assert miss == 0, (nx, ny, nz, na, nb, SO, OD, det, phi, K)
+        if na == 1024:
+            dropped.append(1 - kept.value / total.value)
+    assert min(dropped) > 0.15, dropped
