@@ -752,25 +752,33 @@ __global__ __launch_bounds__(TX *TY *TZ) void atx_sorted_kernel(ViewGeom g,
 }
 
 // atx_sorted_kernel for views of general rays (ViewGeom::general) with fewer
-// than 4096 bins per detector row and column, with a lighter regrouping:
-//  - a voxel's slot carries two words, its footprint's first bin (a_lo,
-//    b_lo) and its widths with its index in the tile, instead of eight; the
-//    thread that takes the slot recomputes the voxel centre from the index
-//    with the expressions footprint_col / footprint_z use (same bits);
+// than 2048 bins per detector row and column.  A workgroup of 256 threads
+// owns a tile of 512 voxels, two per thread:
+//  - the columns' footprints (footprint_col) and the tile's counters and
+//    barriers are shared by twice the voxels;
+//  - the 512 voxels are regrouped by pair count, largest first, and thread t
+//    walks sorted slots t and t + 256 one after the other, so each wave's
+//    lanes hold footprints of similar size in both walks;
+//  - a slot carries two words, the footprint's first bin (a_lo, b_lo) and its
+//    widths with the voxel's index in the tile, instead of eight; the thread
+//    that takes the slot recomputes the voxel centre from the index with the
+//    expressions footprint_col / footprint_z use (same bits);
 //  - the tile index is split by multiply-high division (FastDiv) on the
 //    scalar unit, and the class counters are cleared before the column
-//    barrier, one barrier fewer;
-//  - the walk takes one pair per load batch, evaluated without a mode
+//    barrier;
+//  - each walk takes one pair per load batch, evaluated without a mode
 //    dispatch (atx_voxel_walk_general).
 // Each voxel's terms and their order are those of atx_sorted_kernel.
 template <int TX, int TY, int TZ, class Epi, bool KAHAN = false>
-__global__ __launch_bounds__(TX *TY *TZ) void atx_sorted_general_kernel(
+__global__ __launch_bounds__(256) void atx_sorted_general_kernel(
     ViewGeom g, const float *__restrict__ Y, const RayRec *__restrict__ rt, Epi epi,
     FastDiv div_ntx, FastDiv div_nty)
 {
 #pragma clang fp contract(off)
     constexpr int T = TX * TY * TZ;
-    static_assert(T == 256, "one voxel per thread, tile index in 8 bits");
+    static_assert(T == 512 && TX * TY <= 256 && 256 % (TX * TY) == 0,
+                  "two voxels per thread, tile index in 9 bits");
+    constexpr int ZH = 256 / (TX * TY);  // slices between a thread's two voxels
     __shared__ int s_cnt[SORT_CLASSES], s_off[SORT_CLASSES];
     __shared__ unsigned s_w0[T], s_w1[T];
     __shared__ FootCol s_col[TX * TY];
@@ -779,24 +787,33 @@ __global__ __launch_bounds__(TX *TY *TZ) void atx_sorted_general_kernel(
     const unsigned t = xcd_block(blockIdx.x, gridDim.x);
     const unsigned tq = fastdiv(t, div_ntx), tile_x = t - tq * ntx;
     const unsigned tile_z = fastdiv(tq, div_nty), tile_y = tq - tile_z * nty;
-    const int lx = (int)(threadIdx.x % TX), ly = (int)((threadIdx.x / TX) % TY),
+    const int ix = (int)tile_x * TX + (int)(threadIdx.x % TX),
+              iy = (int)tile_y * TY + (int)((threadIdx.x / TX) % TY),
               lz = (int)(threadIdx.x / (TX * TY));
-    const int ix = (int)tile_x * TX + lx, iy = (int)tile_y * TY + ly, iz = (int)tile_z * TZ + lz;
-    const bool valid = ix < g.nx && iy < g.ny && iz < g.nz;
     if (threadIdx.x < SORT_CLASSES) s_cnt[threadIdx.x] = 0;
     if ((int)threadIdx.x < TX * TY && ix < g.nx && iy < g.ny)
         s_col[threadIdx.x] = footprint_col(g, ix, iy);
     __syncthreads();
-    Footprint f;
-    f.a_lo = f.b_lo = 0;
-    f.a_hi = f.b_hi = -1;
-    if (valid) f = footprint_z(g, s_col[threadIdx.x % (TX * TY)], iz);
-    const bool nonempty = f.a_lo <= f.a_hi && f.b_lo <= f.b_hi;
-    const unsigned wa = nonempty ? (unsigned)(f.a_hi - f.a_lo + 1) : 0u,
-                   wb = nonempty ? (unsigned)(f.b_hi - f.b_lo + 1) : 0u;
-    const int np = (int)(wa * wb);
-    const int cls = np < SORT_CLASSES - 1 ? np : SORT_CLASSES - 1;
-    const int pos = atomicAdd(&s_cnt[cls], 1);
+    unsigned w0[2], w1[2];
+    int cls[2], pos[2];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const int iz = (int)tile_z * TZ + lz + h * ZH;
+        const bool valid = ix < g.nx && iy < g.ny && iz < g.nz;
+        Footprint f;
+        f.a_lo = f.b_lo = 0;
+        f.a_hi = f.b_hi = -1;
+        if (valid) f = footprint_z(g, s_col[threadIdx.x % (TX * TY)], iz);
+        const bool nonempty = f.a_lo <= f.a_hi && f.b_lo <= f.b_hi;
+        const unsigned wa = nonempty ? (unsigned)(f.a_hi - f.a_lo + 1) : 0u,
+                       wb = nonempty ? (unsigned)(f.b_hi - f.b_lo + 1) : 0u;
+        const int np = (int)(wa * wb);
+        cls[h] = np < SORT_CLASSES - 1 ? np : SORT_CLASSES - 1;
+        pos[h] = atomicAdd(&s_cnt[cls[h]], 1);
+        // voxels outside the volume: first bin word all ones (a_lo < 2047)
+        w0[h] = valid ? (unsigned)f.a_lo | (unsigned)f.b_lo << 16 : ~0u;
+        w1[h] = (threadIdx.x + h * 256u) | wa << 9 | wb << 20;
+    }
     __syncthreads();
     if (threadIdx.x < 64) {
         // exclusive offsets of the classes, largest class first
@@ -811,41 +828,45 @@ __global__ __launch_bounds__(TX *TY *TZ) void atx_sorted_general_kernel(
         if (lane < SORT_CLASSES) s_off[c] = incl - n;
     }
     __syncthreads();
-    const int slot = s_off[cls] + pos;
-    // voxels outside the volume: first bin word all ones (a_lo <= na - 1 < 4095)
-    s_w0[slot] = valid ? (unsigned)f.a_lo | (unsigned)f.b_lo << 16 : ~0u;
-    s_w1[slot] = threadIdx.x | wa << 8 | wb << 20;
-    __syncthreads();
-    const unsigned w0 = s_w0[threadIdx.x], w1 = s_w1[threadIdx.x];
-    if (w0 == ~0u) return;
-    const int me = (int)(w1 & 255u);
-    const int jx = (int)tile_x * TX + me % TX, jy = (int)tile_y * TY + (me / TX) % TY,
-              jz = (int)tile_z * TZ + me / (TX * TY);
-    const unsigned vv = (unsigned)(((size_t)jz * g.ny + jy) * g.nx + jx);
-    const typename Epi::Pre pre = epi.load(vv);
-    Footprint fs;
-    fs.a_lo = (int)(w0 & 0xffffu);
-    fs.b_lo = (int)(w0 >> 16);
-    fs.a_hi = fs.a_lo + (int)((w1 >> 8) & 4095u) - 1;
-    fs.b_hi = fs.b_lo + (int)(w1 >> 20) - 1;
-    // the voxel centre as footprint_col and footprint_z form it
-    const float px = g.cos_phi * (-g.SO) - voxel_xc(g, jx),
-                py = g.sin_phi * (-g.SO) - voxel_yc(g, jy), pz = 0.0f - voxel_zc(g, jz);
-    Acc<KAHAN> acc, accl;
-    if (fs.a_lo <= fs.a_hi && fs.b_lo <= fs.b_hi) {
-        const unsigned M = (unsigned)g.na * (unsigned)g.nb;  // < 2^24
-        atx_voxel_walk_general<KAHAN, 1>(g, BinsBuf(rt, Y, M), fs, px, py, pz, g.dz / 2.0f, acc,
-                                         accl);
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const int slot = s_off[cls[h]] + pos[h];
+        s_w0[slot] = w0[h];
+        s_w1[slot] = w1[h];
     }
-    if (!stop) epi(vv, acc.value(), accl.value(), pre);
+    __syncthreads();
+    const unsigned M = (unsigned)g.na * (unsigned)g.nb;  // < 2^22
+    const BinsBuf B(rt, Y, M);
+#pragma unroll 1
+    for (int h = 0; h < 2; h++) {
+        const unsigned a0 = s_w0[threadIdx.x + h * 256], a1 = s_w1[threadIdx.x + h * 256];
+        if (a0 == ~0u) continue;
+        const int me = (int)(a1 & 511u);
+        const int jx = (int)tile_x * TX + me % TX, jy = (int)tile_y * TY + (me / TX) % TY,
+                  jz = (int)tile_z * TZ + me / (TX * TY);
+        const unsigned vv = (unsigned)(((size_t)jz * g.ny + jy) * g.nx + jx);
+        const typename Epi::Pre pre = epi.load(vv);
+        Footprint fs;
+        fs.a_lo = (int)(a0 & 0xffffu);
+        fs.b_lo = (int)(a0 >> 16);
+        fs.a_hi = fs.a_lo + (int)((a1 >> 9) & 2047u) - 1;
+        fs.b_hi = fs.b_lo + (int)(a1 >> 20) - 1;
+        // the voxel centre as footprint_col and footprint_z form it
+        const float px = g.cos_phi * (-g.SO) - voxel_xc(g, jx),
+                    py = g.sin_phi * (-g.SO) - voxel_yc(g, jy), pz = 0.0f - voxel_zc(g, jz);
+        Acc<KAHAN> acc, accl;
+        if (fs.a_lo <= fs.a_hi && fs.b_lo <= fs.b_hi)
+            atx_voxel_walk_general<KAHAN, 1>(g, B, fs, px, py, pz, g.dz / 2.0f, acc, accl);
+        if (!stop) epi(vv, acc.value(), accl.value(), pre);
+    }
 }
 
 // Whether atx_sorted_general_kernel may take a view: general rays, footprint
-// bounds and widths below 4096 (two 16-bit bin indices and two 12-bit widths
-// per slot word), and the whole volume (no z-window).
+// bounds below 2048 (16-bit first bins, 11- and 12-bit widths in the slot
+// words), and the whole volume (no z-window).
 __host__ __device__ __forceinline__ bool atx_general_ok(const ViewGeom &g)
 {
-    return g.general && g.na < 4096 && g.nb < 4096 && g.z0 == 0 && g.nzs == g.nz;
+    return g.general && g.na < 2048 && g.nb < 2048 && g.z0 == 0 && g.nzs == g.nz;
 }
 
 // Backprojector on voxel tiles (TX x TY x TZ = 256 voxels per workgroup):
@@ -978,19 +999,22 @@ int dispatch_atx_variant(int variant, const ViewGeom &g, const float *Y, const R
     case ATX_SORTED16:
     case ATX_SORTED8: {
         const bool t16 = variant == ATX_SORTED16;
-        const TileDims td = t16 ? tile_dims(g, 16, 16, 1) : tile_dims(g, 8, 8, 4);
-        const unsigned nt = (unsigned)td.ntx * td.nty * td.ntz;
         if (atx_general_ok(g)) {
-            const FastDiv dx = make_fastdiv((unsigned)td.ntx), dy = make_fastdiv((unsigned)td.nty);
+            // the same column footprints over twice the slices
+            const TileDims tg = t16 ? tile_dims(g, 16, 16, 2) : tile_dims(g, 8, 8, 8);
+            const unsigned ng = (unsigned)tg.ntx * tg.nty * tg.ntz;
+            const FastDiv dx = make_fastdiv((unsigned)tg.ntx), dy = make_fastdiv((unsigned)tg.nty);
             if (t16)
-                klaunch(atx_sorted_general_kernel<16, 16, 1, Epi, KAHAN>, nt, 256, 0, s, g, Y, rt,
+                klaunch(atx_sorted_general_kernel<16, 16, 2, Epi, KAHAN>, ng, 256, 0, s, g, Y, rt,
                         epi, dx, dy);
             else
-                klaunch(atx_sorted_general_kernel<8, 8, 4, Epi, KAHAN>, nt, 256, 0, s, g, Y, rt,
+                klaunch(atx_sorted_general_kernel<8, 8, 8, Epi, KAHAN>, ng, 256, 0, s, g, Y, rt,
                         epi, dx, dy);
             ASD_LAUNCH_CHECK("atx_sorted_general_kernel");
             return ASD_OK;
         }
+        const TileDims td = t16 ? tile_dims(g, 16, 16, 1) : tile_dims(g, 8, 8, 4);
+        const unsigned nt = (unsigned)td.ntx * td.nty * td.ntz;
         if (t16)
             klaunch(atx_sorted_kernel<16, 16, 1, Epi, KAHAN, 2>, nt, 256, 0, s, g, Y, rt, epi);
         else
@@ -1001,9 +1025,10 @@ int dispatch_atx_variant(int variant, const ViewGeom &g, const float *Y, const R
     case ATX_SORTED8Z: {
         const TileDims td = tile_dims(g, 8, 4, 8);
         if (atx_general_ok(g)) {
-            klaunch(atx_sorted_general_kernel<8, 4, 8, Epi, KAHAN>,
-                    (unsigned)td.ntx * td.nty * td.ntz, 256, 0, s, g, Y, rt, epi,
-                    make_fastdiv((unsigned)td.ntx), make_fastdiv((unsigned)td.nty));
+            const TileDims tg = tile_dims(g, 8, 4, 16);
+            klaunch(atx_sorted_general_kernel<8, 4, 16, Epi, KAHAN>,
+                    (unsigned)tg.ntx * tg.nty * tg.ntz, 256, 0, s, g, Y, rt, epi,
+                    make_fastdiv((unsigned)tg.ntx), make_fastdiv((unsigned)tg.nty));
             ASD_LAUNCH_CHECK("atx_sorted_general_kernel");
             return ASD_OK;
         }

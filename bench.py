@@ -44,7 +44,7 @@ ATX_VARIANTS = ["direct", "tile16x16x1", "tile8x8x4", "tile16x4x4", "pairs", "di
 
 def atx_kernel_name(variant, kahan, precision=0, three_d=True, general=False):
     """The backprojector kernel a plan launches for its views.  general: the
-    views' rays are all general rays (cone views with fewer than 4096 bins per
+    views' rays are all general rays (cone views with fewer than 2048 bins per
     detector axis; a view with an a == 0 or b == 0 ray, which none of the
     workloads has, keeps atx_sorted_kernel), where the sorted variants run
     atx_sorted_general_kernel."""
@@ -56,7 +56,7 @@ def atx_kernel_name(variant, kahan, precision=0, three_d=True, general=False):
         return atx_kernel_name(variant, kahan, 0, three_d, general).replace(
             "EpiSartAtx,", "EpiSartAtx64,").replace("EpiSartAtx>", "EpiSartAtx64>")
     if general and variant in ("sorted16", "sorted8x8x4", "sorted8x4x8"):
-        t = {"sorted16": "16, 16, 1", "sorted8x8x4": "8, 8, 4", "sorted8x4x8": "8, 4, 8"}[variant]
+        t = {"sorted16": "16, 16, 2", "sorted8x8x4": "8, 8, 8", "sorted8x4x8": "8, 4, 16"}[variant]
         return f"atx_sorted_general_kernel<{t}, EpiSartAtx, {k}>"
     return {"direct": f"atx_kernel<EpiSartAtx, {k}, 4>",
             "tile16x16x1": f"atx_tile_plain_kernel<16, 16, 1, {k}, EpiSartAtx>",
@@ -281,7 +281,7 @@ def headline(args):
     lpr, rows, _ = plan.info()
     names = {0: ax_kernel_name(lpr, rows, precision),
              1: atx_kernel_name(tuning["atx_variant"], precision in (1, 2, 3), precision,
-                                p["nz"] > 1, p["nz"] > 1 and max(p["na"], p["nb"]) < 4096),
+                                p["nz"] > 1, p["nz"] > 1 and max(p["na"], p["nb"]) < 2048),
              2: "TV descent (tv_grad_kernel x ng + tv_step_kernel)"}
     per_stage = {}
     for s, (ms, cnt) in stage.items():
