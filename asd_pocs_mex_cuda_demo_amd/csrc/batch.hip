@@ -186,7 +186,8 @@ __global__ __launch_bounds__(256) void atx_batch_kernel(const BItem *__restrict_
     const float sx = g.cos_phi * (-g.SO), sy = g.sin_phi * (-g.SO);
     const float px = sx - f.xc, py = sy - f.yc, pz = 0.0f - f.zc;
     Acc<KAHAN> acc, accl;
-    atx_voxel_walk<KAHAN, CHUNK>(g, it.cor, it.rt, f, px, py, pz, g.dz / 2.0f, acc, accl);
+    // fan views only
+    atx_voxel_walk<KAHAN, CHUNK, true>(g, it.cor, it.rt, f, px, py, pz, g.dz / 2.0f, acc, accl);
     float bp = acc.value() * g.scale, sc = accl.value() * g.scale;
     double cc = (sc == 0.0f) ? 1.0 : (double)sc;
     double nm = (double)m0 + it.ctl->beta * ((double)bp / cc);

@@ -871,26 +871,40 @@ __host__ __device__ __forceinline__ float chord3_sides(const RayRec &R, float px
     return two ? l : 0.0f;
 }
 
+// FAN: for kernels that only see fan views, where every ray is flat or takes
+// an a == 0 / b == 0 fallback: flat rays are tested first and general rays
+// take the ordered rule (chord3_general), so the two-sided chord is not
+// compiled in.  Same lengths either way.
+template <bool FAN = false>
 __host__ __device__ __forceinline__ float find_l_3d_rec(const RayRec &R, float px, float py, float pz,
                                                float hz, float dz)
 {
 #pragma clang fp contract(off)
     const float hx = 0.5f, hy = 0.5f;
+    if constexpr (FAN) {
+        if (R.mode >= RAY_FLAT_X)
+            return chord2(px, py, R.s_ba, R.s_ab, R.s_ba * hx, R.s_ab * hy, hx, hy,
+                          R.mode == RAY_FLAT_X);
+        if (R.mode <= RAY_Y_FIRST)
+            return chord3_general<true>(R, R.mode == RAY_X_FIRST, px, py, pz, hz);
+    }
     // the general rays first, behind one test (the other modes are the
     // a == 0 / b == 0 fallbacks, which only views without a footprint cut
     // have, general rays whose face offsets are inexact products, and rays in
     // the face of a one-slice volume)
     if (R.mode <= RAY_Y_FIRST) return chord3_sides(R, px, py, pz, hz);
+    // A flat ray (c == 0 in a one-slice volume, pz = -hz; every ray of a fan
+    // view), next behind one test: s_ca = s_cb = +-0 put both hits' z at pz,
+    // so every x / y face test's z condition holds (|pz| == hz) and ez = 0;
+    // s_ac, s_bc = +-inf make the z-face tests NaN / inf comparisons that
+    // fail.  What is left is the in-plane chord, evaluated by the same
+    // expressions in the same order.
+    if (R.mode >= RAY_FLAT_X)
+        return chord2(px, py, R.s_ba, R.s_ab, R.s_ba * hx, R.s_ab * hy, hx, hy,
+                      R.mode == RAY_FLAT_X);
     switch (R.mode) {
     case RAY_X_FIRST_P: return chord3_general<false>(R, true, px, py, pz, hz);
     case RAY_Y_FIRST_P: return chord3_general<false>(R, false, px, py, pz, hz);
-    // A flat ray (c == 0 in a one-slice volume, pz = -hz): s_ca = s_cb = +-0
-    // put both hits' z at pz, so every x / y face test's z condition holds
-    // (|pz| == hz) and ez = 0; s_ac, s_bc = +-inf make the z-face tests
-    // NaN / inf comparisons that fail.  What is left is the in-plane
-    // chord, evaluated by the same expressions in the same order.
-    case RAY_FLAT_X: return chord2(px, py, R.s_ba, R.s_ab, R.s_ba * hx, R.s_ab * hy, hx, hy, true);
-    case RAY_FLAT_Y: return chord2(px, py, R.s_ba, R.s_ab, R.s_ba * hx, R.s_ab * hy, hx, hy, false);
     case RAY_A0_B0: return absf(py) <= hy ? dz : 0.0f;
     case RAY_A0_C0: return absf(pz) <= hz ? 1.0f : 0.0f;
     case RAY_A0_YZ_U: return chord2(py, pz, R.s_cb, R.s_bc, R.s_cb * hy, R.s_bc * hz, hy, hz, true);

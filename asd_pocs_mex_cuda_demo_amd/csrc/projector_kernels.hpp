@@ -465,14 +465,18 @@ __host__ __device__ __forceinline__ void atx_voxel_walk_general(const ViewGeom &
     }
 }
 
-template <bool KAHAN, int CHUNK, class Bins>
+// FAN: the kernel only ever sees fan views (the batch), whose rays are flat
+// or take the a == 0 / b == 0 fallbacks: it leaves out the walk for views of
+// general rays and tests flat rays first (find_l_3d_rec<true>), and with them
+// the registers the general chord would hold.
+template <bool KAHAN, int CHUNK, bool FAN, class Bins>
 __device__ __forceinline__ void atx_voxel_walk_t(const ViewGeom &g, const Bins &B,
                                                  const Footprint &f, float px, float py,
                                                  float pz, float hz, Acc<KAHAN> &acc,
                                                  Acc<KAHAN> &accl)
 {
 #pragma clang fp contract(off)
-    if (g.general) {
+    if (!FAN && g.general) {
         atx_voxel_walk_general<KAHAN, CHUNK>(g, B, f, px, py, pz, hz, acc, accl);
         return;
     }
@@ -500,7 +504,7 @@ __device__ __forceinline__ void atx_voxel_walk_t(const ViewGeom &g, const Bins &
 #pragma unroll
         for (int k = 0; k < CHUNK; k++) {
             if (k0 + k >= npairs || (!g.tight && !inplane_may_hit(R[k], px, py))) continue;
-            float l = find_l_3d_rec(R[k], px, py, pz, hz, g.dz);
+            float l = find_l_3d_rec<FAN>(R[k], px, py, pz, hz, g.dz);
             acc.add(l * yv[k]);
             accl.add(l);
         }
@@ -509,7 +513,7 @@ __device__ __forceinline__ void atx_voxel_walk_t(const ViewGeom &g, const Bins &
 
 // The footprint walk of one voxel (shared by atx_kernel, atx_sorted_kernel
 // and the fallback of atx_pairs_kernel), CHUNK pairs per load batch.
-template <bool KAHAN, int CHUNK = ATX_CHUNK>
+template <bool KAHAN, int CHUNK = ATX_CHUNK, bool FAN = false>
 __device__ __forceinline__ void atx_voxel_walk(const ViewGeom &g, const float *__restrict__ Y,
                                                const RayRec *__restrict__ rt, const Footprint &f,
                                                float px, float py, float pz, float hz,
@@ -518,12 +522,12 @@ __device__ __forceinline__ void atx_voxel_walk(const ViewGeom &g, const float *_
     if (!(f.a_lo <= f.a_hi && f.b_lo <= f.b_hi)) return;
     const unsigned M = (unsigned)g.na * (unsigned)g.nb;  // < 2^31 (index_fits)
     if (M < (1u << 27))
-        atx_voxel_walk_t<KAHAN, CHUNK>(g, BinsBuf(rt, Y, M), f, px, py, pz, hz, acc, accl);
+        atx_voxel_walk_t<KAHAN, CHUNK, FAN>(g, BinsBuf(rt, Y, M), f, px, py, pz, hz, acc, accl);
     else
-        atx_voxel_walk_t<KAHAN, CHUNK>(g, BinsPtr{rt, Y}, f, px, py, pz, hz, acc, accl);
+        atx_voxel_walk_t<KAHAN, CHUNK, FAN>(g, BinsPtr{rt, Y}, f, px, py, pz, hz, acc, accl);
 }
 
-template <class Epi, bool KAHAN = false, int CHUNK = ATX_CHUNK>
+template <class Epi, bool KAHAN = false, int CHUNK = ATX_CHUNK, bool FAN = false>
 __global__ __launch_bounds__(256) void atx_kernel(ViewGeom g, const float *__restrict__ Y,
                                                   const RayRec *__restrict__ rt, Epi epi)
 {
@@ -543,7 +547,7 @@ __global__ __launch_bounds__(256) void atx_kernel(ViewGeom g, const float *__res
     const float sx = g.cos_phi * (-g.SO), sy = g.sin_phi * (-g.SO);
     const float px = sx - f.xc, py = sy - f.yc, pz = 0.0f - f.zc;
     Acc<KAHAN> acc, accl;
-    atx_voxel_walk<KAHAN, CHUNK>(g, Y, rt, f, px, py, pz, g.dz / 2.0f, acc, accl);
+    atx_voxel_walk<KAHAN, CHUNK, FAN>(g, Y, rt, f, px, py, pz, g.dz / 2.0f, acc, accl);
     if (!stop) epi(v, acc.value(), accl.value(), pre);
 }
 
@@ -992,7 +996,13 @@ int dispatch_atx_variant(int variant, const ViewGeom &g, const float *Y, const R
     case ATX_T16x4x4: return launch_atx_tile_t<16, 4, 4, KAHAN>(g, Y, rt, epi, s);
     case ATX_DIRECT2: {
         const long N = (long)g.nx * g.ny * g.nzs;
-        klaunch(atx_kernel<Epi, KAHAN, 2>, (unsigned)((N + 255) / 256), 256, 0, s, g, Y, rt, epi);
+        // one-slice views: the walk compiled for fan rays (find_l_3d_rec<true>)
+        if (g.nz == 1)
+            klaunch(atx_kernel<Epi, KAHAN, 2, true>, (unsigned)((N + 255) / 256), 256, 0, s, g, Y,
+                    rt, epi);
+        else
+            klaunch(atx_kernel<Epi, KAHAN, 2>, (unsigned)((N + 255) / 256), 256, 0, s, g, Y, rt,
+                    epi);
         ASD_LAUNCH_CHECK("atx_kernel");
         return ASD_OK;
     }
@@ -1039,7 +1049,12 @@ int dispatch_atx_variant(int variant, const ViewGeom &g, const float *Y, const R
     }
     case ATX_DIRECT1: {
         const long N = (long)g.nx * g.ny * g.nzs;
-        klaunch(atx_kernel<Epi, KAHAN, 1>, (unsigned)((N + 255) / 256), 256, 0, s, g, Y, rt, epi);
+        if (g.nz == 1)
+            klaunch(atx_kernel<Epi, KAHAN, 1, true>, (unsigned)((N + 255) / 256), 256, 0, s, g, Y,
+                    rt, epi);
+        else
+            klaunch(atx_kernel<Epi, KAHAN, 1>, (unsigned)((N + 255) / 256), 256, 0, s, g, Y, rt,
+                    epi);
         ASD_LAUNCH_CHECK("atx_kernel");
         return ASD_OK;
     }
@@ -1051,7 +1066,11 @@ int dispatch_atx_variant(int variant, const ViewGeom &g, const float *Y, const R
     }
     default: {
         const long N = (long)g.nx * g.ny * g.nzs;
-        klaunch(atx_kernel<Epi, KAHAN>, (unsigned)((N + 255) / 256), 256, 0, s, g, Y, rt, epi);
+        if (g.nz == 1)
+            klaunch(atx_kernel<Epi, KAHAN, ATX_CHUNK, true>, (unsigned)((N + 255) / 256), 256, 0,
+                    s, g, Y, rt, epi);
+        else
+            klaunch(atx_kernel<Epi, KAHAN>, (unsigned)((N + 255) / 256), 256, 0, s, g, Y, rt, epi);
         ASD_LAUNCH_CHECK("atx_kernel");
         return ASD_OK;
     }

@@ -55,6 +55,10 @@ def atx_kernel_name(variant, kahan, precision=0, three_d=True, general=False):
     if precision == 4:  # FP64 image and its FP32 shadow
         return atx_kernel_name(variant, kahan, 0, three_d, general).replace(
             "EpiSartAtx,", "EpiSartAtx64,").replace("EpiSartAtx>", "EpiSartAtx64>")
+    if not three_d and variant in ("direct", "direct2", "direct1"):
+        # one-slice views: the walk compiled for fan rays
+        c = {"direct": 4, "direct2": 2, "direct1": 1}[variant]
+        return f"atx_kernel<EpiSartAtx, {k}, {c}, true>"
     if general and variant in ("sorted16", "sorted8x8x4", "sorted8x4x8"):
         t = {"sorted16": "16, 16, 2", "sorted8x8x4": "8, 8, 8", "sorted8x4x8": "8, 4, 16"}[variant]
         return f"atx_sorted_general_kernel<{t}, EpiSartAtx, {k}>"
