@@ -434,10 +434,13 @@ struct BinsPtr {
 template <bool KAHAN, int CHUNK, class Bins>
 __host__ __device__ __forceinline__ void atx_voxel_walk_general(const ViewGeom &g, const Bins &B,
                                                        const Footprint &f, float px, float py,
-                                                       float pz, float hz, Acc<KAHAN> &acc,
+                                                       float pz, Acc<KAHAN> &acc,
                                                        Acc<KAHAN> &accl)
 {
 #pragma clang fp contract(off)
+    // a view of general rays has dz == 1 (view_general): half a voxel on
+    // every axis, as a compile-time constant
+    constexpr float hz = 0.5f;
     const int npairs = (f.a_hi - f.a_lo + 1) * (f.b_hi - f.b_lo + 1);
     int ia = f.a_lo;
     unsigned idx = mad24((unsigned)f.b_lo, (unsigned)g.na, (unsigned)f.a_lo);
@@ -477,7 +480,7 @@ __device__ __forceinline__ void atx_voxel_walk_t(const ViewGeom &g, const Bins &
 {
 #pragma clang fp contract(off)
     if (!FAN && g.general) {
-        atx_voxel_walk_general<KAHAN, CHUNK>(g, B, f, px, py, pz, hz, acc, accl);
+        atx_voxel_walk_general<KAHAN, CHUNK>(g, B, f, px, py, pz, acc, accl);
         return;
     }
     const int npairs = (f.a_hi - f.a_lo + 1) * (f.b_hi - f.b_lo + 1);
@@ -860,7 +863,7 @@ __global__ __launch_bounds__(256) void atx_sorted_general_kernel(
                     py = g.sin_phi * (-g.SO) - voxel_yc(g, jy), pz = 0.0f - voxel_zc(g, jz);
         Acc<KAHAN> acc, accl;
         if (fs.a_lo <= fs.a_hi && fs.b_lo <= fs.b_hi)
-            atx_voxel_walk_general<KAHAN, 1>(g, B, fs, px, py, pz, g.dz / 2.0f, acc, accl);
+            atx_voxel_walk_general<KAHAN, 1>(g, B, fs, px, py, pz, acc, accl);
         if (!stop) epi(vv, acc.value(), accl.value(), pre);
     }
 }

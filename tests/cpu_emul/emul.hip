@@ -301,7 +301,7 @@ void emul_atx(float *X, const float *y, float *sc, float c, float s, const float
         if (g.general) {
             // the kernels' walk for views of general rays (two pairs per load batch)
             if (f.a_lo <= f.a_hi && f.b_lo <= f.b_hi)
-                atx_voxel_walk_general<false, 2>(g, BinsPtr{rt, y}, f, px, py, pz, hz, acc, accl);
+                atx_voxel_walk_general<false, 2>(g, BinsPtr{rt, y}, f, px, py, pz, acc, accl);
             X[v] = acc.value() * scale;
             sc[v] = accl.value() * scale;
             continue;
