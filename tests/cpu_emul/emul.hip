@@ -131,6 +131,45 @@ long emul_ax_slab_range_misses(float c, float s, const float *ydet, const float 
     return miss;
 }
 
+// The footprint cut against the reference's footprint on nsample voxels of
+// one view (drawn with a fixed LCG from seed): returns the number of pairs
+// with a nonzero length (find_l_3d_rec, the reference's value) that lie
+// outside the cut footprint (must be 0); *kept receives the pairs the cut
+// keeps and *nonzero the pairs with a nonzero length.
+long emul_footprint_cut_check(float c, float s, const float *ydet, const float *zdet, float SO,
+                              float OD, float dyd, float dzd, int nx, int ny, int nz, int na,
+                              int nb, int nsample, unsigned seed, long *kept, long *nonzero)
+{
+    ViewGeom g = mk(c, s, SO, OD, 1.0f, dyd, dzd, 1.0f, nx, ny, nz, na, nb);
+    const int tight = view_tight(g, ydet);
+    long miss = 0;
+    *kept = *nonzero = 0;
+    for (int k = 0; k < nsample; k++) {
+        seed = seed * 1664525u + 1013904223u;
+        const int ix = (int)((seed >> 8) % (unsigned)nx);
+        seed = seed * 1664525u + 1013904223u;
+        const int iy = (int)((seed >> 8) % (unsigned)ny);
+        seed = seed * 1664525u + 1013904223u;
+        const int iz = (int)((seed >> 8) % (unsigned)nz);
+        g.tight = 0;
+        const Footprint fr = footprint(g, ix, iy, iz);
+        g.tight = tight;
+        const Footprint fc = footprint(g, ix, iy, iz);
+        if (fc.a_lo <= fc.a_hi && fc.b_lo <= fc.b_hi)
+            *kept += (long)(fc.a_hi - fc.a_lo + 1) * (fc.b_hi - fc.b_lo + 1);
+        const float px = g.cos_phi * (-g.SO) - fr.xc, py = g.sin_phi * (-g.SO) - fr.yc,
+                    pz = 0.0f - fr.zc;
+        for (int ib = fr.b_lo; ib <= fr.b_hi; ib++)
+            for (int ia = fr.a_lo; ia <= fr.a_hi; ia++) {
+                const RayRec R = make_ray_rec(g, ydet[ia], zdet[ib]);
+                if (find_l_3d_rec(R, px, py, pz, g.dz / 2.0f, g.dz) == 0.0f) continue;
+                ++*nonzero;
+                if (ia < fc.a_lo || ia > fc.a_hi || ib < fc.b_lo || ib > fc.b_hi) miss++;
+            }
+    }
+    return miss;
+}
+
 // view_general for a view (after view_tight, as the plans set them).
 int emul_view_general(float c, float s, const float *ydet, const float *zdet, float SO, float OD,
                       float dyd, float dzd, float dz, int nx, int ny, int nz, int na, int nb)

@@ -410,3 +410,75 @@ def test_forward_slab_range_keeps_every_slab_with_a_piece(orc):
         if na == 1024:
             dropped.append(1 - kept.value / total.value)
     assert min(dropped) > 0.15, dropped
+
+
+def test_footprint_cut_keeps_every_nonzero_pair_at_512_cubed(orc):
+    """At the 512^3 workload's geometry (a 1200-voxel source distance, where
+    find_l_3d's FP32 rounding is largest), the cut footprint holds every
+    (voxel, bin) pair whose reference length is nonzero, on 20000 voxels of
+    each of eight views, and keeps at most 10% more pairs than those."""
+    lib = emul()
+    f, i, vp = C.c_float, C.c_int, C.c_void_p
+    lib.emul_footprint_cut_check.restype = C.c_long
+    lib.emul_footprint_cut_check.argtypes = [f, f, vp, vp, f, f, f, f, i, i, i, i, i, i,
+                                             C.c_uint, vp, vp]
+    kept_all = nonzero_all = 0
+    for k, deg in enumerate([0.5, 17.3, 37.3, 45.0, 90.5, 133.0, 211.7, 300.1]):
+        g = orc.gateway_geometry(-deg / 180 * np.pi, 600.0, 500.0, 0.5, 0.6, 1024, 768)
+        kept, nonzero = C.c_long(), C.c_long()
+        miss = lib.emul_footprint_cut_check(g["cos_phi"], g["sin_phi"], P(g["y_det"]),
+                                            P(g["z_det"]), g["SO"], g["OD"], g["det"], g["det"],
+                                            512, 512, 512, 1024, 768, 20000, 11 + k,
+                                            C.byref(kept), C.byref(nonzero))
+        assert miss == 0, deg
+        kept_all += kept.value
+        nonzero_all += nonzero.value
+    assert nonzero_all > 8 * 20000 * 2 and kept_all <= 1.10 * nonzero_all, (kept_all, nonzero_all)
+
+
+def test_face_tests_credit_only_near_misses(orc):
+    """The premise of the footprint cut's 1e-2 voxel margin: rays aimed at
+    voxel edges and corners from a 1200-voxel source distance (detector
+    points moved by up to 3e-3), on the reference's voxel grid, get a nonzero
+    length from find_l_3d only where the exact FP64 ray misses the voxel by
+    less than 1e-3 voxel (observed: about 6e-5)."""
+    lib = emul()
+    lib.emul_find_l_3d_pairs.argtypes = [C.c_long] + [C.c_void_p] * 9 + [C.c_int, C.c_void_p]
+    rng = np.random.default_rng(7)
+    n = 200000
+    phi = rng.uniform(-7, 7, n)
+    c = np.cos(phi).astype(np.float32)
+    s = np.sin(phi).astype(np.float32)
+    SO = np.full(n, 1200, np.float32)
+    OD = np.full(n, 1000, np.float32)
+    Cc = np.floor(rng.uniform(-256, 256, (n, 3))) + 0.5
+    off = rng.choice([-0.5, 0.5], (n, 3))
+    free = rng.integers(0, 4, n)
+    for k in range(3):
+        off[free == k, k] = rng.uniform(-0.5, 0.5, (free == k).sum())
+    S = np.stack([c.astype(np.float64) * -1200, s.astype(np.float64) * -1200, np.zeros(n)], 1)
+    v = Cc + off - S
+    cd, sd = c.astype(np.float64), s.astype(np.float64)
+    t = 2200.0 / (cd * v[:, 0] + sd * v[:, 1])
+    yd = ((-sd * v[:, 0] + cd * v[:, 1]) * t + rng.uniform(-3e-3, 3e-3, n)).astype(np.float32)
+    zd = (v[:, 2] * t + rng.uniform(-3e-3, 3e-3, n)).astype(np.float32)
+    Cf = Cc.astype(np.float32)
+    out = np.zeros(n, np.float32)
+    args = [np.ascontiguousarray(a) for a in (c, s, SO, OD, yd, zd, Cf[:, 0], Cf[:, 1], Cf[:, 2])]
+    lib.emul_find_l_3d_pairs(n, *[a.ctypes.data for a in args], 512, out.ctypes.data)
+    # exact L-inf enlargement of the voxel the ray needs to meet it (bisection)
+    D = np.stack([(c * OD - s * yd).astype(np.float64), (s * OD + c * yd).astype(np.float64),
+                  zd.astype(np.float64)], 1)
+    d = D - S
+    lo, hi = np.full(n, -0.5), np.full(n, 0.1)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        for _ in range(50):
+            mid = (lo + hi) / 2
+            t0 = (Cc - 0.5 - mid[:, None] - S) / d
+            t1 = (Cc + 0.5 + mid[:, None] - S) / d
+            meet = np.minimum(t0, t1).max(1) <= np.maximum(t0, t1).min(1)
+            hi = np.where(meet, mid, hi)
+            lo = np.where(meet, lo, mid)
+    hit = out > 0
+    assert hit.sum() > n // 2 and (hi[hit] > 0).sum() > 10  # some near misses are credited
+    assert hi[hit].max() < 1e-3, hi[hit].max()
