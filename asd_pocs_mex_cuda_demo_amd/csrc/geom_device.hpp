@@ -852,21 +852,23 @@ __host__ __device__ __forceinline__ float chord3_sides(const RayRec &R, float px
     const bool fxx = absf(xy_x) <= hy && absf(xz_x) <= hz;
     const bool fyx = absf(yx_x) <= hx && absf(yz_x) <= hz;
     const bool fzx = absf(zx_x) <= hx && absf(zy_x) <= hy;
-    // exactly one pass on each side
-    const bool one_e = (fxe != fye) != fze && !(fxe && fye && fze);
-    const bool one_x = (fxx != fyx) != fzx && !(fxx && fyx && fzx);
-    const bool any_e = fxe || fye || fze, any_x = fxx || fyx || fzx;
-    const bool two = one_e && one_x;
+    // at least two passes on a side (near an edge or a corner of the voxel),
+    // at least one on each side, and exactly one on each side
+    const bool many_e = (fxe && (fye || fze)) || (fye && fze);
+    const bool many_x = (fxx && (fyx || fzx)) || (fyx && fzx);
+    const bool two = (fxe || fye || fze) && (fxx || fyx || fzx) && !many_e && !many_x;
     const float ex = (fxx ? -hxe : fyx ? yx_x : zx_x) - (fxe ? hxe : fye ? yx_e : zx_e);
     const float ey = (fxx ? xy_x : fyx ? -hye : zy_x) - (fxe ? xy_e : fye ? hye : zy_e);
     const float ez = (fxx ? xz_x : fyx ? yz_x : -hze) - (fxe ? xz_e : fye ? yz_e : hze);
     const float q = ex * ex + ey * ey + ez * ez;
-    // every lane takes the root (of 1 where there is no chord); a chord's q
-    // is finite (|e| <= 1 per axis) and, off the branch below, >= 2^-96
-    const float l = sqrt_finite_ge_tiny(two ? q : 1.0f);
-    // at most one pass in all: no chord
-    const bool none = !(any_e && any_x) && (!any_e || one_e) && (!any_x || one_x);
-    if ((!two && !none) || (two && !(q >= 0x1p-96f)))
+    // every lane takes the root; its value is used only for a chord, whose q
+    // is finite (|e| <= 1 per axis) and, off the branch below, >= 2^-96 (where
+    // there is no chord the argument may be anything: the operations on it
+    // raise no exception and the result is discarded)
+    const float l = sqrt_finite_ge_tiny(q);
+    // several passes on a side, or a chord below 2^-48: the ordered rule.
+    // Otherwise at most one pass in all, or one on each side only (two).
+    if (many_e || many_x || (two && !(q >= 0x1p-96f)))
         return chord3_general<true>(R, R.mode == RAY_X_FIRST, px, py, pz, hz);
     return two ? l : 0.0f;
 }

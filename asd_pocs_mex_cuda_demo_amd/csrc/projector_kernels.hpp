@@ -392,30 +392,44 @@ int dispatch_ax(AxShape sh, const ViewGeom &g, const float *x, const float *y_de
 // share stay in its L2.
 constexpr int ATX_CHUNK = 4;
 
-// Reads of a view's ray records and corrections by bin index.  Buffer form
-// (fewer than 2^27 bins, so every record's byte offset fits 32 bits): one
-// 32-bit shift per read instead of 64-bit address arithmetic.  Pointer form
-// otherwise.
+// Structured buffer loads (llvm.amdgcn.struct.ptr.buffer.load: the element
+// index goes to the instruction's index operand, which the hardware scales by
+// the descriptor's stride).
+using u32x4 = __attribute__((ext_vector_type(4))) unsigned;
+__device__ u32x4 struct_buffer_load_b128(__amdgpu_buffer_rsrc_t rsrc, unsigned index,
+                                         unsigned voffset, unsigned soffset, int aux)
+    __asm("llvm.amdgcn.struct.ptr.buffer.load.v4i32");
+__device__ float struct_buffer_load_f32(__amdgpu_buffer_rsrc_t rsrc, unsigned index,
+                                        unsigned voffset, unsigned soffset, int aux)
+    __asm("llvm.amdgcn.struct.ptr.buffer.load.f32");
+
+// Reads of a view's ray records and corrections by bin index (fewer than
+// 2^27 bins): structured buffer resources with strides of 32 and 4 bytes, so
+// the bin index is the load's operand as it is, with no address arithmetic.
+// Pointer form otherwise.  The record count of each resource is its size in
+// bytes, a bound at least as large as the count of elements; the walks only
+// read bins of the view.
 struct BinsBuf {
     __amdgpu_buffer_rsrc_t rr, ry;
     __device__ __forceinline__ BinsBuf(const RayRec *rt, const float *Y, unsigned M)
     {
+        static_assert(sizeof(RayRec) == 32, "record stride");
         // dword3 0x00020000: 32-bit data format, the form untyped loads use
-        rr = __builtin_amdgcn_make_buffer_rsrc((void *)rt, (short)0, (int)(M * 32u), 0x00020000);
-        ry = __builtin_amdgcn_make_buffer_rsrc((void *)Y, (short)0, (int)(M * 4u), 0x00020000);
+        rr = __builtin_amdgcn_make_buffer_rsrc((void *)rt, (short)32, (int)(M * 32u), 0x00020000);
+        ry = __builtin_amdgcn_make_buffer_rsrc((void *)Y, (short)4, (int)(M * 4u), 0x00020000);
     }
     __device__ __forceinline__ RayRec rec(unsigned i) const
     {
         struct Q {
-            __attribute__((ext_vector_type(4))) unsigned a, b;
+            u32x4 a, b;
         } q;
-        q.a = __builtin_amdgcn_raw_buffer_load_b128(rr, i << 5, 0, 0);
-        q.b = __builtin_amdgcn_raw_buffer_load_b128(rr, i << 5, 16, 0);
+        q.a = struct_buffer_load_b128(rr, i, 0, 0, 0);
+        q.b = struct_buffer_load_b128(rr, i, 16, 0, 0);
         return __builtin_bit_cast(RayRec, q);
     }
     __device__ __forceinline__ float y(unsigned i) const
     {
-        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(ry, i << 2, 0, 0));
+        return struct_buffer_load_f32(ry, i, 0, 0, 0);
     }
 };
 
@@ -452,7 +466,8 @@ __host__ __device__ __forceinline__ void atx_voxel_walk_general(const ViewGeom &
         for (int k = 0; k < CHUNK; k++) {
             R[k] = B.rec(idx);
             yv[k] = B.y(idx);
-            if (k0 + k + 1 < npairs) {
+            // with one pair per batch the cursor past the last pair is never read
+            if (CHUNK == 1 || k0 + k + 1 < npairs) {
                 const bool wrap = ia == f.a_hi;
                 ia = wrap ? f.a_lo : ia + 1;
                 idx += wrap ? jump : 1u;
