@@ -59,14 +59,17 @@ __global__ __launch_bounds__(256) void ax_kernel(ViewGeom g, const float *__rest
                 ax_slabs<U, KAHAN, decltype(ft)::value, VolRead<WIDE>, decltype(flat)::value>(
                     r, A, V, iu0, LPR, nu, g.ny, sa, sl);
         };
-        // fast_t depends on the view only, and every ray of a 2-D view is
-        // flat: one branch for the whole march
+        // fast_t depends on the view only, every ray of a 2-D view is flat,
+        // and no ray of a view of several slices is: one branch for the
+        // whole march
         if (r.fast_t && r.flat)
-            march(std::true_type{}, std::true_type{});
+            march(std::true_type{}, std::integral_constant<int, FLAT_ALL>{});
+        else if (r.fast_t && view_has_no_flat_ray(g))
+            march(std::true_type{}, std::integral_constant<int, FLAT_NONE>{});
         else if (r.fast_t)
-            march(std::true_type{}, std::false_type{});
+            march(std::true_type{}, std::integral_constant<int, FLAT_ANY>{});
         else
-            march(std::false_type{}, std::false_type{});
+            march(std::false_type{}, std::integral_constant<int, FLAT_ANY>{});
     }
     // compensated partials are combined in FP64, plain ones in FP32
     using R = typename std::conditional<KAHAN, double, float>::type;
@@ -154,7 +157,7 @@ __global__ __launch_bounds__(256) void ax_rows_kernel(ViewGeom g, const float *_
         ax_bundle_slabs(r, K, nu, lo, hi);
         const int first = lo + ((lane - lo) % LPR + LPR) % LPR;
         const SlabAddr A = slab_addr(r[0], g.nx, VolRead<WIDE>::unit);
-        auto march = [&](auto ft) {
+        auto march = [&](auto ft, auto flat) {
             for (int iu0 = first; iu0 <= hi; iu0 += U * LPR) {
                 SlabW s[U][K];
                 float xv[U][K][3];
@@ -164,7 +167,8 @@ __global__ __launch_bounds__(256) void ax_rows_kernel(ViewGeom g, const float *_
                     if (iu <= hi) {
                         const SlabT t = ax_slab_t<decltype(ft)::value>(r[0], A, iu);
 #pragma unroll
-                        for (int k = 0; k < K; k++) ax_slab_z(r[k], t, A, g.ny, s[j][k]);
+                        for (int k = 0; k < K; k++)
+                            ax_slab_z<decltype(flat)::value>(r[k], t, A, g.ny, s[j][k]);
                     } else {
 #pragma unroll
                         for (int k = 0; k < K; k++)
@@ -195,10 +199,14 @@ __global__ __launch_bounds__(256) void ax_rows_kernel(ViewGeom g, const float *_
                             }
             }
         };
-        if (r[0].fast_t)
-            march(std::true_type{});
+        // fast_t depends on the view only, and no ray of a view of several
+        // slices is flat: one branch for the whole march
+        if (r[0].fast_t && view_has_no_flat_ray(g))
+            march(std::true_type{}, std::integral_constant<int, FLAT_NONE>{});
+        else if (r[0].fast_t)
+            march(std::true_type{}, std::integral_constant<int, FLAT_ANY>{});
         else
-            march(std::false_type{});
+            march(std::false_type{}, std::integral_constant<int, FLAT_ANY>{});
     }
     using R = typename std::conditional<KAHAN, double, float>::type;
     R a[K], b[K];

@@ -254,6 +254,9 @@ static void emul_ax_t(const float *X, float *y, float *sr, const ViewGeom &g, co
             }
             if (g_general)
                 for (int k = 0; k < rows; k++) r[k].flat = r[k].dz1 = 0;
+            // the kernels' march for views without flat rays (under g_general
+            // every ray takes it, flat ones included)
+            const bool noflat = view_has_no_flat_ray(g) || g_general;
             const int nu = r[0].xdriven ? nx : ny;
             // the buffer form's byte offsets where the kernels use it, else
             // element indices (the same voxels)
@@ -272,6 +275,12 @@ static void emul_ax_t(const float *X, float *y, float *sr, const ViewGeom &g, co
                         else if (r[0].flat)
                             ax_slabs<4, KAHAN, false, VolRead<true>, true>(r[0], A, Vw, iu0, lpr,
                                                                            nu, ny, a[0], l[0]);
+                        else if (noflat && buf)
+                            ax_slabs<4, KAHAN, false, VolRead<false>, FLAT_NONE>(
+                                r[0], A, Vb, iu0, lpr, nu, ny, a[0], l[0]);
+                        else if (noflat)
+                            ax_slabs<4, KAHAN, false, VolRead<true>, FLAT_NONE>(
+                                r[0], A, Vw, iu0, lpr, nu, ny, a[0], l[0]);
                         else if (buf)
                             ax_slabs<4, KAHAN, false>(r[0], A, Vb, iu0, lpr, nu, ny, a[0], l[0]);
                         else
@@ -281,7 +290,10 @@ static void emul_ax_t(const float *X, float *y, float *sr, const ViewGeom &g, co
                         const SlabT t = ax_slab_t(r[0], A, iu);
                         for (int k = 0; k < rows; k++) {
                             SlabW s;
-                            ax_slab_z(r[k], t, A, ny, s);
+                            if (noflat)
+                                ax_slab_z<FLAT_NONE>(r[k], t, A, ny, s);
+                            else
+                                ax_slab_z(r[k], t, A, ny, s);
                             for (int p = 0; p < 3; p++)
                                 if (s.ok[p]) {
                                     const float xv = buf ? Vb(s.off[p], true) : Vw(s.off[p], true);
