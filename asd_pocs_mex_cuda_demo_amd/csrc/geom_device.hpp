@@ -61,6 +61,14 @@ __host__ __device__ __forceinline__ float div_by_tmp(const AxRay &r, float x)
     return fmaf(res, r.rtmp, q0);
 }
 
+// sqrt(x / tmp), the slab length of cu:402: the argument is >= 1, and
+// finite where mk holds; other rays keep the fixup for +inf.
+__host__ __device__ __forceinline__ float slab_root(const AxRay &r, float x)
+{
+    const float v = div_by_tmp(r, x);
+    return r.mk ? sqrt_finite_ge_tiny(v) : sqrt_ge_tiny(v);
+}
+
 // (c - a) / (b - a) for a slab's cell crossing, bit for bit the IEEE
 // quotient: the compiler's expansion of the FP32 division without its
 // operand scaling and special-value fixup, which leave the result unchanged
@@ -479,9 +487,9 @@ __host__ __device__ __forceinline__ void ax_slab_z(const AxRay &r, const SlabT &
     const bool tfirst = !(t.rt > rz);  // the t crossing comes first (ties: t)
     s.ok[1] = (tfirst && t.in_t2 && in_z1) || (!tfirst && t.in_t1 && in_z2);
     // (d*d + 1) * q >= q >= tmp, so the root's argument is >= 1 (rounding is
-    // monotone) and finite for any ray of a real geometry (q < 2^60 where
-    // div_by_tmp uses the reciprocal; the root of +inf is +inf in either form)
-    const float l = sqrt_finite_ge_tiny(div_by_tmp(r, t.d2p1 * r.q));
+    // monotone); where div_by_tmp uses the reciprocal (mk: q <= 2^60,
+    // tmp >= 2^-60, (d*d + 1) <= ~2) it is also finite
+    const float l = slab_root(r, t.d2p1 * r.q);
     const float lo = tfirst ? t.rt : rz, hi = tfirst ? rz : t.rt;
     // slice parts (cz - zlo) * ny * nx * unit, relative to the window's first
     // slice, in two 24-bit products (index_fits: cz * ny < 2^24;
@@ -668,9 +676,9 @@ __host__ __device__ __forceinline__ bool chord_points(const Cands<K> &c, float &
     return two;
 }
 
-// sqrtf of a squared chord: the same bits as sqrtf, without the rescaling
-// of tiny arguments where the argument is at least 2^-96 (sqrt_ge_tiny); a
-// shorter chord (two hits within 2^-48 voxel) takes sqrtf.
+// sqrtf of a squared chord: the same bits as sqrtf, by sqrt_ge_tiny where
+// the argument is at least 2^-96; a shorter chord (two hits within 2^-48
+// voxel) takes sqrtf.
 __host__ __device__ __forceinline__ float chord_sqrt(float x)
 {
     return x >= 0x1p-96f ? sqrt_ge_tiny(x) : sqrtf(x);
