@@ -787,8 +787,9 @@ __global__ __launch_bounds__(TX *TY *TZ) void atx_sorted_kernel(ViewGeom g,
 //  - the columns' footprints (footprint_col) and the tile's counters and
 //    barriers are shared by twice the voxels;
 //  - the 512 voxels are regrouped by pair count, largest first, and thread t
-//    walks sorted slots t and t + 256 one after the other, so each wave's
-//    lanes hold footprints of similar size in both walks;
+//    walks sorted slots t and 511 - t one after the other, so each wave's
+//    lanes hold footprints of similar size in both walks, and the waves'
+//    two walks together are of similar length;
 //  - a slot carries two words, the footprint's first bin (a_lo, b_lo) and its
 //    widths with the voxel's index in the tile, instead of eight; the thread
 //    that takes the slot recomputes the voxel centre from the index with the
@@ -869,7 +870,11 @@ __global__ __launch_bounds__(256) void atx_sorted_general_kernel(
     const BinsBuf B(rt, Y, M);
 #pragma unroll 1
     for (int h = 0; h < 2; h++) {
-        const unsigned a0 = s_w0[threadIdx.x + h * 256], a1 = s_w1[threadIdx.x + h * 256];
+        // the second walk takes the sorted slots in reverse, so wave 0 pairs
+        // the largest footprints of the first half with the smallest of the
+        // second: the four waves of a workgroup finish closer together
+        const unsigned sl = h ? 511u - threadIdx.x : threadIdx.x;
+        const unsigned a0 = s_w0[sl], a1 = s_w1[sl];
         if (a0 == ~0u) continue;
         const int me = (int)(a1 & 511u);
         const int jx = (int)tile_x * TX + me % TX, jy = (int)tile_y * TY + (me / TX) % TY,
