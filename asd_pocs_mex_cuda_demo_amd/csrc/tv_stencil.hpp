@@ -154,9 +154,8 @@ struct VoxelCursor {
 };
 
 // Square root of a stencil denominator eps + (sum of squares) >= eps = 1e-8.
-// In FP32 that argument is far above 2^-96, so the correctly rounded root
-// needs no rescaling of tiny arguments (sqrt_ge_tiny: the same bits as
-// sqrtf); FP64 roots are taken as they are.
+// In FP32 that argument is far above 2^-96, in the range where sqrt_ge_tiny
+// gives the same bits as sqrtf; FP64 roots are taken as they are.
 __device__ __forceinline__ float tv_sqrt(float x) { return sqrt_ge_tiny(x); }
 __device__ __forceinline__ double tv_sqrt(double x) { return sqrt(x); }
 
