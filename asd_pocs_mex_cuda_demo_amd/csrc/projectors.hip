@@ -72,14 +72,17 @@ AxShape ax_shape(int na, int nb, int nu)
 {
     AxShape s;
     s.lpr = ax_lanes_per_ray(na, nb, nu);
-    // Bundles of three rows at the same 8 lanes share each slab's transverse
-    // arithmetic among three rays (a third fewer waves, each three times as
-    // long).  Measured per view on MI355X (kernel alone, four angles): 256^3
-    // with 384 x 384 bins 105 -> 92 us, 512^3 with 1024 x 768 bins 850 ->
-    // 706 us; two rows gave 99 and 709 us.  Views with fewer rays (more than
-    // 8 lanes per ray) keep single rays, and so do detectors of fewer than
-    // 16 rows, where the last bundle's repeated rows would cost more.
-    if (s.lpr == 8 && nb >= 16) s.rows = 3;
+    // Bundles of rows at the same 8 lanes share each slab's transverse
+    // arithmetic among the bundle's rays (fewer waves, each longer); every
+    // ray's sum is the same at 2 and 3 rows.  Measured per view on MI355X in
+    // the plan: 256^3 with 384 x 384 bins 96.8 us with three rows against
+    // 98.6 with two (two slabs per load batch); 512^3 with 1024 x 768 bins
+    // 742 against 728 us, where two rows leave the chip four times as many
+    // waves to schedule.  So views of at least 2^19 rays take two rows, the
+    // others three.  Views with fewer rays (more than 8 lanes per ray) keep
+    // single rays, and so do detectors of fewer than 16 rows, where the last
+    // bundle's repeated rows would cost more.
+    if (s.lpr == 8 && nb >= 16) s.rows = (long)na * nb >= (1L << 19) ? 2 : 3;
     return s;
 }
 

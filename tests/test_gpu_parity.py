@@ -83,6 +83,10 @@ def test_row_bundles_match_single_rays_and_oracle(orc):
     pl = ops.Plan(base, [0.0, 90.0])
     assert pl.info()[:2] == (8, 3)
     pl.close()
+    # a view of at least 2^19 rays takes bundles of two rows (the same sums)
+    pl = ops.Plan(dict(base, na=1024, nb=768), [0.0])
+    assert pl.info()[:2] == (8, 2)
+    pl.close()
     rng = np.random.default_rng(5)
     X = rng.random(nx * ny * nz).astype(np.float32)
     for phi in (-0.4, -2.0):
