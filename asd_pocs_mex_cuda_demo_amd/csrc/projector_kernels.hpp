@@ -414,17 +414,18 @@ __device__ float struct_buffer_load_f32(__amdgpu_buffer_rsrc_t rsrc, unsigned in
 // Reads of a view's ray records and corrections by bin index (fewer than
 // 2^27 bins): structured buffer resources with strides of 32 and 4 bytes, so
 // the bin index is the load's operand as it is, with no address arithmetic.
-// Pointer form otherwise.  The record count of each resource is its size in
-// bytes, a bound at least as large as the count of elements; the walks only
-// read bins of the view.
+// Pointer form otherwise.  Each resource holds the view's M records (the
+// hardware checks a strided load's index against the record count); the
+// walks only read bins of the view.
 struct BinsBuf {
     __amdgpu_buffer_rsrc_t rr, ry;
     __device__ __forceinline__ BinsBuf(const RayRec *rt, const float *Y, unsigned M)
     {
         static_assert(sizeof(RayRec) == 32, "record stride");
-        // dword3 0x00020000: 32-bit data format, the form untyped loads use
-        rr = __builtin_amdgcn_make_buffer_rsrc((void *)rt, (short)32, (int)(M * 32u), 0x00020000);
-        ry = __builtin_amdgcn_make_buffer_rsrc((void *)Y, (short)4, (int)(M * 4u), 0x00020000);
+        // dword3 0x00020000: 32-bit data format, the form untyped loads use;
+        // both buffers end at bin M, so a read past it would return zeros
+        rr = __builtin_amdgcn_make_buffer_rsrc((void *)rt, (short)32, (int)M, 0x00020000);
+        ry = __builtin_amdgcn_make_buffer_rsrc((void *)Y, (short)4, (int)M, 0x00020000);
     }
     __device__ __forceinline__ RayRec rec(unsigned i) const
     {
