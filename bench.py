@@ -452,6 +452,11 @@ def accuracy_fan2d_512(threads):
     }
 
 
+def log(msg):
+    """A progress line on stderr (stdout carries only the JSON result)."""
+    print(f"bench.py: {msg}", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -502,17 +507,22 @@ def main():
                   "batch": r}
         print(json.dumps(result), flush=True)
         return
+    log(f"{args.workload}: {args.warmup} warm-up and {args.steps} timed iterations")
     result = headline(args)
+    log(f"{args.workload}: {result['value']:.4g} {result['unit']}")
     if not args.no_extras:
         threads = cpu_threads()
         extras = {}
         for name in ("fan2d_256", "fan2d_512", "cone3d_256"):
             if name != args.workload:
                 extras[name] = workload_rate(name, not args.no_cpu, threads)
+                log(f"{name}: {extras[name]['iters_per_sec']:.4g} iterations/s")
         result["workloads"] = extras
         result["batch64"] = batch_rates()
+        log("batch64 done")
         if not args.no_cpu:
             result["accuracy_vs_oracle"] = accuracy_fan2d_512(threads)
+            log("accuracy against the oracle done")
     print(json.dumps(result), flush=True)
 
 
