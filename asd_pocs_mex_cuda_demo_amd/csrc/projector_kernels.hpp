@@ -103,6 +103,32 @@ __global__ __launch_bounds__(256) void ax_kernel(ViewGeom g, const float *__rest
     if (active && lane == 0 && !stop) epi(id, (float)a, (float)b, pre);
 }
 
+// The tile (block of bundles) a workgroup of the row-bundle forward projector
+// takes.  xcd_block gives each XCD a contiguous band of row bundles; on views
+// of at least 2^19 rays, whose bands divide evenly into column blocks of RPB
+// bundles, the band's tiles are taken column block by column block from the
+// centre of the detector outwards, all of the band's row bundles for each, so
+// the longest rays (those through the middle of the volume) start first and
+// the short ones at the detector's edges fill the end of the launch.  The map
+// is a bijection on [0, ncb * nbb) (checked on the CPU); otherwise the tile is
+// xcd_block's.  512^3 view with 1024 x 768 bins: 727 -> 684 us; 256^3 with
+// 384 x 384 bins ran 1% slower with it, and keeps the plain order.
+__host__ __device__ __forceinline__ unsigned ax_rows_tile(unsigned b, unsigned nblk, int na, int nb,
+                                                          int nbb, int rpb)
+{
+    const unsigned tile = xcd_block(b, nblk);
+    const unsigned ncb = (unsigned)na / (unsigned)rpb, ubb = (unsigned)nbb;
+    if ((long)na * nb < (1L << 19) || (unsigned)na % (unsigned)rpb != 0 || ubb % 8u != 0 ||
+        nblk != ncb * ubb || ncb < 2)
+        return tile;
+    const unsigned nrbx = ubb / 8u, x = tile / (ncb * nrbx);
+    const unsigned k = tile - x * ncb * nrbx;
+    const unsigned ci = k / nrbx, r = k - ci * nrbx;
+    const unsigned half = ncb / 2u;
+    const unsigned cb = (ci & 1u) ? half - 1u - ci / 2u : half + ci / 2u;
+    return (x * nrbx + r) * ncb + cb;
+}
+
 // Forward projector over bundles of K rays that share a detector column (ia)
 // and take consecutive rows ib0 .. ib0 + K - 1.  Their in-plane lines are the
 // same, so each slab's transverse half (ax_slab_t: extent, cells, crossing
@@ -128,7 +154,8 @@ __global__ __launch_bounds__(256) void ax_rows_kernel(ViewGeom g, const float *_
     const int nbb = (g.nb + K - 1) / K;  // bundles per detector column
     const long nbund = (long)g.na * nbb;
     const int lane = threadIdx.x % LPR;
-    const long id = (long)xcd_block(blockIdx.x, gridDim.x) * RPB + threadIdx.x / LPR;
+    const long id = (long)ax_rows_tile(blockIdx.x, gridDim.x, g.na, g.nb, nbb, RPB) * RPB +
+                    threadIdx.x / LPR;
     const bool active = id < nbund;
     // id < na * nb < 2^31 (index_fits)
     const unsigned ub = active ? fastdiv((unsigned)id, g.div_na) : 0u;

@@ -96,6 +96,25 @@ float emul_det_coord(int i, int n, float det) { return det_coord(i, n, det); }
 // piece in the volume outside the bundle's [lo, hi] (must be 0); *kept and
 // *total receive the slabs the range keeps and all slabs, summed over
 // bundles.
+// ax_rows_tile over every block of a row-bundle launch (RPB = 32 bundles of
+// 8 lanes per block): returns 1 when it maps [0, nblk) onto itself, and the
+// number of blocks it moves in *moved.
+int emul_ax_rows_tile_check(int na, int nb, int rows, long *moved)
+{
+    const int nbb = (nb + rows - 1) / rows;
+    const long nbund = (long)na * nbb;
+    const unsigned nblk = (unsigned)((nbund * 8 + 255) / 256);
+    std::vector<char> hit(nblk, 0);
+    *moved = 0;
+    for (unsigned b = 0; b < nblk; b++) {
+        const unsigned t = ax_rows_tile(b, nblk, na, nb, nbb, 32);
+        if (t >= nblk || hit[t]) return 0;
+        hit[t] = 1;
+        *moved += t != xcd_block(b, nblk);
+    }
+    return 1;
+}
+
 long emul_ax_slab_range_misses(float c, float s, const float *ydet, const float *zdet, float SO,
                                float OD, float dz, int nx, int ny, int nz, int na, int nb, int K,
                                long *kept, long *total)

@@ -412,6 +412,24 @@ def test_forward_slab_range_keeps_every_slab_with_a_piece(orc):
     assert min(dropped) > 0.15, dropped
 
 
+def test_forward_tile_order_is_a_bijection():
+    """The row-bundle forward projector's longest-first tile order
+    (ax_rows_tile) maps the launch's blocks onto its tiles one to one, so
+    every bundle is projected once; it reorders the 1024 x 768 detector of the
+    512^3 workload and leaves smaller views, and shapes whose bands do not
+    divide evenly, in xcd_block's order."""
+    lib = emul()
+    lib.emul_ax_rows_tile_check.restype = C.c_int
+    lib.emul_ax_rows_tile_check.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p]
+    moved = C.c_long()
+    cases = [(1024, 768, 2, True), (1024, 768, 3, True), (2048, 1536, 2, True),
+             (384, 384, 3, False), (384, 384, 2, False), (1000, 768, 2, False),
+             (1024, 770, 2, False), (1024, 1, 1, False), (768, 1, 1, False), (4096, 128, 2, True)]
+    for na, nb, rows, reordered in cases:
+        assert lib.emul_ax_rows_tile_check(na, nb, rows, C.byref(moved)) == 1, (na, nb, rows)
+        assert (moved.value > 0) == reordered, (na, nb, rows, moved.value)
+
+
 def test_footprint_cut_keeps_every_nonzero_pair_at_512_cubed(orc):
     """At the 512^3 workload's geometry (a 1200-voxel source distance, where
     find_l_3d's FP32 rounding is largest), the cut footprint holds every
