@@ -448,8 +448,9 @@ struct EpiTune {
     }
 };
 
-// Times the bit-identical backprojector variants on view 0 (scratch outputs,
-// before the first sweep) and keeps the fastest.  The choice only changes
+// Times the bit-identical backprojector variants on three views (the first,
+// and those an eighth and a quarter of the way round; scratch outputs, before
+// the first sweep) and keeps the fastest on average.  The choice only changes
 // speed, never results.
 template <bool KAHAN>
 int autotune_atx_t(asd_plan *p);
@@ -470,6 +471,12 @@ int autotune_atx(asd_plan *p)
 template <bool KAHAN>
 int autotune_atx_t(asd_plan *p)
 {
+    // the views timed: the first, and those an eighth and a quarter of the
+    // way round (a variant's speed varies with the angle; one view misled
+    // near-ties between sorted tiles)
+    std::vector<int> tv = {0};
+    for (int v : {p->nv / 8, p->nv / 4})
+        if (v > tv.back()) tv.push_back(v);
     const ViewGeom &g = p->views[0];
     int rc = ASD_OK;
     ASD_HIP(hipMemsetAsync(p->cor.p, 0, p->M * 4, p->stream));
@@ -488,21 +495,28 @@ int autotune_atx_t(asd_plan *p)
     int best = ATX_DIRECT;
     double best_us = 1e30;
     for (int v : cand) {
-        float ms_min = 1e30f;
-        for (int rep = 0; rep < 4; rep++) {
-            // the kernel's own execution time (events stamped by the dispatch)
-            LaunchTiming lt{e0, e1, 0};
-            t_launch_timing = &lt;
-            rc = dispatch_atx_variant<KAHAN>(v, g, p->cor.as<float>(), p->rt.as<RayRec>(), ep,
-                                             p->stream);
-            t_launch_timing = nullptr;
+        float ms_sum = 0.0f;
+        for (int iv : tv) {
+            float ms_min = 1e30f;
+            for (int rep = 0; rep < 4; rep++) {
+                // the kernel's own execution time (events stamped by the dispatch)
+                LaunchTiming lt{e0, e1, 0};
+                t_launch_timing = &lt;
+                rc = dispatch_atx_variant<KAHAN>(v, p->views[iv], p->cor.as<float>(),
+                                                 p->rt.as<RayRec>() + (size_t)iv * p->M, ep,
+                                                 p->stream);
+                t_launch_timing = nullptr;
+                if (rc) break;
+                (void)hipEventSynchronize(e1);
+                float ms = 0;
+                if (rep > 0 && hipEventElapsedTime(&ms, e0, e1) == hipSuccess && ms < ms_min)
+                    ms_min = ms;
+            }
             if (rc) break;
-            (void)hipEventSynchronize(e1);
-            float ms = 0;
-            if (rep > 0 && hipEventElapsedTime(&ms, e0, e1) == hipSuccess && ms < ms_min)
-                ms_min = ms;
+            ms_sum += ms_min;
         }
         if (rc) break;
+        const float ms_min = ms_sum / (float)tv.size();  // mean over the timed views
         p->atx_us[v] = ms_min * 1e3;
         // a later candidate must be faster by more than 3%: near-ties keep the
         // earlier one, so the choice does not flip between runs on timing noise

@@ -226,7 +226,9 @@ int asd_plan_set_graph(asd_plan *plan, int enable);
  * with two-pair load batches, 6 direct one pair at a time, 7 to 9 two-pair
  * walks of 16x16x1 / 8x8x4 / 8x4x8 tiles regrouped by footprint size; -1
  * before the first sweep) and the measured time of each (atx_us[10],
- * microseconds, -1 = not tried).  All variants give bit-identical results. */
+ * microseconds per view, the mean over the three views timed: the first and
+ * those an eighth and a quarter of the way round; -1 = not tried).  All
+ * variants give bit-identical results. */
 int asd_plan_tuning(asd_plan *plan, int *atx_variant, double *atx_us);
 /* Forces the backprojector variant (0..9, as above; the tiles with four
  * slices need nz > 1) or, with -1, asks for a new measurement on the next
