@@ -115,6 +115,25 @@ int emul_ax_rows_tile_check(int na, int nb, int rows, long *moved)
     return 1;
 }
 
+// atx_tile_z_interleave over a launch of per_layer * nlayers tiles in
+// xcd_block's order: returns 1 when it maps the tiles onto themselves, and in
+// spread[8] the number of distinct eighths of the volume's height each XCD's
+// tiles come from.
+int emul_atx_tile_z_check(unsigned per_layer, unsigned nlayers, int *spread)
+{
+    const unsigned n = per_layer * nlayers;
+    std::vector<char> hit(n, 0);
+    unsigned seen[8] = {0};
+    for (unsigned b = 0; b < n; b++) {
+        const unsigned t = atx_tile_z_interleave(xcd_block(b, n), per_layer, nlayers);
+        if (t >= n || hit[t]) return 0;
+        hit[t] = 1;
+        if (n >= 16) seen[b % 8] |= 1u << ((t / per_layer) * 8u / nlayers);
+    }
+    for (int x = 0; x < 8; x++) spread[x] = __builtin_popcount(seen[x]);
+    return 1;
+}
+
 long emul_ax_slab_range_misses(float c, float s, const float *ydet, const float *zdet, float SO,
                                float OD, float dz, int nx, int ny, int nz, int na, int nb, int K,
                                long *kept, long *total)

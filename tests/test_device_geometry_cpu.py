@@ -430,6 +430,27 @@ def test_forward_tile_order_is_a_bijection():
         assert (moved.value > 0) == reordered, (na, nb, rows, moved.value)
 
 
+def test_backprojector_tile_order_is_a_bijection():
+    """The sorted 3-D backprojector's z-interleaved tile order
+    (atx_tile_z_interleave) maps a launch's blocks onto its tiles one to one,
+    and where it applies gives every XCD (workgroup b runs on XCD b % 8)
+    tiles from min(8, layers / 8) eighths of the volume's height instead of
+    one; layer counts that do not split into whole blocks keep the plain
+    order."""
+    lib = emul()
+    lib.emul_atx_tile_z_check.restype = C.c_int
+    lib.emul_atx_tile_z_check.argtypes = [C.c_uint, C.c_uint, C.c_void_p]
+    spread = (C.c_int * 8)()
+    # (tiles per layer, layers, interleaved): 512^3 with 16x16x2, 8x8x8 and
+    # 8x4x16 tiles, 256^3 with 16x16x2 and 8x8x8, then shapes left alone
+    for per, nl, inter in [(32 * 32, 256, True), (64 * 64, 64, True), (64 * 128, 32, True),
+                           (16 * 16, 128, True), (32 * 32, 32, True), (8, 24, True),
+                           (6, 8, False), (5, 100, False), (3, 7, False)]:
+        assert lib.emul_atx_tile_z_check(per, nl, spread) == 1, (per, nl)
+        if inter:
+            assert all(v == min(8, nl // 8) for v in spread), (per, nl, list(spread))
+
+
 def test_footprint_cut_keeps_every_nonzero_pair_at_512_cubed(orc):
     """At the 512^3 workload's geometry (a 1200-voxel source distance, where
     find_l_3d's FP32 rounding is largest), the cut footprint holds every
