@@ -843,19 +843,20 @@ __global__ __launch_bounds__(TX *TY *TZ) void atx_sorted_kernel(ViewGeom g,
 // Each voxel's terms and their order are those of atx_sorted_kernel.
 // Tile t (from xcd_block, which gives each XCD a contiguous range of tiles) of
 // a backprojector launch whose tiles are ordered x fastest, then y, then
-// layer (TZ slices): the layers are cut into B blocks (B = 64, or one layer
-// per block when there are fewer) and XCD x takes blocks x, x + 8, x + 16,
-// ... instead of one contiguous band of layers.  Voxels near the top and
+// layer (TZ slices): XCD x takes layers x, x + 8, x + 16, ... instead of one
+// contiguous band of layers.  Voxels near the top and
 // bottom of the volume, where the cone leaves the detector, have fewer pairs
 // (at 512^3 the outer eighths hold 75-78% of the mean work, the others
 // 107-108%), and the launch ends with the XCD holding the most work; spread
 // over the whole height every XCD holds about the same.  The map is a
-// bijection on [0, per_layer * nlayers) (checked on the CPU); counts of
-// layers that do not split into whole blocks keep xcd_block's order.
+// bijection on [0, per_layer * nlayers) (checked on the CPU); layer counts
+// that are not a multiple of 8 (or below 16) keep xcd_block's order.  Blocks
+// of 4 layers per XCD turn (16x16x2 tiles at 512^3) were 1.6% slower than
+// single layers.
 __host__ __device__ __forceinline__ unsigned atx_tile_z_interleave(unsigned t, unsigned per_layer,
                                                                    unsigned nlayers)
 {
-    const unsigned B = nlayers % 64u == 0 ? 64u : nlayers;
+    const unsigned B = nlayers;  // blocks of one layer
     if (B % 8u != 0 || B < 16u) return t;
     const unsigned k = nlayers / B, q = B / 8u;
     const unsigned layer = t / per_layer, in = t - layer * per_layer;
